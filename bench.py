@@ -1,0 +1,205 @@
+"""Throughput of the MI355X CloverLeaf hydro step on one GPU.
+
+One step = one pass of driver/hydro.cpp:57-89 (timestep with the calc_dt
+minimum, PdV predict/correct, accelerate, flux_calc, both advection sweeps,
+reset_field) over the whole chunk, everything resident in HBM.  Default
+workload: InputDecks/clover_bm256_short.in (15360 x 15360 cells, the two-state
+ideal-gas box, FP64).  Metric: cell-steps/s = x_cells * y_cells * steps / wall
+time (driver/hydro.cpp:213-217).
+
+Prints ONE JSON line.  `roofline` is the dominant kernel (largest share of the
+timed region by HIP events on the chunk's stream): achieved = its algorithmic
+bytes per launch / its mean launch duration, against 8 TB/s.  `cpu_baseline`
+times the OpenMP CPU oracle (oracle/liboracle.so) on a bounded sample of the
+same problem.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--plan fused|reference]
+                       [--deck NAME] [--no-cpu] [--profile-out FILE]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, public spec
+
+# Algorithmic bytes per launch: distinct FP64 2D arrays each launch reads or
+# writes once over its own iteration range (reference kernel granularity,
+# src/serial/<kernel>.cpp loop ranges), x 8 bytes.  Halos, 1D mesh arrays and
+# reduction partials are excluded.  Extents: C = nx*ny cells, N = (nx+1)(ny+1)
+# nodes, G = (nx+4)(ny+4) with ghosts, X = (nx+2)*ny / nx*(ny+2) flux faces,
+# NX = (nx+4)(ny+1) node-flux range, NM = (nx+3)(ny+1) node-mass range,
+# MF = (nx+2)(ny+1) momentum-flux range.
+def _extents(nx, ny):
+    return {"C": nx * ny, "N": (nx + 1) * (ny + 1), "G": (nx + 4) * (ny + 4), "X": (nx + 2) * ny,
+            "Y": nx * (ny + 2), "NX": (nx + 4) * (ny + 1), "NY": (nx + 1) * (ny + 4), "NMX": (nx + 3) * (ny + 1),
+            "NMY": (nx + 1) * (ny + 3), "MFX": (nx + 2) * (ny + 1), "MFY": (nx + 1) * (ny + 2)}
+
+
+# name -> list of (arrays, extent) terms per launch
+ALG_TERMS = {
+    "ideal_gas": [(4, "C")],                   # rho, e -> p, c
+    "viscosity": [(5, "C")],                   # u0, v0, p, rho0 -> q
+    "calc_dt": [(8, "C")],                     # c, q, rho0, u0, v0, xarea, yarea, vol
+    "PdV_predict": [(11, "C")],                # areas, vol, rho0, e0, p, q, u0, v0 -> rho1, e1
+    "PdV_correct": [(13, "C")],                # + u1, v1
+    "revert": [(4, "C")],
+    "accelerate": [(10, "N")],                 # rho0, vol, areas, p, q, u0, v0 -> u1, v1
+    "flux_calc": [(8, "N")],                   # areas, u0, v0, u1, v1 -> vfx, vfy
+    "reset_field": [(4, "C"), (4, "N")],
+    "advec_cell_vol": [(5, "G")],              # vol, vfx, vfy -> pre, post (sweep 2: 4)
+    "advec_cell_flux_x": [(6, "X")],           # vfx, pre, rho1, e1 -> mfx, ener_flux
+    "advec_cell_flux_y": [(6, "Y")],
+    "advec_cell_update_x": [(8, "C")],         # rho1, e1, pre, mf, ef, vf -> rho1, e1
+    "advec_cell_update_y": [(8, "C")],
+    "advec_mom_vol": [(5, "G")],               # vol, vfx, vfy -> post, pre (sweep 2: 4)
+    "advec_mom_node_flux_x": [(2, "NX")],
+    "advec_mom_node_flux_y": [(2, "NY")],
+    "advec_mom_node_mass_x": [(5, "NMX")],     # rho1, post_vol, node_flux -> post, pre
+    "advec_mom_node_mass_y": [(5, "NMY")],
+    "advec_mom_flux_x": [(4, "MFX")],          # node_flux, mass_pre, vel -> mom_flux
+    "advec_mom_flux_y": [(4, "MFY")],
+    "advec_mom_update_x": [(5, "N")],          # vel, pre, mom_flux, post -> vel
+    "advec_mom_update_y": [(5, "N")],
+    "field_summary": [(6, "C")],
+}
+B_ALG_PER_CELL_STEP = 1440.0  # 180 array sweeps of 8 B per interior cell-step at reference kernel granularity
+
+
+def alg_bytes(name, nx, ny):
+    ext = _extents(nx, ny)
+    terms = ALG_TERMS.get(name)
+    if terms is None:
+        return None
+    return sum(k * ext[e] for k, e in terms) * 8.0
+
+
+def cpu_baseline(deck_text, steps):
+    """The OpenMP oracle on a bounded sample: same problem (deck geometry and states)
+    on a 3840^2 grid, `steps` steps."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import clover_testlib as T
+    d = T.parse_deck(deck_text)
+    d.x_cells = d.y_cells = 3840
+    # the 1/100-cell state shrink depends on the cell count: re-read with the sample size
+    text = deck_text.replace(f"x_cells={T.parse_deck(deck_text).x_cells}", "x_cells=3840").replace(
+        f"y_cells={T.parse_deck(deck_text).y_cells}", "y_cells=3840")
+    d = T.parse_deck(text)
+    o = T.Oracle(d)
+    o.start()
+    o.step()  # first-touch / thread start-up outside the timed sample
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if o.step():
+            raise RuntimeError("oracle: small timestep")
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {"value": d.x_cells * d.y_cells * steps / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (OpenMP, {threads} threads), same two-state problem on 3840x3840 cells, "
+                      f"steps 2..{steps + 1}, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=87)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plan", choices=["fused", "reference"], default="reference")
+    ap.add_argument("--deck", default="clover_bm256_short")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--profile-out", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (one chunk per GPU); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import cloverleaf_amd as clv
+    deck_path = REPO / "tests" / "golden" / "decks" / f"{args.deck}.in"
+    deck_text = deck_path.read_text()
+    cfg = clv.parse_deck(deck_text)
+    plan = clv.PLAN_FUSED if args.plan == "fused" else clv.PLAN_REFERENCE
+    nx, ny = cfg.x_cells, cfg.y_cells
+    cfg.end_step = 1 << 30  # the bench decides the step count
+    cfg.end_time = 1e30
+
+    print(f"# bench: {args.deck} {nx}x{ny} plan={args.plan} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
+    ch = clv.Chunk(nx, ny)
+    st, s0 = ch.start(cfg)
+    for _ in range(args.warmup):
+        ch.step(cfg, st, plan)
+    ch.synchronize()
+    ch.profile(True)
+    ch.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ch.step(cfg, st, plan)
+    ch.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ch.profile_read()
+    ch.profile(False)
+    s_end = ch.summary()
+    print(f"# bench: {args.steps} steps in {wall:.3f} s, last dt {st.dt:.6g}, step {st.step}", file=sys.stderr, flush=True)
+
+    cell_steps = float(nx) * ny * args.steps
+    value = cell_steps / wall
+    # dominant kernel
+    kern = sorted(((ms, name, n) for name, (n, ms) in prof.items()), reverse=True)
+    total_kernel_ms = sum(ms for ms, _, _ in kern)
+    dom_ms, dom, dom_n = kern[0]
+    dom_bytes = alg_bytes(dom, nx, ny)
+    avg_s = dom_ms / dom_n / 1e3
+    achieved = dom_bytes / avg_s / 1e9 if dom_bytes else None
+    traffic = None
+    pmc = REPO / "profiles" / f"{dom}_{args.deck}_{args.plan}_traffic.json"
+    if pmc.exists():
+        traffic = json.loads(pmc.read_text()).get("bytes_per_launch")
+    per_kernel = {}
+    for ms, name, n in kern:
+        b = alg_bytes(name, nx, ny)
+        per_kernel[name] = {"launches": n, "total_ms": round(ms, 3), "share": round(ms / total_kernel_ms, 4),
+                            "avg_us": round(ms / n * 1e3, 2),
+                            "alg_GBps": round(b / (ms / n / 1e3) / 1e9, 1) if b else None}
+    if args.profile_out:
+        Path(args.profile_out).write_text(json.dumps({"deck": args.deck, "plan": args.plan, "steps": args.steps,
+                                                      "wall_s": wall, "kernels": per_kernel}, indent=1))
+    out = {
+        "metric": "Gcell-steps/s + % HBM peak at 15360^2 FP64 (cell-steps/s, clover_bm256_short)",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic (deck {args.deck}.in generated on the GPU)",
+        "config": {"workload": f"{args.deck}: {nx}x{ny} cells two-state ideal gas, FP64, steps "
+                               f"{args.warmup + 1}..{args.warmup + args.steps}",
+                   "x_cells": nx, "y_cells": ny, "plan": args.plan, "parallelism": "1 GPU, 1 chunk"},
+        "gcell_steps_per_s": value / 1e9,
+        "effective_hbm_frac_at_1440B": value * B_ALG_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "alg_bytes_per_launch": dom_bytes, "avg_launch_us": avg_s * 1e6, "share_of_kernel_time": dom_ms / total_kernel_ms},
+        "kernel_time_fraction_of_wall": total_kernel_ms / 1e3 / wall,
+        "final_summary": {"step": st.step, "ke": s_end[3], "mass": s_end[1]},
+    }
+    ch.close()
+    if not args.no_cpu:
+        print("# bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(deck_text, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,322 @@
+"""cloverleaf_amd -- MI355X-native CloverLeaf hydro kernels (gfx950, FP64).
+
+Python face of the C ABI in include/clover_mi355x.h (libclover_mi355x.so, built
+by ``make -C cloverleaf_amd/csrc``).  The HIP library is the only compute path:
+importing the package does not need a GPU, but every compute call goes through
+the library and raises if it is missing or fails; there is no CPU fallback.
+
+    from cloverleaf_amd import Chunk, parse_deck
+    cfg = parse_deck(open("clover_bm_short.in").read())
+    ch = Chunk(cfg.x_cells, cfg.y_cells)
+    st, s0 = ch.start(cfg)
+    while not st.complete:
+        ch.step(cfg, st)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libclover_mi355x.so"
+
+# field ids (include/clover_mi355x.h; 0..14 = driver/definitions.h:62-78)
+FIELDS = ["density0", "density1", "energy0", "energy1", "pressure", "viscosity", "soundspeed",
+          "xvel0", "xvel1", "yvel0", "yvel1", "vol_flux_x", "vol_flux_y", "mass_flux_x", "mass_flux_y",
+          "work1", "work2", "work3", "work4", "work5", "work6", "work7", "volume", "xarea", "yarea"]
+FID = {n: i for i, n in enumerate(FIELDS)}
+MESH = ["cellx", "celly", "vertexx", "vertexy", "celldx", "celldy", "vertexdx", "vertexdy"]
+MID = {n: i for i, n in enumerate(MESH)}
+PLAN_REFERENCE, PLAN_FUSED = 0, 1
+MAX_STATES = 16
+ERR_SMALL_DT = -4
+
+
+class CloverError(RuntimeError):
+    pass
+
+
+class SmallTimestep(CloverError):
+    """driver/timestep.cpp:104-106 -- dt fell below dtmin."""
+
+
+class _State(C.Structure):
+    _fields_ = [("density", C.c_double), ("energy", C.c_double), ("xvel", C.c_double), ("yvel", C.c_double),
+                ("geometry", C.c_int), ("xmin", C.c_double), ("ymin", C.c_double), ("xmax", C.c_double),
+                ("ymax", C.c_double), ("radius", C.c_double)]
+
+
+class Config(C.Structure):
+    """clv_config: the deck (driver/definitions.h:97-155)."""
+    _fields_ = [("xmin", C.c_double), ("ymin", C.c_double), ("xmax", C.c_double), ("ymax", C.c_double),
+                ("x_cells", C.c_int), ("y_cells", C.c_int),
+                ("dtinit", C.c_double), ("dtmin", C.c_double), ("dtmax", C.c_double), ("dtrise", C.c_double),
+                ("dtc_safe", C.c_double), ("dtu_safe", C.c_double), ("dtv_safe", C.c_double),
+                ("dtdiv_safe", C.c_double), ("end_time", C.c_double), ("end_step", C.c_int),
+                ("summary_frequency", C.c_int), ("visit_frequency", C.c_int), ("tiles_per_chunk", C.c_int),
+                ("test_problem", C.c_int), ("profiler_on", C.c_int), ("number_of_states", C.c_int),
+                ("states", _State * MAX_STATES)]
+
+
+class HydroState(C.Structure):
+    """clv_hydro_state (driver/definitions.h:265-274)."""
+    _fields_ = [("step", C.c_int), ("advect_x", C.c_int), ("time", C.c_double), ("dt", C.c_double),
+                ("dtold", C.c_double), ("jdt", C.c_int), ("kdt", C.c_int), ("dt_control", C.c_int),
+                ("complete", C.c_int)]
+
+
+class ChunkView(C.Structure):
+    _fields_ = [("x_cells", C.c_int), ("y_cells", C.c_int), ("pitch", C.c_int64),
+                ("field", C.c_void_p * 25), ("mesh", C.c_void_p * 8)]
+
+
+_LIB = None
+
+# (name, restype, argtypes) of every exported entry point
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int)
+SIGNATURES = [
+    ("clv_last_error", C.c_char_p, []),
+    ("clv_version", C.c_char_p, []),
+    ("clv_device_count", _I, []),
+    ("clv_chunk_create", _I, [_I, _I, C.POINTER(_P)]),
+    ("clv_chunk_wrap", _I, [C.POINTER(ChunkView), C.POINTER(_P)]),
+    ("clv_chunk_destroy", _I, [_P]),
+    ("clv_chunk_get_view", _I, [_P, C.POINTER(ChunkView)]),
+    ("clv_chunk_set_stream", _I, [_P, _P]),
+    ("clv_chunk_synchronize", _I, [_P]),
+    ("clv_chunk_bytes", C.c_size_t, [_P]),
+    ("clv_field_upload", _I, [_P, _I, _PD, C.c_int64]),
+    ("clv_field_download", _I, [_P, _I, _PD, C.c_int64]),
+    ("clv_mesh_upload", _I, [_P, _I, _PD]),
+    ("clv_mesh_download", _I, [_P, _I, _PD]),
+    ("clv_build_field", _I, [_P]),
+    ("clv_initialise_chunk", _I, [_P, C.POINTER(Config)]),
+    ("clv_generate_chunk", _I, [_P, C.POINTER(Config)]),
+    ("clv_ideal_gas", _I, [_P, _I]),
+    ("clv_viscosity", _I, [_P]),
+    ("clv_calc_dt", _I, [_P, _D, _D, _D, _D, _D, _PD, _PI, _PI, _PI, _PI]),
+    ("clv_PdV", _I, [_P, _I, _D]),
+    ("clv_PdV_driver", _I, [_P, _I, _D]),
+    ("clv_revert", _I, [_P]),
+    ("clv_accelerate", _I, [_P, _D]),
+    ("clv_flux_calc", _I, [_P, _D]),
+    ("clv_advec_cell", _I, [_P, _I, _I]),
+    ("clv_advec_mom", _I, [_P, _I, _I, _I]),
+    ("clv_reset_field", _I, [_P]),
+    ("clv_update_halo", _I, [_P, _PI, _I]),
+    ("clv_field_summary", _I, [_P, _PD]),
+    ("clv_deck_parse", _I, [C.c_char_p, C.POINTER(Config)]),
+    ("clv_hydro_start", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _PD]),
+    ("clv_timestep", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _I]),
+    ("clv_hydro_step", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _I]),
+    ("clv_hydro_summary", _I, [_P, _PD]),
+    ("clv_profile_enable", _I, [_P, _I]),
+    ("clv_profile_read", _I, [_P, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), _PD, _I]),
+    ("clv_profile_reset", _I, [_P]),
+]
+
+
+def lib() -> C.CDLL:
+    """Load libclover_mi355x.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise CloverError(f"{LIB_PATH} is missing: build it with `make -C {PKG / 'csrc'}` "
+                              "(or __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().clv_last_error().decode()
+        if rc == ERR_SMALL_DT:
+            raise SmallTimestep(f"{what}: {msg}")
+        raise CloverError(f"{what} failed ({rc}): {msg}")
+
+
+def parse_deck(text: str) -> Config:
+    """Deck text -> Config, with driver/read_input.cpp semantics (done by the library)."""
+    cfg = Config()
+    _check(lib().clv_deck_parse(text.encode(), C.byref(cfg)), "clv_deck_parse")
+    return cfg
+
+
+def _mask(names) -> "C.Array":
+    m = (C.c_int * 15)()
+    for n in names:
+        m[FID[n] if isinstance(n, str) else int(n)] = 1
+    return m
+
+
+class Chunk:
+    """One chunk resident on the current GPU (one arena allocation)."""
+
+    def __init__(self, x_cells: int, y_cells: int):
+        self.L = lib()
+        self.nx, self.ny = int(x_cells), int(y_cells)
+        h = C.c_void_p()
+        _check(self.L.clv_chunk_create(self.nx, self.ny, C.byref(h)), "clv_chunk_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.clv_chunk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- data movement (host arrays in the oracle layout: (ny+5, nx+5)) ----
+    def download(self, name: str) -> np.ndarray:
+        a = np.empty((self.ny + 5, self.nx + 5), dtype=np.float64)
+        _check(self.L.clv_field_download(self.h, FID[name], a.ctypes.data_as(_PD), self.nx + 5), "download")
+        return a
+
+    def upload(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.ny + 5, self.nx + 5), a.shape
+        _check(self.L.clv_field_upload(self.h, FID[name], a.ctypes.data_as(_PD), self.nx + 5), "upload")
+
+    def download_mesh(self, name: str) -> np.ndarray:
+        n = self.nx + 5 if name in ("cellx", "vertexx", "celldx", "vertexdx") else self.ny + 5
+        a = np.empty(n, dtype=np.float64)
+        _check(self.L.clv_mesh_download(self.h, MID[name], a.ctypes.data_as(_PD)), "download_mesh")
+        return a
+
+    def upload_mesh(self, name: str, a: np.ndarray):
+        n = self.nx + 5 if name in ("cellx", "vertexx", "celldx", "vertexdx") else self.ny + 5
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (n,), a.shape
+        _check(self.L.clv_mesh_upload(self.h, MID[name], a.ctypes.data_as(_PD)), "upload_mesh")
+
+    def synchronize(self):
+        _check(self.L.clv_chunk_synchronize(self.h), "synchronize")
+
+    def view(self) -> ChunkView:
+        v = ChunkView()
+        _check(self.L.clv_chunk_get_view(self.h, C.byref(v)), "view")
+        return v
+
+    # ---- per-kernel entry points ----
+    def ideal_gas(self, predict=False):
+        _check(self.L.clv_ideal_gas(self.h, int(bool(predict))), "ideal_gas")
+
+    def viscosity(self):
+        _check(self.L.clv_viscosity(self.h), "viscosity")
+
+    def calc_dt(self, dtmin=1e-7, dtc_safe=0.7, dtu_safe=0.5, dtv_safe=0.5, dtdiv_safe=0.7):
+        dt = C.c_double()
+        ctl, jl, kl, small = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(self.L.clv_calc_dt(self.h, dtmin, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, C.byref(dt), C.byref(ctl),
+                                  C.byref(jl), C.byref(kl), C.byref(small)), "calc_dt")
+        return dt.value, ctl.value, jl.value, kl.value, small.value
+
+    def PdV(self, predict, dt):
+        _check(self.L.clv_PdV(self.h, int(bool(predict)), float(dt)), "PdV")
+
+    def PdV_driver(self, predict, dt):
+        _check(self.L.clv_PdV_driver(self.h, int(bool(predict)), float(dt)), "PdV_driver")
+
+    def revert(self):
+        _check(self.L.clv_revert(self.h), "revert")
+
+    def accelerate(self, dt):
+        _check(self.L.clv_accelerate(self.h, float(dt)), "accelerate")
+
+    def flux_calc(self, dt):
+        _check(self.L.clv_flux_calc(self.h, float(dt)), "flux_calc")
+
+    def advec_cell(self, sweep_number, direction):
+        _check(self.L.clv_advec_cell(self.h, sweep_number, direction), "advec_cell")
+
+    def advec_mom(self, which_vel, direction, sweep_number):
+        _check(self.L.clv_advec_mom(self.h, which_vel, direction, sweep_number), "advec_mom")
+
+    def reset_field(self):
+        _check(self.L.clv_reset_field(self.h), "reset_field")
+
+    def update_halo(self, fields, depth):
+        _check(self.L.clv_update_halo(self.h, _mask(fields), depth), "update_halo")
+
+    def field_summary(self):
+        out = (C.c_double * 5)()
+        _check(self.L.clv_field_summary(self.h, out), "field_summary")
+        return list(out)
+
+    def build_field(self):
+        _check(self.L.clv_build_field(self.h), "build_field")
+
+    def initialise_chunk(self, cfg: Config):
+        _check(self.L.clv_initialise_chunk(self.h, C.byref(cfg)), "initialise_chunk")
+
+    def generate_chunk(self, cfg: Config):
+        _check(self.L.clv_generate_chunk(self.h, C.byref(cfg)), "generate_chunk")
+
+    # ---- whole-step forms ----
+    def start(self, cfg: Config):
+        st = HydroState()
+        out = (C.c_double * 5)()
+        _check(self.L.clv_hydro_start(self.h, C.byref(cfg), C.byref(st), out), "hydro_start")
+        return st, list(out)
+
+    def step(self, cfg: Config, st: HydroState, plan: int = PLAN_FUSED):
+        _check(self.L.clv_hydro_step(self.h, C.byref(cfg), C.byref(st), plan), "hydro_step")
+        return st
+
+    def summary(self):
+        out = (C.c_double * 5)()
+        _check(self.L.clv_hydro_summary(self.h, out), "hydro_summary")
+        return list(out)
+
+    # ---- measurement ----
+    def profile(self, on=True):
+        _check(self.L.clv_profile_enable(self.h, int(bool(on))), "profile_enable")
+
+    def profile_reset(self):
+        _check(self.L.clv_profile_reset(self.h), "profile_reset")
+
+    def profile_read(self) -> dict:
+        n = 128
+        names = (C.c_char_p * n)()
+        launches = (C.c_int64 * n)()
+        ms = (C.c_double * n)()
+        k = self.L.clv_profile_read(self.h, names, launches, ms, n)
+        return {names[i].decode(): (int(launches[i]), float(ms[i])) for i in range(min(k, n))}
+
+
+def run(cfg: Config, plan: int = PLAN_FUSED, trace: bool = False):
+    """Run a deck to completion (driver/hydro.cpp:53-111); returns the summary records."""
+    recs = []
+    with Chunk(cfg.x_cells, cfg.y_cells) as ch:
+        st, s = ch.start(cfg)
+        recs.append(dict(step=0, time=st.time, dt=st.dt, vol=s[0], mass=s[1], ie=s[2], ke=s[3], press=s[4]))
+        while True:
+            ch.step(cfg, st, plan)
+            if trace or st.complete or (cfg.summary_frequency and st.step % cfg.summary_frequency == 0):
+                s = ch.summary()
+                recs.append(dict(step=st.step, time=st.time, dt=st.dt, vol=s[0], mass=s[1], ie=s[2], ke=s[3],
+                                 press=s[4]))
+            if st.complete:
+                return recs

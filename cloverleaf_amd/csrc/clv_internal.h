@@ -1,0 +1,78 @@
+// Internal declarations shared by the gfx950 kernels and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "clover_mi355x.h"
+
+namespace clv {
+
+constexpr double g_small = 1.0e-16;  // driver/definitions.h:32
+constexpr double g_big = 1.0e+21;    // driver/definitions.h:33
+
+// Kernel-side view of a chunk: one base pointer per array, one row pitch.
+// Passed by value (kernarg), 288 bytes.
+struct Dev {
+  double *f[CLV_NUM_FIELDS];
+  double *m[CLV_NUM_MESH];
+  int nx, ny;
+  int64_t pitch;
+};
+
+// Row pitch for an x_cells-wide chunk: holds nx + 5 doubles, a multiple of 16
+// doubles (128 B) so that with the arena offset below every row's first
+// interior element (i = 2) starts a 128-byte line.
+inline int64_t pitch_for(int nx) { return ((int64_t)nx + 5 + 15) / 16 * 16; }
+// doubles placed before element (0, 0) of each array so that element (2, j) is 128 B aligned
+constexpr int64_t kLeadPad = 14;
+
+// --- launch helpers ----------------------------------------------------------------------
+struct Profiler;
+
+}  // namespace clv
+
+struct clv_chunk {
+  clv::Dev d{};
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
+  bool owns = false;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // reduction scratch: block partials (device) and results (pinned host)
+  double *partials = nullptr;
+  size_t partials_len = 0;
+  double *result_dev = nullptr;
+  double *result_host = nullptr;
+  clv::Profiler *prof = nullptr;
+};
+
+namespace clv {
+
+clv_status set_error(clv_status code, const std::string &msg);
+clv_status hip_check(hipError_t e, const char *what);
+// after a launch: reports launch errors; records profiling events when enabled
+clv_status after_launch(clv_chunk *c, const char *name);
+void before_launch(clv_chunk *c, const char *name);
+
+// Kernel wrappers (defined in the .hip files), each returning CLV_OK or an error.
+clv_status k_fill_zero(clv_chunk *c);
+clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg);
+clv_status k_generate_chunk(clv_chunk *c, const clv_config *cfg);
+clv_status k_ideal_gas(clv_chunk *c, int predict);
+clv_status k_viscosity(clv_chunk *c);
+clv_status k_PdV(clv_chunk *c, int predict, double dt);
+clv_status k_revert(clv_chunk *c);
+clv_status k_accelerate(clv_chunk *c, double dt);
+clv_status k_flux_calc(clv_chunk *c, double dt);
+clv_status k_advec_cell(clv_chunk *c, int sweep_number, int dir);
+clv_status k_advec_mom(clv_chunk *c, int which_vel, int direction, int sweep_number);
+clv_status k_reset_field(clv_chunk *c);
+clv_status k_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth);
+clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, double *dt_min_val);
+clv_status k_field_summary(clv_chunk *c, double out[5]);
+
+}  // namespace clv

@@ -1,0 +1,460 @@
+// gfx950 kernels for the self halo, the two reductions and problem set-up.
+#include "clv_internal.h"
+
+namespace clv {
+namespace {
+
+#define AT(fid, i, j) d.f[fid][(int64_t)(j) * d.pitch + (i)]
+#define M(mid, i) d.m[mid][i]
+
+// ---- update_halo ------------------------------------------------------------------------
+// The reflective external halo of src/serial/update_halo.cpp:33-684 for one chunk
+// and one tile.  The reference updates each field with four passes, bottom, top,
+// left, right; the left/right passes also cover the ghost rows the bottom/top
+// passes just wrote, so a corner ghost ends up as sx * sy * (interior mirror).
+// With both mirrors resolved per ghost cell, every ghost value is a function of
+// interior values only and one launch writes all faces of all requested fields.
+// Per-field offsets (including the reference's irregular ones, e.g. the x-face
+// top source y_max - k at :530/:573 and the y-face right source x_max - j at
+// :636/:679):
+struct HaloRule {
+  signed char bsrc, bstep;  // bottom: dst row 1 - k <- row bsrc + bstep * k
+  signed char tdst, tsrc;   // top:    dst row y_max + tdst + k <- row y_max + tsrc - k
+  signed char lsrc;         // left:   dst col 1 - j <- col lsrc + j
+  signed char rdst, rsrc;   // right:  dst col x_max + rdst + j <- col x_max + rsrc - j
+  signed char xext, yext;   // node-in-x / node-in-y extension of the pass ranges
+  signed char sy, sx;       // signs of the bottom/top and left/right reflections
+};
+// soundspeed's bottom pass reads rows 0 + k (src/serial/update_halo.cpp:305): row 1
+// takes the old row 0 and row 0 is rewritten with that same value -> bstep 0.
+__constant__ HaloRule kRules[CLV_NUM_HALO_FIELDS] = {
+    {2, 1, 2, 1, 2, 2, 1, 0, 0, 1, 1},    // density0
+    {2, 1, 2, 1, 2, 2, 1, 0, 0, 1, 1},    // density1
+    {2, 1, 2, 1, 2, 2, 1, 0, 0, 1, 1},    // energy0
+    {2, 1, 2, 1, 2, 2, 1, 0, 0, 1, 1},    // energy1
+    {2, 1, 2, 1, 2, 2, 1, 0, 0, 1, 1},    // pressure
+    {2, 1, 2, 1, 2, 2, 1, 0, 0, 1, 1},    // viscosity
+    {0, 0, 2, 1, 2, 2, 1, 0, 0, 1, 1},    // soundspeed
+    {3, 1, 3, 1, 3, 3, 1, 1, 1, 1, -1},   // xvel0
+    {3, 1, 3, 1, 3, 3, 1, 1, 1, 1, -1},   // xvel1
+    {3, 1, 3, 1, 3, 3, 1, 1, 1, -1, 1},   // yvel0
+    {3, 1, 3, 1, 3, 3, 1, 1, 1, -1, 1},   // yvel1
+    {3, 1, 2, 0, 3, 3, 1, 1, 0, 1, -1},   // vol_flux_x
+    {3, 1, 3, 1, 3, 2, 0, 0, 1, -1, 1},   // vol_flux_y
+    {3, 1, 2, 0, 3, 3, 1, 1, 0, 1, -1},   // mass_flux_x
+    {3, 1, 3, 1, 3, 2, 0, 0, 1, -1, 1},   // mass_flux_y
+};
+
+struct HaloList {
+  int n;
+  int ids[CLV_NUM_HALO_FIELDS];
+};
+
+// Source coordinate along y for storage row y (identity if y is not a ghost row of
+// the bottom/top passes) and the accumulated sign.
+__device__ __forceinline__ int halo_src_y(const HaloRule &r, int y, int ny, int depth, double &sgn) {
+  if (y <= 1) {  // bottom band rows 1 - k, k < depth
+    sgn *= r.sy;
+    return r.bsrc + r.bstep * (1 - y);
+  }
+  if (y >= ny + r.tdst) {  // top band rows y_max + tdst + k
+    sgn *= r.sy;
+    return ny + r.tsrc - (y - ny - r.tdst);
+  }
+  return y;
+}
+__device__ __forceinline__ int halo_src_x(const HaloRule &r, int x, int nx, double &sgn) {
+  if (x <= 1) {
+    sgn *= r.sx;
+    return r.lsrc + (1 - x);
+  }
+  if (x >= nx + r.rdst) {
+    sgn *= r.sx;
+    return nx + r.rsrc - (x - nx - r.rdst);
+  }
+  return x;
+}
+
+// blockIdx.y = field slot; threads [0, nrow) walk the bottom/top bands column by
+// column (coalesced), threads [nrow, nrow + ncol) walk the left/right bands of the
+// rows between them.  Every ghost cell is written exactly once.
+__global__ __launch_bounds__(256) void update_halo_k(Dev d, HaloList list, int depth) {
+  const int fid = list.ids[blockIdx.y];
+  const HaloRule r = kRules[fid];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nrow = d.nx + r.xext + 2 * depth;  // columns [2 - depth, nx + xext + depth + 1]
+  const int ncol = d.ny + r.yext;              // rows [2, ny + tdst - 1]
+  double *a = d.f[fid];
+  const int64_t P = d.pitch;
+  if (t < nrow) {
+    const int x = 2 - depth + t;
+    double sx = 1.0;
+    const int xs = halo_src_x(r, x, d.nx, sx);
+    for (int k = 0; k < depth; ++k) {
+      const int yb = 1 - k, yt = d.ny + r.tdst + k;
+      double sb = sx, st = sx;
+      const int ysb = halo_src_y(r, yb, d.ny, depth, sb);
+      const int yst = halo_src_y(r, yt, d.ny, depth, st);
+      // sign products are exact (+-1), as are the reference's sequential negations
+      a[(int64_t)yb * P + x] = sb * a[(int64_t)ysb * P + xs];
+      a[(int64_t)yt * P + x] = st * a[(int64_t)yst * P + xs];
+    }
+  } else if (t < nrow + ncol) {
+    const int y = 2 + (t - nrow);
+    for (int k = 0; k < depth; ++k) {
+      const int xl = 1 - k, xr = d.nx + r.rdst + k;
+      double sl = 1.0, sr = 1.0;
+      const int xsl = halo_src_x(r, xl, d.nx, sl);
+      const int xsr = halo_src_x(r, xr, d.nx, sr);
+      a[(int64_t)y * P + xl] = sl * a[(int64_t)y * P + xsl];
+      a[(int64_t)y * P + xr] = sr * a[(int64_t)y * P + xsr];
+    }
+  }
+}
+
+// Small grids (nx or ny < 4): a mirror source can itself be a ghost written by an
+// earlier pass, so the four passes run as four ordered launches, exactly the
+// reference sequence.  pass 0 bottom, 1 top, 2 left, 3 right.
+__global__ __launch_bounds__(256) void update_halo_pass_k(Dev d, HaloList list, int depth, int pass) {
+  const int fid = list.ids[blockIdx.y];
+  const HaloRule r = kRules[fid];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double *a = d.f[fid];
+  const int64_t P = d.pitch;
+  if (pass < 2) {
+    const int x = 2 - depth + t;
+    if (x > d.nx + r.xext + depth + 1) return;
+    for (int k = 0; k < depth; ++k) {
+      if (pass == 0) a[(int64_t)(1 - k) * P + x] = r.sy * a[(int64_t)(r.bsrc + r.bstep * k) * P + x];
+      else a[(int64_t)(d.ny + r.tdst + k) * P + x] = r.sy * a[(int64_t)(d.ny + r.tsrc - k) * P + x];
+    }
+  } else {
+    const int y = 2 - depth + t;
+    if (y > d.ny + r.yext + depth + 1) return;
+    for (int k = 0; k < depth; ++k) {
+      if (pass == 2) a[(int64_t)y * P + (1 - k)] = r.sx * a[(int64_t)y * P + (r.lsrc + k)];
+      else a[(int64_t)y * P + (d.nx + r.rdst + k)] = r.sx * a[(int64_t)y * P + (d.nx + r.rsrc - k)];
+    }
+  }
+}
+
+// ---- reductions -------------------------------------------------------------------------
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAX_BLOCKS = 2048;
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// calc_dt: src/serial/calc_dt.cpp:50-77.  Block b takes rows 2 + b, 2 + b + G, ...
+__global__ __launch_bounds__(RED_THREADS) void calc_dt_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                                                          double *partials) {
+  double dt_min_val0 = g_big;
+  for (int j = 2 + blockIdx.x; j < d.ny + 2; j += gridDim.x) {
+    const double dsy = M(CLV_CELLDY, j);
+    for (int i = 2 + threadIdx.x; i < d.nx + 2; i += RED_THREADS) {
+      double dsx = M(CLV_CELLDX, i);
+      const double ss = AT(CLV_SOUNDSPEED, i, j);
+      double cc = ss * ss;
+      cc = cc + 2.0 * AT(CLV_VISCOSITY, i, j) / AT(CLV_DENSITY0, i, j);
+      cc = fmax(sqrt(cc), g_small);
+      double dtct = dtc_safe * fmin(dsx, dsy) / cc;
+      double div = 0.0;
+      const double vol = AT(CLV_VOLUME, i, j);
+      double dv1 = (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1)) * AT(CLV_XAREA, i, j);
+      double dv2 = (AT(CLV_XVEL0, i + 1, j) + AT(CLV_XVEL0, i + 1, j + 1)) * AT(CLV_XAREA, i + 1, j);
+      div = div + dv2 - dv1;
+      double dtut = dtu_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
+      dv1 = (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j)) * AT(CLV_YAREA, i, j);
+      dv2 = (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1)) * AT(CLV_YAREA, i, j + 1);
+      div = div + dv2 - dv1;
+      double dtvt = dtv_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
+      div = div / (2.0 * vol);
+      double dtdivt;
+      if (div < -g_small) {
+        dtdivt = dtdiv_safe * (-1.0 / div);
+      } else {
+        dtdivt = g_big;
+      }
+      double mins = fmin(dtct, fmin(dtut, fmin(dtvt, fmin(dtdivt, g_big))));
+      dt_min_val0 = fmin(mins, dt_min_val0);
+    }
+  }
+  __shared__ double lds[RED_THREADS / 64];
+  double v = wave_min(dt_min_val0);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = lds[0];
+    for (int w = 1; w < RED_THREADS / 64; ++w) m = fmin(m, lds[w]);
+    partials[blockIdx.x] = m;
+  }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void reduce_min_k(const double *partials, int n, double *out) {
+  double m = g_big;
+  for (int b = threadIdx.x; b < n; b += RED_THREADS) m = fmin(m, partials[b]);
+  __shared__ double lds[RED_THREADS / 64];
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = lds[0];
+    for (int w = 1; w < RED_THREADS / 64; ++w) r = fmin(r, lds[w]);
+    out[0] = r;
+  }
+}
+
+// field_summary: src/serial/field_summary.cpp:76-92.  The per-cell terms keep the
+// reference's expressions; the sum order is a fixed tree (thread stride within a
+// row, rows by block, wave xor-shuffle, waves, blocks in index order), so results
+// are reproducible run to run and differ from the serial order only by rounding.
+__global__ __launch_bounds__(RED_THREADS) void field_summary_k(Dev d, double *partials) {
+  double vol = 0.0, mass = 0.0, ie = 0.0, ke = 0.0, press = 0.0;
+  for (int k = 2 + blockIdx.x; k < d.ny + 2; k += gridDim.x) {
+    for (int j = 2 + threadIdx.x; j < d.nx + 2; j += RED_THREADS) {
+      double vsqrd = 0.0;
+      for (int kv = k; kv <= k + 1; ++kv) {
+        for (int jv = j; jv <= j + 1; ++jv) {
+          const double u = AT(CLV_XVEL0, jv, kv), v = AT(CLV_YVEL0, jv, kv);
+          vsqrd += 0.25 * (u * u + v * v);
+        }
+      }
+      double cell_vol = AT(CLV_VOLUME, j, k);
+      double cell_mass = cell_vol * AT(CLV_DENSITY0, j, k);
+      vol += cell_vol;
+      mass += cell_mass;
+      ie += cell_mass * AT(CLV_ENERGY0, j, k);
+      ke += cell_mass * 0.5 * vsqrd;
+      press += cell_vol * AT(CLV_PRESSURE, j, k);
+    }
+  }
+  __shared__ double lds[5][RED_THREADS / 64];
+  double s[5] = {vol, mass, ie, ke, press};
+  for (int q = 0; q < 5; ++q) {
+    double v = wave_sum(s[q]);
+    if ((threadIdx.x & 63) == 0) lds[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double r = lds[threadIdx.x][0];
+    for (int w = 1; w < RED_THREADS / 64; ++w) r += lds[threadIdx.x][w];
+    partials[(size_t)threadIdx.x * RED_MAX_BLOCKS + blockIdx.x] = r;
+  }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void reduce_sum5_k(const double *partials, int n, double *out) {
+  __shared__ double lds[5][RED_THREADS / 64];
+  for (int q = 0; q < 5; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < n; b += RED_THREADS) s += partials[(size_t)q * RED_MAX_BLOCKS + b];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) lds[q][threadIdx.x >> 6] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double r = lds[threadIdx.x][0];
+    for (int w = 1; w < RED_THREADS / 64; ++w) r += lds[threadIdx.x][w];
+    out[threadIdx.x] = r;
+  }
+}
+
+// ---- set-up -----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fill_zero_k(double *p, size_t n) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+
+// src/serial/initialise_chunk.cpp:51-75 (one tile: t_left = t_bottom = 1)
+__global__ __launch_bounds__(256) void init_mesh_k(Dev d, double xmin, double ymin, double dx, double dy) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x_min = 1, y_min = 1;
+  if (t < d.nx + 5) {
+    M(CLV_VERTEXX, t) = xmin + dx * (t - 1 - x_min);
+    M(CLV_VERTEXDX, t) = dx;
+  }
+  if (t < d.ny + 5) {
+    M(CLV_VERTEXY, t) = ymin + dy * (t - 1 - y_min);
+    M(CLV_VERTEXDY, t) = dy;
+  }
+  if (t < d.nx + 4) {
+    // recomputed vertex pair, same arithmetic as the stored vertexx (:52, :67)
+    const double v0 = xmin + dx * (t - 1 - x_min), v1 = xmin + dx * (t + 1 - 1 - x_min);
+    M(CLV_CELLX, t) = 0.5 * (v0 + v1);
+    M(CLV_CELLDX, t) = dx;
+  }
+  if (t < d.ny + 4) {
+    const double v0 = ymin + dy * (t - 1 - y_min), v1 = ymin + dy * (t + 1 - 1 - y_min);
+    M(CLV_CELLY, t) = 0.5 * (v0 + v1);
+    M(CLV_CELLDY, t) = dy;
+  }
+}
+
+// src/serial/initialise_chunk.cpp:78-84
+__global__ __launch_bounds__(256) void init_geom_k(Dev d, double dxdy) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx + 4 || j >= d.ny + 4) return;
+  AT(CLV_VOLUME, i, j) = dxdy;
+  AT(CLV_XAREA, i, j) = M(CLV_CELLDY, j);
+  AT(CLV_YAREA, i, j) = M(CLV_CELLDX, i);
+}
+
+struct StateTable {
+  int n;
+  clv_state s[CLV_MAX_STATES];
+};
+
+// The last state that covers cell (i, j) (src/serial/generate_chunk.cpp:84-130:
+// states are painted in order, so the last hit wins); 0 = background.
+__device__ int cell_state(const Dev &d, const StateTable &tab, int i, int j) {
+  int hit = 0;
+  for (int s = 1; s < tab.n; ++s) {
+    const clv_state &st = tab.s[s];
+    const double x_cent = st.xmin, y_cent = st.ymin;
+    bool h = false;
+    if (st.geometry == 1) {
+      h = (M(CLV_VERTEXX, i + 1) >= st.xmin && M(CLV_VERTEXX, i) < st.xmax) && (M(CLV_VERTEXY, j + 1) >= st.ymin && M(CLV_VERTEXY, j) < st.ymax);
+    } else if (st.geometry == 2) {
+      const double cx = M(CLV_CELLX, i), cy = M(CLV_CELLY, j);
+      double radius = sqrt((cx - x_cent) * (cx - x_cent) + (cy - y_cent) * (cy - y_cent));
+      h = radius <= st.radius;
+    } else if (st.geometry == 3) {
+      h = M(CLV_VERTEXX, i) == x_cent && M(CLV_VERTEXY, j) == y_cent;
+    }
+    if (h) hit = s;
+  }
+  return hit;
+}
+
+// Cells: energy/density of the covering state (:77-78, :93-94).
+__global__ __launch_bounds__(256) void generate_cells_k(Dev d, StateTable tab) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= d.nx + 4 || j >= d.ny + 4) return;
+  const int s = cell_state(d, tab, i, j);
+  AT(CLV_ENERGY0, i, j) = tab.s[s].energy;
+  AT(CLV_DENSITY0, i, j) = tab.s[s].density;
+}
+
+// Nodes: a state that covers cell (i, j) also paints nodes (i..i+1, j..j+1)
+// (:95-100), so node (x, y) takes the highest state among its (up to) four
+// cells; untouched nodes inside [0, nx+4) x [0, ny+4) keep the background
+// (:79-80), the last node row/column keeps build_field's zero.
+__global__ __launch_bounds__(256) void generate_nodes_k(Dev d, StateTable tab) {
+  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+  if (x >= d.nx + 5 || y >= d.ny + 5) return;
+  int best = 0;
+  for (int cj = y - 1; cj <= y; ++cj) {
+    for (int ci = x - 1; ci <= x; ++ci) {
+      if (ci < 0 || cj < 0 || ci >= d.nx + 4 || cj >= d.ny + 4) continue;
+      const int s = cell_state(d, tab, ci, cj);
+      if (s > best) best = s;
+    }
+  }
+  if (best > 0) {
+    AT(CLV_XVEL0, x, y) = tab.s[best].xvel;
+    AT(CLV_YVEL0, x, y) = tab.s[best].yvel;
+  } else if (x < d.nx + 4 && y < d.ny + 4) {
+    AT(CLV_XVEL0, x, y) = tab.s[0].xvel;
+    AT(CLV_YVEL0, x, y) = tab.s[0].yvel;
+  }
+}
+
+}  // namespace
+
+// ---- host wrappers ------------------------------------------------------------------------
+#define LAUNCH_G(name, kern, grid, block, ...)                                 \
+  do {                                                                         \
+    before_launch(c, name);                                                    \
+    hipLaunchKernelGGL(kern, grid, block, 0, c->stream, __VA_ARGS__);          \
+    clv_status st_ = after_launch(c, name);                                    \
+    if (st_ != CLV_OK) return st_;                                             \
+  } while (0)
+
+clv_status k_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth) {
+  const Dev &d = c->d;
+  HaloList list{};
+  for (int f = 0; f < CLV_NUM_HALO_FIELDS; ++f)
+    if (fields[f] == 1) list.ids[list.n++] = f;
+  if (list.n == 0) return CLV_OK;
+  if (d.nx >= 4 && d.ny >= 4) {
+    const int work = (d.nx + 1 + 2 * depth) + (d.ny + 1);
+    LAUNCH_G("update_halo", update_halo_k, dim3((work + 255) / 256, list.n, 1), dim3(256, 1, 1), d, list, depth);
+  } else {
+    const int work = (d.nx > d.ny ? d.nx : d.ny) + 2 * depth + 2;
+    for (int pass = 0; pass < 4; ++pass)
+      LAUNCH_G("update_halo", update_halo_pass_k, dim3((work + 255) / 256, list.n, 1), dim3(256, 1, 1), d, list, depth, pass);
+  }
+  return CLV_OK;
+}
+
+static int red_blocks(int ny) { return ny < RED_MAX_BLOCKS ? ny : RED_MAX_BLOCKS; }
+
+clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, double *dt_min_val) {
+  const int nb = red_blocks(c->d.ny);
+  LAUNCH_G("calc_dt", calc_dt_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, c->partials);
+  LAUNCH_G("calc_dt_final", reduce_min_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), (const double *)c->partials, nb, c->result_dev);
+  clv_status st = hip_check(hipMemcpyAsync(c->result_host, c->result_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream), "calc_dt copy");
+  if (st != CLV_OK) return st;
+  st = hip_check(hipStreamSynchronize(c->stream), "calc_dt sync");
+  if (st != CLV_OK) return st;
+  *dt_min_val = c->result_host[0];
+  return CLV_OK;
+}
+
+clv_status k_field_summary(clv_chunk *c, double out[5]) {
+  const int nb = red_blocks(c->d.ny);
+  LAUNCH_G("field_summary", field_summary_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->partials);
+  LAUNCH_G("field_summary_final", reduce_sum5_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), (const double *)c->partials, nb, c->result_dev);
+  clv_status st = hip_check(hipMemcpyAsync(c->result_host, c->result_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "summary copy");
+  if (st != CLV_OK) return st;
+  st = hip_check(hipStreamSynchronize(c->stream), "summary sync");
+  if (st != CLV_OK) return st;
+  for (int q = 0; q < 5; ++q) out[q] = c->result_host[q];
+  return CLV_OK;
+}
+
+clv_status k_fill_zero(clv_chunk *c) {
+  const Dev &d = c->d;
+  const size_t n2 = (size_t)(d.ny + 5) * (size_t)d.pitch;
+  for (int f = 0; f < CLV_NUM_FIELDS; ++f) {
+    // the lead pad before element (0, 0) belongs to no element; start at (0, 0)
+    LAUNCH_G("build_field", fill_zero_k, dim3(2048, 1, 1), dim3(256, 1, 1), d.f[f], n2);
+  }
+  for (int m = 0; m < CLV_NUM_MESH; ++m) {
+    const size_t n = (m == CLV_CELLX || m == CLV_VERTEXX || m == CLV_CELLDX || m == CLV_VERTEXDX) ? d.nx + 5 : d.ny + 5;
+    LAUNCH_G("build_field", fill_zero_k, dim3(1, 1, 1), dim3(256, 1, 1), d.m[m], n);
+  }
+  return CLV_OK;
+}
+
+clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg) {
+  const Dev &d = c->d;
+  double dx = (cfg->xmax - cfg->xmin) / (double)(cfg->x_cells);
+  double dy = (cfg->ymax - cfg->ymin) / (double)(cfg->y_cells);
+  double xmin = cfg->xmin + dx * (double)(1 - 1);
+  double ymin = cfg->ymin + dy * (double)(1 - 1);
+  const int n = (d.nx > d.ny ? d.nx : d.ny) + 5;
+  LAUNCH_G("initialise_chunk", init_mesh_k, dim3((n + 255) / 256, 1, 1), dim3(256, 1, 1), d, xmin, ymin, dx, dy);
+  const dim3 g((unsigned)((d.nx + 4 + 63) / 64), (unsigned)((d.ny + 4 + 3) / 4), 1);
+  LAUNCH_G("initialise_chunk", init_geom_k, g, dim3(64, 4, 1), d, dx * dy);
+  return CLV_OK;
+}
+
+clv_status k_generate_chunk(clv_chunk *c, const clv_config *cfg) {
+  const Dev &d = c->d;
+  if (cfg->number_of_states < 1 || cfg->number_of_states > CLV_MAX_STATES) return set_error(CLV_ERR_ARG, "generate_chunk: bad number_of_states");
+  StateTable tab{};
+  tab.n = cfg->number_of_states;
+  for (int s = 0; s < tab.n; ++s) tab.s[s] = cfg->states[s];
+  const dim3 gc((unsigned)((d.nx + 4 + 63) / 64), (unsigned)((d.ny + 4 + 3) / 4), 1);
+  const dim3 gn((unsigned)((d.nx + 5 + 63) / 64), (unsigned)((d.ny + 5 + 3) / 4), 1);
+  LAUNCH_G("generate_chunk", generate_cells_k, gc, dim3(64, 4, 1), d, tab);
+  LAUNCH_G("generate_chunk", generate_nodes_k, gn, dim3(64, 4, 1), d, tab);
+  return CLV_OK;
+}
+
+}  // namespace clv

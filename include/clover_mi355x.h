@@ -1,0 +1,218 @@
+/*
+ * clover_mi355x -- C ABI of the MI355X (gfx950) CloverLeaf hydro kernels.
+ *
+ * One chunk = one 2D staggered grid of x_cells * y_cells cells with two ghost
+ * layers, stored on one GPU as 25 FP64 fields (8 cell, 4 node velocity, 2 x-face,
+ * 2 y-face, 7 work, volume, xarea, yarea) and 8 1D mesh arrays.  Indices follow
+ * the reference's Fortran-heritage convention: x_min = y_min = 1, storage
+ * index 0 is x_min - 2, interior cells are [2, x_cells + 1], interior nodes
+ * [2, x_cells + 2] (driver/definitions.cpp:4-41, src/serial/ideal_gas.cpp:38-39).
+ *
+ * Every entry point returns CLV_OK (0) or a negative clv_status; the message of
+ * the last failure on the calling thread is clv_last_error().  Calls on one
+ * chunk are ordered on that chunk's stream and do not block the host, except
+ * those that return host scalars (clv_calc_dt, clv_field_summary,
+ * clv_timestep, clv_hydro_step) and the explicit copies.  A chunk is not
+ * thread-safe.  No HIP or C++ type crosses this boundary.
+ *
+ * Each per-kernel entry point replaces one reference model entry point for a
+ * single chunk and a single tile; the reference declaration it replaces is cited.
+ */
+#ifndef CLOVER_MI355X_H
+#define CLOVER_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int clv_status;
+enum {
+  CLV_OK = 0,
+  CLV_ERR_ARG = -1,      /* bad argument (null pointer, field id, size, depth ...) */
+  CLV_ERR_HIP = -2,      /* a HIP runtime call failed */
+  CLV_ERR_ALLOC = -3,    /* device or host allocation failed */
+  CLV_ERR_SMALL_DT = -4, /* the timestep fell below dtmin (driver/timestep.cpp:96,104-106) */
+  CLV_ERR_DECK = -5      /* the input deck is malformed (driver/read_input.cpp:110,197) */
+};
+
+/* 2D field ids.  0..14 equal the reference enum field_parameter
+ * (driver/definitions.h:62-78), so a halo field mask is the reference's
+ * int fields[NUM_FIELDS] array unchanged. */
+enum {
+  CLV_DENSITY0 = 0, CLV_DENSITY1 = 1, CLV_ENERGY0 = 2, CLV_ENERGY1 = 3,
+  CLV_PRESSURE = 4, CLV_VISCOSITY = 5, CLV_SOUNDSPEED = 6,
+  CLV_XVEL0 = 7, CLV_XVEL1 = 8, CLV_YVEL0 = 9, CLV_YVEL1 = 10,
+  CLV_VOL_FLUX_X = 11, CLV_VOL_FLUX_Y = 12, CLV_MASS_FLUX_X = 13, CLV_MASS_FLUX_Y = 14,
+  CLV_WORK1 = 15, CLV_WORK2 = 16, CLV_WORK3 = 17, CLV_WORK4 = 18, CLV_WORK5 = 19, CLV_WORK6 = 20, CLV_WORK7 = 21,
+  CLV_VOLUME = 22, CLV_XAREA = 23, CLV_YAREA = 24,
+  CLV_NUM_FIELDS = 25,
+  CLV_NUM_HALO_FIELDS = 15
+};
+/* 1D mesh arrays (driver/definitions.cpp:28-35); x arrays hold x_cells + 5
+ * doubles, y arrays y_cells + 5. */
+enum {
+  CLV_CELLX = 0, CLV_CELLY = 1, CLV_VERTEXX = 2, CLV_VERTEXY = 3,
+  CLV_CELLDX = 4, CLV_CELLDY = 5, CLV_VERTEXDX = 6, CLV_VERTEXDY = 7,
+  CLV_NUM_MESH = 8
+};
+
+/* Sweep directions (driver/definitions.h:84). */
+enum { CLV_XDIR = 1, CLV_YDIR = 2 };
+
+/* Kernel plan for the whole-step entry points.  Both give bit-identical fields. */
+enum {
+  CLV_PLAN_REFERENCE = 0, /* one launch per reference loop, as the per-kernel entry points */
+  CLV_PLAN_FUSED = 1      /* fused, register/LDS-tiled sweeps */
+};
+
+typedef struct clv_chunk clv_chunk;
+
+/* A device view of one chunk: where each array lives.  All 2D arrays share
+ * one row pitch (in doubles); element (i, j) of field f is f[j * pitch + i]. */
+typedef struct clv_chunk_view {
+  int x_cells, y_cells;
+  int64_t pitch;
+  double *field[CLV_NUM_FIELDS];
+  double *mesh[CLV_NUM_MESH];
+} clv_chunk_view;
+
+/* Geometry and dt controls of a problem (driver/definitions.h:97-155). */
+typedef struct clv_state {
+  double density, energy, xvel, yvel;
+  int geometry; /* 1 rectangle, 2 circle, 3 point (driver/definitions.h:50) */
+  double xmin, ymin, xmax, ymax, radius;
+} clv_state;
+
+#define CLV_MAX_STATES 16
+
+typedef struct clv_config {
+  double xmin, ymin, xmax, ymax;
+  int x_cells, y_cells;
+  double dtinit, dtmin, dtmax, dtrise;
+  double dtc_safe, dtu_safe, dtv_safe, dtdiv_safe;
+  double end_time;
+  int end_step;
+  int summary_frequency, visit_frequency, tiles_per_chunk, test_problem, profiler_on;
+  int number_of_states;
+  clv_state states[CLV_MAX_STATES]; /* states[0] = background; extents already shrunk by dx/100 */
+} clv_config;
+
+/* Driver state carried between steps (driver/definitions.h:265-274). */
+typedef struct clv_hydro_state {
+  int step;
+  int advect_x;
+  double time, dt, dtold;
+  int jdt, kdt;          /* location of the controlling cell, as printed */
+  int dt_control;        /* 1 sound, 2 xvel, 3 yvel, 4 div (src/serial/calc_dt.cpp:131-134) */
+  int complete;
+} clv_hydro_state;
+
+/* ---- library / errors ------------------------------------------------------------------- */
+const char *clv_last_error(void);
+const char *clv_version(void);
+/* Current HIP device count (0 if none); does not create a context. */
+int clv_device_count(void);
+
+/* ---- chunks ----------------------------------------------------------------------------- */
+/* Allocates a chunk of x_cells * y_cells on the current device, in one arena.
+ * Replaces the per-field allocations of field_type (driver/definitions.cpp:4-41). */
+clv_status clv_chunk_create(int x_cells, int y_cells, clv_chunk **out);
+/* Wraps device arrays the caller owns (e.g. a reference model's Buffer2D
+ * storage); the caller keeps them alive while the chunk exists. */
+clv_status clv_chunk_wrap(const clv_chunk_view *view, clv_chunk **out);
+clv_status clv_chunk_destroy(clv_chunk *c);
+clv_status clv_chunk_get_view(const clv_chunk *c, clv_chunk_view *out);
+/* Orders the chunk's work on a caller stream (a hipStream_t passed as void*;
+ * NULL = the null stream). */
+clv_status clv_chunk_set_stream(clv_chunk *c, void *hip_stream);
+clv_status clv_chunk_synchronize(clv_chunk *c);
+/* Device bytes held by the chunk's arena (0 for wrapped chunks). */
+size_t clv_chunk_bytes(const clv_chunk *c);
+
+/* Blocking host copies of one array.  2D arrays use host pitch host_pitch
+ * doubles (>= x_cells + 5) and y_cells + 5 rows; 1D arrays their full length. */
+clv_status clv_field_upload(clv_chunk *c, int field, const double *host, int64_t host_pitch);
+clv_status clv_field_download(clv_chunk *c, int field, double *host, int64_t host_pitch);
+clv_status clv_mesh_upload(clv_chunk *c, int mesh, const double *host);
+clv_status clv_mesh_download(clv_chunk *c, int mesh, double *host);
+
+/* ---- set-up kernels --------------------------------------------------------------------- */
+/* void build_field(global_variables&)              driver/build_field.h:24 */
+clv_status clv_build_field(clv_chunk *c);
+/* void initialise_chunk(int tile, global_variables&) driver/initialise_chunk.h:24 */
+clv_status clv_initialise_chunk(clv_chunk *c, const clv_config *cfg);
+/* void generate_chunk(int tile, global_variables&)   driver/generate_chunk.h:24 */
+clv_status clv_generate_chunk(clv_chunk *c, const clv_config *cfg);
+
+/* ---- per-step kernels (one reference entry point each) ---------------------------------- */
+/* void ideal_gas(global_variables&, const int tile, bool predict)   driver/ideal_gas.h:24 */
+clv_status clv_ideal_gas(clv_chunk *c, int predict);
+/* void viscosity(global_variables&)                                 driver/viscosity.h:24 */
+clv_status clv_viscosity(clv_chunk *c);
+/* void calc_dt(global_variables&, int tile, double& local_dt, std::string& local_control,
+ *              double& xl_pos, double& yl_pos, int& jldt, int& kldt) driver/calc_dt.h:26-27
+ * Blocks for the minimum.  control/jldt/kldt are the reference's constant
+ * bookkeeping (src/serial/calc_dt.cpp:80-83); small = dt_min_val < dtmin. */
+clv_status clv_calc_dt(clv_chunk *c, double dtmin, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                       double *dt_min_val, int *dt_control, int *jldt, int *kldt, int *small);
+/* The kernel of void PdV(global_variables&, bool predict)           driver/PdV.h:24
+ * (src/serial/PdV.cpp:36-87); clv_PdV_driver adds the predictor chain
+ * ideal_gas(predict) + update_halo(pressure, 1) + revert (src/serial/PdV.cpp:113-132). */
+clv_status clv_PdV(clv_chunk *c, int predict, double dt);
+clv_status clv_PdV_driver(clv_chunk *c, int predict, double dt);
+/* void revert(global_variables&)                                    driver/revert.h:24 */
+clv_status clv_revert(clv_chunk *c);
+/* void accelerate(global_variables&)                                driver/accelerate.h:24 */
+clv_status clv_accelerate(clv_chunk *c, double dt);
+/* void flux_calc(global_variables&)                                 driver/flux_calc.h:24 */
+clv_status clv_flux_calc(clv_chunk *c, double dt);
+/* void advec_cell_driver(global_variables&, int tile, int sweep_number, int direction)
+ *                                                                   driver/advec_cell.h:24 */
+clv_status clv_advec_cell(clv_chunk *c, int sweep_number, int direction);
+/* void advec_mom_driver(global_variables&, int tile, int which_vel, int direction,
+ *                       int sweep_number)                           driver/advec_mom.h:24 */
+clv_status clv_advec_mom(clv_chunk *c, int which_vel, int direction, int sweep_number);
+/* void reset_field(global_variables&)                               driver/reset_field.h:24 */
+clv_status clv_reset_field(clv_chunk *c);
+/* void update_halo(global_variables&, int fields[NUM_FIELDS], int depth)
+ *                                                                   driver/update_halo.h:24
+ * The reflective self halo of a single chunk and tile (every face external);
+ * fields[f] == 1 selects field f (0..14), depth is 1 or 2. */
+clv_status clv_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth);
+/* The sums of void field_summary(global_variables&, parallel_&)     driver/field_summary.h:25
+ * (src/serial/field_summary.cpp:60-92) over the current fields, without the
+ * ideal_gas call that precedes them in the reference; out = {vol, mass, ie, ke,
+ * press}.  Blocks.  Fixed-order reduction: run-to-run reproducible. */
+clv_status clv_field_summary(clv_chunk *c, double out[5]);
+
+/* ---- whole-step forms (the host loop of driver/hydro.cpp) -------------------------------- */
+/* Parses deck text with driver/read_input.cpp semantics (defaults :39-69). */
+clv_status clv_deck_parse(const char *text, clv_config *out);
+/* driver/start.cpp:69-127: build, initialise, generate, ideal_gas, depth-2 halo.
+ * Resets *st; summary0 (may be NULL) receives the step-0 field summary. */
+clv_status clv_hydro_start(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, double summary0[5]);
+/* driver/timestep.cpp:37-108: ideal_gas, halos, viscosity, calc_dt, dt selection. */
+clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan);
+/* One step of driver/hydro.cpp:57-89 (timestep, PdV predict, accelerate, PdV
+ * correct, flux_calc, advection, reset_field, advect_x flip, time += dt); sets
+ * st->complete when driver/hydro.cpp:108 says so.  Returns CLV_ERR_SMALL_DT
+ * when dt < dtmin. */
+clv_status clv_hydro_step(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan);
+/* field_summary with its ideal_gas (src/serial/field_summary.cpp:51-99). */
+clv_status clv_hydro_summary(clv_chunk *c, double out[5]);
+
+/* ---- measurement ------------------------------------------------------------------------ */
+/* Per-kernel HIP-event timing on the chunk's stream: when enabled, every launch
+ * is bracketed by events and clv_profile_read() returns, for each kernel name,
+ * the launch count and total milliseconds.  names must hold max entries. */
+clv_status clv_profile_enable(clv_chunk *c, int on);
+int clv_profile_read(clv_chunk *c, const char **names, int64_t *launches, double *total_ms, int max);
+clv_status clv_profile_reset(clv_chunk *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

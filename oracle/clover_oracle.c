@@ -1,0 +1,852 @@
+/*
+ * CPU oracle for the CloverLeaf hydro hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restates the reference serial model (src/serial/<kernel>.cpp) expression by
+ * expression in C, on x-contiguous arrays.  Every loop below keeps the
+ * reference's iteration ranges and floating-point expression order; the
+ * OpenMP pragmas only spread independent cells over threads (each iteration
+ * writes its own cells and reads no cell written by another iteration), so
+ * results do not depend on the thread count.  The one order-dependent loop
+ * (the field_summary sums) runs serially in the reference's order.
+ *
+ * Build with -O2 -ffp-contract=off (oracle/Makefile): no fused multiply-add,
+ * no fast-math, exactly like the reference built with g++ -O3.
+ * How each function is pinned is stated in clover_oracle.h.
+ */
+#include "clover_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define G_SMALL 1.0e-16 /* driver/definitions.h:32 */
+#define G_BIG 1.0e+21   /* driver/definitions.h:33 */
+
+/* element (i, j) of 2D array p with pitch px */
+#define AT(p, i, j) (p)[(size_t)(j) * (size_t)px + (size_t)(i)]
+
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* spread a loop over threads only when the grid is large enough to pay for it */
+#define ORC_PAR(c) ((long)(c)->nx * (long)(c)->ny >= 65536L)
+
+int orc_chunk_alloc(orc_chunk *c, int nx, int ny) {
+  memset(c, 0, sizeof(*c));
+  c->nx = nx;
+  c->ny = ny;
+  c->px = nx + 5;
+  size_t n2 = (size_t)(nx + 5) * (size_t)(ny + 5);
+  for (int a = 0; a < ORC_NUM_2D; ++a) {
+    c->f[a] = (double *)calloc(n2, sizeof(double));
+    if (!c->f[a]) return -1;
+  }
+  for (int a = 0; a < ORC_NUM_1D; ++a) {
+    int n = (a == ORC_CELLX || a == ORC_VERTEXX || a == ORC_CELLDX || a == ORC_VERTEXDX) ? nx + 5 : ny + 5;
+    c->m[a] = (double *)calloc((size_t)n, sizeof(double));
+    if (!c->m[a]) return -1;
+  }
+  return 0;
+}
+
+void orc_chunk_free(orc_chunk *c) {
+  for (int a = 0; a < ORC_NUM_2D; ++a) free(c->f[a]);
+  for (int a = 0; a < ORC_NUM_1D; ++a) free(c->m[a]);
+  memset(c, 0, sizeof(*c));
+}
+
+/* src/serial/ideal_gas.cpp:28-48 (kernel), :55-66 (driver: predict selects time level 1). */
+void orc_ideal_gas(orc_chunk *c, int predict) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *density = c->f[predict ? ORC_DENSITY1 : ORC_DENSITY0];
+  const double *energy = c->f[predict ? ORC_ENERGY1 : ORC_ENERGY0];
+  double *pressure = c->f[ORC_PRESSURE], *soundspeed = c->f[ORC_SOUNDSPEED];
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 2; i++) {
+      double v = 1.0 / AT(density, i, j);
+      AT(pressure, i, j) = (1.4 - 1.0) * AT(density, i, j) * AT(energy, i, j);
+      double pressurebyenergy = (1.4 - 1.0) * AT(density, i, j);
+      double pressurebyvolume = -AT(density, i, j) * AT(pressure, i, j);
+      double sound_speed_squared = v * v * (AT(pressure, i, j) * pressurebyenergy - pressurebyvolume);
+      AT(soundspeed, i, j) = sqrt(sound_speed_squared);
+    }
+  }
+}
+
+/* src/serial/viscosity.cpp:30-68.  Keeps the reference's diry test on pgradx (:57). */
+void orc_viscosity(orc_chunk *c) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *celldx = c->m[ORC_CELLDX], *celldy = c->m[ORC_CELLDY];
+  const double *density0 = c->f[ORC_DENSITY0], *pressure = c->f[ORC_PRESSURE];
+  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  double *viscosity = c->f[ORC_VISCOSITY];
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 2; i++) {
+      double ugrad = (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1)) - (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1));
+      double vgrad = (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1)) - (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0));
+      double div = (celldx[i] * (ugrad) + celldy[j] * (vgrad));
+      double strain2 = 0.5 * (AT(xvel0, i + 0, j + 1) + AT(xvel0, i + 1, j + 1) - AT(xvel0, i, j) - AT(xvel0, i + 1, j + 0)) / celldy[j] +
+                       0.5 * (AT(yvel0, i + 1, j + 0) + AT(yvel0, i + 1, j + 1) - AT(yvel0, i, j) - AT(yvel0, i + 0, j + 1)) / celldx[i];
+      double pgradx = (AT(pressure, i + 1, j + 0) - AT(pressure, i - 1, j + 0)) / (celldx[i] + celldx[i + 1]);
+      double pgrady = (AT(pressure, i + 0, j + 1) - AT(pressure, i + 0, j - 1)) / (celldy[j] + celldy[j + 2]);
+      double pgradx2 = pgradx * pgradx;
+      double pgrady2 = pgrady * pgrady;
+      double limiter = ((0.5 * (ugrad) / celldx[i]) * pgradx2 + (0.5 * (vgrad) / celldy[j]) * pgrady2 + strain2 * pgradx * pgrady) /
+                       fmax(pgradx2 + pgrady2, G_SMALL);
+      if ((limiter > 0.0) || (div >= 0.0)) {
+        AT(viscosity, i, j) = 0.0;
+      } else {
+        double dirx = 1.0;
+        if (pgradx < 0.0) dirx = -1.0;
+        pgradx = dirx * fmax(G_SMALL, fabs(pgradx));
+        double diry = 1.0;
+        if (pgradx < 0.0) diry = -1.0;
+        pgrady = diry * fmax(G_SMALL, fabs(pgrady));
+        double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
+        double xgrad = fabs(celldx[i] * pgrad / pgradx);
+        double ygrad = fabs(celldy[j] * pgrad / pgrady);
+        double grad = fmin(xgrad, ygrad);
+        double grad2 = grad * grad;
+        AT(viscosity, i, j) = 2.0 * AT(density0, i, j) * grad2 * limiter * limiter;
+      }
+    }
+  }
+}
+
+/* src/serial/calc_dt.cpp:30-111 (kernel) and :116-135 (driver).  The min is
+ * order-independent, so the OpenMP min reduction returns the serial value. */
+void orc_calc_dt(orc_chunk *c, double dtmin, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                 double *dt_min_val, int *dtl_control, int *jldt, int *kldt, int *small) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *celldx = c->m[ORC_CELLDX], *celldy = c->m[ORC_CELLDY];
+  const double *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA], *volume = c->f[ORC_VOLUME];
+  const double *density0 = c->f[ORC_DENSITY0], *viscosity_a = c->f[ORC_VISCOSITY], *soundspeed = c->f[ORC_SOUNDSPEED];
+  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  *small = 0;
+  double jk_control = 1.1;
+  double dt_min_val0 = G_BIG;
+#pragma omp parallel for reduction(min : dt_min_val0) if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 2; i++) {
+      double dsx = celldx[i];
+      double dsy = celldy[j];
+      double cc = AT(soundspeed, i, j) * AT(soundspeed, i, j);
+      cc = cc + 2.0 * AT(viscosity_a, i, j) / AT(density0, i, j);
+      cc = fmax(sqrt(cc), G_SMALL);
+      double dtct = dtc_safe * fmin(dsx, dsy) / cc;
+      double div = 0.0;
+      double dv1 = (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1)) * AT(xarea, i, j);
+      double dv2 = (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1)) * AT(xarea, i + 1, j + 0);
+      div = div + dv2 - dv1;
+      double dtut = dtu_safe * 2.0 * AT(volume, i, j) / fmax(fmax(fabs(dv1), fabs(dv2)), G_SMALL * AT(volume, i, j));
+      dv1 = (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0)) * AT(yarea, i, j);
+      dv2 = (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1)) * AT(yarea, i + 0, j + 1);
+      div = div + dv2 - dv1;
+      double dtvt = dtv_safe * 2.0 * AT(volume, i, j) / fmax(fmax(fabs(dv1), fabs(dv2)), G_SMALL * AT(volume, i, j));
+      div = div / (2.0 * AT(volume, i, j));
+      double dtdivt;
+      if (div < -G_SMALL) {
+        dtdivt = dtdiv_safe * (-1.0 / div);
+      } else {
+        dtdivt = G_BIG;
+      }
+      double mins = fmin(dtct, fmin(dtut, fmin(dtvt, fmin(dtdivt, G_BIG))));
+      dt_min_val0 = fmin(mins, dt_min_val0);
+    }
+  }
+  *dt_min_val = dt_min_val0;
+  /* :80-83, constant control/location bookkeeping */
+  *dtl_control = (int)(10.01 * (jk_control - (int)(jk_control)));
+  jk_control = jk_control - (jk_control - (int)(jk_control));
+  *jldt = ((int)jk_control) % x_max;
+  *kldt = (int)(1.f + (jk_control / x_max));
+  if (*dt_min_val < dtmin) *small = 1;
+}
+
+/* src/serial/PdV.cpp:36-87 */
+void orc_PdV(orc_chunk *c, int predict, double dt) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA], *volume = c->f[ORC_VOLUME];
+  const double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0];
+  const double *pressure = c->f[ORC_PRESSURE], *viscosity = c->f[ORC_VISCOSITY];
+  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  const double *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+  double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
+  if (predict) {
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 2; i++) {
+        double left_flux = (AT(xarea, i, j) * (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1) + AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1))) * 0.25 * dt * 0.5;
+        double right_flux = (AT(xarea, i + 1, j + 0) * (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1) + AT(xvel0, i + 1, j + 0) +
+                                                      AT(xvel0, i + 1, j + 1))) *
+                            0.25 * dt * 0.5;
+        double bottom_flux = (AT(yarea, i, j) * (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0) + AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0))) * 0.25 * dt * 0.5;
+        double top_flux = (AT(yarea, i + 0, j + 1) * (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1) + AT(yvel0, i + 0, j + 1) +
+                                                    AT(yvel0, i + 1, j + 1))) *
+                          0.25 * dt * 0.5;
+        double total_flux = right_flux - left_flux + top_flux - bottom_flux;
+        double volume_change_s = AT(volume, i, j) / (AT(volume, i, j) + total_flux);
+        double recip_volume = 1.0 / AT(volume, i, j);
+        double energy_change = (AT(pressure, i, j) / AT(density0, i, j) + AT(viscosity, i, j) / AT(density0, i, j)) * total_flux * recip_volume;
+        AT(energy1, i, j) = AT(energy0, i, j) - energy_change;
+        AT(density1, i, j) = AT(density0, i, j) * volume_change_s;
+      }
+    }
+  } else {
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 2; i++) {
+        double left_flux = (AT(xarea, i, j) * (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1) + AT(xvel1, i, j) + AT(xvel1, i + 0, j + 1))) * 0.25 * dt;
+        double right_flux = (AT(xarea, i + 1, j + 0) * (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1) + AT(xvel1, i + 1, j + 0) +
+                                                      AT(xvel1, i + 1, j + 1))) *
+                            0.25 * dt;
+        double bottom_flux = (AT(yarea, i, j) * (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0) + AT(yvel1, i, j) + AT(yvel1, i + 1, j + 0))) * 0.25 * dt;
+        double top_flux = (AT(yarea, i + 0, j + 1) * (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1) + AT(yvel1, i + 0, j + 1) +
+                                                    AT(yvel1, i + 1, j + 1))) *
+                          0.25 * dt;
+        double total_flux = right_flux - left_flux + top_flux - bottom_flux;
+        double volume_change_s = AT(volume, i, j) / (AT(volume, i, j) + total_flux);
+        double recip_volume = 1.0 / AT(volume, i, j);
+        double energy_change = (AT(pressure, i, j) / AT(density0, i, j) + AT(viscosity, i, j) / AT(density0, i, j)) * total_flux * recip_volume;
+        AT(energy1, i, j) = AT(energy0, i, j) - energy_change;
+        AT(density1, i, j) = AT(density0, i, j) * volume_change_s;
+      }
+    }
+  }
+}
+
+/* src/serial/revert.cpp:29-41 */
+void orc_revert(orc_chunk *c) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0];
+  double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 2; i++) {
+      AT(density1, i, j) = AT(density0, i, j);
+      AT(energy1, i, j) = AT(energy0, i, j);
+    }
+  }
+}
+
+/* src/serial/accelerate.cpp:28-56.  Pressure then viscosity, two separate updates (:46-53). */
+void orc_accelerate(orc_chunk *c, double dt) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA], *volume = c->f[ORC_VOLUME];
+  const double *density0 = c->f[ORC_DENSITY0], *pressure = c->f[ORC_PRESSURE], *viscosity = c->f[ORC_VISCOSITY];
+  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  double *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+  double halfdt = 0.5 * dt;
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+      double stepbymass_s = halfdt / ((AT(density0, i - 1, j - 1) * AT(volume, i - 1, j - 1) + AT(density0, i - 1, j + 0) * AT(volume, i - 1, j + 0) +
+                                       AT(density0, i, j) * AT(volume, i, j) + AT(density0, i + 0, j - 1) * AT(volume, i + 0, j - 1)) *
+                                      0.25);
+      AT(xvel1, i, j) = AT(xvel0, i, j) - stepbymass_s * (AT(xarea, i, j) * (AT(pressure, i, j) - AT(pressure, i - 1, j + 0)) +
+                                                          AT(xarea, i + 0, j - 1) * (AT(pressure, i + 0, j - 1) - AT(pressure, i - 1, j - 1)));
+      AT(yvel1, i, j) = AT(yvel0, i, j) - stepbymass_s * (AT(yarea, i, j) * (AT(pressure, i, j) - AT(pressure, i + 0, j - 1)) +
+                                                          AT(yarea, i - 1, j + 0) * (AT(pressure, i - 1, j + 0) - AT(pressure, i - 1, j - 1)));
+      AT(xvel1, i, j) = AT(xvel1, i, j) - stepbymass_s * (AT(xarea, i, j) * (AT(viscosity, i, j) - AT(viscosity, i - 1, j + 0)) +
+                                                          AT(xarea, i + 0, j - 1) * (AT(viscosity, i + 0, j - 1) - AT(viscosity, i - 1, j - 1)));
+      AT(yvel1, i, j) = AT(yvel1, i, j) - stepbymass_s * (AT(yarea, i, j) * (AT(viscosity, i, j) - AT(viscosity, i + 0, j - 1)) +
+                                                          AT(yarea, i - 1, j + 0) * (AT(viscosity, i - 1, j + 0) - AT(viscosity, i - 1, j - 1)));
+    }
+  }
+}
+
+/* src/serial/flux_calc.cpp:27-43 */
+void orc_flux_calc(orc_chunk *c, double dt) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA];
+  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  const double *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+  double *vol_flux_x = c->f[ORC_VOL_FLUX_X], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+      AT(vol_flux_x, i, j) = 0.25 * dt * AT(xarea, i, j) * (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1) + AT(xvel1, i, j) + AT(xvel1, i + 0, j + 1));
+      AT(vol_flux_y, i, j) = 0.25 * dt * AT(yarea, i, j) * (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0) + AT(yvel1, i, j) + AT(yvel1, i + 1, j + 0));
+    }
+  }
+}
+
+/* One upwind limited flux of the van Leer remap, shared by both directions of
+ * advec_cell (src/serial/advec_cell.cpp:86-110 for x, :178-202 for y).
+ * a_* are density1 (or energy1) at upwind/donor/downwind. */
+static inline double orc_vl_limiter(double sigma, double sigma3, double sigma4, double uw, double dn, double dw) {
+  const double one_by_six = 1.0 / 6.0;
+  double diffuw = dn - uw;
+  double diffdw = dw - dn;
+  double wind = 1.0;
+  if (diffdw <= 0.0) wind = -1.0;
+  double limiter;
+  if (diffuw * diffdw > 0.0) {
+    limiter = (1.0 - sigma) * wind * fmin(fmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
+  } else {
+    limiter = 0.0;
+  }
+  return limiter;
+}
+
+/* src/serial/advec_cell.cpp:28-220.  Work arrays as bound by the driver
+ * (:228-231): pre_vol = work1, post_vol = work2, ener_flux = work7. */
+void orc_advec_cell(orc_chunk *c, int sweep_number, int dir) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const double *vertexdx = c->m[ORC_VERTEXDX], *vertexdy = c->m[ORC_VERTEXDY];
+  const double *volume = c->f[ORC_VOLUME];
+  const double *vol_flux_x = c->f[ORC_VOL_FLUX_X], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
+  double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
+  double *mass_flux_x = c->f[ORC_MASS_FLUX_X], *mass_flux_y = c->f[ORC_MASS_FLUX_Y];
+  double *pre_vol = c->f[ORC_WORK1], *post_vol = c->f[ORC_WORK2], *ener_flux = c->f[ORC_WORK7];
+
+  if (dir == 1) {
+    if (sweep_number == 1) {
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min - 2 + 1; j < y_max + 2 + 2; j++) {
+        for (int i = x_min - 2 + 1; i < x_max + 2 + 2; i++) {
+          AT(pre_vol, i, j) = AT(volume, i, j) + (AT(vol_flux_x, i + 1, j + 0) - AT(vol_flux_x, i, j) + AT(vol_flux_y, i + 0, j + 1) - AT(vol_flux_y, i, j));
+          AT(post_vol, i, j) = AT(pre_vol, i, j) - (AT(vol_flux_x, i + 1, j + 0) - AT(vol_flux_x, i, j));
+        }
+      }
+    } else {
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min - 2 + 1; j < y_max + 2 + 2; j++) {
+        for (int i = x_min - 2 + 1; i < x_max + 2 + 2; i++) {
+          AT(pre_vol, i, j) = AT(volume, i, j) + AT(vol_flux_x, i + 1, j + 0) - AT(vol_flux_x, i, j);
+          AT(post_vol, i, j) = AT(volume, i, j);
+        }
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 2 + 2; i++) {
+        int upwind, donor, downwind, dif;
+        if (AT(vol_flux_x, i, j) > 0.0) {
+          upwind = i - 2;
+          donor = i - 1;
+          downwind = i;
+          dif = donor;
+        } else {
+          upwind = imin(i + 1, x_max + 2);
+          donor = i;
+          downwind = i - 1;
+          dif = upwind;
+        }
+        double sigmat = fabs(AT(vol_flux_x, i, j)) / AT(pre_vol, donor, j);
+        double sigma3 = (1.0 + sigmat) * (vertexdx[i] / vertexdx[dif]);
+        double sigma4 = 2.0 - sigmat;
+        double sigmav = sigmat;
+        double limiter = orc_vl_limiter(sigmav, sigma3, sigma4, AT(density1, upwind, j), AT(density1, donor, j), AT(density1, downwind, j));
+        AT(mass_flux_x, i, j) = AT(vol_flux_x, i, j) * (AT(density1, donor, j) + limiter);
+        double sigmam = fabs(AT(mass_flux_x, i, j)) / (AT(density1, donor, j) * AT(pre_vol, donor, j));
+        limiter = orc_vl_limiter(sigmam, sigma3, sigma4, AT(energy1, upwind, j), AT(energy1, donor, j), AT(energy1, downwind, j));
+        AT(ener_flux, i, j) = AT(mass_flux_x, i, j) * (AT(energy1, donor, j) + limiter);
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 2; i++) {
+        double pre_mass_s = AT(density1, i, j) * AT(pre_vol, i, j);
+        double post_mass_s = pre_mass_s + AT(mass_flux_x, i, j) - AT(mass_flux_x, i + 1, j + 0);
+        double post_ener_s = (AT(energy1, i, j) * pre_mass_s + AT(ener_flux, i, j) - AT(ener_flux, i + 1, j + 0)) / post_mass_s;
+        double advec_vol_s = AT(pre_vol, i, j) + AT(vol_flux_x, i, j) - AT(vol_flux_x, i + 1, j + 0);
+        AT(density1, i, j) = post_mass_s / advec_vol_s;
+        AT(energy1, i, j) = post_ener_s;
+      }
+    }
+  } else if (dir == 2) {
+    if (sweep_number == 1) {
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min - 2 + 1; j < y_max + 2 + 2; j++) {
+        for (int i = x_min - 2 + 1; i < x_max + 2 + 2; i++) {
+          AT(pre_vol, i, j) = AT(volume, i, j) + (AT(vol_flux_y, i + 0, j + 1) - AT(vol_flux_y, i, j) + AT(vol_flux_x, i + 1, j + 0) - AT(vol_flux_x, i, j));
+          AT(post_vol, i, j) = AT(pre_vol, i, j) - (AT(vol_flux_y, i + 0, j + 1) - AT(vol_flux_y, i, j));
+        }
+      }
+    } else {
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min - 2 + 1; j < y_max + 2 + 2; j++) {
+        for (int i = x_min - 2 + 1; i < x_max + 2 + 2; i++) {
+          AT(pre_vol, i, j) = AT(volume, i, j) + AT(vol_flux_y, i + 0, j + 1) - AT(vol_flux_y, i, j);
+          AT(post_vol, i, j) = AT(volume, i, j);
+        }
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 2 + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 2; i++) {
+        int upwind, donor, downwind, dif;
+        if (AT(vol_flux_y, i, j) > 0.0) {
+          upwind = j - 2;
+          donor = j - 1;
+          downwind = j;
+          dif = donor;
+        } else {
+          upwind = imin(j + 1, y_max + 2);
+          donor = j;
+          downwind = j - 1;
+          dif = upwind;
+        }
+        double sigmat = fabs(AT(vol_flux_y, i, j)) / AT(pre_vol, i, donor);
+        double sigma3 = (1.0 + sigmat) * (vertexdy[j] / vertexdy[dif]);
+        double sigma4 = 2.0 - sigmat;
+        double sigmav = sigmat;
+        double limiter = orc_vl_limiter(sigmav, sigma3, sigma4, AT(density1, i, upwind), AT(density1, i, donor), AT(density1, i, downwind));
+        AT(mass_flux_y, i, j) = AT(vol_flux_y, i, j) * (AT(density1, i, donor) + limiter);
+        double sigmam = fabs(AT(mass_flux_y, i, j)) / (AT(density1, i, donor) * AT(pre_vol, i, donor));
+        limiter = orc_vl_limiter(sigmam, sigma3, sigma4, AT(energy1, i, upwind), AT(energy1, i, donor), AT(energy1, i, downwind));
+        AT(ener_flux, i, j) = AT(mass_flux_y, i, j) * (AT(energy1, i, donor) + limiter);
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 2; i++) {
+        double pre_mass_s = AT(density1, i, j) * AT(pre_vol, i, j);
+        double post_mass_s = pre_mass_s + AT(mass_flux_y, i, j) - AT(mass_flux_y, i + 0, j + 1);
+        double post_ener_s = (AT(energy1, i, j) * pre_mass_s + AT(ener_flux, i, j) - AT(ener_flux, i + 0, j + 1)) / post_mass_s;
+        double advec_vol_s = AT(pre_vol, i, j) + AT(vol_flux_y, i, j) - AT(vol_flux_y, i + 0, j + 1);
+        AT(density1, i, j) = post_mass_s / advec_vol_s;
+        AT(energy1, i, j) = post_ener_s;
+      }
+    }
+  }
+}
+
+/* Momentum limiter of src/serial/advec_mom.cpp:125-139 (x) / :197-211 (y). */
+static inline double orc_mom_flux(double flux, double mass_pre_donor, double width, double width_dif, double v_uw, double v_dn, double v_dw) {
+  double sigma = fabs(flux) / (mass_pre_donor);
+  double vdiffuw = v_dn - v_uw;
+  double vdiffdw = v_dw - v_dn;
+  double limiter = 0.0;
+  if (vdiffuw * vdiffdw > 0.0) {
+    double auw = fabs(vdiffuw);
+    double adw = fabs(vdiffdw);
+    double wind = 1.0;
+    if (vdiffdw <= 0.0) wind = -1.0;
+    limiter = wind * fmin(fmin(width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / width_dif) / 6.0, auw), adw);
+  }
+  double advec_vel_s = v_dn + (1.0 - sigma) * limiter;
+  return advec_vel_s * flux;
+}
+
+/* src/serial/advec_mom.cpp:30-225.  Work arrays as bound by the driver
+ * (:234-242): node_flux = work1, node_mass_post = work2, node_mass_pre = work3,
+ * mom_flux = work4, pre_vol = work5, post_vol = work6. */
+void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  double *vel1 = c->f[which_vel == 1 ? ORC_XVEL1 : ORC_YVEL1];
+  const double *mass_flux_x = c->f[ORC_MASS_FLUX_X], *vol_flux_x = c->f[ORC_VOL_FLUX_X];
+  const double *mass_flux_y = c->f[ORC_MASS_FLUX_Y], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
+  const double *volume = c->f[ORC_VOLUME], *density1 = c->f[ORC_DENSITY1];
+  double *node_flux = c->f[ORC_WORK1], *node_mass_post = c->f[ORC_WORK2], *node_mass_pre = c->f[ORC_WORK3];
+  double *mom_flux = c->f[ORC_WORK4], *pre_vol = c->f[ORC_WORK5], *post_vol = c->f[ORC_WORK6];
+  const double *celldx = c->m[ORC_CELLDX], *celldy = c->m[ORC_CELLDY];
+
+  int mom_sweep = direction + 2 * (sweep_number - 1);
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min - 2 + 1; j < y_max + 2 + 2; j++) {
+    for (int i = x_min - 2 + 1; i < x_max + 2 + 2; i++) {
+      if (mom_sweep == 1) { /* x 1 */
+        AT(post_vol, i, j) = AT(volume, i, j) + AT(vol_flux_y, i + 0, j + 1) - AT(vol_flux_y, i, j);
+        AT(pre_vol, i, j) = AT(post_vol, i, j) + AT(vol_flux_x, i + 1, j + 0) - AT(vol_flux_x, i, j);
+      } else if (mom_sweep == 2) { /* y 1 */
+        AT(post_vol, i, j) = AT(volume, i, j) + AT(vol_flux_x, i + 1, j + 0) - AT(vol_flux_x, i, j);
+        AT(pre_vol, i, j) = AT(post_vol, i, j) + AT(vol_flux_y, i + 0, j + 1) - AT(vol_flux_y, i, j);
+      } else if (mom_sweep == 3) { /* x 2 */
+        AT(post_vol, i, j) = AT(volume, i, j);
+        AT(pre_vol, i, j) = AT(post_vol, i, j) + AT(vol_flux_y, i + 0, j + 1) - AT(vol_flux_y, i, j);
+      } else if (mom_sweep == 4) { /* y 2 */
+        AT(post_vol, i, j) = AT(volume, i, j);
+        AT(pre_vol, i, j) = AT(post_vol, i, j) + AT(vol_flux_x, i + 1, j + 0) - AT(vol_flux_x, i, j);
+      }
+    }
+  }
+
+  if (direction == 1) {
+    if (which_vel == 1) {
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+        for (int i = x_min - 2 + 1; i < x_max + 2 + 2; i++) {
+          AT(node_flux, i, j) = 0.25 * (AT(mass_flux_x, i + 0, j - 1) + AT(mass_flux_x, i, j) + AT(mass_flux_x, i + 1, j - 1) + AT(mass_flux_x, i + 1, j + 0));
+        }
+      }
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+        for (int i = x_min - 1 + 1; i < x_max + 2 + 2; i++) {
+          AT(node_mass_post, i, j) = 0.25 * (AT(density1, i + 0, j - 1) * AT(post_vol, i + 0, j - 1) + AT(density1, i, j) * AT(post_vol, i, j) +
+                                             AT(density1, i - 1, j - 1) * AT(post_vol, i - 1, j - 1) + AT(density1, i - 1, j + 0) * AT(post_vol, i - 1, j + 0));
+          AT(node_mass_pre, i, j) = AT(node_mass_post, i, j) - AT(node_flux, i - 1, j + 0) + AT(node_flux, i, j);
+        }
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+      for (int i = x_min - 1 + 1; i < x_max + 1 + 2; i++) {
+        int upwind, donor, downwind, dif;
+        if (AT(node_flux, i, j) < 0.0) {
+          upwind = i + 2;
+          donor = i + 1;
+          downwind = i;
+          dif = donor;
+        } else {
+          upwind = i - 1;
+          donor = i;
+          downwind = i + 1;
+          dif = upwind;
+        }
+        AT(mom_flux, i, j) = orc_mom_flux(AT(node_flux, i, j), AT(node_mass_pre, donor, j), celldx[i], celldx[dif], AT(vel1, upwind, j),
+                                          AT(vel1, donor, j), AT(vel1, downwind, j));
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+        AT(vel1, i, j) = (AT(vel1, i, j) * AT(node_mass_pre, i, j) + AT(mom_flux, i - 1, j + 0) - AT(mom_flux, i, j)) / AT(node_mass_post, i, j);
+      }
+    }
+  } else if (direction == 2) {
+    if (which_vel == 1) {
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min - 2 + 1; j < y_max + 2 + 2; j++) {
+        for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+          AT(node_flux, i, j) = 0.25 * (AT(mass_flux_y, i - 1, j + 0) + AT(mass_flux_y, i, j) + AT(mass_flux_y, i - 1, j + 1) + AT(mass_flux_y, i + 0, j + 1));
+        }
+      }
+#pragma omp parallel for if (ORC_PAR(c))
+      for (int j = y_min - 1 + 1; j < y_max + 2 + 2; j++) {
+        for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+          AT(node_mass_post, i, j) = 0.25 * (AT(density1, i + 0, j - 1) * AT(post_vol, i + 0, j - 1) + AT(density1, i, j) * AT(post_vol, i, j) +
+                                             AT(density1, i - 1, j - 1) * AT(post_vol, i - 1, j - 1) + AT(density1, i - 1, j + 0) * AT(post_vol, i - 1, j + 0));
+          AT(node_mass_pre, i, j) = AT(node_mass_post, i, j) - AT(node_flux, i + 0, j - 1) + AT(node_flux, i, j);
+        }
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min - 1 + 1; j < y_max + 1 + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+        int upwind, donor, downwind, dif;
+        if (AT(node_flux, i, j) < 0.0) {
+          upwind = j + 2;
+          donor = j + 1;
+          downwind = j;
+          dif = donor;
+        } else {
+          upwind = j - 1;
+          donor = j;
+          downwind = j + 1;
+          dif = upwind;
+        }
+        AT(mom_flux, i, j) = orc_mom_flux(AT(node_flux, i, j), AT(node_mass_pre, i, donor), celldy[j], celldy[dif], AT(vel1, i, upwind),
+                                          AT(vel1, i, donor), AT(vel1, i, downwind));
+      }
+    }
+#pragma omp parallel for if (ORC_PAR(c))
+    for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+      for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+        AT(vel1, i, j) = (AT(vel1, i, j) * AT(node_mass_pre, i, j) + AT(mom_flux, i + 0, j - 1) - AT(mom_flux, i, j)) / AT(node_mass_post, i, j);
+      }
+    }
+  }
+}
+
+/* src/serial/reset_field.cpp:28-51 */
+void orc_reset_field(orc_chunk *c) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0], *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  const double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1], *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 2; i++) {
+      AT(density0, i, j) = AT(density1, i, j);
+      AT(energy0, i, j) = AT(energy1, i, j);
+    }
+  }
+#pragma omp parallel for if (ORC_PAR(c))
+  for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
+    for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
+      AT(xvel0, i, j) = AT(xvel1, i, j);
+      AT(yvel0, i, j) = AT(yvel1, i, j);
+    }
+  }
+}
+
+/* Reflective halo of one field, src/serial/update_halo.cpp:33-684, for one
+ * chunk and one tile (every face external).  The four passes run in the
+ * reference's order (bottom, top, left, right); that order fixes the corners.
+ * Per-field offsets, including the reference's irregular ones (soundspeed
+ * bottom :305, the x-face top :530/:573 and the y-face right :636/:679), come
+ * from one table:
+ *   bsrc: bottom source row = bsrc + k       (dst row 1 - k)
+ *   tdst/tsrc: top dst row y_max + tdst + k, source row y_max + tsrc - k
+ *   lsrc: left source col = lsrc + j         (dst col 1 - j)
+ *   rdst/rsrc: right dst col x_max + rdst + j, source col x_max + rsrc - j
+ *   xext: 1 if the bottom/top passes run one column further (node in x)
+ *   yext: 1 if the left/right passes run one row further (node in y)
+ *   sy / sx: sign applied by the bottom/top and left/right passes. */
+typedef struct { int bsrc, tdst, tsrc, lsrc, rdst, rsrc, xext, yext; double sy, sx; } orc_halo_rule;
+
+static const orc_halo_rule ORC_HALO_RULES[15] = {
+    /* density0   */ {2, 2, 1, 2, 2, 1, 0, 0, 1.0, 1.0},
+    /* density1   */ {2, 2, 1, 2, 2, 1, 0, 0, 1.0, 1.0},
+    /* energy0    */ {2, 2, 1, 2, 2, 1, 0, 0, 1.0, 1.0},
+    /* energy1    */ {2, 2, 1, 2, 2, 1, 0, 0, 1.0, 1.0},
+    /* pressure   */ {2, 2, 1, 2, 2, 1, 0, 0, 1.0, 1.0},
+    /* viscosity  */ {2, 2, 1, 2, 2, 1, 0, 0, 1.0, 1.0},
+    /* soundspeed */ {0, 2, 1, 2, 2, 1, 0, 0, 1.0, 1.0},
+    /* xvel0      */ {3, 3, 1, 3, 3, 1, 1, 1, 1.0, -1.0},
+    /* xvel1      */ {3, 3, 1, 3, 3, 1, 1, 1, 1.0, -1.0},
+    /* yvel0      */ {3, 3, 1, 3, 3, 1, 1, 1, -1.0, 1.0},
+    /* yvel1      */ {3, 3, 1, 3, 3, 1, 1, 1, -1.0, 1.0},
+    /* vol_flux_x */ {3, 2, 0, 3, 3, 1, 1, 0, 1.0, -1.0},
+    /* vol_flux_y */ {3, 3, 1, 3, 2, 0, 0, 1, -1.0, 1.0},
+    /* mass_flux_x*/ {3, 2, 0, 3, 3, 1, 1, 0, 1.0, -1.0},
+    /* mass_flux_y*/ {3, 3, 1, 3, 2, 0, 0, 1, -1.0, 1.0},
+};
+
+void orc_update_halo(orc_chunk *c, const int fields[15], int depth) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  for (int f = 0; f < 15; ++f) {
+    if (fields[f] != 1) continue;
+    const orc_halo_rule *r = &ORC_HALO_RULES[f];
+    double *a = c->f[f];
+    /* bottom then top: j over x_min-depth .. x_max(+1)+depth */
+    for (int j = x_min - depth + 1; j < x_max + r->xext + depth + 2; j++)
+      for (int k = 0; k < depth; ++k) AT(a, j, 1 - k) = r->sy * AT(a, j, r->bsrc + k);
+    for (int j = x_min - depth + 1; j < x_max + r->xext + depth + 2; j++)
+      for (int k = 0; k < depth; ++k) AT(a, j, y_max + r->tdst + k) = r->sy * AT(a, j, y_max + r->tsrc - k);
+    /* left then right: k over y_min-depth .. y_max(+1)+depth */
+    for (int k = y_min - depth + 1; k < y_max + r->yext + depth + 2; k++)
+      for (int j = 0; j < depth; ++j) AT(a, 1 - j, k) = r->sx * AT(a, r->lsrc + j, k);
+    for (int k = y_min - depth + 1; k < y_max + r->yext + depth + 2; k++)
+      for (int j = 0; j < depth; ++j) AT(a, x_max + r->rdst + j, k) = r->sx * AT(a, x_max + r->rsrc - j, k);
+  }
+}
+
+/* src/serial/field_summary.cpp:60-92, serial order (linear index, x fastest). */
+void orc_field_summary_kernel(orc_chunk *c, double out[5]) {
+  const int px = c->px, xmin = 1, ymin = 1, xmax = c->nx, ymax = c->ny;
+  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0], *volume = c->f[ORC_VOLUME];
+  const double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0], *pressure = c->f[ORC_PRESSURE];
+  double vol = 0.0, mass = 0.0, ie = 0.0, ke = 0.0, press = 0.0;
+  for (int idx = 0; idx < ((ymax - ymin + 1) * (xmax - xmin + 1)); idx++) {
+    const int j = xmin + 1 + idx % (xmax - xmin + 1);
+    const int k = ymin + 1 + idx / (xmax - xmin + 1);
+    double vsqrd = 0.0;
+    for (int kv = k; kv <= k + 1; ++kv) {
+      for (int jv = j; jv <= j + 1; ++jv) {
+        vsqrd += 0.25 * (AT(xvel0, jv, kv) * AT(xvel0, jv, kv) + AT(yvel0, jv, kv) * AT(yvel0, jv, kv));
+      }
+    }
+    double cell_vol = AT(volume, j, k);
+    double cell_mass = cell_vol * AT(density0, j, k);
+    vol += cell_vol;
+    mass += cell_mass;
+    ie += cell_mass * AT(energy0, j, k);
+    ke += cell_mass * 0.5 * vsqrd;
+    press += cell_vol * AT(pressure, j, k);
+  }
+  out[0] = vol;
+  out[1] = mass;
+  out[2] = ie;
+  out[3] = ke;
+  out[4] = press;
+}
+
+void orc_field_summary(orc_chunk *c, double out[5]) {
+  orc_ideal_gas(c, 0);
+  orc_field_summary_kernel(c, out);
+}
+
+/* src/serial/build_field.cpp:29-180: zero fill (calloc already zeroes; kept for re-use). */
+void orc_build_field(orc_chunk *c) {
+  size_t n2 = (size_t)(c->nx + 5) * (size_t)(c->ny + 5);
+  for (int a = 0; a < ORC_NUM_2D; ++a) memset(c->f[a], 0, n2 * sizeof(double));
+  for (int a = 0; a < ORC_NUM_1D; ++a) {
+    int n = (a == ORC_CELLX || a == ORC_VERTEXX || a == ORC_CELLDX || a == ORC_VERTEXDX) ? c->nx + 5 : c->ny + 5;
+    memset(c->m[a], 0, (size_t)n * sizeof(double));
+  }
+}
+
+/* src/serial/initialise_chunk.cpp:30-85 (one tile, t_left = t_bottom = 1). */
+void orc_initialise_chunk(orc_chunk *c, const orc_config *cfg) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  double dx = (cfg->xmax - cfg->xmin) / (double)(cfg->x_cells);
+  double dy = (cfg->ymax - cfg->ymin) / (double)(cfg->y_cells);
+  double xmin = cfg->xmin + dx * (double)(1 - 1);
+  double ymin = cfg->ymin + dy * (double)(1 - 1);
+  double *vertexx = c->m[ORC_VERTEXX], *vertexdx = c->m[ORC_VERTEXDX], *vertexy = c->m[ORC_VERTEXY], *vertexdy = c->m[ORC_VERTEXDY];
+  double *cellx = c->m[ORC_CELLX], *celldx = c->m[ORC_CELLDX], *celly = c->m[ORC_CELLY], *celldy = c->m[ORC_CELLDY];
+  const int xrange = (x_max + 3) - (x_min - 2) + 1;
+  const int yrange = (y_max + 3) - (y_min - 2) + 1;
+  for (int j = 0; j < xrange; j++) {
+    vertexx[j] = xmin + dx * (j - 1 - x_min);
+    vertexdx[j] = dx;
+  }
+  for (int k = 0; k < yrange; k++) {
+    vertexy[k] = ymin + dy * (k - 1 - y_min);
+    vertexdy[k] = dy;
+  }
+  const int xrange1 = (x_max + 2) - (x_min - 2) + 1;
+  const int yrange1 = (y_max + 2) - (y_min - 2) + 1;
+  for (int j = 0; j < xrange1; j++) {
+    cellx[j] = 0.5 * (vertexx[j] + vertexx[j + 1]);
+    celldx[j] = dx;
+  }
+  for (int k = 0; k < yrange1; k++) {
+    celly[k] = 0.5 * (vertexy[k] + vertexy[k + 1]);
+    celldy[k] = dy;
+  }
+  double *volume = c->f[ORC_VOLUME], *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA];
+  for (int j = 0; j < yrange1; j++) {
+    for (int i = 0; i < xrange1; i++) {
+      AT(volume, i, j) = dx * dy;
+      AT(xarea, i, j) = celldy[j];
+      AT(yarea, i, j) = celldx[i];
+    }
+  }
+}
+
+/* src/serial/generate_chunk.cpp:31-131 */
+void orc_generate_chunk(orc_chunk *c, const orc_config *cfg) {
+  const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
+  const int xrange = (x_max + 2) - (x_min - 2) + 1;
+  const int yrange = (y_max + 2) - (y_min - 2) + 1;
+  double *energy0 = c->f[ORC_ENERGY0], *density0 = c->f[ORC_DENSITY0], *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  const double *vertexx = c->m[ORC_VERTEXX], *vertexy = c->m[ORC_VERTEXY], *cellx = c->m[ORC_CELLX], *celly = c->m[ORC_CELLY];
+  const orc_state *st = cfg->states;
+  for (int j = 0; j < yrange; j++) {
+    for (int i = 0; i < xrange; i++) {
+      AT(energy0, i, j) = st[0].energy;
+      AT(density0, i, j) = st[0].density;
+      AT(xvel0, i, j) = st[0].xvel;
+      AT(yvel0, i, j) = st[0].yvel;
+    }
+  }
+  for (int s = 1; s < cfg->number_of_states; ++s) {
+    for (int j = 0; j < yrange; j++) {
+      for (int i = 0; i < xrange; i++) {
+        double x_cent = st[s].xmin;
+        double y_cent = st[s].ymin;
+        int hit = 0;
+        if (st[s].geometry == 1) {
+          hit = (vertexx[i + 1] >= st[s].xmin && vertexx[i] < st[s].xmax) && (vertexy[j + 1] >= st[s].ymin && vertexy[j] < st[s].ymax);
+        } else if (st[s].geometry == 2) {
+          double radius = sqrt((cellx[i] - x_cent) * (cellx[i] - x_cent) + (celly[j] - y_cent) * (celly[j] - y_cent));
+          hit = radius <= st[s].radius;
+        } else if (st[s].geometry == 3) {
+          hit = vertexx[i] == x_cent && vertexy[j] == y_cent;
+        }
+        if (hit) {
+          AT(energy0, i, j) = st[s].energy;
+          AT(density0, i, j) = st[s].density;
+          for (int kt = j; kt <= j + 1; ++kt) {
+            for (int jt = i; jt <= i + 1; ++jt) {
+              AT(xvel0, jt, kt) = st[s].xvel;
+              AT(yvel0, jt, kt) = st[s].yvel;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+static void orc_mask(int fields[15], const int *ids, int n) {
+  for (int i = 0; i < 15; ++i) fields[i] = 0;
+  for (int i = 0; i < n; ++i) fields[ids[i]] = 1;
+}
+
+/* driver/start.cpp:69-127 for one chunk and one tile. */
+void orc_start(orc_chunk *c, const orc_config *cfg, orc_globals *g, double summary0[5]) {
+  orc_build_field(c);
+  orc_initialise_chunk(c, cfg);
+  orc_generate_chunk(c, cfg);
+  orc_ideal_gas(c, 0);
+  int fields[15];
+  const int ids[] = {ORC_DENSITY0, ORC_ENERGY0, ORC_PRESSURE, ORC_VISCOSITY, ORC_DENSITY1,
+                     ORC_ENERGY1,  ORC_XVEL0,   ORC_YVEL0,    ORC_XVEL1,     ORC_YVEL1};
+  orc_mask(fields, ids, 10);
+  orc_update_halo(c, fields, 2);
+  g->step = 0;
+  g->advect_x = 1;
+  g->time = 0.0;
+  g->dt = cfg->dtinit;    /* driver/definitions.cpp:54 */
+  g->dtold = cfg->dtinit;
+  if (summary0) orc_field_summary(c, summary0);
+}
+
+/* driver/timestep.cpp:37-109 */
+int orc_timestep(orc_chunk *c, const orc_config *cfg, orc_globals *g) {
+  int fields[15];
+  g->dt = G_BIG;
+  orc_ideal_gas(c, 0);
+  const int ids1[] = {ORC_PRESSURE, ORC_ENERGY0, ORC_DENSITY0, ORC_XVEL0, ORC_YVEL0};
+  orc_mask(fields, ids1, 5);
+  orc_update_halo(c, fields, 1);
+  orc_viscosity(c);
+  const int ids2[] = {ORC_VISCOSITY};
+  orc_mask(fields, ids2, 1);
+  orc_update_halo(c, fields, 1);
+  double dtlp;
+  int l_control, jldt, kldt, small;
+  orc_calc_dt(c, cfg->dtmin, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe, &dtlp, &l_control, &jldt, &kldt, &small);
+  if (dtlp <= g->dt) g->dt = dtlp;
+  /* std::min(std::min(dt, dtold * dtrise), dtmax): std::min(a, b) = (b < a) ? b : a */
+  double a = g->dt, b = g->dtold * cfg->dtrise;
+  double m = (b < a) ? b : a;
+  g->dt = (cfg->dtmax < m) ? cfg->dtmax : m;
+  int too_small = g->dt < cfg->dtmin;
+  g->dtold = g->dt;
+  return too_small;
+}
+
+/* src/serial/PdV.cpp:92-133 */
+void orc_PdV_driver(orc_chunk *c, int predict, double dt) {
+  orc_PdV(c, predict, dt);
+  if (predict) {
+    orc_ideal_gas(c, 1);
+    int fields[15];
+    const int ids[] = {ORC_PRESSURE};
+    orc_mask(fields, ids, 1);
+    orc_update_halo(c, fields, 1);
+    orc_revert(c);
+  }
+}
+
+/* driver/advection.cpp:29-105 */
+void orc_advection(orc_chunk *c, int advect_x) {
+  int fields[15];
+  int direction = advect_x ? 1 : 2;
+  const int ids1[] = {ORC_ENERGY1, ORC_DENSITY1, ORC_VOL_FLUX_X, ORC_VOL_FLUX_Y};
+  orc_mask(fields, ids1, 4);
+  orc_update_halo(c, fields, 2);
+  orc_advec_cell(c, 1, direction);
+  const int ids2[] = {ORC_DENSITY1, ORC_ENERGY1, ORC_XVEL1, ORC_YVEL1, ORC_MASS_FLUX_X, ORC_MASS_FLUX_Y};
+  orc_mask(fields, ids2, 6);
+  orc_update_halo(c, fields, 2);
+  orc_advec_mom(c, 1, 1, direction);
+  orc_advec_mom(c, 2, 1, direction);
+  direction = advect_x ? 2 : 1;
+  orc_advec_cell(c, 2, direction);
+  orc_update_halo(c, fields, 2);
+  orc_advec_mom(c, 1, 2, direction);
+  orc_advec_mom(c, 2, 2, direction);
+}
+
+/* driver/hydro.cpp:57-89 */
+int orc_hydro_step(orc_chunk *c, const orc_config *cfg, orc_globals *g) {
+  g->step += 1;
+  int small = orc_timestep(c, cfg, g);
+  if (small) return 1;
+  orc_PdV_driver(c, 1, g->dt);
+  orc_accelerate(c, g->dt);
+  orc_PdV_driver(c, 0, g->dt);
+  orc_flux_calc(c, g->dt);
+  orc_advection(c, g->advect_x);
+  orc_reset_field(c);
+  g->advect_x = !g->advect_x;
+  g->time += g->dt;
+  return 0;
+}

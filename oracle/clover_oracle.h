@@ -1,0 +1,127 @@
+/*
+ * CPU oracle for the CloverLeaf hydro hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference's serial model (UoB-HPC/CloverLeaf,
+ * src/serial/<kernel>.cpp) and of its step sequencing (driver/hydro.cpp,
+ * driver/timestep.cpp, driver/advection.cpp, driver/start.cpp).  Each function
+ * cites the reference file:line it follows and keeps the reference's
+ * floating-point expression order, so that results are bit-identical.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * this library, and only as the checker.  The product library never calls it.
+ *
+ * Pinning (how each function is checked; the tests live in tests/):
+ *   - every kernel: bitwise against the reference serial kernel compiled from
+ *     its own sources (oracle/_ref/librefkern.so, built by oracle/Makefile
+ *     where the reference tree is present) on random grids, every direction,
+ *     sweep and branch;
+ *   - full runs: dt sequence and field summaries against the full-precision
+ *     output of the reference serial model built here (tests/golden/<deck>.json)
+ *     and the reference's own known answers (driver/report.cpp:77-92);
+ *   - closed forms and invariants: ideal gas c^2 = gamma p / rho; a uniform
+ *     single-state deck stays at rest and its dt equals
+ *     min(0.7 min(dx,dy) / sqrt(1.4 * 0.4 * e), dtold * dtrise, dtmax);
+ *     volume and mass are conserved; halos reflect with the reference's signs.
+ *
+ * Layout: every 2D array is x-contiguous with one row pitch px = nx + 5 and
+ * ny + 5 rows; element (i, j) (reference storage indices, 0 = x_min - 2) lives
+ * at [j * px + i].  Cell arrays use (nx+4) x (ny+4) of that, node arrays
+ * (nx+5) x (ny+5), x-face (nx+5) x (ny+4), y-face (nx+4) x (ny+5).
+ */
+#ifndef CLOVER_ORACLE_H
+#define CLOVER_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 2D field ids; 0..14 follow the reference enum field_parameter
+ * (driver/definitions.h:62-78) so that update_halo field masks match. */
+enum {
+  ORC_DENSITY0 = 0, ORC_DENSITY1 = 1, ORC_ENERGY0 = 2, ORC_ENERGY1 = 3,
+  ORC_PRESSURE = 4, ORC_VISCOSITY = 5, ORC_SOUNDSPEED = 6,
+  ORC_XVEL0 = 7, ORC_XVEL1 = 8, ORC_YVEL0 = 9, ORC_YVEL1 = 10,
+  ORC_VOL_FLUX_X = 11, ORC_VOL_FLUX_Y = 12, ORC_MASS_FLUX_X = 13, ORC_MASS_FLUX_Y = 14,
+  ORC_WORK1 = 15, ORC_WORK2, ORC_WORK3, ORC_WORK4, ORC_WORK5, ORC_WORK6, ORC_WORK7,
+  ORC_VOLUME = 22, ORC_XAREA = 23, ORC_YAREA = 24,
+  ORC_NUM_2D = 25
+};
+/* 1D mesh arrays (driver/definitions.cpp:28-35). */
+enum {
+  ORC_CELLX = 0, ORC_CELLY, ORC_VERTEXX, ORC_VERTEXY,
+  ORC_CELLDX, ORC_CELLDY, ORC_VERTEXDX, ORC_VERTEXDY,
+  ORC_NUM_1D = 8
+};
+
+typedef struct orc_chunk {
+  int nx, ny;             /* x_max, y_max; x_min = y_min = 1 */
+  int px;                 /* row pitch of every 2D array = nx + 5 */
+  double *f[ORC_NUM_2D];  /* (ny + 5) * px doubles each */
+  double *m[ORC_NUM_1D];  /* x arrays nx + 5, y arrays ny + 5 doubles */
+} orc_chunk;
+
+typedef struct orc_state {
+  double density, energy, xvel, yvel;
+  int geometry;           /* 1 rectangle, 2 circle, 3 point (driver/definitions.h:50) */
+  double xmin, ymin, xmax, ymax, radius;
+} orc_state;
+
+typedef struct orc_config {
+  double xmin, ymin, xmax, ymax;
+  int x_cells, y_cells;
+  double dtinit, dtmax, dtmin, dtrise;
+  double dtc_safe, dtu_safe, dtv_safe, dtdiv_safe;
+  double end_time;
+  int end_step;
+  int summary_frequency;
+  int number_of_states;
+  const orc_state *states;  /* states[0] is the background; extents already shrunk (read_input.cpp:268-276) */
+} orc_config;
+
+/* Per-step driver state (driver/definitions.h:265-274). */
+typedef struct orc_globals {
+  int step;
+  int advect_x;
+  double time, dt, dtold;
+} orc_globals;
+
+/* allocation (zeroed) */
+int orc_chunk_alloc(orc_chunk *c, int nx, int ny);
+void orc_chunk_free(orc_chunk *c);
+
+/* kernels */
+void orc_ideal_gas(orc_chunk *c, int predict);
+void orc_viscosity(orc_chunk *c);
+void orc_calc_dt(orc_chunk *c, double dtmin, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                 double *dt_min_val, int *dtl_control, int *jldt, int *kldt, int *small);
+void orc_PdV(orc_chunk *c, int predict, double dt);  /* kernel only */
+void orc_revert(orc_chunk *c);
+void orc_accelerate(orc_chunk *c, double dt);
+void orc_flux_calc(orc_chunk *c, double dt);
+void orc_advec_cell(orc_chunk *c, int sweep_number, int dir);
+void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction);
+void orc_reset_field(orc_chunk *c);
+void orc_update_halo(orc_chunk *c, const int fields[15], int depth);
+void orc_field_summary_kernel(orc_chunk *c, double out[5]);
+void orc_build_field(orc_chunk *c);
+void orc_initialise_chunk(orc_chunk *c, const orc_config *cfg);
+void orc_generate_chunk(orc_chunk *c, const orc_config *cfg);
+
+/* sequencing */
+/* driver/start.cpp:69-127: build, init, generate, ideal_gas, depth-2 halo, step-0 summary. */
+void orc_start(orc_chunk *c, const orc_config *cfg, orc_globals *g, double summary0[5]);
+/* driver/timestep.cpp:37-109 (dt selection; returns 1 if dt < dtmin). */
+int orc_timestep(orc_chunk *c, const orc_config *cfg, orc_globals *g);
+/* driver/PdV driver src/serial/PdV.cpp:92-133 (kernel + predictor chain). */
+void orc_PdV_driver(orc_chunk *c, int predict, double dt);
+/* driver/advection.cpp:29-105 */
+void orc_advection(orc_chunk *c, int advect_x);
+/* One pass of driver/hydro.cpp:57-89. Returns 1 if dt fell below dtmin. */
+int orc_hydro_step(orc_chunk *c, const orc_config *cfg, orc_globals *g);
+/* src/serial/field_summary.cpp:44-104: ideal_gas then the five sums. */
+void orc_field_summary(orc_chunk *c, double out[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
