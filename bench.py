@@ -112,7 +112,7 @@ def main():
     ap.add_argument("--plan", choices=["fused", "reference"], default="reference")
     ap.add_argument("--deck", default="clover_bm256_short")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--profile-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

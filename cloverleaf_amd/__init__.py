@@ -31,6 +31,7 @@ FID = {n: i for i, n in enumerate(FIELDS)}
 MESH = ["cellx", "celly", "vertexx", "vertexy", "celldx", "celldy", "vertexdx", "vertexdy"]
 MID = {n: i for i, n in enumerate(MESH)}
 PLAN_REFERENCE, PLAN_FUSED = 0, 1
+SUMMARY_TREE, SUMMARY_SERIAL = 0, 1
 MAX_STATES = 16
 ERR_SMALL_DT = -4
 
@@ -112,6 +113,7 @@ SIGNATURES = [
     ("clv_reset_field", _I, [_P]),
     ("clv_update_halo", _I, [_P, _PI, _I]),
     ("clv_field_summary", _I, [_P, _PD]),
+    ("clv_chunk_set_summary_order", _I, [_P, _I]),
     ("clv_deck_parse", _I, [C.c_char_p, C.POINTER(Config)]),
     ("clv_hydro_start", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _PD]),
     ("clv_timestep", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _I]),
@@ -265,6 +267,10 @@ class Chunk:
         _check(self.L.clv_field_summary(self.h, out), "field_summary")
         return list(out)
 
+    def set_summary_order(self, order: int):
+        """SUMMARY_TREE (default) or SUMMARY_SERIAL (bit-identical to the serial model)."""
+        _check(self.L.clv_chunk_set_summary_order(self.h, order), "set_summary_order")
+
     def build_field(self):
         _check(self.L.clv_build_field(self.h), "build_field")
 
@@ -306,10 +312,11 @@ class Chunk:
         return {names[i].decode(): (int(launches[i]), float(ms[i])) for i in range(min(k, n))}
 
 
-def run(cfg: Config, plan: int = PLAN_FUSED, trace: bool = False):
+def run(cfg: Config, plan: int = PLAN_FUSED, trace: bool = False, summary_order: int = 0):
     """Run a deck to completion (driver/hydro.cpp:53-111); returns the summary records."""
     recs = []
     with Chunk(cfg.x_cells, cfg.y_cells) as ch:
+        ch.set_summary_order(summary_order)
         st, s = ch.start(cfg)
         recs.append(dict(step=0, time=st.time, dt=st.dt, vol=s[0], mass=s[1], ie=s[2], ke=s[3], press=s[4]))
         while True:

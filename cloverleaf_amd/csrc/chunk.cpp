@@ -372,6 +372,13 @@ clv_status clv_field_summary(clv_chunk *c, double out[5]) {
   return k_field_summary(c, out);
 }
 
+clv_status clv_chunk_set_summary_order(clv_chunk *c, int order) {
+  CHECK_CHUNK(c);
+  if (order != CLV_SUMMARY_TREE && order != CLV_SUMMARY_SERIAL) return set_error(CLV_ERR_ARG, "clv_chunk_set_summary_order: bad order");
+  c->summary_serial = order == CLV_SUMMARY_SERIAL;
+  return CLV_OK;
+}
+
 clv_status clv_profile_enable(clv_chunk *c, int on) {
   CHECK_CHUNK(c);
   if (on && !c->prof) c->prof = new Profiler();

@@ -48,6 +48,7 @@ struct clv_chunk {
   double *result_dev = nullptr;
   double *result_host = nullptr;
   clv::Profiler *prof = nullptr;
+  int summary_serial = 0;  // CLV_SUMMARY_SERIAL: reproduce the serial model's summation order
 };
 
 namespace clv {

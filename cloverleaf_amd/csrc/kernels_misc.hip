@@ -264,6 +264,60 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_sum5_k(const double *parti
   }
 }
 
+// The serial model's own summation order (src/serial/field_summary.cpp:76-91: one
+// left-to-right sum over the linear cell index, x fastest), reproduced exactly:
+// the block computes the per-cell terms of 1024 consecutive cells into LDS, then
+// one lane folds them in index order.  The fold is sequential by definition, so
+// this runs at a few hundred M cells/s; it exists to reproduce the reference's
+// printed summaries bit-for-bit, not for throughput.
+constexpr int SER_CHUNK = 1024;
+__global__ __launch_bounds__(RED_THREADS) void field_summary_serial_k(Dev d, double *out) {
+  __shared__ double terms[5][SER_CHUNK];
+  const int64_t width = d.nx, n = (int64_t)d.nx * d.ny;
+  double vol = 0.0, mass = 0.0, ie = 0.0, ke = 0.0, press = 0.0;
+  for (int64_t base = 0; base < n; base += SER_CHUNK) {
+    for (int t = threadIdx.x; t < SER_CHUNK; t += RED_THREADS) {
+      const int64_t idx = base + t;
+      if (idx >= n) break;
+      const int j = 2 + (int)(idx % width);
+      const int k = 2 + (int)(idx / width);
+      double vsqrd = 0.0;
+      for (int kv = k; kv <= k + 1; ++kv) {
+        for (int jv = j; jv <= j + 1; ++jv) {
+          const double u = AT(CLV_XVEL0, jv, kv), v = AT(CLV_YVEL0, jv, kv);
+          vsqrd += 0.25 * (u * u + v * v);
+        }
+      }
+      const double cell_vol = AT(CLV_VOLUME, j, k);
+      const double cell_mass = cell_vol * AT(CLV_DENSITY0, j, k);
+      terms[0][t] = cell_vol;
+      terms[1][t] = cell_mass;
+      terms[2][t] = cell_mass * AT(CLV_ENERGY0, j, k);
+      terms[3][t] = cell_mass * 0.5 * vsqrd;
+      terms[4][t] = cell_vol * AT(CLV_PRESSURE, j, k);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int m = (int)((n - base) < SER_CHUNK ? (n - base) : SER_CHUNK);
+      for (int t = 0; t < m; ++t) {
+        vol += terms[0][t];
+        mass += terms[1][t];
+        ie += terms[2][t];
+        ke += terms[3][t];
+        press += terms[4][t];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = vol;
+    out[1] = mass;
+    out[2] = ie;
+    out[3] = ke;
+    out[4] = press;
+  }
+}
+
 // ---- set-up -----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void fill_zero_k(double *p, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.0;
@@ -407,8 +461,13 @@ clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_
 
 clv_status k_field_summary(clv_chunk *c, double out[5]) {
   const int nb = red_blocks(c->d.ny);
-  LAUNCH_G("field_summary", field_summary_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->partials);
-  LAUNCH_G("field_summary_final", reduce_sum5_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), (const double *)c->partials, nb, c->result_dev);
+  if (c->summary_serial) {
+    LAUNCH_G("field_summary_serial", field_summary_serial_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->result_dev);
+  } else {
+    LAUNCH_G("field_summary", field_summary_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->partials);
+    LAUNCH_G("field_summary_final", reduce_sum5_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), (const double *)c->partials, nb,
+             c->result_dev);
+  }
   clv_status st = hip_check(hipMemcpyAsync(c->result_host, c->result_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "summary copy");
   if (st != CLV_OK) return st;
   st = hip_check(hipStreamSynchronize(c->stream), "summary sync");

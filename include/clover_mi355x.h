@@ -187,6 +187,14 @@ clv_status clv_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], 
  * ideal_gas call that precedes them in the reference; out = {vol, mass, ie, ke,
  * press}.  Blocks.  Fixed-order reduction: run-to-run reproducible. */
 clv_status clv_field_summary(clv_chunk *c, double out[5]);
+/* Summation order of clv_field_summary / clv_hydro_summary on this chunk.
+ * CLV_SUMMARY_TREE (default): fixed-order tree, reproducible, within ~1e-15
+ * relative of the exact sums.  CLV_SUMMARY_SERIAL: the serial model's own
+ * left-to-right order over the linear cell index, so the five sums are
+ * bit-identical to the reference serial model's; the fold is sequential
+ * (a few hundred M cells/s), meant for validation runs. */
+enum { CLV_SUMMARY_TREE = 0, CLV_SUMMARY_SERIAL = 1 };
+clv_status clv_chunk_set_summary_order(clv_chunk *c, int order);
 
 /* ---- whole-step forms (the host loop of driver/hydro.cpp) -------------------------------- */
 /* Parses deck text with driver/read_input.cpp semantics (defaults :39-69). */
