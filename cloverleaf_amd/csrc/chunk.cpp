@@ -104,9 +104,14 @@ clv_status after_launch(clv_chunk *c, const char *name) {
 }
 
 static clv_status alloc_scratch(clv_chunk *c) {
-  clv_status st = hip_check(hipMalloc(&c->partials, sizeof(double) * 5 * 2048), "hipMalloc partials");
+  // block partials of the reductions: 5 sums x 2048 blocks, or one minimum per
+  // wave of the fused timestep kernel, whichever is larger
+  size_t n = 5 * 2048;
+  const size_t nf = fused_partials_needed(c->d.nx, c->d.ny);
+  if (nf > n) n = nf;
+  clv_status st = hip_check(hipMalloc(&c->partials, sizeof(double) * n), "hipMalloc partials");
   if (st != CLV_OK) return st;
-  c->partials_len = 5 * 2048;
+  c->partials_len = n;
   st = hip_check(hipMalloc(&c->result_dev, sizeof(double) * 8), "hipMalloc result");
   if (st != CLV_OK) return st;
   return hip_check(hipHostMalloc((void **)&c->result_host, sizeof(double) * 8, hipHostMallocDefault), "hipHostMalloc result");

@@ -76,4 +76,10 @@ clv_status k_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], in
 clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, double *dt_min_val);
 clv_status k_field_summary(clv_chunk *c, double out[5]);
 
+// fused plan (kernels_fused.hip)
+size_t fused_partials_needed(int nx, int ny);
+clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val);
+clv_status fused_lagrange(clv_chunk *c, double dt);
+clv_status fused_advection(clv_chunk *c, int advect_x);
+
 }  // namespace clv

@@ -1,12 +1,797 @@
 // Fused sweeps for CLV_PLAN_FUSED.
+//
+// One step of driver/hydro.cpp:57-89 in six single-pass kernels:
+//
+//   timestep_k  ideal_gas(rho0,e0) + viscosity + calc_dt partial minimum
+//               (driver/timestep.cpp:47-78)        writes p, q
+//   -- global minimum -> dt on the host --
+//   lagrange_k  PdV predictor + ideal_gas + pressure halo + revert, accelerate,
+//               PdV corrector, flux_calc (driver/hydro.cpp:63-75)
+//                                                   writes u1, v1, rho1, e1, vol_flux_x/y
+//   cell_x_k / cell_y_k   advec_cell, one direction (src/serial/advec_cell.cpp)
+//                                                   writes rho1, e1, mass_flux_d
+//   mom_x_k / mom_y_k     advec_mom for xvel1 AND yvel1 (the yvel pass reuses the
+//               xvel pass's node arrays, src/serial/advec_mom.cpp:81-103, so both
+//               velocities share one set)           writes u1, v1
+//   reset_field: pointer swaps 1 <-> 0.
+//
+// Every value a kernel writes is computed by the reference expression on the
+// same inputs, so the persistent fields are bit-identical to the reference
+// plan.  Intermediates the reference keeps in work arrays (pre/post volumes,
+// node fluxes and masses, momentum and energy fluxes, the predictor's
+// rho1/e1/p) stay in registers; dead stores are not made (revert's copy, the
+// predictor's soundspeed, advec_mom's pre_vol, advec_cell's post_vol).
+//
+// Launch shape.  One wave64 owns a strip of consecutive columns (x-contiguous,
+// coalesced) and marches up a segment of SEG rows, so neighbours in y come
+// from registers of the previous row and every input element is loaded once.
+// Neighbours in x of computed values come from the adjacent lane (wave
+// shuffles); the lanes at a strip's edges only supply those values, so strips
+// overlap by the stencil width and each lane of the interior writes results.
+// Inputs needed at x+1 / x-1 also come from shuffles where the lane ranges
+// allow, or from a second (cache-hitting) load.
+//
+// In-place hazards: a kernel that updates a field its neighbours' stencils read
+// (rho1, e1 in advec_cell; u1, v1 in advec_mom) writes into a work array, and
+// the host swaps the two pointers afterwards.
 #include "clv_internal.h"
 
 namespace clv {
+namespace {
 
-clv_status fused_timestep_kernels(clv_chunk *, const clv_config *, double *) {
-  return set_error(CLV_ERR_ARG, "CLV_PLAN_FUSED is not available in this build");
+#define AT(fid, i, j) d.f[fid][(int64_t)(j) * d.pitch + (i)]
+#define M(mid, i) d.m[mid][i]
+
+constexpr int SEG = 64;        // rows per wave segment
+constexpr int WAVES = 4;       // waves (strips) per 256-thread block
+
+__device__ __forceinline__ double up1(double v) { return __shfl_up(v, 1, 64); }      // value of lane - 1
+__device__ __forceinline__ double up2(double v) { return __shfl_up(v, 2, 64); }      // value of lane - 2
+__device__ __forceinline__ double dn1(double v) { return __shfl_down(v, 1, 64); }    // value of lane + 1
+__device__ __forceinline__ double dn2(double v) { return __shfl_down(v, 2, 64); }    // value of lane + 2
+
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
 }
-clv_status fused_lagrange(clv_chunk *, double) { return set_error(CLV_ERR_ARG, "CLV_PLAN_FUSED is not available in this build"); }
-clv_status fused_advection(clv_chunk *, int) { return set_error(CLV_ERR_ARG, "CLV_PLAN_FUSED is not available in this build"); }
+
+// Reflective mirror of a cell index (src/serial/update_halo.cpp:47-77: ghost
+// 1 - k <- 2 + k, ghost max + 2 + k <- max + 1 - k).  Reading a cell field's
+// ghost through its mirror gives what the reference's halo update would have
+// put there, whatever the ghost layer holds.
+__device__ __forceinline__ int mirror_cell(int i, int n) { return i < 2 ? 3 - i : (i > n + 1 ? 2 * n + 3 - i : i); }
+
+// ---- timestep: ideal_gas + viscosity + calc_dt ----------------------------------------
+// Lanes: column x0 + L, x0 = 1 + 62 * strip; lanes 1..62 own cells (p of the
+// neighbouring columns comes from lanes 0 and 63).
+__device__ __forceinline__ void ideal_gas_cell(double rho, double e, double &p, double &c) {
+  // src/serial/ideal_gas.cpp:40-45
+  double v = 1.0 / rho;
+  p = (1.4 - 1.0) * rho * e;
+  double pressurebyenergy = (1.4 - 1.0) * rho;
+  double pressurebyvolume = -rho * p;
+  double sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
+  c = sqrt(sound_speed_squared);
+}
+
+__device__ __forceinline__ double pressure_of(double rho, double e) { return (1.4 - 1.0) * rho * e; }
+
+__global__ __launch_bounds__(256) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                                                  double *partials, int nstrips) {
+  const int lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;  // whole wave
+  const int x0 = 1 + 62 * strip;
+  const int i = x0 + lane;
+  const int ic = i < d.nx + 4 ? i : d.nx + 4;  // clamped load column (lanes past the chunk)
+  const bool own = lane >= 1 && lane <= 62 && i <= d.nx + 1;
+  const int j0 = 2 + SEG * blockIdx.y;
+  const int j1 = min(j0 + SEG, d.ny + 2);
+
+  const double cdx = M(CLV_CELLDX, ic), cdx1 = M(CLV_CELLDX, ic + 1 <= d.nx + 4 ? ic + 1 : ic);
+  // rho0/e0 of ghost cells through their mirrors (the halo of driver/timestep.cpp:53-60)
+  const int im = mirror_cell(ic, d.nx);
+  // p at rows r-1, r (window) and r+1
+  double p_m = pressure_of(AT(CLV_DENSITY0, im, mirror_cell(j0 - 1, d.ny)), AT(CLV_ENERGY0, im, mirror_cell(j0 - 1, d.ny)));
+  double rho_c = AT(CLV_DENSITY0, im, j0), e_c = AT(CLV_ENERGY0, im, j0);
+  double p_c, c_c;
+  ideal_gas_cell(rho_c, e_c, p_c, c_c);
+  // node row r velocities (own column and column + 1 via shuffle)
+  double u_r = AT(CLV_XVEL0, ic, j0), v_r = AT(CLV_YVEL0, ic, j0);
+  double ya_r = AT(CLV_YAREA, ic, j0);
+  double dtmin = g_big;
+  for (int r = j0; r < j1; ++r) {
+    const int rm = mirror_cell(r + 1, d.ny);
+    const double rho_p = AT(CLV_DENSITY0, im, rm), e_p = AT(CLV_ENERGY0, im, rm);
+    double p_p, c_p;
+    ideal_gas_cell(rho_p, e_p, p_p, c_p);
+    const double u_r1 = AT(CLV_XVEL0, ic, r + 1), v_r1 = AT(CLV_YVEL0, ic, r + 1);
+    const double ya_r1 = AT(CLV_YAREA, ic, r + 1);
+    const double xa = AT(CLV_XAREA, ic, r);
+    const double vol = AT(CLV_VOLUME, ic, r);
+    // neighbours along x
+    const double p_w = up1(p_c), p_e = dn1(p_c);
+    const double u10 = dn1(u_r), u11 = dn1(u_r1), v10 = dn1(v_r), v11 = dn1(v_r1), xa1 = dn1(xa);
+    const double u00 = u_r, u01 = u_r1, v00 = v_r, v01 = v_r1;
+    const double cdy = M(CLV_CELLDY, r), cdy2 = M(CLV_CELLDY, r + 2);
+    // viscosity, src/serial/viscosity.cpp:39-65
+    double ugrad = (u10 + u11) - (u00 + u01);
+    double vgrad = (v01 + v11) - (v00 + v10);
+    double div = (cdx * (ugrad) + cdy * (vgrad));
+    double strain2 = 0.5 * (u01 + u11 - u00 - u10) / cdy + 0.5 * (v10 + v11 - v00 - v01) / cdx;
+    double pgradx = (p_e - p_w) / (cdx + cdx1);
+    double pgrady = (p_p - p_m) / (cdy + cdy2);
+    double pgradx2 = pgradx * pgradx;
+    double pgrady2 = pgrady * pgrady;
+    double limiter = ((0.5 * (ugrad) / cdx) * pgradx2 + (0.5 * (vgrad) / cdy) * pgrady2 + strain2 * pgradx * pgrady) /
+                     fmax(pgradx2 + pgrady2, g_small);
+    double q;
+    if ((limiter > 0.0) || (div >= 0.0)) {
+      q = 0.0;
+    } else {
+      double dirx = 1.0;
+      if (pgradx < 0.0) dirx = -1.0;
+      pgradx = dirx * fmax(g_small, fabs(pgradx));
+      double diry = 1.0;
+      if (pgradx < 0.0) diry = -1.0;
+      pgrady = diry * fmax(g_small, fabs(pgrady));
+      double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
+      double xgrad = fabs(cdx * pgrad / pgradx);
+      double ygrad = fabs(cdy * pgrad / pgrady);
+      double grad = fmin(xgrad, ygrad);
+      double grad2 = grad * grad;
+      q = 2.0 * rho_c * grad2 * limiter * limiter;
+    }
+    // calc_dt, src/serial/calc_dt.cpp:52-75
+    {
+      double dsx = cdx;
+      double dsy = cdy;
+      double cc = c_c * c_c;
+      cc = cc + 2.0 * q / rho_c;
+      cc = fmax(sqrt(cc), g_small);
+      double dtct = dtc_safe * fmin(dsx, dsy) / cc;
+      double dv = 0.0;
+      double dv1 = (u00 + u01) * xa;
+      double dv2 = (u10 + u11) * xa1;
+      dv = dv + dv2 - dv1;
+      double dtut = dtu_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
+      dv1 = (v00 + v10) * ya_r;
+      dv2 = (v01 + v11) * ya_r1;
+      dv = dv + dv2 - dv1;
+      double dtvt = dtv_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
+      dv = dv / (2.0 * vol);
+      double dtdivt;
+      if (dv < -g_small) {
+        dtdivt = dtdiv_safe * (-1.0 / dv);
+      } else {
+        dtdivt = g_big;
+      }
+      double mins = fmin(dtct, fmin(dtut, fmin(dtvt, fmin(dtdivt, g_big))));
+      if (own) dtmin = fmin(mins, dtmin);
+    }
+    if (own) {
+      AT(CLV_PRESSURE, i, r) = p_c;
+      AT(CLV_VISCOSITY, i, r) = q;
+    }
+    p_m = p_c;
+    p_c = p_p;
+    c_c = c_p;
+    rho_c = rho_p;
+    u_r = u_r1;
+    v_r = v_r1;
+    ya_r = ya_r1;
+  }
+  dtmin = wave_min(dtmin);
+  if (lane == 0) partials[(size_t)blockIdx.y * nstrips + strip] = dtmin;
+}
+
+__global__ __launch_bounds__(1024) void reduce_min_big_k(const double *partials, int n, double *out) {
+  double m = g_big;
+  for (int b = threadIdx.x; b < n; b += 1024) m = fmin(m, partials[b]);
+  __shared__ double lds[16];
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = lds[0];
+    for (int w = 1; w < 16; ++w) r = fmin(r, lds[w]);
+    out[0] = r;
+  }
+}
+
+// ---- lagrange: PdV predictor chain, accelerate, PdV corrector, flux_calc ----------------
+// Lanes: column x0 + L, x0 = 1 + 61 * strip.  Cells are owned by lanes 1..61;
+// nodes by lanes 1..61 plus the boundary node column nx + 2 (lane <= 62).
+// March: cell/node row r from j0 - 1 (or 2) to j1; at row r the predictor
+// pressure p'(r), the node velocities of node row r, then the corrector of cell
+// row r - 1 and the fluxes.
+struct PredIn {
+  double xa0, xa1, ya0, ya1, u00, u01, u10, u11, v00, v01, v10, v11, vol, rho0, e0, p, q;
+};
+
+// src/serial/PdV.cpp:49-63 (predict) then src/serial/ideal_gas.cpp:41 (pressure only: the
+// predictor's soundspeed is overwritten before use).
+__device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt) {
+  double left_flux = (a.xa0 * (a.u00 + a.u01 + a.u00 + a.u01)) * 0.25 * dt * 0.5;
+  double right_flux = (a.xa1 * (a.u10 + a.u11 + a.u10 + a.u11)) * 0.25 * dt * 0.5;
+  double bottom_flux = (a.ya0 * (a.v00 + a.v10 + a.v00 + a.v10)) * 0.25 * dt * 0.5;
+  double top_flux = (a.ya1 * (a.v01 + a.v11 + a.v01 + a.v11)) * 0.25 * dt * 0.5;
+  double total_flux = right_flux - left_flux + top_flux - bottom_flux;
+  double volume_change_s = a.vol / (a.vol + total_flux);
+  double recip_volume = 1.0 / a.vol;
+  double energy_change = (a.p / a.rho0 + a.q / a.rho0) * total_flux * recip_volume;
+  double e1 = a.e0 - energy_change;
+  double rho1 = a.rho0 * volume_change_s;
+  return (1.4 - 1.0) * rho1 * e1;
+}
+
+__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips) {
+  const int lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  const int nx = d.nx, ny = d.ny;
+  const int x0 = 1 + 61 * strip;
+  const int i = x0 + lane;
+  const int ic = i < nx + 4 ? i : nx + 4;
+  const bool own_cell = lane >= 1 && lane <= 61 && i <= nx + 1;
+  const bool own_node = (lane >= 1 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
+  const int j0 = 2 + SEG * blockIdx.y;
+  const int jc1 = min(j0 + SEG, ny + 2);           // cells [j0, jc1)
+  const int jn1 = jc1 == ny + 2 ? ny + 3 : jc1;    // owned node rows [j0, jn1)
+  const double halfdt = 0.5 * dt;                  // src/serial/accelerate.cpp:33
+
+  // per-row loads of column ic: node velocities (rows r, r+1), areas, cell data
+  auto load_pred = [&](int r, double u_r, double v_r, double u_r1, double v_r1, PredIn &a) {
+    a.xa0 = AT(CLV_XAREA, ic, r);
+    a.xa1 = dn1(a.xa0);
+    a.ya0 = AT(CLV_YAREA, ic, r);
+    a.ya1 = AT(CLV_YAREA, ic, r + 1);
+    a.u00 = u_r;
+    a.u01 = u_r1;
+    a.u10 = dn1(u_r);
+    a.u11 = dn1(u_r1);
+    a.v00 = v_r;
+    a.v01 = v_r1;
+    a.v10 = dn1(v_r);
+    a.v11 = dn1(v_r1);
+    a.vol = AT(CLV_VOLUME, ic, r);
+    // ghost rho0 through the mirror (the halo of driver/timestep.cpp:53-60)
+    a.rho0 = AT(CLV_DENSITY0, mirror_cell(ic, nx), mirror_cell(r, ny));
+    a.e0 = AT(CLV_ENERGY0, ic, r);
+    a.p = AT(CLV_PRESSURE, ic, r);
+    a.q = AT(CLV_VISCOSITY, ic, r);
+  };
+  // predictor pressure of column i, with the reflective ghost columns taking the
+  // interior value (the pressure halo of src/serial/PdV.cpp:121-125)
+  auto mirror_cols = [&](double pp) {
+    const double from_right = dn1(pp), from_left = up1(pp);
+    return i == 1 ? from_right : (i == nx + 2 ? from_left : pp);
+  };
+
+  // window: previous cell row (r - 1) and previous node row (r - 1)
+  double pp_prev = 0.0, m_prev = 0.0, q_prev = 0.0, xa_prev = 0.0;
+  double u0_prev = 0.0, v0_prev = 0.0, u1_prev = 0.0, v1_prev = 0.0;
+  double rho0_prev = 0.0, e0_prev = 0.0, vol_prev = 0.0, q0_prev = 0.0, ya_prev = 0.0;
+  // node row r velocities; the march starts one row below the segment (the
+  // ghost row 1 for the first segment, whose p' is the mirror of row 2)
+  int r = j0 - 1;
+  double u_r = AT(CLV_XVEL0, ic, r), v_r = AT(CLV_YVEL0, ic, r);
+  for (; r <= jc1; ++r) {
+    const double u_r1 = AT(CLV_XVEL0, ic, r + 1);
+    const double v_r1 = AT(CLV_YVEL0, ic, r + 1);
+    // cell row r inputs
+    PredIn a;
+    load_pred(r, u_r, v_r, u_r1, v_r1, a);
+    double pp;
+    if (r == 1) {
+      pp = 0.0;  // set from row 2 below, before any use
+    } else if (r == ny + 2) {
+      pp = pp_prev;  // ghost row: p'(ny+2) = p'(ny+1)
+    } else {
+      pp = mirror_cols(predicted_pressure(a, dt));
+    }
+    if (r == 2) pp_prev = pp;  // ghost row 1 mirrors row 2
+    const double m = a.rho0 * a.vol;
+    if (r >= j0) {
+      // accelerate at node (i, r), src/serial/accelerate.cpp:43-53
+      const double m_w = up1(m), m_w_prev = up1(m_prev);
+      double stepbymass_s = halfdt / ((m_w_prev + m_w + m + m_prev) * 0.25);
+      const double pp_w = up1(pp), pp_w_prev = up1(pp_prev);
+      const double q_w = up1(a.q), q_w_prev = up1(q_prev);
+      const double ya_w = up1(a.ya0);
+      double u1 = u_r - stepbymass_s * (a.xa0 * (pp - pp_w) + xa_prev * (pp_prev - pp_w_prev));
+      double v1 = v_r - stepbymass_s * (a.ya0 * (pp - pp_prev) + ya_w * (pp_w - pp_w_prev));
+      u1 = u1 - stepbymass_s * (a.xa0 * (a.q - q_w) + xa_prev * (q_prev - q_w_prev));
+      v1 = v1 - stepbymass_s * (a.ya0 * (a.q - q_prev) + ya_w * (q_w - q_w_prev));
+      const double u1_e = dn1(u1), v1_e = dn1(v1), u1_e_prev = dn1(u1_prev), v1_e_prev = dn1(v1_prev);
+      const double u0_e = dn1(u_r), v0_e = dn1(v_r), u0_e_prev = dn1(u0_prev), v0_e_prev = dn1(v0_prev);
+      const double xa_e_prev = dn1(xa_prev);
+      if (r > j0) {
+        // corrector for cell (i, r - 1), src/serial/PdV.cpp:72-83
+        const int c = r - 1;
+        double left_flux = (xa_prev * (u0_prev + u_r + u1_prev + u1)) * 0.25 * dt;
+        double right_flux = (xa_e_prev * (u0_e_prev + u0_e + u1_e_prev + u1_e)) * 0.25 * dt;
+        double bottom_flux = (ya_prev * (v0_prev + v0_e_prev + v1_prev + v1_e_prev)) * 0.25 * dt;
+        double top_flux = (a.ya0 * (v_r + v0_e + v1 + v1_e)) * 0.25 * dt;
+        double total_flux = right_flux - left_flux + top_flux - bottom_flux;
+        double volume_change_s = vol_prev / (vol_prev + total_flux);
+        double recip_volume = 1.0 / vol_prev;
+        double energy_change = (pp_prev / rho0_prev + q0_prev / rho0_prev) * total_flux * recip_volume;
+        if (own_cell && c < jc1) {
+          AT(CLV_ENERGY1, i, c) = e0_prev - energy_change;
+          AT(CLV_DENSITY1, i, c) = rho0_prev * volume_change_s;
+        }
+        // flux_calc, src/serial/flux_calc.cpp:39: x-face (i, r - 1)
+        if (own_node && c < jc1) AT(CLV_VOL_FLUX_X, i, c) = 0.25 * dt * xa_prev * (u0_prev + u_r + u1_prev + u1);
+      }
+      if (r < jn1) {
+        if (own_node) {
+          AT(CLV_XVEL1, i, r) = u1;
+          AT(CLV_YVEL1, i, r) = v1;
+        }
+        // flux_calc, src/serial/flux_calc.cpp:40: y-face (i, r)
+        if (own_cell) AT(CLV_VOL_FLUX_Y, i, r) = 0.25 * dt * a.ya0 * (v_r + v0_e + v1 + v1_e);
+      }
+      u1_prev = u1;
+      v1_prev = v1;
+    }
+    pp_prev = pp;
+    m_prev = m;
+    q_prev = a.q;
+    xa_prev = a.xa0;
+    ya_prev = a.ya0;
+    u0_prev = u_r;
+    v0_prev = v_r;
+    rho0_prev = a.rho0;
+    e0_prev = a.e0;
+    vol_prev = a.vol;
+    q0_prev = a.q;
+    u_r = u_r1;
+    v_r = v_r1;
+  }
+}
+
+// ---- advec_cell -------------------------------------------------------------------------
+__device__ __forceinline__ double vl_limiter(double sigma, double sigma3, double sigma4, double uw, double dn, double dw) {
+  const double one_by_six = 1.0 / 6.0;
+  double diffuw = dn - uw;
+  double diffdw = dw - dn;
+  double wind = 1.0;
+  if (diffdw <= 0.0) wind = -1.0;
+  double limiter;
+  if (diffuw * diffdw > 0.0) {
+    limiter = (1.0 - sigma) * wind * fmin(fmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
+  } else {
+    limiter = 0.0;
+  }
+  return limiter;
+}
+
+// Face flux of src/serial/advec_cell.cpp:71-110 given the cell values around the face.
+// *_m2, *_m1, *_0, *_p1 are the cells at face - 2, face - 1, face, face + 1 (the +1
+// one already clamped by the caller, :77).
+__device__ __forceinline__ void cell_face_flux(double vf, double pre_m1, double pre_0, double rho_m2, double rho_m1, double rho_0,
+                                               double rho_p1, double e_m2, double e_m1, double e_0, double e_p1, double vd_face,
+                                               double vd_m1, double vd_p1, double &mf, double &ef) {
+  double pre_d, rho_uw, rho_d, rho_dw, e_uw, e_d, e_dw, vd_dif;
+  if (vf > 0.0) {
+    rho_uw = rho_m2; rho_d = rho_m1; rho_dw = rho_0;
+    e_uw = e_m2; e_d = e_m1; e_dw = e_0;
+    pre_d = pre_m1;
+    vd_dif = vd_m1;
+  } else {
+    rho_uw = rho_p1; rho_d = rho_0; rho_dw = rho_m1;
+    e_uw = e_p1; e_d = e_0; e_dw = e_m1;
+    pre_d = pre_0;
+    vd_dif = vd_p1;
+  }
+  double sigmat = fabs(vf) / pre_d;
+  double sigma3 = (1.0 + sigmat) * (vd_face / vd_dif);
+  double sigma4 = 2.0 - sigmat;
+  double sigmav = sigmat;
+  double limiter = vl_limiter(sigmav, sigma3, sigma4, rho_uw, rho_d, rho_dw);
+  mf = vf * (rho_d + limiter);
+  double sigmam = fabs(mf) / (rho_d * pre_d);
+  limiter = vl_limiter(sigmam, sigma3, sigma4, e_uw, e_d, e_dw);
+  ef = mf * (e_d + limiter);
+}
+
+// x direction.  Lanes: column x0 + L, x0 = 60 * strip; faces valid on lanes 2..62,
+// cells updated on lanes 2..61.  Rows independent (no y stencil besides the own
+// cell's top face of vol_flux_y in sweep 1).
+template <int SWEEP>
+__global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+  const int lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  const int nx = d.nx, ny = d.ny;
+  const int x0 = 60 * strip;
+  const int i = x0 + lane;
+  const int ic = i < nx + 4 ? i : nx + 4;
+  const bool own_cell = lane >= 2 && lane <= 61 && i <= nx + 1;
+  const bool own_face = (lane >= 2 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
+  const int j0 = 2 + SEG * blockIdx.y;
+  const int j1 = min(j0 + SEG, ny + 2);
+  const double vd_face = M(CLV_VERTEXDX, ic);
+  // shuffles run on every lane; the clamp selects afterwards
+  const double vd_m1 = up1(vd_face), vd_e = dn1(vd_face);
+  const double vd_p1 = i + 1 <= nx + 2 ? vd_e : vd_face;
+  double vfy_r = SWEEP == 1 ? AT(CLV_VOL_FLUX_Y, ic, j0) : 0.0;
+  for (int r = j0; r < j1; ++r) {
+    const double rho = AT(CLV_DENSITY1, ic, r), e = AT(CLV_ENERGY1, ic, r);
+    const double vol = AT(CLV_VOLUME, ic, r), vfx = AT(CLV_VOL_FLUX_X, ic, r);
+    const double vfx_e = dn1(vfx);
+    double pre;
+    if (SWEEP == 1) {
+      const double vfy_r1 = AT(CLV_VOL_FLUX_Y, ic, r + 1);
+      pre = vol + (vfx_e - vfx + vfy_r1 - vfy_r);  // src/serial/advec_cell.cpp:47
+      vfy_r = vfy_r1;
+    } else {
+      pre = vol + vfx_e - vfx;  // :57
+    }
+    const double rho_m1 = up1(rho), rho_m2 = up2(rho), e_m1 = up1(e), e_m2 = up2(e);
+    const double rho_e = dn1(rho), e_e = dn1(e);
+    // upwind = min(i + 1, x_max + 2) (:77): at face nx + 2 the upwind cell is the donor
+    const double rho_p1 = i + 1 <= nx + 2 ? rho_e : rho, e_p1 = i + 1 <= nx + 2 ? e_e : e;
+    const double pre_m1 = up1(pre);
+    double mf, ef;
+    cell_face_flux(vfx, pre_m1, pre, rho_m2, rho_m1, rho, rho_p1, e_m2, e_m1, e, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
+    const double mf_e = dn1(mf), ef_e = dn1(ef);
+    if (own_face) AT(CLV_MASS_FLUX_X, i, r) = mf;
+    // src/serial/advec_cell.cpp:120-125
+    double pre_mass_s = rho * pre;
+    double post_mass_s = pre_mass_s + mf - mf_e;
+    double post_ener_s = (e * pre_mass_s + ef - ef_e) / post_mass_s;
+    double advec_vol_s = pre + vfx - vfx_e;
+    if (own_cell) {
+      rho_out[(int64_t)r * d.pitch + i] = post_mass_s / advec_vol_s;
+      e_out[(int64_t)r * d.pitch + i] = post_ener_s;
+    }
+  }
+}
+
+// y direction.  Lanes: column x0 + L, x0 = 2 + 64 * strip (no x stencil besides
+// vol_flux_x at i + 1 in sweep 1, loaded directly).  March faces r = j0 .. j1:
+// face r needs cells r-2 .. r+1; cell r - 1 is updated from faces r - 1 and r.
+template <int SWEEP>
+__global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+  const int lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  const int nx = d.nx, ny = d.ny;
+  const int i = 2 + 64 * strip + lane;
+  if (i > nx + 1) return;  // per-lane: no cross-lane traffic in this kernel
+  const int j0 = 2 + SEG * blockIdx.y;
+  const int j1 = min(j0 + SEG, ny + 2);
+  auto pre_of = [&](int row) {
+    const double vol = AT(CLV_VOLUME, i, row);
+    if (SWEEP == 1)  // src/serial/advec_cell.cpp:139
+      return vol + (AT(CLV_VOL_FLUX_Y, i, row + 1) - AT(CLV_VOL_FLUX_Y, i, row) + AT(CLV_VOL_FLUX_X, i + 1, row) - AT(CLV_VOL_FLUX_X, i, row));
+    return vol + AT(CLV_VOL_FLUX_Y, i, row + 1) - AT(CLV_VOL_FLUX_Y, i, row);  // :149
+  };
+  // window rows r-2, r-1, r (r = face index), pre at r-1
+  double rho_m2 = AT(CLV_DENSITY1, i, j0 - 2), rho_m1 = AT(CLV_DENSITY1, i, j0 - 1), rho_0 = AT(CLV_DENSITY1, i, j0);
+  double e_m2 = AT(CLV_ENERGY1, i, j0 - 2), e_m1 = AT(CLV_ENERGY1, i, j0 - 1), e_0 = AT(CLV_ENERGY1, i, j0);
+  double pre_m1 = pre_of(j0 - 1);
+  double vd_m1 = M(CLV_VERTEXDY, j0 - 1);
+  double mf_prev = 0.0, ef_prev = 0.0, vf_prev = 0.0;
+  for (int r = j0; r <= j1; ++r) {
+    const double pre_0 = pre_of(r);
+    const double vf = AT(CLV_VOL_FLUX_Y, i, r);
+    const double vd_face = M(CLV_VERTEXDY, r);
+    // upwind = min(j + 1, y_max + 2) (:169)
+    const int rp = r + 1 <= ny + 2 ? r + 1 : ny + 2;
+    const double rho_p1 = AT(CLV_DENSITY1, i, rp), e_p1 = AT(CLV_ENERGY1, i, rp);
+    const double vd_p1 = M(CLV_VERTEXDY, rp);
+    double mf, ef;
+    cell_face_flux(vf, pre_m1, pre_0, rho_m2, rho_m1, rho_0, rho_p1, e_m2, e_m1, e_0, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
+    if (r < j1 || r == ny + 2) AT(CLV_MASS_FLUX_Y, i, r) = mf;
+    if (r > j0) {
+      // update cell r - 1 from faces r - 1 and r, src/serial/advec_cell.cpp:211-216
+      const double pre = pre_m1, rho = rho_m1, e = e_m1;
+      double pre_mass_s = rho * pre;
+      double post_mass_s = pre_mass_s + mf_prev - mf;
+      double post_ener_s = (e * pre_mass_s + ef_prev - ef) / post_mass_s;
+      double advec_vol_s = pre + vf_prev - vf;
+      rho_out[(int64_t)(r - 1) * d.pitch + i] = post_mass_s / advec_vol_s;
+      e_out[(int64_t)(r - 1) * d.pitch + i] = post_ener_s;
+    }
+    mf_prev = mf;
+    ef_prev = ef;
+    vf_prev = vf;
+    rho_m2 = rho_m1;
+    rho_m1 = rho_0;
+    e_m2 = e_m1;
+    e_m1 = e_0;
+    // next face r + 1 needs cell r + 1 (unclamped: rows <= ny + 3 exist)
+    if (r + 1 <= j1) {
+      rho_0 = AT(CLV_DENSITY1, i, r + 1);
+      e_0 = AT(CLV_ENERGY1, i, r + 1);
+    }
+    pre_m1 = pre_0;
+    vd_m1 = vd_face;
+  }
+}
+
+// ---- advec_mom --------------------------------------------------------------------------
+// Limited momentum flux, src/serial/advec_mom.cpp:114-139 (x) / :186-211 (y).
+__device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor, double width, double width_dif, double v_uw,
+                                              double v_dn, double v_dw) {
+  double sigma = fabs(flux) / (mass_pre_donor);
+  double vdiffuw = v_dn - v_uw;
+  double vdiffdw = v_dw - v_dn;
+  double limiter = 0.0;
+  if (vdiffuw * vdiffdw > 0.0) {
+    double auw = fabs(vdiffuw);
+    double adw = fabs(vdiffdw);
+    double wind = 1.0;
+    if (vdiffdw <= 0.0) wind = -1.0;
+    limiter = wind * fmin(fmin(width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / width_dif) / 6.0, auw), adw);
+  }
+  double advec_vel_s = v_dn + (1.0 - sigma) * limiter;
+  return advec_vel_s * flux;
+}
+
+// x direction (mom_sweep 1 = x first, 3 = x second).  Lanes: node column x0 + L,
+// x0 = 60 * strip; node velocities updated on lanes 2..61.
+template <int MOM_SWEEP>
+__global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out, double *v_out) {
+  const int lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  const int nx = d.nx, ny = d.ny;
+  const int x0 = 60 * strip;
+  const int i = x0 + lane;
+  const int ic = i < nx + 4 ? i : nx + 4;
+  const bool own = lane >= 2 && lane <= 61 && i <= nx + 2;
+  const int j0 = 2 + SEG * blockIdx.y;
+  const int j1 = min(j0 + SEG, ny + 3);  // node rows [j0, j1)
+  const double width = M(CLV_CELLDX, ic);
+  const double w_m1 = up1(width), w_p1 = dn1(width);
+  // cell row c products density1 * post_vol (src/serial/advec_mom.cpp:98-99)
+  auto mass_of = [&](int c) {
+    double post;
+    if (MOM_SWEEP == 1) post = AT(CLV_VOLUME, ic, c) + AT(CLV_VOL_FLUX_Y, ic, c + 1) - AT(CLV_VOL_FLUX_Y, ic, c);  // :47
+    else post = AT(CLV_VOLUME, ic, c);                                                                               // :65
+    return AT(CLV_DENSITY1, ic, c) * post;
+  };
+  double mp_prev = mass_of(j0 - 1);
+  double mfx_prev = AT(CLV_MASS_FLUX_X, ic, j0 - 1);
+  for (int r = j0; r < j1; ++r) {
+    const double mp = mass_of(r);
+    const double mfx = AT(CLV_MASS_FLUX_X, ic, r);
+    // node_flux (:88), node_mass_post / pre (:98-100)
+    const double node_flux = 0.25 * (mfx_prev + mfx + dn1(mfx_prev) + dn1(mfx));
+    const double node_mass_post = 0.25 * (mp_prev + mp + up1(mp_prev) + up1(mp));
+    const double node_mass_pre = node_mass_post - up1(node_flux) + node_flux;
+    const double pre_e = dn1(node_mass_pre);
+    const double u = AT(CLV_XVEL1, ic, r), v = AT(CLV_YVEL1, ic, r);
+    const double u_w = up1(u), u_e = dn1(u), u_ee = dn2(u);
+    const double v_w = up1(v), v_e = dn1(v), v_ee = dn2(v);
+    double mu, mv;
+    if (node_flux < 0.0) {  // upwind i+2, donor i+1, downwind i, dif = donor
+      mu = mom_flux_of(node_flux, pre_e, width, w_p1, u_ee, u_e, u);
+      mv = mom_flux_of(node_flux, pre_e, width, w_p1, v_ee, v_e, v);
+    } else {  // upwind i-1, donor i, downwind i+1, dif = upwind
+      mu = mom_flux_of(node_flux, node_mass_pre, width, w_m1, u_w, u, u_e);
+      mv = mom_flux_of(node_flux, node_mass_pre, width, w_m1, v_w, v, v_e);
+    }
+    const double mu_w = up1(mu), mv_w = up1(mv);
+    // :149
+    if (own) {
+      u_out[(int64_t)r * d.pitch + i] = (u * node_mass_pre + mu_w - mu) / node_mass_post;
+      v_out[(int64_t)r * d.pitch + i] = (v * node_mass_pre + mv_w - mv) / node_mass_post;
+    }
+    mp_prev = mp;
+    mfx_prev = mfx;
+  }
+}
+
+// y direction (mom_sweep 2 = y first, 4 = y second).  Lanes: node column
+// x0 + L, x0 = 2 + 64 * strip; the cells and faces at column i - 1 are loaded
+// directly, so every lane is independent.  Iteration k computes node_flux and
+// the node masses of node row k, the momentum flux of face f = k - 1 and the
+// update of node row k - 1 (src/serial/advec_mom.cpp:158-222):
+//   nf(n)      = 0.25 (mfy(i-1,n) + mfy(i,n) + mfy(i-1,n+1) + mfy(i,n+1))
+//   nm_post(n) = 0.25 (mo(n-1) + mo(n) + mw(n-1) + mw(n)), mo/mw = density1 * post_vol
+//                at cell (i, .) / (i - 1, .)
+//   nm_pre(n)  = nm_post(n) - nf(n-1) + nf(n)
+//   mom_flux(f) from nf(f), nm_pre(donor), vel(f-1 .. f+2)
+//   vel(n)     = (vel(n) nm_pre(n) + mom_flux(n-1) - mom_flux(n)) / nm_post(n)
+template <int MOM_SWEEP>
+__global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, double *u_out, double *v_out) {
+  const int lane = threadIdx.x & 63;
+  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  const int nx = d.nx, ny = d.ny;
+  const int i = 2 + 64 * strip + lane;
+  if (i > nx + 2) return;  // per-lane kernel
+  const int j0 = 2 + SEG * blockIdx.y;
+  const int j1 = min(j0 + SEG, ny + 3);  // node rows updated [j0, j1)
+  auto post_of = [&](int col, int c) {
+    if (MOM_SWEEP == 2) return AT(CLV_VOLUME, col, c) + AT(CLV_VOL_FLUX_X, col + 1, c) - AT(CLV_VOL_FLUX_X, col, c);  // :56
+    return AT(CLV_VOLUME, col, c);                                                                                    // :74
+  };
+  auto node_flux_of = [&](int n) {  // :160
+    return 0.25 * (AT(CLV_MASS_FLUX_Y, i - 1, n) + AT(CLV_MASS_FLUX_Y, i, n) + AT(CLV_MASS_FLUX_Y, i - 1, n + 1) + AT(CLV_MASS_FLUX_Y, i, n + 1));
+  };
+  // state entering iteration k: nf(k-1), mo/mw(k-1), nm_pre(k-1), nm_post(k-1),
+  // mom_flux(k-2), velocities of rows k-2, k-1, k
+  int k = j0 - 1;
+  double nf_prev = node_flux_of(k - 1);
+  double mo_prev = AT(CLV_DENSITY1, i, k - 1) * post_of(i, k - 1);
+  double mw_prev = AT(CLV_DENSITY1, i - 1, k - 1) * post_of(i - 1, k - 1);
+  double pre_prev = 0.0, post_prev = 0.0, mu_prev = 0.0, mv_prev = 0.0;
+  double u_m2 = 0.0, v_m2 = 0.0;
+  double u_m1 = AT(CLV_XVEL1, i, k - 1), v_m1 = AT(CLV_YVEL1, i, k - 1);
+  double u_0 = AT(CLV_XVEL1, i, k), v_0 = AT(CLV_YVEL1, i, k);
+  for (; k <= j1; ++k) {
+    const double nf = node_flux_of(k);
+    const double mo = AT(CLV_DENSITY1, i, k) * post_of(i, k);
+    const double mw = AT(CLV_DENSITY1, i - 1, k) * post_of(i - 1, k);
+    const double nm_post = 0.25 * (mo_prev + mo + mw_prev + mw);  // :170-171
+    const double nm_pre = nm_post - nf_prev + nf;                 // :172
+    const double u_p1 = AT(CLV_XVEL1, i, k + 1), v_p1 = AT(CLV_YVEL1, i, k + 1);
+    if (k >= j0) {
+      // momentum flux of face f = k - 1: vel(f-1) = row k-2 .. vel(f+2) = row k+1
+      const int f = k - 1;
+      const double wd = M(CLV_CELLDY, f);
+      double mu, mv;
+      if (nf_prev < 0.0) {  // upwind f+2, donor f+1, downwind f, dif = donor (:186-191)
+        const double wdif = M(CLV_CELLDY, f + 1);
+        mu = mom_flux_of(nf_prev, nm_pre, wd, wdif, u_p1, u_0, u_m1);
+        mv = mom_flux_of(nf_prev, nm_pre, wd, wdif, v_p1, v_0, v_m1);
+      } else {  // upwind f-1, donor f, downwind f+1, dif = upwind (:192-196)
+        const double wdif = M(CLV_CELLDY, f - 1);
+        mu = mom_flux_of(nf_prev, pre_prev, wd, wdif, u_m2, u_m1, u_0);
+        mv = mom_flux_of(nf_prev, pre_prev, wd, wdif, v_m2, v_m1, v_0);
+      }
+      if (k >= j0 + 1) {
+        // update node row f = k - 1 from faces f - 1 and f (:221)
+        u_out[(int64_t)f * d.pitch + i] = (u_m1 * pre_prev + mu_prev - mu) / post_prev;
+        v_out[(int64_t)f * d.pitch + i] = (v_m1 * pre_prev + mv_prev - mv) / post_prev;
+      }
+      mu_prev = mu;
+      mv_prev = mv;
+    }
+    nf_prev = nf;
+    mo_prev = mo;
+    mw_prev = mw;
+    pre_prev = nm_pre;
+    post_prev = nm_post;
+    u_m2 = u_m1;
+    u_m1 = u_0;
+    u_0 = u_p1;
+    v_m2 = v_m1;
+    v_m1 = v_0;
+    v_0 = v_p1;
+  }
+}
+
+}  // namespace
+
+// ---- host side ----------------------------------------------------------------------------
+#define LAUNCH_F(name, kern, grid, ...)                                          \
+  do {                                                                           \
+    before_launch(c, name);                                                      \
+    hipLaunchKernelGGL(kern, grid, dim3(256, 1, 1), 0, c->stream, __VA_ARGS__);  \
+    clv_status st_ = after_launch(c, name);                                      \
+    if (st_ != CLV_OK) return st_;                                               \
+  } while (0)
+
+static int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// strips of the timestep kernel (62 owned columns each) times row segments
+size_t fused_partials_needed(int nx, int ny) { return (size_t)ceil_div(nx, 62) * (size_t)ceil_div(ny, SEG); }
+
+static clv_status halo(clv_chunk *c, std::initializer_list<int> ids, int depth) {
+  int fields[CLV_NUM_HALO_FIELDS] = {0};
+  for (int id : ids) fields[id] = 1;
+  return k_update_halo(c, fields, depth);
+}
+
+static void swap_fields(clv_chunk *c, int a, int b) {
+  double *t = c->d.f[a];
+  c->d.f[a] = c->d.f[b];
+  c->d.f[b] = t;
+}
+
+static clv_status require_owning(clv_chunk *c) {
+  // the fused plan renames buffers (double buffering, reset_field by swap), which
+  // only a chunk that owns its arena may do
+  if (!c->owns) return set_error(CLV_ERR_ARG, "CLV_PLAN_FUSED needs a chunk created by clv_chunk_create (not clv_chunk_wrap)");
+  if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "CLV_PLAN_FUSED needs x_cells, y_cells >= 4");
+  return CLV_OK;
+}
+
+clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val) {
+  clv_status st = require_owning(c);
+  if (st != CLV_OK) return st;
+  const Dev &d = c->d;
+  const int ns = ceil_div(d.nx, 62), nseg = ceil_div(d.ny, SEG);
+  if ((size_t)ns * nseg > c->partials_len) return set_error(CLV_ERR_ARG, "fused timestep: partials buffer too small");
+  LAUNCH_F("f_timestep", timestep_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe,
+           c->partials, ns);
+  before_launch(c, "calc_dt_final");
+  hipLaunchKernelGGL(reduce_min_big_k, dim3(1, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)c->partials, ns * nseg, c->result_dev);
+  if ((st = after_launch(c, "calc_dt_final")) != CLV_OK) return st;
+  st = hip_check(hipMemcpyAsync(c->result_host, c->result_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream), "fused dt copy");
+  if (st != CLV_OK) return st;
+  // q's reflective ghosts for the accelerate stencil (driver/timestep.cpp:66-69)
+  if ((st = halo(c, {CLV_VISCOSITY}, 1)) != CLV_OK) return st;
+  if ((st = hip_check(hipStreamSynchronize(c->stream), "fused dt sync")) != CLV_OK) return st;
+  *dt_min_val = c->result_host[0];
+  return CLV_OK;
+}
+
+clv_status fused_lagrange(clv_chunk *c, double dt) {
+  clv_status st = require_owning(c);
+  if (st != CLV_OK) return st;
+  const Dev &d = c->d;
+  const int ns = ceil_div(d.nx, 61), nseg = ceil_div(d.ny, SEG);
+  LAUNCH_F("f_lagrange", lagrange_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns);
+  return CLV_OK;
+}
+
+static clv_status cell_sweep(clv_chunk *c, int sweep, int dir) {
+  const Dev &d = c->d;
+  const int nseg = ceil_div(d.ny, SEG);
+  double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2];
+  if (dir == CLV_XDIR) {
+    const int ns = ceil_div(d.nx, 60);
+    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    if (sweep == 1) LAUNCH_F("f_cell_x", cell_x_k<1>, g, d, ns, rho_out, e_out);
+    else LAUNCH_F("f_cell_x", cell_x_k<2>, g, d, ns, rho_out, e_out);
+  } else {
+    const int ns = ceil_div(d.nx, 64);
+    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    if (sweep == 1) LAUNCH_F("f_cell_y", cell_y_k<1>, g, d, ns, rho_out, e_out);
+    else LAUNCH_F("f_cell_y", cell_y_k<2>, g, d, ns, rho_out, e_out);
+  }
+  swap_fields(c, CLV_DENSITY1, CLV_WORK1);
+  swap_fields(c, CLV_ENERGY1, CLV_WORK2);
+  return CLV_OK;
+}
+
+static clv_status mom_sweep(clv_chunk *c, int sweep, int dir) {
+  const Dev &d = c->d;
+  const int nseg = ceil_div(d.ny + 1, SEG);
+  double *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
+  const int mom = dir + 2 * (sweep - 1);  // src/serial/advec_mom.cpp:37
+  if (dir == CLV_XDIR) {
+    const int ns = ceil_div(d.nx + 1, 60);
+    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    if (mom == 1) LAUNCH_F("f_mom_x", mom_x_k<1>, g, d, ns, u_out, v_out);
+    else LAUNCH_F("f_mom_x", mom_x_k<3>, g, d, ns, u_out, v_out);
+  } else {
+    const int ns = ceil_div(d.nx + 1, 64);
+    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    if (mom == 2) LAUNCH_F("f_mom_y", mom_y_k<2>, g, d, ns, u_out, v_out);
+    else LAUNCH_F("f_mom_y", mom_y_k<4>, g, d, ns, u_out, v_out);
+  }
+  swap_fields(c, CLV_XVEL1, CLV_WORK3);
+  swap_fields(c, CLV_YVEL1, CLV_WORK4);
+  return CLV_OK;
+}
+
+// driver/advection.cpp:29-105 followed by reset_field (src/serial/reset_field.cpp)
+clv_status fused_advection(clv_chunk *c, int advect_x) {
+  clv_status st = require_owning(c);
+  if (st != CLV_OK) return st;
+  const int dir1 = advect_x ? CLV_XDIR : CLV_YDIR, dir2 = advect_x ? CLV_YDIR : CLV_XDIR;
+  if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}, 2)) != CLV_OK) return st;
+  if ((st = cell_sweep(c, 1, dir1)) != CLV_OK) return st;
+  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
+  if ((st = mom_sweep(c, 1, dir1)) != CLV_OK) return st;
+  if ((st = cell_sweep(c, 2, dir2)) != CLV_OK) return st;
+  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
+  if ((st = mom_sweep(c, 2, dir2)) != CLV_OK) return st;
+  // reset_field: the end-of-step fields become the start-of-step fields
+  swap_fields(c, CLV_DENSITY0, CLV_DENSITY1);
+  swap_fields(c, CLV_ENERGY0, CLV_ENERGY1);
+  swap_fields(c, CLV_XVEL0, CLV_XVEL1);
+  swap_fields(c, CLV_YVEL0, CLV_YVEL1);
+  return CLV_OK;
+}
 
 }  // namespace clv
