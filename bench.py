@@ -67,7 +67,21 @@ ALG_TERMS = {
     "advec_mom_update_x": [(5, "N")],          # vel, pre, mom_flux, post -> vel
     "advec_mom_update_y": [(5, "N")],
     "field_summary": [(6, "C")],
+    # fused plan: compulsory traffic of each fused kernel (every input array read
+    # once, every output written once over the interior)
+    "f_timestep": [(9, "C")],                  # rho0, e0, u0, v0, xarea, yarea, vol -> p, q
+    "f_lagrange": [(15, "C")],                 # rho0, e0, p, q, u0, v0, vol, areas -> u1, v1, rho1, e1, vfx, vfy
+    "f_cell_x1": [(8, "C")],                   # vol, vfx, vfy, rho1, e1 -> rho1, e1, mfx
+    "f_cell_y1": [(8, "C")],
+    "f_cell_x2": [(7, "C")],                   # vol, vfx, rho1, e1 -> rho1, e1, mfx
+    "f_cell_y2": [(7, "C")],
+    "f_mom_x1": [(8, "N")],                    # vol, vfy, mfx, rho1, u1, v1 -> u1, v1
+    "f_mom_y1": [(8, "N")],
+    "f_mom_x2": [(7, "N")],                    # vol, mfx, rho1, u1, v1 -> u1, v1
+    "f_mom_y2": [(7, "N")],
 }
+# compulsory traffic of one fused step: 9 + 15 + 8 + 7 + 8 + 7 array sweeps
+B_FUSED_PER_CELL_STEP = 54 * 8.0
 B_ALG_PER_CELL_STEP = 1440.0  # 180 array sweeps of 8 B per interior cell-step at reference kernel granularity
 
 
@@ -109,10 +123,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=87)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--plan", choices=["fused", "reference"], default="reference")
+    ap.add_argument("--plan", choices=["fused", "reference"], default="fused")
     ap.add_argument("--deck", default="clover_bm256_short")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--no-probe", action="store_true", help="skip the HBM copy-bandwidth probe")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="only run the copy kernel (4 GiB, 20 launches): the counter calibration run")
+    ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--profile-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -120,6 +137,11 @@ def main():
         sys.exit(2)
 
     import cloverleaf_amd as clv
+    if args.probe_only:
+        gbps = clv.bandwidth_probe(4 << 30, 20)
+        print(json.dumps({"probe": "copy", "bytes_per_launch_read": 4 << 30, "bytes_per_launch_write": 4 << 30,
+                          "launches": 21, "copy_GBps": gbps}))
+        return
     deck_path = REPO / "tests" / "golden" / "decks" / f"{args.deck}.in"
     deck_text = deck_path.read_text()
     cfg = clv.parse_deck(deck_text)
@@ -186,6 +208,7 @@ def main():
                    "x_cells": nx, "y_cells": ny, "plan": args.plan, "parallelism": "1 GPU, 1 chunk"},
         "gcell_steps_per_s": value / 1e9,
         "effective_hbm_frac_at_1440B": value * B_ALG_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
+        "compulsory_hbm_frac_fused_432B": value * B_FUSED_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9) if args.plan == "fused" else None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_us": avg_s * 1e6, "share_of_kernel_time": dom_ms / total_kernel_ms},
@@ -193,6 +216,10 @@ def main():
         "final_summary": {"step": st.step, "ke": s_end[3], "mass": s_end[1]},
     }
     ch.close()
+    if not args.no_probe:
+        out["hbm_copy_GBps_measured"] = clv.bandwidth_probe(4 << 30, 20)
+        if achieved:
+            out["roofline"]["frac_of_measured_copy"] = achieved / out["hbm_copy_GBps_measured"]
     if not args.no_cpu:
         print("# bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(deck_text, args.cpu_steps)

@@ -122,7 +122,19 @@ SIGNATURES = [
     ("clv_profile_enable", _I, [_P, _I]),
     ("clv_profile_read", _I, [_P, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), _PD, _I]),
     ("clv_profile_reset", _I, [_P]),
+    ("clv_bandwidth_probe", _I, [C.c_size_t, _I, _PD]),
+    ("clv_device_alloc", _I, [C.c_size_t, C.POINTER(_P)]),
+    ("clv_device_free", _I, [_P]),
+    ("clv_copy_to_host", _I, [_P, _P, C.c_size_t]),
+    ("clv_copy_to_device", _I, [_P, _P, C.c_size_t]),
 ]
+
+
+def bandwidth_probe(nbytes: int = 4 << 30, iters: int = 20) -> float:
+    """Copy bandwidth (GB/s, read + write) of the current GPU's HBM."""
+    out = C.c_double()
+    _check(lib().clv_bandwidth_probe(int(nbytes), int(iters), C.byref(out)), "clv_bandwidth_probe")
+    return out.value
 
 
 def lib() -> C.CDLL:
@@ -201,15 +213,20 @@ class Chunk:
         assert a.shape == (self.ny + 5, self.nx + 5), a.shape
         _check(self.L.clv_field_upload(self.h, FID[name], a.ctypes.data_as(_PD), self.nx + 5), "upload")
 
+    def mesh_length(self, name: str) -> int:
+        """Cell arrays n + 4, vertex arrays n + 5 (driver/definitions.cpp:28-35)."""
+        n = self.nx if name in ("cellx", "vertexx", "celldx", "vertexdx") else self.ny
+        return n + (5 if name.startswith("vertex") else 4)
+
     def download_mesh(self, name: str) -> np.ndarray:
-        n = self.nx + 5 if name in ("cellx", "vertexx", "celldx", "vertexdx") else self.ny + 5
+        n = self.mesh_length(name)
         a = np.empty(n, dtype=np.float64)
         _check(self.L.clv_mesh_download(self.h, MID[name], a.ctypes.data_as(_PD)), "download_mesh")
         return a
 
     def upload_mesh(self, name: str, a: np.ndarray):
-        n = self.nx + 5 if name in ("cellx", "vertexx", "celldx", "vertexdx") else self.ny + 5
-        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = self.mesh_length(name)
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64)[:n])
         assert a.shape == (n,), a.shape
         _check(self.L.clv_mesh_upload(self.h, MID[name], a.ctypes.data_as(_PD)), "upload_mesh")
 

@@ -119,9 +119,7 @@ static clv_status alloc_scratch(clv_chunk *c) {
 
 static bool valid_field(int f) { return f >= 0 && f < CLV_NUM_FIELDS; }
 static bool valid_mesh(int m) { return m >= 0 && m < CLV_NUM_MESH; }
-static size_t mesh_len(const clv_chunk *c, int m) {
-  return (m == CLV_CELLX || m == CLV_VERTEXX || m == CLV_CELLDX || m == CLV_VERTEXDX) ? (size_t)c->d.nx + 5 : (size_t)c->d.ny + 5;
-}
+static size_t mesh_len(const clv_chunk *c, int m) { return clv_mesh_length(c->d.nx, c->d.ny, m); }
 
 }  // namespace clv
 
@@ -231,6 +229,29 @@ clv_status clv_chunk_synchronize(clv_chunk *c) {
 }
 
 size_t clv_chunk_bytes(const clv_chunk *c) { return c && c->owns ? c->arena_bytes : 0; }
+
+clv_status clv_device_alloc(size_t bytes, void **out) {
+  if (!out) return set_error(CLV_ERR_ARG, "clv_device_alloc: null out");
+  *out = nullptr;
+  if (bytes == 0) return CLV_OK;
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess) return set_error(CLV_ERR_ALLOC, std::string("clv_device_alloc: ") + hipGetErrorString(e));
+  return CLV_OK;
+}
+
+clv_status clv_device_free(void *p) { return p ? hip_check(hipFree(p), "clv_device_free") : CLV_OK; }
+
+clv_status clv_copy_to_host(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return CLV_OK;
+  if (!dst || !src) return set_error(CLV_ERR_ARG, "clv_copy_to_host: null pointer");
+  return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "clv_copy_to_host");
+}
+
+clv_status clv_copy_to_device(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return CLV_OK;
+  if (!dst || !src) return set_error(CLV_ERR_ARG, "clv_copy_to_device: null pointer");
+  return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "clv_copy_to_device");
+}
 
 clv_status clv_field_upload(clv_chunk *c, int field, const double *host, int64_t host_pitch) {
   CHECK_CHUNK(c);

@@ -64,7 +64,7 @@ void report_step(std::ostream &out, const clv_hydro_state &st, const double s[5]
 
 int main(int argc, char **argv) {
   std::string in = "clover.in", outname = "clover.out";
-  int plan = CLV_PLAN_REFERENCE;
+  int plan = CLV_PLAN_FUSED;
   bool profile = false, quiet = false;
   for (int a = 1; a < argc; ++a) {
     std::string arg = argv[a];

@@ -740,13 +740,13 @@ static clv_status cell_sweep(clv_chunk *c, int sweep, int dir) {
   if (dir == CLV_XDIR) {
     const int ns = ceil_div(d.nx, 60);
     const dim3 g(ceil_div(ns, WAVES), nseg, 1);
-    if (sweep == 1) LAUNCH_F("f_cell_x", cell_x_k<1>, g, d, ns, rho_out, e_out);
-    else LAUNCH_F("f_cell_x", cell_x_k<2>, g, d, ns, rho_out, e_out);
+    if (sweep == 1) LAUNCH_F("f_cell_x1", cell_x_k<1>, g, d, ns, rho_out, e_out);
+    else LAUNCH_F("f_cell_x2", cell_x_k<2>, g, d, ns, rho_out, e_out);
   } else {
     const int ns = ceil_div(d.nx, 64);
     const dim3 g(ceil_div(ns, WAVES), nseg, 1);
-    if (sweep == 1) LAUNCH_F("f_cell_y", cell_y_k<1>, g, d, ns, rho_out, e_out);
-    else LAUNCH_F("f_cell_y", cell_y_k<2>, g, d, ns, rho_out, e_out);
+    if (sweep == 1) LAUNCH_F("f_cell_y1", cell_y_k<1>, g, d, ns, rho_out, e_out);
+    else LAUNCH_F("f_cell_y2", cell_y_k<2>, g, d, ns, rho_out, e_out);
   }
   swap_fields(c, CLV_DENSITY1, CLV_WORK1);
   swap_fields(c, CLV_ENERGY1, CLV_WORK2);
@@ -761,13 +761,13 @@ static clv_status mom_sweep(clv_chunk *c, int sweep, int dir) {
   if (dir == CLV_XDIR) {
     const int ns = ceil_div(d.nx + 1, 60);
     const dim3 g(ceil_div(ns, WAVES), nseg, 1);
-    if (mom == 1) LAUNCH_F("f_mom_x", mom_x_k<1>, g, d, ns, u_out, v_out);
-    else LAUNCH_F("f_mom_x", mom_x_k<3>, g, d, ns, u_out, v_out);
+    if (mom == 1) LAUNCH_F("f_mom_x1", mom_x_k<1>, g, d, ns, u_out, v_out);
+    else LAUNCH_F("f_mom_x2", mom_x_k<3>, g, d, ns, u_out, v_out);
   } else {
     const int ns = ceil_div(d.nx + 1, 64);
     const dim3 g(ceil_div(ns, WAVES), nseg, 1);
-    if (mom == 2) LAUNCH_F("f_mom_y", mom_y_k<2>, g, d, ns, u_out, v_out);
-    else LAUNCH_F("f_mom_y", mom_y_k<4>, g, d, ns, u_out, v_out);
+    if (mom == 2) LAUNCH_F("f_mom_y1", mom_y_k<2>, g, d, ns, u_out, v_out);
+    else LAUNCH_F("f_mom_y2", mom_y_k<4>, g, d, ns, u_out, v_out);
   }
   swap_fields(c, CLV_XVEL1, CLV_WORK3);
   swap_fields(c, CLV_YVEL1, CLV_WORK4);

@@ -318,6 +318,17 @@ __global__ __launch_bounds__(RED_THREADS) void field_summary_serial_k(Dev d, dou
   }
 }
 
+// ---- HBM probe ------------------------------------------------------------------------
+// A plain copy of n doubles (16-byte vector accesses, grid-stride): the
+// bandwidth reference for the roofline and the known-size kernel that scales
+// the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void stream_copy_k(const double2 *__restrict__ a, double2 *__restrict__ b, size_t n2) {
+  // one 16-byte element per thread and many blocks: the access pattern of the
+  // streaming kernels (one element per thread) at full occupancy
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n2) b[t] = a[t];
+}
+
 // ---- set-up -----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void fill_zero_k(double *p, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.0;
@@ -476,6 +487,41 @@ clv_status k_field_summary(clv_chunk *c, double out[5]) {
   return CLV_OK;
 }
 
+}  // namespace clv
+
+extern "C" clv_status clv_bandwidth_probe(size_t bytes, int iters, double *copy_gbps) {
+  using namespace clv;
+  if (!copy_gbps || bytes < 1024 || iters < 1) return set_error(CLV_ERR_ARG, "clv_bandwidth_probe: bad argument");
+  const size_t n2 = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  clv_status st = hip_check(hipMalloc(&a, n2 * 16), "probe alloc");
+  if (st == CLV_OK) st = hip_check(hipMalloc(&b, n2 * 16), "probe alloc");
+  if (st == CLV_OK) st = hip_check(hipMemset(a, 0, n2 * 16), "probe memset");
+  if (st == CLV_OK) st = hip_check(hipEventCreate(&e0), "probe event");
+  if (st == CLV_OK) st = hip_check(hipEventCreate(&e1), "probe event");
+  const dim3 grid((unsigned)((n2 + 255) / 256)), block(256);
+  if (st == CLV_OK) {
+    hipLaunchKernelGGL(stream_copy_k, grid, block, 0, nullptr, (const double2 *)a, b, n2);
+    (void)hipEventRecord(e0, nullptr);
+    for (int k = 0; k < iters; ++k) hipLaunchKernelGGL(stream_copy_k, grid, block, 0, nullptr, (const double2 *)a, b, n2);
+    (void)hipEventRecord(e1, nullptr);
+    st = hip_check(hipEventSynchronize(e1), "probe sync");
+  }
+  if (st == CLV_OK) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *copy_gbps = 2.0 * (double)n2 * 16.0 * iters / (ms * 1e-3) / 1e9;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return st;
+}
+
+namespace clv {
+
 clv_status k_fill_zero(clv_chunk *c) {
   const Dev &d = c->d;
   const size_t n2 = (size_t)(d.ny + 5) * (size_t)d.pitch;
@@ -484,7 +530,7 @@ clv_status k_fill_zero(clv_chunk *c) {
     LAUNCH_G("build_field", fill_zero_k, dim3(2048, 1, 1), dim3(256, 1, 1), d.f[f], n2);
   }
   for (int m = 0; m < CLV_NUM_MESH; ++m) {
-    const size_t n = (m == CLV_CELLX || m == CLV_VERTEXX || m == CLV_CELLDX || m == CLV_VERTEXDX) ? d.nx + 5 : d.ny + 5;
+    const size_t n = clv_mesh_length(d.nx, d.ny, m);
     LAUNCH_G("build_field", fill_zero_k, dim3(1, 1, 1), dim3(256, 1, 1), d.m[m], n);
   }
   return CLV_OK;

@@ -1,0 +1,9 @@
+// void advec_cell_driver(global_variables&, int tile, int sweep_number, int direction)   driver/advec_cell.h:24
+#include "advec_cell.h"
+
+#include "mi355x_model.h"
+
+void advec_cell_driver(global_variables &globals, int tile, int sweep_number, int direction) {
+  mi355x::check(clv_advec_cell(mi355x::chunk(globals, tile), sweep_number, direction), "advec_cell");
+  mi355x::sync_if_profiling(globals, tile);
+}

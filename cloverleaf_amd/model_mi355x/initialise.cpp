@@ -1,0 +1,84 @@
+// create_context / report_context (driver/initialise.h:160-162) and the tile ->
+// chunk binding of the MI355X model.
+#include "initialise.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <string>
+
+#include "mi355x_model.h"
+
+model create_context(bool silent, const std::vector<std::string> &args) {
+  const int n = clv_device_count();
+  std::vector<std::string> devices;
+  for (int d = 0; d < (n > 0 ? n : 1); ++d) devices.push_back("HIP device " + std::to_string(d) + " (gfx950, clover_mi355x)");
+  auto [device, parsed] = list_and_parse<std::string>(
+      silent, devices, [](const auto &d) { return d; }, args);
+  if (n < 1) {
+    std::fprintf(stderr, "mi355x model: no HIP device visible\n");
+    std::exit(EXIT_FAILURE);
+  }
+  clover::context ctx{};
+  ctx.device = 0;
+  for (int d = 0; d < n; ++d)
+    if (devices[d] == device) ctx.device = d;
+  return {ctx, "MI355X (clover_mi355x, gfx950 HIP kernels)", true, parsed};
+}
+
+void report_context(const clover::context &ctx) {
+  std::printf(" - Device:    %d (%s)\n", ctx.device, clv_version());
+}
+
+namespace mi355x {
+
+static std::map<const void *, clv_chunk *> g_chunks;
+
+[[noreturn]] void unsupported(const char *what) {
+  std::fprintf(stderr, "mi355x model: %s needs more than one chunk or tile, which this model does not implement\n", what);
+  std::abort();
+}
+
+clv_chunk *chunk(global_variables &globals, int tile) {
+  tile_type &t = globals.chunk.tiles[tile];
+  field_type &f = t.field;
+  auto it = g_chunks.find(f.density0.data);
+  if (it != g_chunks.end()) return it->second;
+  if (globals.config.tiles_per_chunk != 1 || t.info.t_xmin != 1 || t.info.t_ymin != 1 || t.info.t_left != 1 || t.info.t_bottom != 1)
+    unsupported("a tiled or decomposed chunk");
+  clv_chunk_view v{};
+  v.x_cells = t.info.t_xmax - t.info.t_xmin + 1;
+  v.y_cells = t.info.t_ymax - t.info.t_ymin + 1;
+  v.pitch = (int64_t)f.density0.pitch;
+  clover::Buffer2D<double> *b2[CLV_NUM_FIELDS] = {
+      &f.density0,   &f.density1,    &f.energy0,     &f.energy1,     &f.pressure,    &f.viscosity,   &f.soundspeed,
+      &f.xvel0,      &f.xvel1,       &f.yvel0,       &f.yvel1,       &f.vol_flux_x,  &f.vol_flux_y,  &f.mass_flux_x,
+      &f.mass_flux_y, &f.work_array1, &f.work_array2, &f.work_array3, &f.work_array4, &f.work_array5, &f.work_array6,
+      &f.work_array7, &f.volume,      &f.xarea,       &f.yarea};
+  for (int k = 0; k < CLV_NUM_FIELDS; ++k) {
+    if ((int64_t)b2[k]->pitch != v.pitch) {
+      std::fprintf(stderr, "mi355x model: field %d has pitch %zu, density0 %zu\n", k, b2[k]->pitch, f.density0.pitch);
+      std::abort();
+    }
+    v.field[k] = b2[k]->data;
+  }
+  clover::Buffer1D<double> *b1[CLV_NUM_MESH] = {&f.cellx, &f.celly, &f.vertexx, &f.vertexy, &f.celldx, &f.celldy, &f.vertexdx, &f.vertexdy};
+  for (int k = 0; k < CLV_NUM_MESH; ++k) v.mesh[k] = b1[k]->data;
+  clv_chunk *c = nullptr;
+  check(clv_chunk_wrap(&v, &c), "clv_chunk_wrap");
+  if (const char *order = std::getenv("CLOVER_MI355X_SUMMARY"); order && std::string(order) == "serial")
+    check(clv_chunk_set_summary_order(c, CLV_SUMMARY_SERIAL), "clv_chunk_set_summary_order");
+  g_chunks[f.density0.data] = c;
+  return c;
+}
+
+void sync_if_profiling(global_variables &globals, int tile) {
+  if (globals.profiler_on) check(clv_chunk_synchronize(chunk(globals, tile)), "clv_chunk_synchronize");
+}
+
+void release_chunks() {
+  for (auto &kv : g_chunks) clv_chunk_destroy(kv.second);
+  g_chunks.clear();
+}
+
+}  // namespace mi355x

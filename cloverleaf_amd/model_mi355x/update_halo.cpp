@@ -1,0 +1,34 @@
+// void update_halo(global_variables&, int fields[NUM_FIELDS], int depth)   driver/update_halo.h:24
+// The order of src/serial/update_halo.cpp:690-718: tile halo (a no-op for one
+// tile, driver/update_tile_halo.cpp:33-36), chunk exchange (a no-op for one
+// chunk), then the external reflective halo on the GPU in one launch.
+#include "update_halo.h"
+
+#include "comms_kernel.h"
+#include "mi355x_model.h"
+#include "timer.h"
+#include "update_tile_halo.h"
+
+void update_halo(global_variables &globals, int fields[NUM_FIELDS], const int depth) {
+  double kernel_time = 0;
+  if (globals.profiler_on) kernel_time = timer();
+  update_tile_halo(globals, fields, depth);
+  if (globals.profiler_on) {
+    globals.profiler.tile_halo_exchange += timer() - kernel_time;
+    kernel_time = timer();
+  }
+  clover_exchange(globals, fields, depth);
+  if (globals.profiler_on) {
+    globals.profiler.mpi_halo_exchange += timer() - kernel_time;
+    kernel_time = timer();
+  }
+  const auto &nb = globals.chunk.chunk_neighbours;
+  if (nb[chunk_left] == external_face || nb[chunk_right] == external_face || nb[chunk_bottom] == external_face ||
+      nb[chunk_top] == external_face) {
+    for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {
+      mi355x::check(clv_update_halo(mi355x::chunk(globals, tile), fields, depth), "update_halo");
+      mi355x::sync_if_profiling(globals, tile);
+    }
+  }
+  if (globals.profiler_on) globals.profiler.self_halo_exchange += timer() - kernel_time;
+}

@@ -1,0 +1,11 @@
+// void viscosity(global_variables&)   driver/viscosity.h:24
+#include "viscosity.h"
+
+#include "mi355x_model.h"
+
+void viscosity(global_variables &globals) {
+  for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {
+    mi355x::check(clv_viscosity(mi355x::chunk(globals, tile)), "viscosity");
+    mi355x::sync_if_profiling(globals, tile);
+  }
+}

@@ -51,13 +51,17 @@ enum {
   CLV_NUM_FIELDS = 25,
   CLV_NUM_HALO_FIELDS = 15
 };
-/* 1D mesh arrays (driver/definitions.cpp:28-35); x arrays hold x_cells + 5
- * doubles, y arrays y_cells + 5. */
+/* 1D mesh arrays (driver/definitions.cpp:28-35): cell arrays hold n + 4
+ * doubles, vertex arrays n + 5 (n = x_cells or y_cells). */
 enum {
   CLV_CELLX = 0, CLV_CELLY = 1, CLV_VERTEXX = 2, CLV_VERTEXY = 3,
   CLV_CELLDX = 4, CLV_CELLDY = 5, CLV_VERTEXDX = 6, CLV_VERTEXDY = 7,
   CLV_NUM_MESH = 8
 };
+static inline size_t clv_mesh_length(int x_cells, int y_cells, int mesh) {
+  const int n = (mesh == CLV_CELLX || mesh == CLV_VERTEXX || mesh == CLV_CELLDX || mesh == CLV_VERTEXDX) ? x_cells : y_cells;
+  return (size_t)n + ((mesh == CLV_VERTEXX || mesh == CLV_VERTEXY || mesh == CLV_VERTEXDX || mesh == CLV_VERTEXDY) ? 5 : 4);
+}
 
 /* Sweep directions (driver/definitions.h:84). */
 enum { CLV_XDIR = 1, CLV_YDIR = 2 };
@@ -131,6 +135,14 @@ clv_status clv_chunk_set_stream(clv_chunk *c, void *hip_stream);
 clv_status clv_chunk_synchronize(clv_chunk *c);
 /* Device bytes held by the chunk's arena (0 for wrapped chunks). */
 size_t clv_chunk_bytes(const clv_chunk *c);
+
+/* Device memory for a plugin model's buffer types: the allocation/free of the
+ * reference's Buffer1D/Buffer2D (src/hip/context.h:67-79 alloc/dealloc) and the
+ * host copies behind mirrored() (src/hip/context.h:115-123). */
+clv_status clv_device_alloc(size_t bytes, void **out);
+clv_status clv_device_free(void *p);
+clv_status clv_copy_to_host(void *dst, const void *device_src, size_t bytes);
+clv_status clv_copy_to_device(void *device_dst, const void *src, size_t bytes);
 
 /* Blocking host copies of one array.  2D arrays use host pitch host_pitch
  * doubles (>= x_cells + 5) and y_cells + 5 rows; 1D arrays their full length. */
@@ -219,6 +231,11 @@ clv_status clv_hydro_summary(clv_chunk *c, double out[5]);
 clv_status clv_profile_enable(clv_chunk *c, int on);
 int clv_profile_read(clv_chunk *c, const char **names, int64_t *launches, double *total_ms, int max);
 clv_status clv_profile_reset(clv_chunk *c);
+/* HBM bandwidth probe on the current device (no counterpart in the reference):
+ * a plain copy of `bytes` (read + write, 16-byte accesses), timed over `iters`
+ * launches with HIP events; *copy_gbps = 2 * bytes * iters / time.  Allocates and
+ * frees two buffers of `bytes`. */
+clv_status clv_bandwidth_probe(size_t bytes, int iters, double *copy_gbps);
 
 #ifdef __cplusplus
 }

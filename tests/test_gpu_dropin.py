@@ -1,0 +1,71 @@
+"""The drop-in: the reference's own, unchanged driver (driver/*.cpp) linked with the
+MI355X plugin model (cloverleaf_amd/model_mi355x) prints what the reference
+prints.
+
+The binary is built by cloverleaf_amd/model_mi355x/Makefile from the reference
+sources where the reference tree is present (this file skips where it was not
+built).  tests/golden/clover_bm_short.reference_lines.txt holds the reference's
+own printed dt lines and field-summary table (the reference omp binary built
+here; the serial binary prints identical lines).
+"""
+import os
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+import clover_testlib as T
+
+REPO = Path(__file__).resolve().parent.parent
+DROPIN = REPO / "cloverleaf_amd" / "model_mi355x" / "_dropin" / "mi355x-cloverleaf"
+
+pytestmark = pytest.mark.skipif(not DROPIN.exists(), reason="drop-in binary not built (needs the reference tree)")
+
+
+def _printed_lines(stdout: str, clover_out: str):
+    steps = [l for l in stdout.splitlines() if re.match(r"^ Step \d+ time", l)]
+    table = [l for l in clover_out.splitlines() if l.startswith(" step:")]
+    return steps + table
+
+
+def _run(tmp_path, deck, env_extra=None, args=()):
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    return subprocess.run([str(DROPIN), "--file", str(T.DECKS / f"{deck}.in"), "--out", str(tmp_path / "clover.out"), *args],
+                          cwd=tmp_path, env=env, capture_output=True, text=True, timeout=600)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", ["tree", "serial"])
+def test_dropin_prints_the_reference_output(gpu, tmp_path, order):
+    """clover_bm_short (960^2, 87 steps, test problem 2): every dt line and every
+    field-summary line equal to the reference's, and the QA check passes."""
+    p = _run(tmp_path, "clover_bm_short", {"CLOVER_MI355X_SUMMARY": order})
+    assert p.returncode == 0, p.stderr[-2000:]
+    got = _printed_lines(p.stdout, (tmp_path / "clover.out").read_text())
+    want = (T.GOLDEN / "clover_bm_short.reference_lines.txt").read_text().splitlines()
+    assert got == want
+    assert "This test is considered PASSED" in p.stdout
+    assert "MI355X" in p.stdout
+
+
+@pytest.mark.gpu
+def test_dropin_profiler_and_known_answer(gpu, tmp_path):
+    """clover_bm16_very_short (3840^2, 8 steps, test problem 168) with --profile: the
+    reference's per-phase profile table is produced and the QA check passes."""
+    p = _run(tmp_path, "clover_bm16_very_short", args=("--profile",))
+    assert p.returncode == 0, p.stderr[-2000:]
+    assert "This test is considered PASSED" in p.stdout
+    out = (tmp_path / "clover.out").read_text()
+    assert "Profiler Output" in out and "Cell Advection" in out  # driver/hydro.cpp:177-194
+
+
+def test_dropin_without_gpu_fails_cleanly(tmp_path):
+    """Without a HIP device the model says so and exits non-zero (no CPU fallback)."""
+    import cloverleaf_amd as clv
+    if clv.lib().clv_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    p = _run(tmp_path, "clover_default")
+    assert p.returncode != 0
+    assert "no HIP device" in p.stderr

@@ -206,14 +206,16 @@ def test_known_answer_decks(gpu, deck):
     assert abs(100.0 * (recs[-1]["ke"] / KNOWN[d.test_problem]) - 100.0) < 0.001
 
 
-def test_driver_binary_prints_reference_dt_lines(gpu, tmp_path):
+@pytest.mark.parametrize("plan", ["reference", "fused"])
+def test_driver_binary_prints_reference_lines(gpu, tmp_path, plan):
+    """The host driver prints the reference's dt lines and summary table verbatim
+    (tests/golden/clover_bm_short.reference_lines.txt, the reference binary's output)."""
     exe = clv.PKG / "bin" / "clover_mi355x"
     out = subprocess.run([str(exe), "--in", str(T.DECKS / "clover_bm_short.in"), "--out", str(tmp_path / "clover.out"),
-                          "--plan", "reference"], capture_output=True, text=True, timeout=300)
+                          "--plan", plan], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
-    lines = [l for l in out.stdout.splitlines() if l.startswith(" Step ")]
-    assert len(lines) == 87
-    # README.md sample (clover_bm_short): step 1 and step 87 lines
-    assert lines[0].split()[:8] == ["Step", "1", "time", "0", "control", "sound", "timestep", "0.00616258"]
-    assert lines[-1].split()[7] == "0.005848"
+    steps = [l for l in out.stdout.splitlines() if l.startswith(" Step ")]
+    table = [l for l in (tmp_path / "clover.out").read_text().splitlines() if l.startswith(" step:")]
+    want = (T.GOLDEN / "clover_bm_short.reference_lines.txt").read_text().splitlines()
+    assert steps + table == want
     assert "This test is considered PASSED" in out.stdout
