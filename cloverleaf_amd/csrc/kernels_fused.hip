@@ -45,10 +45,25 @@ namespace {
 constexpr int SEG = 64;        // rows per wave segment
 constexpr int WAVES = 4;       // waves (strips) per 256-thread block
 
-__device__ __forceinline__ double up1(double v) { return __shfl_up(v, 1, 64); }      // value of lane - 1
-__device__ __forceinline__ double up2(double v) { return __shfl_up(v, 2, 64); }      // value of lane - 2
-__device__ __forceinline__ double dn1(double v) { return __shfl_down(v, 1, 64); }    // value of lane + 1
-__device__ __forceinline__ double dn2(double v) { return __shfl_down(v, 2, 64); }    // value of lane + 2
+// Lane shifts across the whole wave64 with DPP (GFX9 wave_shr:1 / wave_shl:1):
+// one VALU move per 32-bit half instead of an LDS-crossbar bpermute.  The lane
+// shifted in at the wave's edge keeps its own value; edge lanes never produce
+// owned results, so that value is never used.
+__device__ __forceinline__ double dpp_shift(double v, int ctrl_is_shr) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  if (ctrl_is_shr) {
+    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);  // wave_shr:1
+    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);  // wave_shl:1
+    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  }
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double up1(double v) { return dpp_shift(v, 1); }          // value of lane - 1
+__device__ __forceinline__ double up2(double v) { return dpp_shift(dpp_shift(v, 1), 1); }  // value of lane - 2
+__device__ __forceinline__ double dn1(double v) { return dpp_shift(v, 0); }          // value of lane + 1
+__device__ __forceinline__ double dn2(double v) { return dpp_shift(dpp_shift(v, 0), 0); }  // value of lane + 2
 
 __device__ __forceinline__ double wave_min(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -60,6 +75,22 @@ __device__ __forceinline__ double wave_min(double v) {
 // ghost through its mirror gives what the reference's halo update would have
 // put there, whatever the ghost layer holds.
 __device__ __forceinline__ int mirror_cell(int i, int n) { return i < 2 ? 3 - i : (i > n + 1 ? 2 * n + 3 - i : i); }
+
+// XCD-aware tile order.  Workgroups are dispatched in linear order (x fastest)
+// and dealt round-robin to the 8 XCDs, each with its own L2.  Renumbering so that
+// XCD k receives the k-th contiguous eighth of the tiles keeps neighbouring
+// strips and row segments -- which re-read each other's edge columns and rows --
+// in one L2.  The map is a bijection on [0, n).
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ void xcd_tile(int &bx, int &by) {
+  const int n = gridDim.x * gridDim.y;
+  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  const int xcd = b % NUM_XCD, k = b / NUM_XCD;
+  const int per = n / NUM_XCD, rem = n % NUM_XCD;
+  const int t = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+  bx = t % gridDim.x;
+  by = t / gridDim.x;
+}
 
 // ---- timestep: ideal_gas + viscosity + calc_dt ----------------------------------------
 // Lanes: column x0 + L, x0 = 1 + 62 * strip; lanes 1..62 own cells (p of the
@@ -76,16 +107,19 @@ __device__ __forceinline__ void ideal_gas_cell(double rho, double e, double &p, 
 
 __device__ __forceinline__ double pressure_of(double rho, double e) { return (1.4 - 1.0) * rho * e; }
 
-__global__ __launch_bounds__(256) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+// 5 waves per SIMD fit without spilling (96 VGPRs)
+__global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
                                                   double *partials, int nstrips) {
   const int lane = threadIdx.x & 63;
-  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  int bx, by;
+  xcd_tile(bx, by);
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;  // whole wave
   const int x0 = 1 + 62 * strip;
   const int i = x0 + lane;
   const int ic = i < d.nx + 4 ? i : d.nx + 4;  // clamped load column (lanes past the chunk)
   const bool own = lane >= 1 && lane <= 62 && i <= d.nx + 1;
-  const int j0 = 2 + SEG * blockIdx.y;
+  const int j0 = 2 + SEG * by;
   const int j1 = min(j0 + SEG, d.ny + 2);
 
   const double cdx = M(CLV_CELLDX, ic), cdx1 = M(CLV_CELLDX, ic + 1 <= d.nx + 4 ? ic + 1 : ic);
@@ -182,7 +216,7 @@ __global__ __launch_bounds__(256) void timestep_k(Dev d, double dtc_safe, double
     ya_r = ya_r1;
   }
   dtmin = wave_min(dtmin);
-  if (lane == 0) partials[(size_t)blockIdx.y * nstrips + strip] = dtmin;
+  if (lane == 0) partials[(size_t)by * nstrips + strip] = dtmin;
 }
 
 __global__ __launch_bounds__(1024) void reduce_min_big_k(const double *partials, int n, double *out) {
@@ -227,7 +261,9 @@ __device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt)
 
 __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips) {
   const int lane = threadIdx.x & 63;
-  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  int bx, by;
+  xcd_tile(bx, by);
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
   const int x0 = 1 + 61 * strip;
@@ -235,7 +271,7 @@ __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips)
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own_cell = lane >= 1 && lane <= 61 && i <= nx + 1;
   const bool own_node = (lane >= 1 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
-  const int j0 = 2 + SEG * blockIdx.y;
+  const int j0 = 2 + SEG * by;
   const int jc1 = min(j0 + SEG, ny + 2);           // cells [j0, jc1)
   const int jn1 = jc1 == ny + 2 ? ny + 3 : jc1;    // owned node rows [j0, jn1)
   const double halfdt = 0.5 * dt;                  // src/serial/accelerate.cpp:33
@@ -402,7 +438,9 @@ __device__ __forceinline__ void cell_face_flux(double vf, double pre_m1, double 
 template <int SWEEP>
 __global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_out, double *e_out) {
   const int lane = threadIdx.x & 63;
-  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  int bx, by;
+  xcd_tile(bx, by);
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
   const int x0 = 60 * strip;
@@ -410,7 +448,7 @@ __global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own_cell = lane >= 2 && lane <= 61 && i <= nx + 1;
   const bool own_face = (lane >= 2 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
-  const int j0 = 2 + SEG * blockIdx.y;
+  const int j0 = 2 + SEG * by;
   const int j1 = min(j0 + SEG, ny + 2);
   const double vd_face = M(CLV_VERTEXDX, ic);
   // shuffles run on every lane; the clamp selects afterwards
@@ -456,12 +494,14 @@ __global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_
 template <int SWEEP>
 __global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, double *rho_out, double *e_out) {
   const int lane = threadIdx.x & 63;
-  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  int bx, by;
+  xcd_tile(bx, by);
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
   const int i = 2 + 64 * strip + lane;
   if (i > nx + 1) return;  // per-lane: no cross-lane traffic in this kernel
-  const int j0 = 2 + SEG * blockIdx.y;
+  const int j0 = 2 + SEG * by;
   const int j1 = min(j0 + SEG, ny + 2);
   auto pre_of = [&](int row) {
     const double vol = AT(CLV_VOLUME, i, row);
@@ -537,14 +577,16 @@ __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor
 template <int MOM_SWEEP>
 __global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out, double *v_out) {
   const int lane = threadIdx.x & 63;
-  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  int bx, by;
+  xcd_tile(bx, by);
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
   const int x0 = 60 * strip;
   const int i = x0 + lane;
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own = lane >= 2 && lane <= 61 && i <= nx + 2;
-  const int j0 = 2 + SEG * blockIdx.y;
+  const int j0 = 2 + SEG * by;
   const int j1 = min(j0 + SEG, ny + 3);  // node rows [j0, j1)
   const double width = M(CLV_CELLDX, ic);
   const double w_m1 = up1(width), w_p1 = dn1(width);
@@ -601,12 +643,14 @@ __global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out
 template <int MOM_SWEEP>
 __global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, double *u_out, double *v_out) {
   const int lane = threadIdx.x & 63;
-  const int strip = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  int bx, by;
+  xcd_tile(bx, by);
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
   const int i = 2 + 64 * strip + lane;
   if (i > nx + 2) return;  // per-lane kernel
-  const int j0 = 2 + SEG * blockIdx.y;
+  const int j0 = 2 + SEG * by;
   const int j1 = min(j0 + SEG, ny + 3);  // node rows updated [j0, j1)
   auto post_of = [&](int col, int c) {
     if (MOM_SWEEP == 2) return AT(CLV_VOLUME, col, c) + AT(CLV_VOL_FLUX_X, col + 1, c) - AT(CLV_VOL_FLUX_X, col, c);  // :56
