@@ -13,8 +13,12 @@ bytes per launch / its mean launch duration, against 8 TB/s.  `cpu_baseline`
 times the OpenMP CPU oracle (oracle/liboracle.so) on a bounded sample of the
 same problem.
 
+`batched` (unless --no-batch) runs the seeded 64-problem workload (decks from
+clv_batch_deck, seed 355, 100 steps each) one problem at a time on the plain
+path and as one batch, and reports both times.
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--plan fused|reference]
-                       [--deck NAME] [--no-cpu] [--profile-out FILE]
+                       [--deck NAME] [--no-cpu] [--no-batch] [--batch-only] [--profile-out FILE]
 """
 import argparse
 import json
@@ -118,6 +122,89 @@ def cpu_baseline(deck_text, steps):
                       f"steps 2..{steps + 1}, {dt:.1f} s"}
 
 
+BATCH_SEED = 355
+
+
+def batch_leg(clv, n=64, seed=BATCH_SEED, reps=3):
+    """The seeded batch workload: n independent problems (decks from
+    clv_batch_deck), each to end_step 100 with its own dt.  Times (a) the problems
+    one at a time on the plain fused path, each with its host loop, and (b) all of
+    them as one batch (persistent launch per phase, hipGraph per step, autotuned
+    schedule); checks that both give every problem the same dt sequence."""
+    decks = [clv.batch_deck(seed, i) for i in range(n)]
+    cfgs = [clv.parse_deck(t) for t in decks]
+    cell_steps = float(sum(c.x_cells * c.y_cells * c.end_step for c in cfgs))
+    # (a) one at a time
+    seq_s, seq_dts = 0.0, []
+    for i, c in enumerate(cfgs):
+        with clv.Chunk(c.x_cells, c.y_cells) as ch:
+            st, _ = ch.start(c)
+            ch.synchronize()
+            dts = []
+            t0 = time.perf_counter()
+            while not st.complete:
+                ch.step(c, st, clv.PLAN_FUSED)
+                dts.append(st.dt)
+            ch.synchronize()
+            seq_s += time.perf_counter() - t0
+            seq_dts.append(dts)
+    print(f"# bench: batch workload one at a time: {seq_s:.3f} s", file=sys.stderr, flush=True)
+
+    def started():
+        chunks, states = [], []
+        for c in cfgs:
+            ch = clv.Chunk(c.x_cells, c.y_cells)
+            st, _ = ch.start(c)
+            chunks.append(ch)
+            states.append(st)
+        return chunks, states
+
+    # (b) batched, graph replay; the schedule tuned first (untimed)
+    chunks, states = started()
+    b = clv.Batch(chunks, cfgs, states, hist_len=128)
+    t0 = time.perf_counter()
+    sched, fam_ms = b.autotune(reps=reps)
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    steps = b.run(1000, use_graph=True)
+    bat_s = time.perf_counter() - t0
+    same = all(b.dt_history(i).tolist() == seq_dts[i] for i in range(n))
+    states_end = b.states()
+    b.close()
+    for ch in chunks:
+        ch.close()
+    print(f"# bench: batched (graph): {bat_s:.3f} s, dt identical: {same}", file=sys.stderr, flush=True)
+    # (c) batched, direct launches (same schedule): the graph's share
+    chunks, states = started()
+    b = clv.Batch(chunks, cfgs, states, hist_len=0)
+    b.set_schedule(sched)
+    t0 = time.perf_counter()
+    b.run(1000, use_graph=False)
+    direct_s = time.perf_counter() - t0
+    b.close()
+    for ch in chunks:
+        ch.close()
+    sides = [c.x_cells for c in cfgs]
+    return {
+        "workload": f"{n} problems from seed {seed}: square grids, side uniform in {{256,384,...,2048}} "
+                    f"(drawn: min {min(sides)}, max {max(sides)}), two-state ideal gas with a random state-2 rectangle, "
+                    f"100 steps each with its own dt, FP64",
+        "cell_steps": cell_steps,
+        "one_at_a_time_s": seq_s,
+        "batched_s": bat_s,
+        "batched_direct_launch_s": direct_s,
+        "speedup_vs_one_at_a_time": seq_s / bat_s,
+        "batched_cell_steps_per_s": cell_steps / bat_s,
+        "one_at_a_time_cell_steps_per_s": cell_steps / seq_s,
+        "batch_steps": steps,
+        "all_complete": all(s.complete for s in states_end),
+        "dt_sequences_identical": same,
+        "autotune_s": tune_s,
+        "schedule": sched.as_dict(),
+        "family_ms_per_launch": dict(zip(clv.BATCH_FAMILIES, [round(x, 4) for x in fam_ms])),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +214,8 @@ def main():
     ap.add_argument("--deck", default="clover_bm256_short")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-probe", action="store_true", help="skip the HBM copy-bandwidth probe")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched workload")
+    ap.add_argument("--batch-only", action="store_true", help="only the 64-problem batched workload")
     ap.add_argument("--probe-only", action="store_true",
                     help="only run the copy kernel (4 GiB, 20 launches): the counter calibration run")
     ap.add_argument("--cpu-steps", type=int, default=120)
@@ -141,6 +230,9 @@ def main():
         gbps = clv.bandwidth_probe(4 << 30, 20)
         print(json.dumps({"probe": "copy", "bytes_per_launch_read": 4 << 30, "bytes_per_launch_write": 4 << 30,
                           "launches": 21, "copy_GBps": gbps}))
+        return
+    if args.batch_only:
+        print(json.dumps({"batched": batch_leg(clv)}))
         return
     deck_path = REPO / "tests" / "golden" / "decks" / f"{args.deck}.in"
     deck_text = deck_path.read_text()
@@ -220,6 +312,9 @@ def main():
         out["hbm_copy_GBps_measured"] = clv.bandwidth_probe(4 << 30, 20)
         if achieved:
             out["roofline"]["frac_of_measured_copy"] = achieved / out["hbm_copy_GBps_measured"]
+    if not args.no_batch:
+        print("# bench: batched workload ...", file=sys.stderr, flush=True)
+        out["batched"] = batch_leg(clv)
     if not args.no_cpu:
         print("# bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(deck_text, args.cpu_steps)

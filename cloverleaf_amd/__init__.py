@@ -69,6 +69,23 @@ class HydroState(C.Structure):
                 ("complete", C.c_int)]
 
 
+class BatchFamily(C.Structure):
+    """clv_batch_family: rows per tile, persistent blocks per CU (0: one per tile), member order."""
+    _fields_ = [("seg", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int)]
+
+
+BATCH_FAMILIES = ["timestep", "lagrange", "cell_x", "cell_y", "mom_x", "mom_y"]
+ORDER_GIVEN, ORDER_LARGEST_FIRST = 0, 1
+
+
+class BatchSchedule(C.Structure):
+    _fields_ = [("fam", BatchFamily * len(BATCH_FAMILIES))]
+
+    def as_dict(self) -> dict:
+        return {n: dict(seg=f.seg, blocks_per_cu=f.blocks_per_cu, order=f.order)
+                for n, f in zip(BATCH_FAMILIES, self.fam)}
+
+
 class ChunkView(C.Structure):
     _fields_ = [("x_cells", C.c_int), ("y_cells", C.c_int), ("pitch", C.c_int64),
                 ("field", C.c_void_p * 25), ("mesh", C.c_void_p * 8)]
@@ -127,6 +144,15 @@ SIGNATURES = [
     ("clv_device_free", _I, [_P]),
     ("clv_copy_to_host", _I, [_P, _P, C.c_size_t]),
     ("clv_copy_to_device", _I, [_P, _P, C.c_size_t]),
+    ("clv_batch_deck", _I, [C.c_uint64, _I, C.c_char_p, C.c_size_t]),
+    ("clv_batch_create", _I, [C.POINTER(_P), C.POINTER(Config), C.POINTER(HydroState), _I, _I, C.POINTER(_P)]),
+    ("clv_batch_destroy", _I, [_P]),
+    ("clv_batch_get_schedule", _I, [_P, C.POINTER(BatchSchedule)]),
+    ("clv_batch_set_schedule", _I, [_P, C.POINTER(BatchSchedule)]),
+    ("clv_batch_autotune", _I, [_P, _I, C.POINTER(BatchSchedule), _PD]),
+    ("clv_batch_run", _I, [_P, _I, _I, _PI]),
+    ("clv_batch_states", _I, [_P, C.POINTER(HydroState)]),
+    ("clv_batch_dt_history", _I, [_P, _I, _PD, _I, _PI]),
 ]
 
 
@@ -344,3 +370,79 @@ def run(cfg: Config, plan: int = PLAN_FUSED, trace: bool = False, summary_order:
                                  press=s[4]))
             if st.complete:
                 return recs
+
+
+def batch_deck(seed: int, index: int) -> str:
+    """Deck text of problem `index` of the seeded batch workload (clv_batch_deck)."""
+    buf = C.create_string_buffer(4096)
+    _check(lib().clv_batch_deck(C.c_uint64(seed), int(index), buf, len(buf)), "clv_batch_deck")
+    return buf.value.decode()
+
+
+class Batch:
+    """Many independent problems advanced together (clv_batch_*).
+
+    Each member is a started Chunk; the batch owns nothing but its schedule,
+    views and per-member driver state.  Members must not be stepped outside the
+    batch while it exists (reading them is fine between runs).
+    """
+
+    def __init__(self, chunks, cfgs, states, hist_len: int = 0):
+        self.L = lib()
+        n = len(chunks)
+        self.chunks = list(chunks)
+        self.n = n
+        hs = (_P * n)(*[c.h.value for c in chunks])
+        cf = (Config * n)(*cfgs)
+        st = (HydroState * n)(*states)
+        h = C.c_void_p()
+        _check(self.L.clv_batch_create(hs, cf, st, n, int(hist_len), C.byref(h)), "clv_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.clv_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def schedule(self) -> BatchSchedule:
+        s = BatchSchedule()
+        _check(self.L.clv_batch_get_schedule(self.h, C.byref(s)), "clv_batch_get_schedule")
+        return s
+
+    def set_schedule(self, s: BatchSchedule):
+        _check(self.L.clv_batch_set_schedule(self.h, C.byref(s)), "clv_batch_set_schedule")
+
+    def autotune(self, reps: int = 3):
+        s = BatchSchedule()
+        ms = (C.c_double * len(BATCH_FAMILIES))()
+        _check(self.L.clv_batch_autotune(self.h, int(reps), C.byref(s), ms), "clv_batch_autotune")
+        return s, list(ms)
+
+    def run(self, max_steps: int, use_graph: bool = True) -> int:
+        done = C.c_int()
+        _check(self.L.clv_batch_run(self.h, int(max_steps), int(bool(use_graph)), C.byref(done)), "clv_batch_run")
+        return done.value
+
+    def states(self):
+        out = (HydroState * self.n)()
+        _check(self.L.clv_batch_states(self.h, out), "clv_batch_states")
+        return list(out)
+
+    def dt_history(self, member: int, max_steps: int = 1 << 20) -> np.ndarray:
+        a = np.empty(max_steps, dtype=np.float64)
+        k = C.c_int()
+        _check(self.L.clv_batch_dt_history(self.h, int(member), a.ctypes.data_as(_PD), int(max_steps), C.byref(k)),
+               "clv_batch_dt_history")
+        return a[:k.value].copy()

@@ -82,4 +82,50 @@ clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *d
 clv_status fused_lagrange(clv_chunk *c, double dt);
 clv_status fused_advection(clv_chunk *c, int advect_x);
 
+// Location/label bookkeeping of calc_dt: constants (src/serial/calc_dt.cpp:40,80-83).
+inline void dt_location(int nx, int *control, int *jldt, int *kldt) {
+  double jk_control = 1.1;
+  *control = (int)(10.01 * (jk_control - (int)(jk_control)));
+  jk_control = jk_control - (jk_control - (int)(jk_control));
+  *jldt = ((int)jk_control) % nx;
+  *kldt = (int)(1.f + (jk_control / nx));
+}
+
+// ---- batched form (kernels_fused.hip, kernels_misc.hip, batch.cpp) ------------------------
+// Tile kernels of a batched step, one per fused kernel and sweep variant.
+enum BatchKind {
+  BK_TIMESTEP, BK_LAGRANGE,
+  BK_CELL_X1, BK_CELL_X2, BK_CELL_Y1, BK_CELL_Y2,
+  BK_MOM_X1, BK_MOM_X3, BK_MOM_Y2, BK_MOM_Y4,
+  BK_COUNT
+};
+// One tile of a persistent launch: strip block bx of problem p, rows
+// [2 + k * seg, 2 + (k + 1) * seg) clipped to the problem.
+struct BatchItem {
+  uint16_t p, bx, k, seg;
+};
+// Per problem, constant while a schedule holds.
+struct BatchConst {
+  double dtc_safe, dtu_safe, dtv_safe, dtdiv_safe;
+  double dtrise, dtmax, dtmin, end_time;
+  double *partials;  // timestep minima: strips x row segments of the timestep schedule
+  double *dt_hist;   // dt of step s at [s - 1], s <= hist_len
+  int nparts, end_step, hist_len, pad;
+};
+// Per problem, advanced on the device by batch_dt_k.
+struct BatchState {
+  double dt, dtold, time;
+  int step;         // steps begun (driver/hydro.cpp:57)
+  int complete;     // driver/hydro.cpp:108 held after the last step
+  int run;          // this step's kernels run for the problem
+  int ran;          // steps completed through the batch
+  int failed_step;  // step whose dt fell below dtmin (0: none)
+  int pad;
+};
+hipError_t batch_launch_tiles(int kind, hipStream_t s, int grid, const Dev *tab, const BatchConst *pc, const BatchState *ps,
+                              const BatchItem *items, int nitems);
+hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, BatchState *ps);
+// halo of the listed fields for every running problem (grid: problems x fields)
+hipError_t batch_launch_halo(hipStream_t s, int nprob, const Dev *tab, const BatchState *ps, const int *ids, int nids, int depth);
+
 }  // namespace clv

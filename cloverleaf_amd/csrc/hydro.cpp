@@ -219,14 +219,7 @@ clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st
     TRY(k_calc_dt(c, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe, &dtlp));
   }
   int dt_control = 0, jldt = 0, kldt = 0;
-  {
-    // constant bookkeeping of src/serial/calc_dt.cpp:40,80-83
-    double jk_control = 1.1;
-    dt_control = (int)(10.01 * (jk_control - (int)(jk_control)));
-    jk_control = jk_control - (jk_control - (int)(jk_control));
-    jldt = ((int)jk_control) % c->d.nx;
-    kldt = (int)(1.f + (jk_control / c->d.nx));
-  }
+  dt_location(c->d.nx, &dt_control, &jldt, &kldt);
   if (dtlp <= st->dt) {
     st->dt = dtlp;
     st->dt_control = dt_control;

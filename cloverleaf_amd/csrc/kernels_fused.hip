@@ -107,20 +107,22 @@ __device__ __forceinline__ void ideal_gas_cell(double rho, double e, double &p, 
 
 __device__ __forceinline__ double pressure_of(double rho, double e) { return (1.4 - 1.0) * rho * e; }
 
-// 5 waves per SIMD fit without spilling (96 VGPRs)
-__global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
-                                                  double *partials, int nstrips) {
+// Every kernel below is a tile function -- strip block bx (WAVES strips of one
+// 256-thread block) over rows [j0, j1) of one chunk -- called either once per
+// block of a plain launch (<name>_k) or in the loop of a persistent batched launch
+// over many chunks (batch_<name>_k).  Both run the same instructions per tile.
+
+// The wave's minimum goes to part[strip].
+template <class V>
+__device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1, double dtc_safe, double dtu_safe, double dtv_safe,
+                                              double dtdiv_safe, double *part, int nstrips) {
   const int lane = threadIdx.x & 63;
-  int bx, by;
-  xcd_tile(bx, by);
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;  // whole wave
   const int x0 = 1 + 62 * strip;
   const int i = x0 + lane;
   const int ic = i < d.nx + 4 ? i : d.nx + 4;  // clamped load column (lanes past the chunk)
   const bool own = lane >= 1 && lane <= 62 && i <= d.nx + 1;
-  const int j0 = 2 + SEG * by;
-  const int j1 = min(j0 + SEG, d.ny + 2);
 
   const double cdx = M(CLV_CELLDX, ic), cdx1 = M(CLV_CELLDX, ic + 1 <= d.nx + 4 ? ic + 1 : ic);
   // rho0/e0 of ghost cells through their mirrors (the halo of driver/timestep.cpp:53-60)
@@ -216,7 +218,16 @@ __global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, dou
     ya_r = ya_r1;
   }
   dtmin = wave_min(dtmin);
-  if (lane == 0) partials[(size_t)by * nstrips + strip] = dtmin;
+  if (lane == 0) part[strip] = dtmin;
+}
+
+// 5 waves per SIMD fit without spilling (96 VGPRs)
+__global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                                                  double *partials, int nstrips) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + SEG * by;
+  timestep_tile(d, bx, j0, min(j0 + SEG, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
 }
 
 __global__ __launch_bounds__(1024) void reduce_min_big_k(const double *partials, int n, double *out) {
@@ -259,10 +270,10 @@ __device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt)
   return (1.4 - 1.0) * rho1 * e1;
 }
 
-__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips) {
+// cells [j0, jc1)
+template <class V>
+__device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc1, double dt, int nstrips) {
   const int lane = threadIdx.x & 63;
-  int bx, by;
-  xcd_tile(bx, by);
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
@@ -271,8 +282,6 @@ __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips)
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own_cell = lane >= 1 && lane <= 61 && i <= nx + 1;
   const bool own_node = (lane >= 1 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
-  const int j0 = 2 + SEG * by;
-  const int jc1 = min(j0 + SEG, ny + 2);           // cells [j0, jc1)
   const int jn1 = jc1 == ny + 2 ? ny + 3 : jc1;    // owned node rows [j0, jn1)
   const double halfdt = 0.5 * dt;                  // src/serial/accelerate.cpp:33
 
@@ -387,6 +396,13 @@ __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips)
   }
 }
 
+__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + SEG * by;
+  lagrange_tile(d, bx, j0, min(j0 + SEG, d.ny + 2), dt, nstrips);
+}
+
 // ---- advec_cell -------------------------------------------------------------------------
 __device__ __forceinline__ double vl_limiter(double sigma, double sigma3, double sigma4, double uw, double dn, double dw) {
   const double one_by_six = 1.0 / 6.0;
@@ -435,21 +451,17 @@ __device__ __forceinline__ void cell_face_flux(double vf, double pre_m1, double 
 // x direction.  Lanes: column x0 + L, x0 = 60 * strip; faces valid on lanes 2..62,
 // cells updated on lanes 2..61.  Rows independent (no y stencil besides the own
 // cell's top face of vol_flux_y in sweep 1).
-template <int SWEEP>
-__global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+template <int SWEEP, class V, class O>
+__device__ __forceinline__ void cell_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out) {
   const int lane = threadIdx.x & 63;
-  int bx, by;
-  xcd_tile(bx, by);
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
-  const int nx = d.nx, ny = d.ny;
+  const int nx = d.nx;
   const int x0 = 60 * strip;
   const int i = x0 + lane;
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own_cell = lane >= 2 && lane <= 61 && i <= nx + 1;
   const bool own_face = (lane >= 2 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
-  const int j0 = 2 + SEG * by;
-  const int j1 = min(j0 + SEG, ny + 2);
   const double vd_face = M(CLV_VERTEXDX, ic);
   // shuffles run on every lane; the clamp selects afterwards
   const double vd_m1 = up1(vd_face), vd_e = dn1(vd_face);
@@ -488,21 +500,25 @@ __global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_
   }
 }
 
+template <int SWEEP>
+__global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + SEG * by;
+  cell_x_tile<SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 2), nstrips, rho_out, e_out);
+}
+
 // y direction.  Lanes: column x0 + L, x0 = 2 + 64 * strip (no x stencil besides
 // vol_flux_x at i + 1 in sweep 1, loaded directly).  March faces r = j0 .. j1:
 // face r needs cells r-2 .. r+1; cell r - 1 is updated from faces r - 1 and r.
-template <int SWEEP>
-__global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+template <int SWEEP, class V, class O>
+__device__ __forceinline__ void cell_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out) {
   const int lane = threadIdx.x & 63;
-  int bx, by;
-  xcd_tile(bx, by);
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
   const int i = 2 + 64 * strip + lane;
   if (i > nx + 1) return;  // per-lane: no cross-lane traffic in this kernel
-  const int j0 = 2 + SEG * by;
-  const int j1 = min(j0 + SEG, ny + 2);
   auto pre_of = [&](int row) {
     const double vol = AT(CLV_VOLUME, i, row);
     if (SWEEP == 1)  // src/serial/advec_cell.cpp:139
@@ -553,6 +569,14 @@ __global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, double *rho_
   }
 }
 
+template <int SWEEP>
+__global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + SEG * by;
+  cell_y_tile<SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 2), nstrips, rho_out, e_out);
+}
+
 // ---- advec_mom --------------------------------------------------------------------------
 // Limited momentum flux, src/serial/advec_mom.cpp:114-139 (x) / :186-211 (y).
 __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor, double width, double width_dif, double v_uw,
@@ -574,20 +598,17 @@ __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor
 
 // x direction (mom_sweep 1 = x first, 3 = x second).  Lanes: node column x0 + L,
 // x0 = 60 * strip; node velocities updated on lanes 2..61.
-template <int MOM_SWEEP>
-__global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out, double *v_out) {
+// node rows [j0, j1)
+template <int MOM_SWEEP, class V, class O>
+__device__ __forceinline__ void mom_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O u_out, O v_out) {
   const int lane = threadIdx.x & 63;
-  int bx, by;
-  xcd_tile(bx, by);
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
-  const int nx = d.nx, ny = d.ny;
+  const int nx = d.nx;
   const int x0 = 60 * strip;
   const int i = x0 + lane;
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own = lane >= 2 && lane <= 61 && i <= nx + 2;
-  const int j0 = 2 + SEG * by;
-  const int j1 = min(j0 + SEG, ny + 3);  // node rows [j0, j1)
   const double width = M(CLV_CELLDX, ic);
   const double w_m1 = up1(width), w_p1 = dn1(width);
   // cell row c products density1 * post_vol (src/serial/advec_mom.cpp:98-99)
@@ -629,6 +650,14 @@ __global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out
   }
 }
 
+template <int MOM_SWEEP>
+__global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out, double *v_out) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + SEG * by;
+  mom_x_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 3), nstrips, u_out, v_out);
+}
+
 // y direction (mom_sweep 2 = y first, 4 = y second).  Lanes: node column
 // x0 + L, x0 = 2 + 64 * strip; the cells and faces at column i - 1 are loaded
 // directly, so every lane is independent.  Iteration k computes node_flux and
@@ -640,18 +669,15 @@ __global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out
 //   nm_pre(n)  = nm_post(n) - nf(n-1) + nf(n)
 //   mom_flux(f) from nf(f), nm_pre(donor), vel(f-1 .. f+2)
 //   vel(n)     = (vel(n) nm_pre(n) + mom_flux(n-1) - mom_flux(n)) / nm_post(n)
-template <int MOM_SWEEP>
-__global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, double *u_out, double *v_out) {
+// node rows updated [j0, j1)
+template <int MOM_SWEEP, class V, class O>
+__device__ __forceinline__ void mom_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O u_out, O v_out) {
   const int lane = threadIdx.x & 63;
-  int bx, by;
-  xcd_tile(bx, by);
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
-  const int nx = d.nx, ny = d.ny;
+  const int nx = d.nx;
   const int i = 2 + 64 * strip + lane;
   if (i > nx + 2) return;  // per-lane kernel
-  const int j0 = 2 + SEG * by;
-  const int j1 = min(j0 + SEG, ny + 3);  // node rows updated [j0, j1)
   auto post_of = [&](int col, int c) {
     if (MOM_SWEEP == 2) return AT(CLV_VOLUME, col, c) + AT(CLV_VOL_FLUX_X, col + 1, c) - AT(CLV_VOL_FLUX_X, col, c);  // :56
     return AT(CLV_VOLUME, col, c);                                                                                    // :74
@@ -712,7 +738,189 @@ __global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, double *u_out
   }
 }
 
+template <int MOM_SWEEP>
+__global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, double *u_out, double *v_out) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + SEG * by;
+  mom_y_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 3), nstrips, u_out, v_out);
+}
+
+// ---- batched form: persistent launches over the tiles of many chunks ----------------------
+// A launch walks a list of BatchItems (problem, strip block, row segment).  Its
+// blocks are dealt round-robin to the 8 XCDs (block b on XCD b % 8), so the
+// blocks of XCD x take the x-th contiguous eighth of the list, keeping
+// neighbouring tiles of one problem -- which re-read each other's edge rows and
+// columns -- in one L2.  With fewer blocks than XCDs: a plain block stride.
+struct ItemRange {
+  int first, end, stride;
+};
+__device__ __forceinline__ ItemRange batch_items(int n) {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (G < NUM_XCD) return {b, n, G};
+  const int x = b % NUM_XCD, k = b / NUM_XCD;
+  const int nbx = (G - x + NUM_XCD - 1) / NUM_XCD;  // blocks on XCD x
+  const int lo = (int)((int64_t)n * x / NUM_XCD), hi = (int)((int64_t)n * (x + 1) / NUM_XCD);
+  return {lo + k, hi, nbx};
+}
+
+// strips of each tile family (owned columns per wave: 62, 61, 60, 64)
+__device__ __forceinline__ int strips_of(int kind, int nx) {
+  switch (kind) {
+    case BK_TIMESTEP: return (nx + 61) / 62;
+    case BK_LAGRANGE: return (nx + 60) / 61;
+    case BK_CELL_X1: case BK_CELL_X2: return (nx + 59) / 60;
+    case BK_CELL_Y1: case BK_CELL_Y2: return (nx + 63) / 64;
+    case BK_MOM_X1: case BK_MOM_X3: return (nx + 1 + 59) / 60;
+    default: return (nx + 1 + 63) / 64;  // mom y
+  }
+}
+
+// A view loaded from memory holds generic pointers, which would make every
+// access a flat one with 64-bit vector address arithmetic.  They all point to
+// global memory: the batched kernels use this copy typed so (global loads and
+// stores with scalar bases, as a view passed by value gets).
+typedef __attribute__((address_space(1))) double gdouble;
+struct GDev {
+  gdouble *f[CLV_NUM_FIELDS];
+  gdouble *m[CLV_NUM_MESH];
+  int nx, ny;
+  int64_t pitch;
+};
+__device__ __forceinline__ GDev global_view(const Dev &s) {
+  GDev d;
+  for (int k = 0; k < CLV_NUM_FIELDS; ++k) d.f[k] = (gdouble *)s.f[k];
+  for (int k = 0; k < CLV_NUM_MESH; ++k) d.m[k] = (gdouble *)s.m[k];
+  d.nx = s.nx;
+  d.ny = s.ny;
+  d.pitch = s.pitch;
+  return d;
+}
+
+// One 8-byte load (a scalar load: the list is read-only for the launch).
+__device__ __forceinline__ BatchItem load_item(const BatchItem *__restrict__ items, int w) {
+  const uint64_t v = reinterpret_cast<const uint64_t *>(items)[w];
+  return BatchItem{(uint16_t)v, (uint16_t)(v >> 16), (uint16_t)(v >> 32), (uint16_t)(v >> 48)};
+}
+
+__global__ __launch_bounds__(256, 5) void batch_timestep_k(const Dev *__restrict__ tab, const BatchConst *__restrict__ pc,
+                                                        const BatchState *__restrict__ ps, const BatchItem *__restrict__ items,
+                                                        int nitems) {
+  const ItemRange rg = batch_items(nitems);
+  for (int w = rg.first; w < rg.end; w += rg.stride) {
+    const BatchItem it = load_item(items, w);
+    if (ps[it.p].complete) continue;
+    const GDev d = global_view(tab[it.p]);
+    const BatchConst c = pc[it.p];
+    const int ns = strips_of(BK_TIMESTEP, d.nx);
+    const int j0 = 2 + (int)it.k * it.seg;
+    timestep_tile(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), c.dtc_safe, c.dtu_safe, c.dtv_safe, c.dtdiv_safe,
+                  c.partials + (size_t)it.k * ns, ns);
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_lagrange_k(const Dev *__restrict__ tab, const BatchState *__restrict__ ps,
+                                                      const BatchItem *__restrict__ items, int nitems) {
+  const ItemRange rg = batch_items(nitems);
+  for (int w = rg.first; w < rg.end; w += rg.stride) {
+    const BatchItem it = load_item(items, w);
+    if (!ps[it.p].run) continue;
+    const GDev d = global_view(tab[it.p]);
+    const double dt = ps[it.p].dt;
+    const int j0 = 2 + (int)it.k * it.seg;
+    lagrange_tile(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), dt, strips_of(BK_LAGRANGE, d.nx));
+  }
+}
+
+// advection sweeps write the updated pair into work arrays 1/2 (cells) or 3/4
+// (nodes) of the stage's view; the next stage's view has them renamed.
+template <int KIND>
+__global__ __launch_bounds__(256) void batch_advec_k(const Dev *__restrict__ tab, const BatchState *__restrict__ ps,
+                                                   const BatchItem *__restrict__ items, int nitems) {
+  const ItemRange rg = batch_items(nitems);
+  for (int w = rg.first; w < rg.end; w += rg.stride) {
+    const BatchItem it = load_item(items, w);
+    if (!ps[it.p].run) continue;
+    const GDev d = global_view(tab[it.p]);
+    const int ns = strips_of(KIND, d.nx);
+    const int j0 = 2 + (int)it.k * it.seg;
+    const int jc1 = min(j0 + (int)it.seg, d.ny + 2), jn1 = min(j0 + (int)it.seg, d.ny + 3);
+    if constexpr (KIND == BK_CELL_X1) cell_x_tile<1>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
+    if constexpr (KIND == BK_CELL_X2) cell_x_tile<2>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
+    if constexpr (KIND == BK_CELL_Y1) cell_y_tile<1>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
+    if constexpr (KIND == BK_CELL_Y2) cell_y_tile<2>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
+    if constexpr (KIND == BK_MOM_X1) mom_x_tile<1>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_MOM_X3) mom_x_tile<3>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_MOM_Y2) mom_y_tile<2>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_MOM_Y4) mom_y_tile<4>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
+  }
+}
+
+// Per problem: the step's dt from its timestep minima, with the host logic of
+// clv_timestep / clv_hydro_step (driver/timestep.cpp:78-90, driver/hydro.cpp:108).
+__global__ __launch_bounds__(256) void batch_dt_k(const BatchConst *pc, BatchState *ps) {
+  const int p = blockIdx.x;
+  BatchState s = ps[p];
+  if (s.complete) {
+    if (threadIdx.x == 0) ps[p].run = 0;
+    return;
+  }
+  const BatchConst c = pc[p];
+  double m = g_big;
+  for (int b = threadIdx.x; b < c.nparts; b += 256) m = fmin(m, c.partials[b]);
+  __shared__ double lds[4];
+  m = wave_min(m);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double dtlp = fmin(fmin(lds[0], lds[1]), fmin(lds[2], lds[3]));
+  double dt = g_big;
+  if (dtlp <= dt) dt = dtlp;
+  const double a = dt, b = s.dtold * c.dtrise;  // std::min(std::min(dt, dtold * dtrise), dtmax)
+  const double mm = (b < a) ? b : a;
+  dt = (c.dtmax < mm) ? c.dtmax : mm;
+  s.step += 1;
+  s.dt = dt;
+  s.dtold = dt;
+  if (dt < c.dtmin) {  // the host path stops with CLV_ERR_SMALL_DT before PdV
+    s.failed_step = s.step;
+    s.complete = 1;
+    s.run = 0;
+  } else {
+    s.run = 1;
+    s.ran += 1;
+    if (s.step <= c.hist_len) c.dt_hist[s.step - 1] = dt;
+    s.time += dt;
+    s.complete = (s.time + g_small > c.end_time || s.step >= c.end_step) ? 1 : 0;
+  }
+  ps[p] = s;
+}
+
 }  // namespace
+
+hipError_t batch_launch_tiles(int kind, hipStream_t s, int grid, const Dev *tab, const BatchConst *pc, const BatchState *ps,
+                              const BatchItem *items, int nitems) {
+  const dim3 g(grid, 1, 1), blk(256, 1, 1);
+  switch (kind) {
+    case BK_TIMESTEP: hipLaunchKernelGGL(batch_timestep_k, g, blk, 0, s, tab, pc, ps, items, nitems); break;
+    case BK_LAGRANGE: hipLaunchKernelGGL(batch_lagrange_k, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CELL_X1: hipLaunchKernelGGL(batch_advec_k<BK_CELL_X1>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CELL_X2: hipLaunchKernelGGL(batch_advec_k<BK_CELL_X2>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CELL_Y1: hipLaunchKernelGGL(batch_advec_k<BK_CELL_Y1>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CELL_Y2: hipLaunchKernelGGL(batch_advec_k<BK_CELL_Y2>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_MOM_X1: hipLaunchKernelGGL(batch_advec_k<BK_MOM_X1>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_MOM_X3: hipLaunchKernelGGL(batch_advec_k<BK_MOM_X3>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_MOM_Y2: hipLaunchKernelGGL(batch_advec_k<BK_MOM_Y2>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_MOM_Y4: hipLaunchKernelGGL(batch_advec_k<BK_MOM_Y4>, g, blk, 0, s, tab, ps, items, nitems); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, BatchState *ps) {
+  hipLaunchKernelGGL(batch_dt_k, dim3(nprob, 1, 1), dim3(256, 1, 1), 0, s, pc, ps);
+  return hipGetLastError();
+}
 
 // ---- host side ----------------------------------------------------------------------------
 #define LAUNCH_F(name, kern, grid, ...)                                          \

@@ -75,13 +75,13 @@ __device__ __forceinline__ int halo_src_x(const HaloRule &r, int x, int nx, doub
   return x;
 }
 
-// blockIdx.y = field slot; threads [0, nrow) walk the bottom/top bands column by
-// column (coalesced), threads [nrow, nrow + ncol) walk the left/right bands of the
-// rows between them.  Every ghost cell is written exactly once.
-__global__ __launch_bounds__(256) void update_halo_k(Dev d, HaloList list, int depth) {
-  const int fid = list.ids[blockIdx.y];
-  const HaloRule r = kRules[fid];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// Work item t of field fid: t in [0, nrow) walks the bottom/top bands column by
+// column (coalesced), t in [nrow, nrow + ncol) the left/right bands of the rows
+// between them.  Every ghost cell is written exactly once.
+__device__ __forceinline__ int halo_items(const Dev &d, const HaloRule &r, int depth) {
+  return (d.nx + r.xext + 2 * depth) + (d.ny + r.yext);
+}
+__device__ __forceinline__ void halo_item(const Dev &d, const HaloRule &r, int fid, int depth, int t) {
   const int nrow = d.nx + r.xext + 2 * depth;  // columns [2 - depth, nx + xext + depth + 1]
   const int ncol = d.ny + r.yext;              // rows [2, ny + tdst - 1]
   double *a = d.f[fid];
@@ -110,6 +110,26 @@ __global__ __launch_bounds__(256) void update_halo_k(Dev d, HaloList list, int d
       a[(int64_t)y * P + xr] = sr * a[(int64_t)y * P + xsr];
     }
   }
+}
+
+// blockIdx.y = field slot
+__global__ __launch_bounds__(256) void update_halo_k(Dev d, HaloList list, int depth) {
+  const int fid = list.ids[blockIdx.y];
+  const HaloRule r = kRules[fid];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < halo_items(d, r, depth)) halo_item(d, r, fid, depth, t);
+}
+
+// Batched: blockIdx.x = problem, blockIdx.y = field slot; the block strides over
+// the problem's ghost cells.  Problems not running this step are skipped.
+__global__ __launch_bounds__(256) void batch_halo_k(const Dev *tab, const BatchState *ps, HaloList list, int depth) {
+  const int p = blockIdx.x;
+  if (!ps[p].run) return;
+  const Dev d = tab[p];
+  const int fid = list.ids[blockIdx.y];
+  const HaloRule r = kRules[fid];
+  const int n = halo_items(d, r, depth);
+  for (int t = threadIdx.x; t < n; t += 256) halo_item(d, r, fid, depth, t);
 }
 
 // Small grids (nx or ny < 4): a mirror source can itself be a ghost written by an
@@ -454,6 +474,15 @@ clv_status k_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], in
       LAUNCH_G("update_halo", update_halo_pass_k, dim3((work + 255) / 256, list.n, 1), dim3(256, 1, 1), d, list, depth, pass);
   }
   return CLV_OK;
+}
+
+// every problem of a batch has x_cells, y_cells >= 4: the single-pass form holds
+hipError_t batch_launch_halo(hipStream_t s, int nprob, const Dev *tab, const BatchState *ps, const int *ids, int nids, int depth) {
+  HaloList list{};
+  for (int k = 0; k < nids && k < CLV_NUM_HALO_FIELDS; ++k) list.ids[list.n++] = ids[k];
+  if (list.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_halo_k, dim3(nprob, list.n, 1), dim3(256, 1, 1), 0, s, tab, ps, list, depth);
+  return hipGetLastError();
 }
 
 static int red_blocks(int ny) { return ny < RED_MAX_BLOCKS ? ny : RED_MAX_BLOCKS; }

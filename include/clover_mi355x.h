@@ -224,6 +224,65 @@ clv_status clv_hydro_step(clv_chunk *c, const clv_config *cfg, clv_hydro_state *
 /* field_summary with its ideal_gas (src/serial/field_summary.cpp:51-99). */
 clv_status clv_hydro_summary(clv_chunk *c, double out[5]);
 
+/* ---- batched form: many independent problems advanced together ------------------------- */
+/* No counterpart in the reference, whose driver advances one problem
+ * (driver/hydro.cpp:53-223).  Each member is a chunk of its own (clv_chunk_create
+ * + clv_hydro_start); a batch step runs the step of clv_hydro_step(...,
+ * CLV_PLAN_FUSED) for every member that is not complete, with one persistent
+ * launch per phase over the tiles of all members, dt chosen per member on the
+ * device, and the whole step replayed as a captured hipGraph.  Each member's
+ * fields and dt sequence are bit-identical to stepping it alone. */
+typedef struct clv_batch clv_batch;
+
+/* Tile families of a batched step (one per fused kernel; sweep variants share). */
+enum {
+  CLV_BF_TIMESTEP = 0, CLV_BF_LAGRANGE, CLV_BF_CELL_X, CLV_BF_CELL_Y, CLV_BF_MOM_X, CLV_BF_MOM_Y,
+  CLV_BF_COUNT
+};
+enum { CLV_BATCH_ORDER_GIVEN = 0, CLV_BATCH_ORDER_LARGEST_FIRST = 1 };
+/* Work schedule of one family: rows per tile (16 ... 4096), persistent grid in
+ * blocks per CU (0: one block per tile) and the order of members in the tile list. */
+typedef struct clv_batch_family {
+  int seg;
+  int blocks_per_cu;
+  int order;
+} clv_batch_family;
+typedef struct clv_batch_schedule {
+  clv_batch_family fam[CLV_BF_COUNT];
+} clv_batch_schedule;
+
+/* Deck text of problem `index` of the seeded batch workload: a square grid of
+ * side 256 + 128 k (k uniform in 0..14) on [0,10]^2, state 1 density 0.2 energy
+ * 1.0, a state-2 rectangle with corners from two sorted U(0,10) draws per axis,
+ * density U[0.5,2.0], energy U[1.5,3.5], dt controls of InputDecks/clover_bm_short.in,
+ * end_step 100.  Draws come from splitmix64 streams keyed by (seed, index).
+ * Writes at most len bytes including the terminating NUL. */
+clv_status clv_batch_deck(uint64_t seed, int index, char *text, size_t len);
+/* Members: n owned chunks on the current device, already started, with their
+ * configs and driver states (all with the same advect_x).  hist_len: dt values
+ * kept per member (step s at index s - 1).  While a chunk is in a batch, step it
+ * only through the batch; reading it (downloads, summaries) is fine between runs. */
+clv_status clv_batch_create(clv_chunk *const *chunks, const clv_config *cfgs, const clv_hydro_state *states, int n, int hist_len,
+                            clv_batch **out);
+clv_status clv_batch_destroy(clv_batch *b);
+clv_status clv_batch_get_schedule(const clv_batch *b, clv_batch_schedule *out);
+clv_status clv_batch_set_schedule(clv_batch *b, const clv_batch_schedule *s);
+/* Times each family's kernel on the current fields over candidate schedules
+ * (rows per tile, persistent grid, member order; `reps` launches each, by
+ * coordinate descent) and sets the fastest.  Does not change the members'
+ * state.  out / family_ms (may be NULL) receive the schedule and its per-launch
+ * times in ms. */
+clv_status clv_batch_autotune(clv_batch *b, int reps, clv_batch_schedule *out, double family_ms[CLV_BF_COUNT]);
+/* Runs up to max_steps batch steps, stopping early once every member is
+ * complete.  use_graph: replay the step as a hipGraph (else direct launches).
+ * *steps_done (may be NULL) receives the batch steps issued.  Blocks.  Returns
+ * CLV_ERR_SMALL_DT if a member's dt fell below its dtmin (that member stops). */
+clv_status clv_batch_run(clv_batch *b, int max_steps, int use_graph, int *steps_done);
+/* Driver state of every member (n entries), as clv_hydro_step would have left it. */
+clv_status clv_batch_states(clv_batch *b, clv_hydro_state *out);
+/* dt of steps 1 .. *count of member `member` (at most max values). */
+clv_status clv_batch_dt_history(clv_batch *b, int member, double *dt, int max, int *count);
+
 /* ---- measurement ------------------------------------------------------------------------ */
 /* Per-kernel HIP-event timing on the chunk's stream: when enabled, every launch
  * is bracketed by events and clv_profile_read() returns, for each kernel name,
