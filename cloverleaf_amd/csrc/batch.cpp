@@ -40,7 +40,7 @@ using namespace clv;
 namespace {
 
 constexpr int kStages = 4;  // views before cell sweep 1, mom sweep 1, cell sweep 2, mom sweep 2
-constexpr int kMinSeg = 16, kMaxSeg = 4096;
+constexpr int kMinSeg = 8, kMaxSeg = 4096;
 constexpr int kCheckEvery = 16;  // batch steps between completion checks
 
 void swap_in(Dev &d, int a, int b) { std::swap(d.f[a], d.f[b]); }
@@ -177,7 +177,7 @@ clv_status upload_items(BatchItem **buf, size_t *cap, const std::vector<BatchIte
 clv_status check_schedule(const clv_batch *b, const clv_batch_schedule &s) {
   for (int f = 0; f < CLV_BF_COUNT; ++f) {
     const clv_batch_family &fs = s.fam[f];
-    if (fs.seg < kMinSeg || fs.seg > kMaxSeg) return set_error(CLV_ERR_ARG, "batch schedule: seg must be in [16, 4096]");
+    if (fs.seg < kMinSeg || fs.seg > kMaxSeg) return set_error(CLV_ERR_ARG, "batch schedule: seg must be in [8, 4096]");
     if (fs.blocks_per_cu < 0 || fs.blocks_per_cu > 64) return set_error(CLV_ERR_ARG, "batch schedule: blocks_per_cu must be in [0, 64]");
     if (fs.order != CLV_BATCH_ORDER_GIVEN && fs.order != CLV_BATCH_ORDER_LARGEST_FIRST)
       return set_error(CLV_ERR_ARG, "batch schedule: unknown member order");
@@ -412,8 +412,10 @@ clv_status clv_batch_create(clv_chunk *const *chunks, const clv_config *cfgs, co
   if (hist_len > 0 &&
       (st = hip_check(hipMemset(b->hist, 0, (size_t)n * hist_len * sizeof(double)), "clv_batch_create: history clear")) != CLV_OK)
     return fail(st);
+  // the plain launches' segments (measured at 15360^2), one block per tile
   clv_batch_schedule s{};
-  for (auto &f : s.fam) f = clv_batch_family{64, 4, CLV_BATCH_ORDER_GIVEN};
+  static const int seg0[CLV_BF_COUNT] = {16, 16, 8, 128, 8, 128};
+  for (int f = 0; f < CLV_BF_COUNT; ++f) s.fam[f] = clv_batch_family{seg0[f], 0, CLV_BATCH_ORDER_GIVEN};
   if ((st = apply_schedule(b, s)) != CLV_OK) return fail(st);
   *out = b;
   return CLV_OK;
@@ -477,7 +479,7 @@ clv_status clv_batch_autotune(clv_batch *b, int reps, clv_batch_schedule *out, d
     *ms = t / reps;
     return CLV_OK;
   };
-  static const int segs[] = {32, 64, 128, 256};
+  static const int segs[] = {8, 16, 32, 64, 128, 256};
   static const int bpcs[] = {0, 1, 2, 4, 8};
   static const int orders[] = {CLV_BATCH_ORDER_GIVEN, CLV_BATCH_ORDER_LARGEST_FIRST};
   clv_batch_schedule best = b->sched;

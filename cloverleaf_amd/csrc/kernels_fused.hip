@@ -23,7 +23,7 @@
 // predictor's soundspeed, advec_mom's pre_vol, advec_cell's post_vol).
 //
 // Launch shape.  One wave64 owns a strip of consecutive columns (x-contiguous,
-// coalesced) and marches up a segment of SEG rows, so neighbours in y come
+// coalesced) and marches up a segment of rows, so neighbours in y come
 // from registers of the previous row and every input element is loaded once.
 // Neighbours in x of computed values come from the adjacent lane (wave
 // shuffles); the lanes at a strip's edges only supply those values, so strips
@@ -42,7 +42,31 @@ namespace {
 #define AT(fid, i, j) d.f[fid][(int64_t)(j) * d.pitch + (i)]
 #define M(mid, i) d.m[mid][i]
 
-constexpr int SEG = 64;        // rows per wave segment
+// Rows per wave segment of each plain launch (a wave marches one segment).
+// Longer segments re-read fewer window rows at segment edges; shorter ones give
+// more, shorter workgroups (smaller tail, tighter L2 working set per XCD).
+// Measured at 15360^2 (ms per launch, segments 8/16/32/64/128/256): the kernels
+// with x shuffles are fastest at 8-16 rows, the per-lane y sweeps at 128.
+#ifndef CLV_SEG_TIMESTEP
+#define CLV_SEG_TIMESTEP 16
+#endif
+#ifndef CLV_SEG_LAGRANGE
+#define CLV_SEG_LAGRANGE 16
+#endif
+#ifndef CLV_SEG_CELL_X
+#define CLV_SEG_CELL_X 8
+#endif
+#ifndef CLV_SEG_CELL_Y
+#define CLV_SEG_CELL_Y 128
+#endif
+#ifndef CLV_SEG_MOM_X
+#define CLV_SEG_MOM_X 8
+#endif
+#ifndef CLV_SEG_MOM_Y
+#define CLV_SEG_MOM_Y 128
+#endif
+constexpr int SEG_TS = CLV_SEG_TIMESTEP, SEG_LAG = CLV_SEG_LAGRANGE, SEG_CX = CLV_SEG_CELL_X, SEG_CY = CLV_SEG_CELL_Y,
+              SEG_MX = CLV_SEG_MOM_X, SEG_MY = CLV_SEG_MOM_Y;
 constexpr int WAVES = 4;       // waves (strips) per 256-thread block
 
 // Lane shifts across the whole wave64 with DPP (GFX9 wave_shr:1 / wave_shl:1):
@@ -226,13 +250,17 @@ __global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, dou
                                                   double *partials, int nstrips) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG * by;
-  timestep_tile(d, bx, j0, min(j0 + SEG, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
+  const int j0 = 2 + SEG_TS * by;
+  timestep_tile(d, bx, j0, min(j0 + SEG_TS, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
 }
 
+// Minimum of partials[0, n): block b takes the b-th of gridDim.x contiguous
+// parts and writes out[b].
 __global__ __launch_bounds__(1024) void reduce_min_big_k(const double *partials, int n, double *out) {
+  const int part = (n + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * part, hi = min(n, lo + part);
   double m = g_big;
-  for (int b = threadIdx.x; b < n; b += 1024) m = fmin(m, partials[b]);
+  for (int b = lo + threadIdx.x; b < hi; b += 1024) m = fmin(m, partials[b]);
   __shared__ double lds[16];
   m = wave_min(m);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
@@ -240,7 +268,7 @@ __global__ __launch_bounds__(1024) void reduce_min_big_k(const double *partials,
   if (threadIdx.x == 0) {
     double r = lds[0];
     for (int w = 1; w < 16; ++w) r = fmin(r, lds[w]);
-    out[0] = r;
+    out[blockIdx.x] = r;
   }
 }
 
@@ -399,8 +427,8 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
 __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG * by;
-  lagrange_tile(d, bx, j0, min(j0 + SEG, d.ny + 2), dt, nstrips);
+  const int j0 = 2 + SEG_LAG * by;
+  lagrange_tile(d, bx, j0, min(j0 + SEG_LAG, d.ny + 2), dt, nstrips);
 }
 
 // ---- advec_cell -------------------------------------------------------------------------
@@ -504,8 +532,8 @@ template <int SWEEP>
 __global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_out, double *e_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG * by;
-  cell_x_tile<SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 2), nstrips, rho_out, e_out);
+  const int j0 = 2 + SEG_CX * by;
+  cell_x_tile<SWEEP>(d, bx, j0, min(j0 + SEG_CX, d.ny + 2), nstrips, rho_out, e_out);
 }
 
 // y direction.  Lanes: column x0 + L, x0 = 2 + 64 * strip (no x stencil besides
@@ -573,8 +601,8 @@ template <int SWEEP>
 __global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, double *rho_out, double *e_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG * by;
-  cell_y_tile<SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 2), nstrips, rho_out, e_out);
+  const int j0 = 2 + SEG_CY * by;
+  cell_y_tile<SWEEP>(d, bx, j0, min(j0 + SEG_CY, d.ny + 2), nstrips, rho_out, e_out);
 }
 
 // ---- advec_mom --------------------------------------------------------------------------
@@ -654,8 +682,8 @@ template <int MOM_SWEEP>
 __global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG * by;
-  mom_x_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 3), nstrips, u_out, v_out);
+  const int j0 = 2 + SEG_MX * by;
+  mom_x_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG_MX, d.ny + 3), nstrips, u_out, v_out);
 }
 
 // y direction (mom_sweep 2 = y first, 4 = y second).  Lanes: node column
@@ -742,8 +770,8 @@ template <int MOM_SWEEP>
 __global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG * by;
-  mom_y_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG, d.ny + 3), nstrips, u_out, v_out);
+  const int j0 = 2 + SEG_MY * by;
+  mom_y_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG_MY, d.ny + 3), nstrips, u_out, v_out);
 }
 
 // ---- batched form: persistent launches over the tiles of many chunks ----------------------
@@ -933,8 +961,10 @@ hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, Batch
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-// strips of the timestep kernel (62 owned columns each) times row segments
-size_t fused_partials_needed(int nx, int ny) { return (size_t)ceil_div(nx, 62) * (size_t)ceil_div(ny, SEG); }
+// strips of the timestep kernel (62 owned columns each) times row segments, and
+// the second level of the minimum
+constexpr int kMinLevel2 = 256;
+size_t fused_partials_needed(int nx, int ny) { return (size_t)ceil_div(nx, 62) * (size_t)ceil_div(ny, SEG_TS) + kMinLevel2; }
 
 static clv_status halo(clv_chunk *c, std::initializer_list<int> ids, int depth) {
   int fields[CLV_NUM_HALO_FIELDS] = {0};
@@ -960,12 +990,19 @@ clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *d
   clv_status st = require_owning(c);
   if (st != CLV_OK) return st;
   const Dev &d = c->d;
-  const int ns = ceil_div(d.nx, 62), nseg = ceil_div(d.ny, SEG);
-  if ((size_t)ns * nseg > c->partials_len) return set_error(CLV_ERR_ARG, "fused timestep: partials buffer too small");
+  const int ns = ceil_div(d.nx, 62), nseg = ceil_div(d.ny, SEG_TS);
+  const int n1 = ns * nseg;
+  if ((size_t)n1 + kMinLevel2 > c->partials_len) return set_error(CLV_ERR_ARG, "fused timestep: partials buffer too small");
   LAUNCH_F("f_timestep", timestep_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe,
            c->partials, ns);
   before_launch(c, "calc_dt_final");
-  hipLaunchKernelGGL(reduce_min_big_k, dim3(1, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)c->partials, ns * nseg, c->result_dev);
+  if (n1 <= 16384) {
+    hipLaunchKernelGGL(reduce_min_big_k, dim3(1, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)c->partials, n1, c->result_dev);
+  } else {  // two levels: kMinLevel2 block minima, then one
+    double *level2 = c->partials + n1;
+    hipLaunchKernelGGL(reduce_min_big_k, dim3(kMinLevel2, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)c->partials, n1, level2);
+    hipLaunchKernelGGL(reduce_min_big_k, dim3(1, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)level2, kMinLevel2, c->result_dev);
+  }
   if ((st = after_launch(c, "calc_dt_final")) != CLV_OK) return st;
   st = hip_check(hipMemcpyAsync(c->result_host, c->result_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream), "fused dt copy");
   if (st != CLV_OK) return st;
@@ -980,23 +1017,22 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
   clv_status st = require_owning(c);
   if (st != CLV_OK) return st;
   const Dev &d = c->d;
-  const int ns = ceil_div(d.nx, 61), nseg = ceil_div(d.ny, SEG);
+  const int ns = ceil_div(d.nx, 61), nseg = ceil_div(d.ny, SEG_LAG);
   LAUNCH_F("f_lagrange", lagrange_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns);
   return CLV_OK;
 }
 
 static clv_status cell_sweep(clv_chunk *c, int sweep, int dir) {
   const Dev &d = c->d;
-  const int nseg = ceil_div(d.ny, SEG);
   double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2];
   if (dir == CLV_XDIR) {
     const int ns = ceil_div(d.nx, 60);
-    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, SEG_CX), 1);
     if (sweep == 1) LAUNCH_F("f_cell_x1", cell_x_k<1>, g, d, ns, rho_out, e_out);
     else LAUNCH_F("f_cell_x2", cell_x_k<2>, g, d, ns, rho_out, e_out);
   } else {
     const int ns = ceil_div(d.nx, 64);
-    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, SEG_CY), 1);
     if (sweep == 1) LAUNCH_F("f_cell_y1", cell_y_k<1>, g, d, ns, rho_out, e_out);
     else LAUNCH_F("f_cell_y2", cell_y_k<2>, g, d, ns, rho_out, e_out);
   }
@@ -1007,17 +1043,16 @@ static clv_status cell_sweep(clv_chunk *c, int sweep, int dir) {
 
 static clv_status mom_sweep(clv_chunk *c, int sweep, int dir) {
   const Dev &d = c->d;
-  const int nseg = ceil_div(d.ny + 1, SEG);
   double *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
   const int mom = dir + 2 * (sweep - 1);  // src/serial/advec_mom.cpp:37
   if (dir == CLV_XDIR) {
     const int ns = ceil_div(d.nx + 1, 60);
-    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, SEG_MX), 1);
     if (mom == 1) LAUNCH_F("f_mom_x1", mom_x_k<1>, g, d, ns, u_out, v_out);
     else LAUNCH_F("f_mom_x2", mom_x_k<3>, g, d, ns, u_out, v_out);
   } else {
     const int ns = ceil_div(d.nx + 1, 64);
-    const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, SEG_MY), 1);
     if (mom == 2) LAUNCH_F("f_mom_y1", mom_y_k<2>, g, d, ns, u_out, v_out);
     else LAUNCH_F("f_mom_y2", mom_y_k<4>, g, d, ns, u_out, v_out);
   }

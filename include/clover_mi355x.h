@@ -240,7 +240,7 @@ enum {
   CLV_BF_COUNT
 };
 enum { CLV_BATCH_ORDER_GIVEN = 0, CLV_BATCH_ORDER_LARGEST_FIRST = 1 };
-/* Work schedule of one family: rows per tile (16 ... 4096), persistent grid in
+/* Work schedule of one family: rows per tile (8 ... 4096), persistent grid in
  * blocks per CU (0: one block per tile) and the order of members in the tile list. */
 typedef struct clv_batch_family {
   int seg;

@@ -111,7 +111,7 @@ def test_batch_bitwise_vs_plain(gpu, small_decks, small_plain, use_graph):
 
 
 SCHEDULES = [
-    dict(seg=16, blocks_per_cu=1, order=clv.ORDER_GIVEN),
+    dict(seg=8, blocks_per_cu=1, order=clv.ORDER_GIVEN),
     dict(seg=32, blocks_per_cu=0, order=clv.ORDER_LARGEST_FIRST),
     dict(seg=128, blocks_per_cu=8, order=clv.ORDER_GIVEN),
     dict(seg=4096, blocks_per_cu=2, order=clv.ORDER_LARGEST_FIRST),
@@ -139,7 +139,7 @@ def test_batch_pieces_and_autotune(gpu, small_decks, small_plain):
         assert b.run(3) == 3
         sched, ms = b.autotune(reps=2)
         assert all(m > 0 for m in ms)
-        assert all(16 <= f.seg <= 4096 for f in sched.fam)
+        assert all(8 <= f.seg <= 4096 for f in sched.fam)
         assert b.run(5, use_graph=False) == 5
         # members' own views are kept current between runs: a summary reads them
         s_mid = chunks[3].summary()
@@ -159,7 +159,7 @@ def test_batch_rejects_bad_use(gpu, small_decks):
         clv.Batch([chunks[0], chunks[0]], [cfgs[0], cfgs[0]], [states[0], states[0]])
     with clv.Batch(chunks, cfgs, states) as b:
         s = b.schedule()
-        s.fam[0].seg = 8
+        s.fam[0].seg = 4
         with pytest.raises(clv.CloverError, match="seg"):
             b.set_schedule(s)
         b.run(2)
