@@ -73,8 +73,8 @@ ALG_TERMS = {
     "field_summary": [(6, "C")],
     # fused plan: compulsory traffic of each fused kernel (every input array read
     # once, every output written once over the interior)
-    "f_timestep": [(9, "C")],                  # rho0, e0, u0, v0, xarea, yarea, vol -> p, q
-    "f_lagrange": [(15, "C")],                 # rho0, e0, p, q, u0, v0, vol, areas -> u1, v1, rho1, e1, vfx, vfy
+    "f_timestep": [(8, "C")],                  # rho0, e0, u0, v0, xarea, yarea, vol -> q
+    "f_lagrange": [(14, "C")],                 # rho0, e0, q, u0, v0, vol, areas -> u1, v1, rho1, e1, vfx, vfy
     "f_cell_x1": [(8, "C")],                   # vol, vfx, vfy, rho1, e1 -> rho1, e1, mfx
     "f_cell_y1": [(8, "C")],
     "f_cell_x2": [(7, "C")],                   # vol, vfx, rho1, e1 -> rho1, e1, mfx
@@ -84,8 +84,8 @@ ALG_TERMS = {
     "f_mom_x2": [(7, "N")],                    # vol, mfx, rho1, u1, v1 -> u1, v1
     "f_mom_y2": [(7, "N")],
 }
-# compulsory traffic of one fused step: 9 + 15 + 8 + 7 + 8 + 7 array sweeps
-B_FUSED_PER_CELL_STEP = 54 * 8.0
+# compulsory traffic of one fused step: 8 + 14 + 8 + 7 + 8 + 7 array sweeps
+B_FUSED_PER_CELL_STEP = 52 * 8.0
 B_ALG_PER_CELL_STEP = 1440.0  # 180 array sweeps of 8 B per interior cell-step at reference kernel granularity
 
 
@@ -300,7 +300,7 @@ def main():
                    "x_cells": nx, "y_cells": ny, "plan": args.plan, "parallelism": "1 GPU, 1 chunk"},
         "gcell_steps_per_s": value / 1e9,
         "effective_hbm_frac_at_1440B": value * B_ALG_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
-        "compulsory_hbm_frac_fused_432B": value * B_FUSED_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9) if args.plan == "fused" else None,
+        "compulsory_hbm_frac_fused_416B": value * B_FUSED_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9) if args.plan == "fused" else None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_us": avg_s * 1e6, "share_of_kernel_time": dom_ms / total_kernel_ms},

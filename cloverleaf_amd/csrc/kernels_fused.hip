@@ -3,7 +3,7 @@
 // One step of driver/hydro.cpp:57-89 in six single-pass kernels:
 //
 //   timestep_k  ideal_gas(rho0,e0) + viscosity + calc_dt partial minimum
-//               (driver/timestep.cpp:47-78)        writes p, q
+//               (driver/timestep.cpp:47-78)        writes q (p stays in registers)
 //   -- global minimum -> dt on the host --
 //   lagrange_k  PdV predictor + ideal_gas + pressure halo + revert, accelerate,
 //               PdV corrector, flux_calc (driver/hydro.cpp:63-75)
@@ -229,10 +229,8 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
       double mins = fmin(dtct, fmin(dtut, fmin(dtvt, fmin(dtdivt, g_big))));
       if (own) dtmin = fmin(mins, dtmin);
     }
-    if (own) {
-      AT(CLV_PRESSURE, i, r) = p_c;
-      AT(CLV_VISCOSITY, i, r) = q;
-    }
+    // p is not stored: lagrange_tile recomputes it from the same rho0, e0
+    if (own) AT(CLV_VISCOSITY, i, r) = q;
     p_m = p_c;
     p_c = p_p;
     c_c = c_p;
@@ -331,7 +329,11 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
     // ghost rho0 through the mirror (the halo of driver/timestep.cpp:53-60)
     a.rho0 = AT(CLV_DENSITY0, mirror_cell(ic, nx), mirror_cell(r, ny));
     a.e0 = AT(CLV_ENERGY0, ic, r);
-    a.p = AT(CLV_PRESSURE, ic, r);
+    // ideal_gas of the timestep (driver/timestep.cpp:48), recomputed from the
+    // same operands with the same expression instead of re-read: bit-identical
+    // on the interior cells, the only cells whose predictor result is used (the
+    // ghost rows and columns of p' are mirrored from the interior below)
+    a.p = pressure_of(a.rho0, a.e0);
     a.q = AT(CLV_VISCOSITY, ic, r);
   };
   // predictor pressure of column i, with the reflective ghost columns taking the

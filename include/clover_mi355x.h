@@ -214,7 +214,10 @@ clv_status clv_deck_parse(const char *text, clv_config *out);
 /* driver/start.cpp:69-127: build, initialise, generate, ideal_gas, depth-2 halo.
  * Resets *st; summary0 (may be NULL) receives the step-0 field summary. */
 clv_status clv_hydro_start(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, double summary0[5]);
-/* driver/timestep.cpp:37-108: ideal_gas, halos, viscosity, calc_dt, dt selection. */
+/* driver/timestep.cpp:37-108: ideal_gas, halos, viscosity, calc_dt, dt selection.
+ * With CLV_PLAN_FUSED the pressure and soundspeed arrays are not written (the
+ * fused step recomputes them where it needs them); follow it with the fused
+ * step, not with the per-kernel entry points. */
 clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan);
 /* One step of driver/hydro.cpp:57-89 (timestep, PdV predict, accelerate, PdV
  * correct, flux_calc, advection, reset_field, advect_x flip, time += dt); sets
