@@ -42,31 +42,6 @@ namespace {
 #define AT(fid, i, j) d.f[fid][(int64_t)(j) * d.pitch + (i)]
 #define M(mid, i) d.m[mid][i]
 
-// Rows per wave segment of each plain launch (a wave marches one segment).
-// Longer segments re-read fewer window rows at segment edges; shorter ones give
-// more, shorter workgroups (smaller tail, tighter L2 working set per XCD).
-// Measured at 15360^2 (ms per launch, segments 8/16/32/64/128/256): the kernels
-// with x shuffles are fastest at 8-16 rows, the per-lane y sweeps at 128.
-#ifndef CLV_SEG_TIMESTEP
-#define CLV_SEG_TIMESTEP 16
-#endif
-#ifndef CLV_SEG_LAGRANGE
-#define CLV_SEG_LAGRANGE 16
-#endif
-#ifndef CLV_SEG_CELL_X
-#define CLV_SEG_CELL_X 8
-#endif
-#ifndef CLV_SEG_CELL_Y
-#define CLV_SEG_CELL_Y 128
-#endif
-#ifndef CLV_SEG_MOM_X
-#define CLV_SEG_MOM_X 8
-#endif
-#ifndef CLV_SEG_MOM_Y
-#define CLV_SEG_MOM_Y 128
-#endif
-constexpr int SEG_TS = CLV_SEG_TIMESTEP, SEG_LAG = CLV_SEG_LAGRANGE, SEG_CX = CLV_SEG_CELL_X, SEG_CY = CLV_SEG_CELL_Y,
-              SEG_MX = CLV_SEG_MOM_X, SEG_MY = CLV_SEG_MOM_Y;
 constexpr int WAVES = 4;       // waves (strips) per 256-thread block
 
 // Lane shifts across the whole wave64 with DPP (GFX9 wave_shr:1 / wave_shl:1):
@@ -245,11 +220,11 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
 
 // 5 waves per SIMD fit without spilling (96 VGPRs)
 __global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
-                                                  double *partials, int nstrips) {
+                                                  double *partials, int nstrips, int seg) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG_TS * by;
-  timestep_tile(d, bx, j0, min(j0 + SEG_TS, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
+  const int j0 = 2 + seg * by;
+  timestep_tile(d, bx, j0, min(j0 + seg, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
 }
 
 // Minimum of partials[0, n): block b takes the b-th of gridDim.x contiguous
@@ -426,11 +401,11 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   }
 }
 
-__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips) {
+__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips, int seg) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG_LAG * by;
-  lagrange_tile(d, bx, j0, min(j0 + SEG_LAG, d.ny + 2), dt, nstrips);
+  const int j0 = 2 + seg * by;
+  lagrange_tile(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips);
 }
 
 // ---- advec_cell -------------------------------------------------------------------------
@@ -531,11 +506,11 @@ __device__ __forceinline__ void cell_x_tile(const V &d, int bx, int j0, int j1, 
 }
 
 template <int SWEEP>
-__global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+__global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG_CX * by;
-  cell_x_tile<SWEEP>(d, bx, j0, min(j0 + SEG_CX, d.ny + 2), nstrips, rho_out, e_out);
+  const int j0 = 2 + seg * by;
+  cell_x_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 2), nstrips, rho_out, e_out);
 }
 
 // y direction.  Lanes: column x0 + L, x0 = 2 + 64 * strip (no x stencil besides
@@ -600,11 +575,11 @@ __device__ __forceinline__ void cell_y_tile(const V &d, int bx, int j0, int j1, 
 }
 
 template <int SWEEP>
-__global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, double *rho_out, double *e_out) {
+__global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG_CY * by;
-  cell_y_tile<SWEEP>(d, bx, j0, min(j0 + SEG_CY, d.ny + 2), nstrips, rho_out, e_out);
+  const int j0 = 2 + seg * by;
+  cell_y_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 2), nstrips, rho_out, e_out);
 }
 
 // ---- advec_mom --------------------------------------------------------------------------
@@ -681,11 +656,11 @@ __device__ __forceinline__ void mom_x_tile(const V &d, int bx, int j0, int j1, i
 }
 
 template <int MOM_SWEEP>
-__global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, double *u_out, double *v_out) {
+__global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, int seg, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG_MX * by;
-  mom_x_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG_MX, d.ny + 3), nstrips, u_out, v_out);
+  const int j0 = 2 + seg * by;
+  mom_x_tile<MOM_SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, u_out, v_out);
 }
 
 // y direction (mom_sweep 2 = y first, 4 = y second).  Lanes: node column
@@ -769,11 +744,11 @@ __device__ __forceinline__ void mom_y_tile(const V &d, int bx, int j0, int j1, i
 }
 
 template <int MOM_SWEEP>
-__global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, double *u_out, double *v_out) {
+__global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, int seg, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
-  const int j0 = 2 + SEG_MY * by;
-  mom_y_tile<MOM_SWEEP>(d, bx, j0, min(j0 + SEG_MY, d.ny + 3), nstrips, u_out, v_out);
+  const int j0 = 2 + seg * by;
+  mom_y_tile<MOM_SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, u_out, v_out);
 }
 
 // ---- batched form: persistent launches over the tiles of many chunks ----------------------
@@ -963,10 +938,35 @@ hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, Batch
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-// strips of the timestep kernel (62 owned columns each) times row segments, and
-// the second level of the minimum
+// Rows per wave segment of each plain launch, per tile family (timestep,
+// lagrange, cell x, cell y, mom x, mom y).  Longer segments re-read fewer window
+// rows at segment edges; shorter ones give more, shorter workgroups (a smaller
+// tail, a tighter working set per XCD L2).  At 15360^2 (ms per launch over
+// segments of 8 ... 256 rows) the kernels with x shuffles are fastest at 8-16
+// rows and the per-lane y sweeps at 128.  On a smaller grid a family's segment
+// is halved until it has 4 workgroups per CU (or 8 rows).
+enum { SF_TIMESTEP, SF_LAGRANGE, SF_CELL_X, SF_CELL_Y, SF_MOM_X, SF_MOM_Y };
+constexpr int kSegLarge[6] = {16, 16, 8, 128, 8, 128};
+constexpr int kSegMin = 8;
+
+static void choose_segments(clv_chunk *c) {
+  if (c->seg[0]) return;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
+  const int nx = c->d.nx, ny = c->d.ny;
+  const int strips[6] = {ceil_div(nx, 62), ceil_div(nx, 61), ceil_div(nx, 60), ceil_div(nx, 64), ceil_div(nx + 1, 60), ceil_div(nx + 1, 64)};
+  const int rows[6] = {ny, ny, ny, ny, ny + 1, ny + 1};
+  for (int f = 0; f < 6; ++f) {
+    int seg = kSegLarge[f];
+    while (seg > kSegMin && ceil_div(strips[f], WAVES) * ceil_div(rows[f], seg) < 4 * cus) seg /= 2;
+    c->seg[f] = seg;
+  }
+}
+
+// strips of the timestep kernel (62 owned columns each) times row segments (of
+// at least kSegMin rows), and the second level of the minimum
 constexpr int kMinLevel2 = 256;
-size_t fused_partials_needed(int nx, int ny) { return (size_t)ceil_div(nx, 62) * (size_t)ceil_div(ny, SEG_TS) + kMinLevel2; }
+size_t fused_partials_needed(int nx, int ny) { return (size_t)ceil_div(nx, 62) * (size_t)ceil_div(ny, kSegMin) + kMinLevel2; }
 
 static clv_status halo(clv_chunk *c, std::initializer_list<int> ids, int depth) {
   int fields[CLV_NUM_HALO_FIELDS] = {0};
@@ -985,6 +985,7 @@ static clv_status require_owning(clv_chunk *c) {
   // only a chunk that owns its arena may do
   if (!c->owns) return set_error(CLV_ERR_ARG, "CLV_PLAN_FUSED needs a chunk created by clv_chunk_create (not clv_chunk_wrap)");
   if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "CLV_PLAN_FUSED needs x_cells, y_cells >= 4");
+  choose_segments(c);
   return CLV_OK;
 }
 
@@ -992,11 +993,12 @@ clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *d
   clv_status st = require_owning(c);
   if (st != CLV_OK) return st;
   const Dev &d = c->d;
-  const int ns = ceil_div(d.nx, 62), nseg = ceil_div(d.ny, SEG_TS);
+  const int seg = c->seg[SF_TIMESTEP];
+  const int ns = ceil_div(d.nx, 62), nseg = ceil_div(d.ny, seg);
   const int n1 = ns * nseg;
   if ((size_t)n1 + kMinLevel2 > c->partials_len) return set_error(CLV_ERR_ARG, "fused timestep: partials buffer too small");
   LAUNCH_F("f_timestep", timestep_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe,
-           c->partials, ns);
+           c->partials, ns, seg);
   before_launch(c, "calc_dt_final");
   if (n1 <= 16384) {
     hipLaunchKernelGGL(reduce_min_big_k, dim3(1, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)c->partials, n1, c->result_dev);
@@ -1019,8 +1021,9 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
   clv_status st = require_owning(c);
   if (st != CLV_OK) return st;
   const Dev &d = c->d;
-  const int ns = ceil_div(d.nx, 61), nseg = ceil_div(d.ny, SEG_LAG);
-  LAUNCH_F("f_lagrange", lagrange_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns);
+  const int seg = c->seg[SF_LAGRANGE];
+  const int ns = ceil_div(d.nx, 61), nseg = ceil_div(d.ny, seg);
+  LAUNCH_F("f_lagrange", lagrange_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg);
   return CLV_OK;
 }
 
@@ -1028,15 +1031,15 @@ static clv_status cell_sweep(clv_chunk *c, int sweep, int dir) {
   const Dev &d = c->d;
   double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2];
   if (dir == CLV_XDIR) {
-    const int ns = ceil_div(d.nx, 60);
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, SEG_CX), 1);
-    if (sweep == 1) LAUNCH_F("f_cell_x1", cell_x_k<1>, g, d, ns, rho_out, e_out);
-    else LAUNCH_F("f_cell_x2", cell_x_k<2>, g, d, ns, rho_out, e_out);
+    const int ns = ceil_div(d.nx, 60), seg = c->seg[SF_CELL_X];
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, seg), 1);
+    if (sweep == 1) LAUNCH_F("f_cell_x1", cell_x_k<1>, g, d, ns, seg, rho_out, e_out);
+    else LAUNCH_F("f_cell_x2", cell_x_k<2>, g, d, ns, seg, rho_out, e_out);
   } else {
-    const int ns = ceil_div(d.nx, 64);
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, SEG_CY), 1);
-    if (sweep == 1) LAUNCH_F("f_cell_y1", cell_y_k<1>, g, d, ns, rho_out, e_out);
-    else LAUNCH_F("f_cell_y2", cell_y_k<2>, g, d, ns, rho_out, e_out);
+    const int ns = ceil_div(d.nx, 64), seg = c->seg[SF_CELL_Y];
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, seg), 1);
+    if (sweep == 1) LAUNCH_F("f_cell_y1", cell_y_k<1>, g, d, ns, seg, rho_out, e_out);
+    else LAUNCH_F("f_cell_y2", cell_y_k<2>, g, d, ns, seg, rho_out, e_out);
   }
   swap_fields(c, CLV_DENSITY1, CLV_WORK1);
   swap_fields(c, CLV_ENERGY1, CLV_WORK2);
@@ -1048,15 +1051,15 @@ static clv_status mom_sweep(clv_chunk *c, int sweep, int dir) {
   double *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
   const int mom = dir + 2 * (sweep - 1);  // src/serial/advec_mom.cpp:37
   if (dir == CLV_XDIR) {
-    const int ns = ceil_div(d.nx + 1, 60);
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, SEG_MX), 1);
-    if (mom == 1) LAUNCH_F("f_mom_x1", mom_x_k<1>, g, d, ns, u_out, v_out);
-    else LAUNCH_F("f_mom_x2", mom_x_k<3>, g, d, ns, u_out, v_out);
+    const int ns = ceil_div(d.nx + 1, 60), seg = c->seg[SF_MOM_X];
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
+    if (mom == 1) LAUNCH_F("f_mom_x1", mom_x_k<1>, g, d, ns, seg, u_out, v_out);
+    else LAUNCH_F("f_mom_x2", mom_x_k<3>, g, d, ns, seg, u_out, v_out);
   } else {
-    const int ns = ceil_div(d.nx + 1, 64);
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, SEG_MY), 1);
-    if (mom == 2) LAUNCH_F("f_mom_y1", mom_y_k<2>, g, d, ns, u_out, v_out);
-    else LAUNCH_F("f_mom_y2", mom_y_k<4>, g, d, ns, u_out, v_out);
+    const int ns = ceil_div(d.nx + 1, 64), seg = c->seg[SF_MOM_Y];
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
+    if (mom == 2) LAUNCH_F("f_mom_y1", mom_y_k<2>, g, d, ns, seg, u_out, v_out);
+    else LAUNCH_F("f_mom_y2", mom_y_k<4>, g, d, ns, seg, u_out, v_out);
   }
   swap_fields(c, CLV_XVEL1, CLV_WORK3);
   swap_fields(c, CLV_YVEL1, CLV_WORK4);
