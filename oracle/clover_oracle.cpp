@@ -2,12 +2,17 @@
  * CPU oracle for the CloverLeaf hydro hot path -- TEST INFRASTRUCTURE ONLY.
  *
  * Restates the reference serial model (src/serial/<kernel>.cpp) expression by
- * expression in C, on x-contiguous arrays.  Every loop below keeps the
- * reference's iteration ranges and floating-point expression order; the
- * OpenMP pragmas only spread independent cells over threads (each iteration
- * writes its own cells and reads no cell written by another iteration), so
- * results do not depend on the thread count.  The one order-dependent loop
- * (the field_summary sums) runs serially in the reference's order.
+ * expression, on x-contiguous arrays.  Every loop below keeps the reference's
+ * iteration ranges and floating-point expression order; the OpenMP pragmas
+ * only spread independent cells over threads (each iteration writes its own
+ * cells and reads no cell written by another iteration), so results do not
+ * depend on the thread count.  The one order-dependent loop (the
+ * field_summary sums) runs serially in the reference's order.
+ *
+ * The hydro kernels are templates over the field type: T = double is the
+ * oracle (the orc_* entry points); T = Dual, a forward-mode dual number, is
+ * its tangent-linear model (the orc_tl_* entry points), which carries one
+ * directional derivative through the same expressions.
  *
  * Build with -O2 -ffp-contract=off (oracle/Makefile): no fused multiply-add,
  * no fast-math, exactly like the reference built with g++ -O3.
@@ -18,6 +23,56 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+namespace {
+
+// Forward-mode dual number: a value and one directional derivative (the
+// tangent-linear model of every expression it passes through).  Branches and
+// comparisons follow the value, so a piecewise definition (van Leer limiter,
+// upwind choice, viscosity switch, fmin/fmax, fabs) is differentiated on its
+// active piece.  Ties: fmin/fmax return their first argument, fabs(0) returns
+// its argument unchanged (derivative +1).
+struct Dual {
+  double v, d;
+  Dual() = default;
+  constexpr Dual(double x) : v(x), d(0.0) {}
+  constexpr Dual(double x, double dx) : v(x), d(dx) {}
+};
+inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+inline Dual &operator+=(Dual &a, Dual b) { return a = a + b; }
+inline Dual &operator-=(Dual &a, Dual b) { return a = a - b; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+inline Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
+inline Dual fmin(Dual a, Dual b) { return b.v < a.v ? b : a; }
+inline Dual fmax(Dual a, Dual b) { return b.v > a.v ? b : a; }
+inline Dual sqrt(Dual a) {
+  const double s = ::sqrt(a.v);
+  return Dual(s, s > 0.0 ? a.d / (2.0 * s) : 0.0);
+}
+static_assert(sizeof(Dual) == sizeof(orc_dual), "Dual and orc_dual share a layout");
+
+// A chunk whose 2D arrays hold T (double, or Dual for the tangent-linear model);
+// the layout of orc_chunk (T = double) and orc_tl_chunk (T = Dual).
+template <class T>
+struct View {
+  int nx, ny, px;
+  T *f[ORC_NUM_2D];
+  double *m[ORC_NUM_1D];
+};
+inline View<double> *V(orc_chunk *c) { return reinterpret_cast<View<double> *>(c); }
+inline View<Dual> *V(orc_tl_chunk *c) { return reinterpret_cast<View<Dual> *>(c); }
+
+}  // namespace
 
 #define G_SMALL 1.0e-16 /* driver/definitions.h:32 */
 #define G_BIG 1.0e+21   /* driver/definitions.h:33 */
@@ -55,59 +110,61 @@ void orc_chunk_free(orc_chunk *c) {
 }
 
 /* src/serial/ideal_gas.cpp:28-48 (kernel), :55-66 (driver: predict selects time level 1). */
-void orc_ideal_gas(orc_chunk *c, int predict) {
+template <class T>
+static void k_ideal_gas(View<T> *c, int predict) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
-  const double *density = c->f[predict ? ORC_DENSITY1 : ORC_DENSITY0];
-  const double *energy = c->f[predict ? ORC_ENERGY1 : ORC_ENERGY0];
-  double *pressure = c->f[ORC_PRESSURE], *soundspeed = c->f[ORC_SOUNDSPEED];
+  const T *density = c->f[predict ? ORC_DENSITY1 : ORC_DENSITY0];
+  const T *energy = c->f[predict ? ORC_ENERGY1 : ORC_ENERGY0];
+  T *pressure = c->f[ORC_PRESSURE], *soundspeed = c->f[ORC_SOUNDSPEED];
 #pragma omp parallel for if (ORC_PAR(c))
   for (int j = y_min + 1; j < y_max + 2; j++) {
     for (int i = x_min + 1; i < x_max + 2; i++) {
-      double v = 1.0 / AT(density, i, j);
+      T v = 1.0 / AT(density, i, j);
       AT(pressure, i, j) = (1.4 - 1.0) * AT(density, i, j) * AT(energy, i, j);
-      double pressurebyenergy = (1.4 - 1.0) * AT(density, i, j);
-      double pressurebyvolume = -AT(density, i, j) * AT(pressure, i, j);
-      double sound_speed_squared = v * v * (AT(pressure, i, j) * pressurebyenergy - pressurebyvolume);
+      T pressurebyenergy = (1.4 - 1.0) * AT(density, i, j);
+      T pressurebyvolume = -AT(density, i, j) * AT(pressure, i, j);
+      T sound_speed_squared = v * v * (AT(pressure, i, j) * pressurebyenergy - pressurebyvolume);
       AT(soundspeed, i, j) = sqrt(sound_speed_squared);
     }
   }
 }
 
 /* src/serial/viscosity.cpp:30-68.  Keeps the reference's diry test on pgradx (:57). */
-void orc_viscosity(orc_chunk *c) {
+template <class T>
+static void k_viscosity(View<T> *c) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
   const double *celldx = c->m[ORC_CELLDX], *celldy = c->m[ORC_CELLDY];
-  const double *density0 = c->f[ORC_DENSITY0], *pressure = c->f[ORC_PRESSURE];
-  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
-  double *viscosity = c->f[ORC_VISCOSITY];
+  const T *density0 = c->f[ORC_DENSITY0], *pressure = c->f[ORC_PRESSURE];
+  const T *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  T *viscosity = c->f[ORC_VISCOSITY];
 #pragma omp parallel for if (ORC_PAR(c))
   for (int j = y_min + 1; j < y_max + 2; j++) {
     for (int i = x_min + 1; i < x_max + 2; i++) {
-      double ugrad = (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1)) - (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1));
-      double vgrad = (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1)) - (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0));
-      double div = (celldx[i] * (ugrad) + celldy[j] * (vgrad));
-      double strain2 = 0.5 * (AT(xvel0, i + 0, j + 1) + AT(xvel0, i + 1, j + 1) - AT(xvel0, i, j) - AT(xvel0, i + 1, j + 0)) / celldy[j] +
+      T ugrad = (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1)) - (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1));
+      T vgrad = (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1)) - (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0));
+      T div = (celldx[i] * (ugrad) + celldy[j] * (vgrad));
+      T strain2 = 0.5 * (AT(xvel0, i + 0, j + 1) + AT(xvel0, i + 1, j + 1) - AT(xvel0, i, j) - AT(xvel0, i + 1, j + 0)) / celldy[j] +
                        0.5 * (AT(yvel0, i + 1, j + 0) + AT(yvel0, i + 1, j + 1) - AT(yvel0, i, j) - AT(yvel0, i + 0, j + 1)) / celldx[i];
-      double pgradx = (AT(pressure, i + 1, j + 0) - AT(pressure, i - 1, j + 0)) / (celldx[i] + celldx[i + 1]);
-      double pgrady = (AT(pressure, i + 0, j + 1) - AT(pressure, i + 0, j - 1)) / (celldy[j] + celldy[j + 2]);
-      double pgradx2 = pgradx * pgradx;
-      double pgrady2 = pgrady * pgrady;
-      double limiter = ((0.5 * (ugrad) / celldx[i]) * pgradx2 + (0.5 * (vgrad) / celldy[j]) * pgrady2 + strain2 * pgradx * pgrady) /
+      T pgradx = (AT(pressure, i + 1, j + 0) - AT(pressure, i - 1, j + 0)) / (celldx[i] + celldx[i + 1]);
+      T pgrady = (AT(pressure, i + 0, j + 1) - AT(pressure, i + 0, j - 1)) / (celldy[j] + celldy[j + 2]);
+      T pgradx2 = pgradx * pgradx;
+      T pgrady2 = pgrady * pgrady;
+      T limiter = ((0.5 * (ugrad) / celldx[i]) * pgradx2 + (0.5 * (vgrad) / celldy[j]) * pgrady2 + strain2 * pgradx * pgrady) /
                        fmax(pgradx2 + pgrady2, G_SMALL);
       if ((limiter > 0.0) || (div >= 0.0)) {
         AT(viscosity, i, j) = 0.0;
       } else {
-        double dirx = 1.0;
+        T dirx = 1.0;
         if (pgradx < 0.0) dirx = -1.0;
         pgradx = dirx * fmax(G_SMALL, fabs(pgradx));
-        double diry = 1.0;
+        T diry = 1.0;
         if (pgradx < 0.0) diry = -1.0;
         pgrady = diry * fmax(G_SMALL, fabs(pgrady));
-        double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
-        double xgrad = fabs(celldx[i] * pgrad / pgradx);
-        double ygrad = fabs(celldy[j] * pgrad / pgrady);
-        double grad = fmin(xgrad, ygrad);
-        double grad2 = grad * grad;
+        T pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
+        T xgrad = fabs(celldx[i] * pgrad / pgradx);
+        T ygrad = fabs(celldy[j] * pgrad / pgrady);
+        T grad = fmin(xgrad, ygrad);
+        T grad2 = grad * grad;
         AT(viscosity, i, j) = 2.0 * AT(density0, i, j) * grad2 * limiter * limiter;
       }
     }
@@ -165,30 +222,31 @@ void orc_calc_dt(orc_chunk *c, double dtmin, double dtc_safe, double dtu_safe, d
 }
 
 /* src/serial/PdV.cpp:36-87 */
-void orc_PdV(orc_chunk *c, int predict, double dt) {
+template <class T>
+static void k_PdV(View<T> *c, int predict, double dt) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
-  const double *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA], *volume = c->f[ORC_VOLUME];
-  const double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0];
-  const double *pressure = c->f[ORC_PRESSURE], *viscosity = c->f[ORC_VISCOSITY];
-  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
-  const double *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
-  double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
+  const T *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA], *volume = c->f[ORC_VOLUME];
+  const T *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0];
+  const T *pressure = c->f[ORC_PRESSURE], *viscosity = c->f[ORC_VISCOSITY];
+  const T *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  const T *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+  T *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
   if (predict) {
 #pragma omp parallel for if (ORC_PAR(c))
     for (int j = y_min + 1; j < y_max + 2; j++) {
       for (int i = x_min + 1; i < x_max + 2; i++) {
-        double left_flux = (AT(xarea, i, j) * (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1) + AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1))) * 0.25 * dt * 0.5;
-        double right_flux = (AT(xarea, i + 1, j + 0) * (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1) + AT(xvel0, i + 1, j + 0) +
+        T left_flux = (AT(xarea, i, j) * (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1) + AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1))) * 0.25 * dt * 0.5;
+        T right_flux = (AT(xarea, i + 1, j + 0) * (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1) + AT(xvel0, i + 1, j + 0) +
                                                       AT(xvel0, i + 1, j + 1))) *
                             0.25 * dt * 0.5;
-        double bottom_flux = (AT(yarea, i, j) * (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0) + AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0))) * 0.25 * dt * 0.5;
-        double top_flux = (AT(yarea, i + 0, j + 1) * (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1) + AT(yvel0, i + 0, j + 1) +
+        T bottom_flux = (AT(yarea, i, j) * (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0) + AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0))) * 0.25 * dt * 0.5;
+        T top_flux = (AT(yarea, i + 0, j + 1) * (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1) + AT(yvel0, i + 0, j + 1) +
                                                     AT(yvel0, i + 1, j + 1))) *
                           0.25 * dt * 0.5;
-        double total_flux = right_flux - left_flux + top_flux - bottom_flux;
-        double volume_change_s = AT(volume, i, j) / (AT(volume, i, j) + total_flux);
-        double recip_volume = 1.0 / AT(volume, i, j);
-        double energy_change = (AT(pressure, i, j) / AT(density0, i, j) + AT(viscosity, i, j) / AT(density0, i, j)) * total_flux * recip_volume;
+        T total_flux = right_flux - left_flux + top_flux - bottom_flux;
+        T volume_change_s = AT(volume, i, j) / (AT(volume, i, j) + total_flux);
+        T recip_volume = 1.0 / AT(volume, i, j);
+        T energy_change = (AT(pressure, i, j) / AT(density0, i, j) + AT(viscosity, i, j) / AT(density0, i, j)) * total_flux * recip_volume;
         AT(energy1, i, j) = AT(energy0, i, j) - energy_change;
         AT(density1, i, j) = AT(density0, i, j) * volume_change_s;
       }
@@ -197,18 +255,18 @@ void orc_PdV(orc_chunk *c, int predict, double dt) {
 #pragma omp parallel for if (ORC_PAR(c))
     for (int j = y_min + 1; j < y_max + 2; j++) {
       for (int i = x_min + 1; i < x_max + 2; i++) {
-        double left_flux = (AT(xarea, i, j) * (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1) + AT(xvel1, i, j) + AT(xvel1, i + 0, j + 1))) * 0.25 * dt;
-        double right_flux = (AT(xarea, i + 1, j + 0) * (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1) + AT(xvel1, i + 1, j + 0) +
+        T left_flux = (AT(xarea, i, j) * (AT(xvel0, i, j) + AT(xvel0, i + 0, j + 1) + AT(xvel1, i, j) + AT(xvel1, i + 0, j + 1))) * 0.25 * dt;
+        T right_flux = (AT(xarea, i + 1, j + 0) * (AT(xvel0, i + 1, j + 0) + AT(xvel0, i + 1, j + 1) + AT(xvel1, i + 1, j + 0) +
                                                       AT(xvel1, i + 1, j + 1))) *
                             0.25 * dt;
-        double bottom_flux = (AT(yarea, i, j) * (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0) + AT(yvel1, i, j) + AT(yvel1, i + 1, j + 0))) * 0.25 * dt;
-        double top_flux = (AT(yarea, i + 0, j + 1) * (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1) + AT(yvel1, i + 0, j + 1) +
+        T bottom_flux = (AT(yarea, i, j) * (AT(yvel0, i, j) + AT(yvel0, i + 1, j + 0) + AT(yvel1, i, j) + AT(yvel1, i + 1, j + 0))) * 0.25 * dt;
+        T top_flux = (AT(yarea, i + 0, j + 1) * (AT(yvel0, i + 0, j + 1) + AT(yvel0, i + 1, j + 1) + AT(yvel1, i + 0, j + 1) +
                                                     AT(yvel1, i + 1, j + 1))) *
                           0.25 * dt;
-        double total_flux = right_flux - left_flux + top_flux - bottom_flux;
-        double volume_change_s = AT(volume, i, j) / (AT(volume, i, j) + total_flux);
-        double recip_volume = 1.0 / AT(volume, i, j);
-        double energy_change = (AT(pressure, i, j) / AT(density0, i, j) + AT(viscosity, i, j) / AT(density0, i, j)) * total_flux * recip_volume;
+        T total_flux = right_flux - left_flux + top_flux - bottom_flux;
+        T volume_change_s = AT(volume, i, j) / (AT(volume, i, j) + total_flux);
+        T recip_volume = 1.0 / AT(volume, i, j);
+        T energy_change = (AT(pressure, i, j) / AT(density0, i, j) + AT(viscosity, i, j) / AT(density0, i, j)) * total_flux * recip_volume;
         AT(energy1, i, j) = AT(energy0, i, j) - energy_change;
         AT(density1, i, j) = AT(density0, i, j) * volume_change_s;
       }
@@ -217,10 +275,11 @@ void orc_PdV(orc_chunk *c, int predict, double dt) {
 }
 
 /* src/serial/revert.cpp:29-41 */
-void orc_revert(orc_chunk *c) {
+template <class T>
+static void k_revert(View<T> *c) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
-  const double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0];
-  double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
+  const T *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0];
+  T *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
 #pragma omp parallel for if (ORC_PAR(c))
   for (int j = y_min + 1; j < y_max + 2; j++) {
     for (int i = x_min + 1; i < x_max + 2; i++) {
@@ -231,17 +290,18 @@ void orc_revert(orc_chunk *c) {
 }
 
 /* src/serial/accelerate.cpp:28-56.  Pressure then viscosity, two separate updates (:46-53). */
-void orc_accelerate(orc_chunk *c, double dt) {
+template <class T>
+static void k_accelerate(View<T> *c, double dt) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
-  const double *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA], *volume = c->f[ORC_VOLUME];
-  const double *density0 = c->f[ORC_DENSITY0], *pressure = c->f[ORC_PRESSURE], *viscosity = c->f[ORC_VISCOSITY];
-  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
-  double *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
-  double halfdt = 0.5 * dt;
+  const T *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA], *volume = c->f[ORC_VOLUME];
+  const T *density0 = c->f[ORC_DENSITY0], *pressure = c->f[ORC_PRESSURE], *viscosity = c->f[ORC_VISCOSITY];
+  const T *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  T *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+  T halfdt = 0.5 * dt;
 #pragma omp parallel for if (ORC_PAR(c))
   for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
     for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
-      double stepbymass_s = halfdt / ((AT(density0, i - 1, j - 1) * AT(volume, i - 1, j - 1) + AT(density0, i - 1, j + 0) * AT(volume, i - 1, j + 0) +
+      T stepbymass_s = halfdt / ((AT(density0, i - 1, j - 1) * AT(volume, i - 1, j - 1) + AT(density0, i - 1, j + 0) * AT(volume, i - 1, j + 0) +
                                        AT(density0, i, j) * AT(volume, i, j) + AT(density0, i + 0, j - 1) * AT(volume, i + 0, j - 1)) *
                                       0.25);
       AT(xvel1, i, j) = AT(xvel0, i, j) - stepbymass_s * (AT(xarea, i, j) * (AT(pressure, i, j) - AT(pressure, i - 1, j + 0)) +
@@ -257,12 +317,13 @@ void orc_accelerate(orc_chunk *c, double dt) {
 }
 
 /* src/serial/flux_calc.cpp:27-43 */
-void orc_flux_calc(orc_chunk *c, double dt) {
+template <class T>
+static void k_flux_calc(View<T> *c, double dt) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
-  const double *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA];
-  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
-  const double *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
-  double *vol_flux_x = c->f[ORC_VOL_FLUX_X], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
+  const T *xarea = c->f[ORC_XAREA], *yarea = c->f[ORC_YAREA];
+  const T *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  const T *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+  T *vol_flux_x = c->f[ORC_VOL_FLUX_X], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
 #pragma omp parallel for if (ORC_PAR(c))
   for (int j = y_min + 1; j < y_max + 1 + 2; j++) {
     for (int i = x_min + 1; i < x_max + 1 + 2; i++) {
@@ -275,13 +336,14 @@ void orc_flux_calc(orc_chunk *c, double dt) {
 /* One upwind limited flux of the van Leer remap, shared by both directions of
  * advec_cell (src/serial/advec_cell.cpp:86-110 for x, :178-202 for y).
  * a_* are density1 (or energy1) at upwind/donor/downwind. */
-static inline double orc_vl_limiter(double sigma, double sigma3, double sigma4, double uw, double dn, double dw) {
-  const double one_by_six = 1.0 / 6.0;
-  double diffuw = dn - uw;
-  double diffdw = dw - dn;
-  double wind = 1.0;
+template <class T>
+static inline T vl_limiter(T sigma, T sigma3, T sigma4, T uw, T dn, T dw) {
+  const T one_by_six = 1.0 / 6.0;
+  T diffuw = dn - uw;
+  T diffdw = dw - dn;
+  T wind = 1.0;
   if (diffdw <= 0.0) wind = -1.0;
-  double limiter;
+  T limiter;
   if (diffuw * diffdw > 0.0) {
     limiter = (1.0 - sigma) * wind * fmin(fmin(fabs(diffuw), fabs(diffdw)), one_by_six * (sigma3 * fabs(diffuw) + sigma4 * fabs(diffdw)));
   } else {
@@ -292,14 +354,15 @@ static inline double orc_vl_limiter(double sigma, double sigma3, double sigma4, 
 
 /* src/serial/advec_cell.cpp:28-220.  Work arrays as bound by the driver
  * (:228-231): pre_vol = work1, post_vol = work2, ener_flux = work7. */
-void orc_advec_cell(orc_chunk *c, int sweep_number, int dir) {
+template <class T>
+static void k_advec_cell(View<T> *c, int sweep_number, int dir) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
   const double *vertexdx = c->m[ORC_VERTEXDX], *vertexdy = c->m[ORC_VERTEXDY];
-  const double *volume = c->f[ORC_VOLUME];
-  const double *vol_flux_x = c->f[ORC_VOL_FLUX_X], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
-  double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
-  double *mass_flux_x = c->f[ORC_MASS_FLUX_X], *mass_flux_y = c->f[ORC_MASS_FLUX_Y];
-  double *pre_vol = c->f[ORC_WORK1], *post_vol = c->f[ORC_WORK2], *ener_flux = c->f[ORC_WORK7];
+  const T *volume = c->f[ORC_VOLUME];
+  const T *vol_flux_x = c->f[ORC_VOL_FLUX_X], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
+  T *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1];
+  T *mass_flux_x = c->f[ORC_MASS_FLUX_X], *mass_flux_y = c->f[ORC_MASS_FLUX_Y];
+  T *pre_vol = c->f[ORC_WORK1], *post_vol = c->f[ORC_WORK2], *ener_flux = c->f[ORC_WORK7];
 
   if (dir == 1) {
     if (sweep_number == 1) {
@@ -334,24 +397,24 @@ void orc_advec_cell(orc_chunk *c, int sweep_number, int dir) {
           downwind = i - 1;
           dif = upwind;
         }
-        double sigmat = fabs(AT(vol_flux_x, i, j)) / AT(pre_vol, donor, j);
-        double sigma3 = (1.0 + sigmat) * (vertexdx[i] / vertexdx[dif]);
-        double sigma4 = 2.0 - sigmat;
-        double sigmav = sigmat;
-        double limiter = orc_vl_limiter(sigmav, sigma3, sigma4, AT(density1, upwind, j), AT(density1, donor, j), AT(density1, downwind, j));
+        T sigmat = fabs(AT(vol_flux_x, i, j)) / AT(pre_vol, donor, j);
+        T sigma3 = (1.0 + sigmat) * (vertexdx[i] / vertexdx[dif]);
+        T sigma4 = 2.0 - sigmat;
+        T sigmav = sigmat;
+        T limiter = vl_limiter<T>(sigmav, sigma3, sigma4, AT(density1, upwind, j), AT(density1, donor, j), AT(density1, downwind, j));
         AT(mass_flux_x, i, j) = AT(vol_flux_x, i, j) * (AT(density1, donor, j) + limiter);
-        double sigmam = fabs(AT(mass_flux_x, i, j)) / (AT(density1, donor, j) * AT(pre_vol, donor, j));
-        limiter = orc_vl_limiter(sigmam, sigma3, sigma4, AT(energy1, upwind, j), AT(energy1, donor, j), AT(energy1, downwind, j));
+        T sigmam = fabs(AT(mass_flux_x, i, j)) / (AT(density1, donor, j) * AT(pre_vol, donor, j));
+        limiter = vl_limiter<T>(sigmam, sigma3, sigma4, AT(energy1, upwind, j), AT(energy1, donor, j), AT(energy1, downwind, j));
         AT(ener_flux, i, j) = AT(mass_flux_x, i, j) * (AT(energy1, donor, j) + limiter);
       }
     }
 #pragma omp parallel for if (ORC_PAR(c))
     for (int j = y_min + 1; j < y_max + 2; j++) {
       for (int i = x_min + 1; i < x_max + 2; i++) {
-        double pre_mass_s = AT(density1, i, j) * AT(pre_vol, i, j);
-        double post_mass_s = pre_mass_s + AT(mass_flux_x, i, j) - AT(mass_flux_x, i + 1, j + 0);
-        double post_ener_s = (AT(energy1, i, j) * pre_mass_s + AT(ener_flux, i, j) - AT(ener_flux, i + 1, j + 0)) / post_mass_s;
-        double advec_vol_s = AT(pre_vol, i, j) + AT(vol_flux_x, i, j) - AT(vol_flux_x, i + 1, j + 0);
+        T pre_mass_s = AT(density1, i, j) * AT(pre_vol, i, j);
+        T post_mass_s = pre_mass_s + AT(mass_flux_x, i, j) - AT(mass_flux_x, i + 1, j + 0);
+        T post_ener_s = (AT(energy1, i, j) * pre_mass_s + AT(ener_flux, i, j) - AT(ener_flux, i + 1, j + 0)) / post_mass_s;
+        T advec_vol_s = AT(pre_vol, i, j) + AT(vol_flux_x, i, j) - AT(vol_flux_x, i + 1, j + 0);
         AT(density1, i, j) = post_mass_s / advec_vol_s;
         AT(energy1, i, j) = post_ener_s;
       }
@@ -389,24 +452,24 @@ void orc_advec_cell(orc_chunk *c, int sweep_number, int dir) {
           downwind = j - 1;
           dif = upwind;
         }
-        double sigmat = fabs(AT(vol_flux_y, i, j)) / AT(pre_vol, i, donor);
-        double sigma3 = (1.0 + sigmat) * (vertexdy[j] / vertexdy[dif]);
-        double sigma4 = 2.0 - sigmat;
-        double sigmav = sigmat;
-        double limiter = orc_vl_limiter(sigmav, sigma3, sigma4, AT(density1, i, upwind), AT(density1, i, donor), AT(density1, i, downwind));
+        T sigmat = fabs(AT(vol_flux_y, i, j)) / AT(pre_vol, i, donor);
+        T sigma3 = (1.0 + sigmat) * (vertexdy[j] / vertexdy[dif]);
+        T sigma4 = 2.0 - sigmat;
+        T sigmav = sigmat;
+        T limiter = vl_limiter<T>(sigmav, sigma3, sigma4, AT(density1, i, upwind), AT(density1, i, donor), AT(density1, i, downwind));
         AT(mass_flux_y, i, j) = AT(vol_flux_y, i, j) * (AT(density1, i, donor) + limiter);
-        double sigmam = fabs(AT(mass_flux_y, i, j)) / (AT(density1, i, donor) * AT(pre_vol, i, donor));
-        limiter = orc_vl_limiter(sigmam, sigma3, sigma4, AT(energy1, i, upwind), AT(energy1, i, donor), AT(energy1, i, downwind));
+        T sigmam = fabs(AT(mass_flux_y, i, j)) / (AT(density1, i, donor) * AT(pre_vol, i, donor));
+        limiter = vl_limiter<T>(sigmam, sigma3, sigma4, AT(energy1, i, upwind), AT(energy1, i, donor), AT(energy1, i, downwind));
         AT(ener_flux, i, j) = AT(mass_flux_y, i, j) * (AT(energy1, i, donor) + limiter);
       }
     }
 #pragma omp parallel for if (ORC_PAR(c))
     for (int j = y_min + 1; j < y_max + 2; j++) {
       for (int i = x_min + 1; i < x_max + 2; i++) {
-        double pre_mass_s = AT(density1, i, j) * AT(pre_vol, i, j);
-        double post_mass_s = pre_mass_s + AT(mass_flux_y, i, j) - AT(mass_flux_y, i + 0, j + 1);
-        double post_ener_s = (AT(energy1, i, j) * pre_mass_s + AT(ener_flux, i, j) - AT(ener_flux, i + 0, j + 1)) / post_mass_s;
-        double advec_vol_s = AT(pre_vol, i, j) + AT(vol_flux_y, i, j) - AT(vol_flux_y, i + 0, j + 1);
+        T pre_mass_s = AT(density1, i, j) * AT(pre_vol, i, j);
+        T post_mass_s = pre_mass_s + AT(mass_flux_y, i, j) - AT(mass_flux_y, i + 0, j + 1);
+        T post_ener_s = (AT(energy1, i, j) * pre_mass_s + AT(ener_flux, i, j) - AT(ener_flux, i + 0, j + 1)) / post_mass_s;
+        T advec_vol_s = AT(pre_vol, i, j) + AT(vol_flux_y, i, j) - AT(vol_flux_y, i + 0, j + 1);
         AT(density1, i, j) = post_mass_s / advec_vol_s;
         AT(energy1, i, j) = post_ener_s;
       }
@@ -415,33 +478,35 @@ void orc_advec_cell(orc_chunk *c, int sweep_number, int dir) {
 }
 
 /* Momentum limiter of src/serial/advec_mom.cpp:125-139 (x) / :197-211 (y). */
-static inline double orc_mom_flux(double flux, double mass_pre_donor, double width, double width_dif, double v_uw, double v_dn, double v_dw) {
-  double sigma = fabs(flux) / (mass_pre_donor);
-  double vdiffuw = v_dn - v_uw;
-  double vdiffdw = v_dw - v_dn;
-  double limiter = 0.0;
+template <class T>
+static inline T mom_flux_of(T flux, T mass_pre_donor, T width, T width_dif, T v_uw, T v_dn, T v_dw) {
+  T sigma = fabs(flux) / (mass_pre_donor);
+  T vdiffuw = v_dn - v_uw;
+  T vdiffdw = v_dw - v_dn;
+  T limiter = 0.0;
   if (vdiffuw * vdiffdw > 0.0) {
-    double auw = fabs(vdiffuw);
-    double adw = fabs(vdiffdw);
-    double wind = 1.0;
+    T auw = fabs(vdiffuw);
+    T adw = fabs(vdiffdw);
+    T wind = 1.0;
     if (vdiffdw <= 0.0) wind = -1.0;
     limiter = wind * fmin(fmin(width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / width_dif) / 6.0, auw), adw);
   }
-  double advec_vel_s = v_dn + (1.0 - sigma) * limiter;
+  T advec_vel_s = v_dn + (1.0 - sigma) * limiter;
   return advec_vel_s * flux;
 }
 
 /* src/serial/advec_mom.cpp:30-225.  Work arrays as bound by the driver
  * (:234-242): node_flux = work1, node_mass_post = work2, node_mass_pre = work3,
  * mom_flux = work4, pre_vol = work5, post_vol = work6. */
-void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction) {
+template <class T>
+static void k_advec_mom(View<T> *c, int which_vel, int sweep_number, int direction) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
-  double *vel1 = c->f[which_vel == 1 ? ORC_XVEL1 : ORC_YVEL1];
-  const double *mass_flux_x = c->f[ORC_MASS_FLUX_X], *vol_flux_x = c->f[ORC_VOL_FLUX_X];
-  const double *mass_flux_y = c->f[ORC_MASS_FLUX_Y], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
-  const double *volume = c->f[ORC_VOLUME], *density1 = c->f[ORC_DENSITY1];
-  double *node_flux = c->f[ORC_WORK1], *node_mass_post = c->f[ORC_WORK2], *node_mass_pre = c->f[ORC_WORK3];
-  double *mom_flux = c->f[ORC_WORK4], *pre_vol = c->f[ORC_WORK5], *post_vol = c->f[ORC_WORK6];
+  T *vel1 = c->f[which_vel == 1 ? ORC_XVEL1 : ORC_YVEL1];
+  const T *mass_flux_x = c->f[ORC_MASS_FLUX_X], *vol_flux_x = c->f[ORC_VOL_FLUX_X];
+  const T *mass_flux_y = c->f[ORC_MASS_FLUX_Y], *vol_flux_y = c->f[ORC_VOL_FLUX_Y];
+  const T *volume = c->f[ORC_VOLUME], *density1 = c->f[ORC_DENSITY1];
+  T *node_flux = c->f[ORC_WORK1], *node_mass_post = c->f[ORC_WORK2], *node_mass_pre = c->f[ORC_WORK3];
+  T *mom_flux = c->f[ORC_WORK4], *pre_vol = c->f[ORC_WORK5], *post_vol = c->f[ORC_WORK6];
   const double *celldx = c->m[ORC_CELLDX], *celldy = c->m[ORC_CELLDY];
 
   int mom_sweep = direction + 2 * (sweep_number - 1);
@@ -496,7 +561,7 @@ void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction)
           downwind = i + 1;
           dif = upwind;
         }
-        AT(mom_flux, i, j) = orc_mom_flux(AT(node_flux, i, j), AT(node_mass_pre, donor, j), celldx[i], celldx[dif], AT(vel1, upwind, j),
+        AT(mom_flux, i, j) = mom_flux_of<T>(AT(node_flux, i, j), AT(node_mass_pre, donor, j), celldx[i], celldx[dif], AT(vel1, upwind, j),
                                           AT(vel1, donor, j), AT(vel1, downwind, j));
       }
     }
@@ -538,7 +603,7 @@ void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction)
           downwind = j + 1;
           dif = upwind;
         }
-        AT(mom_flux, i, j) = orc_mom_flux(AT(node_flux, i, j), AT(node_mass_pre, i, donor), celldy[j], celldy[dif], AT(vel1, i, upwind),
+        AT(mom_flux, i, j) = mom_flux_of<T>(AT(node_flux, i, j), AT(node_mass_pre, i, donor), celldy[j], celldy[dif], AT(vel1, i, upwind),
                                           AT(vel1, i, donor), AT(vel1, i, downwind));
       }
     }
@@ -552,10 +617,11 @@ void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction)
 }
 
 /* src/serial/reset_field.cpp:28-51 */
-void orc_reset_field(orc_chunk *c) {
+template <class T>
+static void k_reset_field(View<T> *c) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
-  double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0], *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
-  const double *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1], *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
+  T *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0], *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0];
+  const T *density1 = c->f[ORC_DENSITY1], *energy1 = c->f[ORC_ENERGY1], *xvel1 = c->f[ORC_XVEL1], *yvel1 = c->f[ORC_YVEL1];
 #pragma omp parallel for if (ORC_PAR(c))
   for (int j = y_min + 1; j < y_max + 2; j++) {
     for (int i = x_min + 1; i < x_max + 2; i++) {
@@ -605,12 +671,13 @@ static const orc_halo_rule ORC_HALO_RULES[15] = {
     /* mass_flux_y*/ {3, 3, 1, 3, 2, 0, 0, 1, -1.0, 1.0},
 };
 
-void orc_update_halo(orc_chunk *c, const int fields[15], int depth) {
+template <class T>
+static void k_update_halo(View<T> *c, const int fields[15], int depth) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
   for (int f = 0; f < 15; ++f) {
     if (fields[f] != 1) continue;
     const orc_halo_rule *r = &ORC_HALO_RULES[f];
-    double *a = c->f[f];
+    T *a = c->f[f];
     /* bottom then top: j over x_min-depth .. x_max(+1)+depth */
     for (int j = x_min - depth + 1; j < x_max + r->xext + depth + 2; j++)
       for (int k = 0; k < depth; ++k) AT(a, j, 1 - k) = r->sy * AT(a, j, r->bsrc + k);
@@ -625,22 +692,23 @@ void orc_update_halo(orc_chunk *c, const int fields[15], int depth) {
 }
 
 /* src/serial/field_summary.cpp:60-92, serial order (linear index, x fastest). */
-void orc_field_summary_kernel(orc_chunk *c, double out[5]) {
+template <class T>
+static void k_field_summary_kernel(View<T> *c, T out[5]) {
   const int px = c->px, xmin = 1, ymin = 1, xmax = c->nx, ymax = c->ny;
-  const double *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0], *volume = c->f[ORC_VOLUME];
-  const double *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0], *pressure = c->f[ORC_PRESSURE];
-  double vol = 0.0, mass = 0.0, ie = 0.0, ke = 0.0, press = 0.0;
+  const T *xvel0 = c->f[ORC_XVEL0], *yvel0 = c->f[ORC_YVEL0], *volume = c->f[ORC_VOLUME];
+  const T *density0 = c->f[ORC_DENSITY0], *energy0 = c->f[ORC_ENERGY0], *pressure = c->f[ORC_PRESSURE];
+  T vol = 0.0, mass = 0.0, ie = 0.0, ke = 0.0, press = 0.0;
   for (int idx = 0; idx < ((ymax - ymin + 1) * (xmax - xmin + 1)); idx++) {
     const int j = xmin + 1 + idx % (xmax - xmin + 1);
     const int k = ymin + 1 + idx / (xmax - xmin + 1);
-    double vsqrd = 0.0;
+    T vsqrd = 0.0;
     for (int kv = k; kv <= k + 1; ++kv) {
       for (int jv = j; jv <= j + 1; ++jv) {
         vsqrd += 0.25 * (AT(xvel0, jv, kv) * AT(xvel0, jv, kv) + AT(yvel0, jv, kv) * AT(yvel0, jv, kv));
       }
     }
-    double cell_vol = AT(volume, j, k);
-    double cell_mass = cell_vol * AT(density0, j, k);
+    T cell_vol = AT(volume, j, k);
+    T cell_mass = cell_vol * AT(density0, j, k);
     vol += cell_vol;
     mass += cell_mass;
     ie += cell_mass * AT(energy0, j, k);
@@ -803,36 +871,38 @@ int orc_timestep(orc_chunk *c, const orc_config *cfg, orc_globals *g) {
 }
 
 /* src/serial/PdV.cpp:92-133 */
-void orc_PdV_driver(orc_chunk *c, int predict, double dt) {
-  orc_PdV(c, predict, dt);
+template <class T>
+static void k_PdV_driver(View<T> *c, int predict, double dt) {
+  k_PdV(c, predict, dt);
   if (predict) {
-    orc_ideal_gas(c, 1);
+    k_ideal_gas(c, 1);
     int fields[15];
     const int ids[] = {ORC_PRESSURE};
     orc_mask(fields, ids, 1);
-    orc_update_halo(c, fields, 1);
-    orc_revert(c);
+    k_update_halo(c, fields, 1);
+    k_revert(c);
   }
 }
 
 /* driver/advection.cpp:29-105 */
-void orc_advection(orc_chunk *c, int advect_x) {
+template <class T>
+static void k_advection(View<T> *c, int advect_x) {
   int fields[15];
   int direction = advect_x ? 1 : 2;
   const int ids1[] = {ORC_ENERGY1, ORC_DENSITY1, ORC_VOL_FLUX_X, ORC_VOL_FLUX_Y};
   orc_mask(fields, ids1, 4);
-  orc_update_halo(c, fields, 2);
-  orc_advec_cell(c, 1, direction);
+  k_update_halo(c, fields, 2);
+  k_advec_cell(c, 1, direction);
   const int ids2[] = {ORC_DENSITY1, ORC_ENERGY1, ORC_XVEL1, ORC_YVEL1, ORC_MASS_FLUX_X, ORC_MASS_FLUX_Y};
   orc_mask(fields, ids2, 6);
-  orc_update_halo(c, fields, 2);
-  orc_advec_mom(c, 1, 1, direction);
-  orc_advec_mom(c, 2, 1, direction);
+  k_update_halo(c, fields, 2);
+  k_advec_mom(c, 1, 1, direction);
+  k_advec_mom(c, 2, 1, direction);
   direction = advect_x ? 2 : 1;
-  orc_advec_cell(c, 2, direction);
-  orc_update_halo(c, fields, 2);
-  orc_advec_mom(c, 1, 2, direction);
-  orc_advec_mom(c, 2, 2, direction);
+  k_advec_cell(c, 2, direction);
+  k_update_halo(c, fields, 2);
+  k_advec_mom(c, 1, 2, direction);
+  k_advec_mom(c, 2, 2, direction);
 }
 
 /* driver/hydro.cpp:57-89 */
@@ -850,3 +920,103 @@ int orc_hydro_step(orc_chunk *c, const orc_config *cfg, orc_globals *g) {
   g->time += g->dt;
   return 0;
 }
+
+/* ---- fixed-dt step ------------------------------------------------------------------------
+ * driver/timestep.cpp:47-69 without calc_dt (the step's dt is given), then
+ * driver/hydro.cpp:63-89.  The same sequence as orc_hydro_step whenever calc_dt
+ * would not limit dt. */
+namespace {
+template <class T>
+void k_step_fixed(View<T> *c, double dt, int advect_x) {
+  int fields[15];
+  k_ideal_gas(c, 0);
+  const int ids1[] = {ORC_PRESSURE, ORC_ENERGY0, ORC_DENSITY0, ORC_XVEL0, ORC_YVEL0};
+  orc_mask(fields, ids1, 5);
+  k_update_halo(c, fields, 1);
+  k_viscosity(c);
+  const int ids2[] = {ORC_VISCOSITY};
+  orc_mask(fields, ids2, 1);
+  k_update_halo(c, fields, 1);
+  k_PdV_driver(c, 1, dt);
+  k_accelerate(c, dt);
+  k_PdV_driver(c, 0, dt);
+  k_flux_calc(c, dt);
+  k_advection(c, advect_x);
+  k_reset_field(c);
+}
+}  // namespace
+
+/* ---- C entry points: double (the oracle) ------------------------------------------------ */
+void orc_ideal_gas(orc_chunk *c, int predict) { k_ideal_gas(V(c), predict); }
+void orc_viscosity(orc_chunk *c) { k_viscosity(V(c)); }
+void orc_PdV(orc_chunk *c, int predict, double dt) { k_PdV(V(c), predict, dt); }
+void orc_revert(orc_chunk *c) { k_revert(V(c)); }
+void orc_accelerate(orc_chunk *c, double dt) { k_accelerate(V(c), dt); }
+void orc_flux_calc(orc_chunk *c, double dt) { k_flux_calc(V(c), dt); }
+void orc_advec_cell(orc_chunk *c, int sweep_number, int dir) { k_advec_cell(V(c), sweep_number, dir); }
+void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction) { k_advec_mom(V(c), which_vel, sweep_number, direction); }
+void orc_reset_field(orc_chunk *c) { k_reset_field(V(c)); }
+void orc_update_halo(orc_chunk *c, const int fields[15], int depth) { k_update_halo(V(c), fields, depth); }
+void orc_field_summary_kernel(orc_chunk *c, double out[5]) { k_field_summary_kernel(V(c), out); }
+void orc_PdV_driver(orc_chunk *c, int predict, double dt) { k_PdV_driver(V(c), predict, dt); }
+void orc_advection(orc_chunk *c, int advect_x) { k_advection(V(c), advect_x); }
+void orc_step_fixed(orc_chunk *c, double dt, int advect_x) { k_step_fixed(V(c), dt, advect_x); }
+
+/* ---- C entry points: the tangent-linear model ------------------------------------------- */
+int orc_tl_alloc(orc_tl_chunk *t, int nx, int ny) {
+  memset(t, 0, sizeof(*t));
+  t->nx = nx;
+  t->ny = ny;
+  t->px = nx + 5;
+  const size_t n2 = (size_t)(nx + 5) * (size_t)(ny + 5);
+  for (int a = 0; a < ORC_NUM_2D; ++a) {
+    t->f[a] = (orc_dual *)calloc(n2, sizeof(orc_dual));
+    if (!t->f[a]) return -1;
+  }
+  for (int a = 0; a < ORC_NUM_1D; ++a) {
+    const int n = (a == ORC_CELLX || a == ORC_VERTEXX || a == ORC_CELLDX || a == ORC_VERTEXDX) ? nx + 5 : ny + 5;
+    t->m[a] = (double *)calloc((size_t)n, sizeof(double));
+    if (!t->m[a]) return -1;
+  }
+  return 0;
+}
+
+void orc_tl_free(orc_tl_chunk *t) {
+  for (int a = 0; a < ORC_NUM_2D; ++a) free(t->f[a]);
+  for (int a = 0; a < ORC_NUM_1D; ++a) free(t->m[a]);
+  memset(t, 0, sizeof(*t));
+}
+
+void orc_tl_load(orc_tl_chunk *t, const orc_chunk *c) {
+  const size_t n2 = (size_t)(c->nx + 5) * (size_t)(c->ny + 5);
+  for (int a = 0; a < ORC_NUM_2D; ++a)
+    for (size_t k = 0; k < n2; ++k) {
+      t->f[a][k].v = c->f[a][k];
+      t->f[a][k].d = 0.0;
+    }
+  for (int a = 0; a < ORC_NUM_1D; ++a) {
+    const int n = (a == ORC_CELLX || a == ORC_VERTEXX || a == ORC_CELLDX || a == ORC_VERTEXDX) ? c->nx + 5 : c->ny + 5;
+    memcpy(t->m[a], c->m[a], (size_t)n * sizeof(double));
+  }
+}
+
+void orc_tl_ideal_gas(orc_tl_chunk *t, int predict) { k_ideal_gas(V(t), predict); }
+void orc_tl_viscosity(orc_tl_chunk *t) { k_viscosity(V(t)); }
+void orc_tl_PdV(orc_tl_chunk *t, int predict, double dt) { k_PdV(V(t), predict, dt); }
+void orc_tl_revert(orc_tl_chunk *t) { k_revert(V(t)); }
+void orc_tl_accelerate(orc_tl_chunk *t, double dt) { k_accelerate(V(t), dt); }
+void orc_tl_flux_calc(orc_tl_chunk *t, double dt) { k_flux_calc(V(t), dt); }
+void orc_tl_advec_cell(orc_tl_chunk *t, int sweep_number, int dir) { k_advec_cell(V(t), sweep_number, dir); }
+void orc_tl_advec_mom(orc_tl_chunk *t, int which_vel, int sweep_number, int direction) {
+  k_advec_mom(V(t), which_vel, sweep_number, direction);
+}
+void orc_tl_reset_field(orc_tl_chunk *t) { k_reset_field(V(t)); }
+void orc_tl_update_halo(orc_tl_chunk *t, const int fields[15], int depth) { k_update_halo(V(t), fields, depth); }
+void orc_tl_field_summary_kernel(orc_tl_chunk *t, orc_dual out[5]) { k_field_summary_kernel(V(t), reinterpret_cast<Dual *>(out)); }
+void orc_tl_field_summary(orc_tl_chunk *t, orc_dual out[5]) {
+  k_ideal_gas(V(t), 0);
+  k_field_summary_kernel(V(t), reinterpret_cast<Dual *>(out));
+}
+void orc_tl_PdV_driver(orc_tl_chunk *t, int predict, double dt) { k_PdV_driver(V(t), predict, dt); }
+void orc_tl_advection(orc_tl_chunk *t, int advect_x) { k_advection(V(t), advect_x); }
+void orc_tl_step_fixed(orc_tl_chunk *t, double dt, int advect_x) { k_step_fixed(V(t), dt, advect_x); }

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the CloverLeaf hydro hot path -- TEST INFRASTRUCTURE ONLY.
  *
- * A plain-C restatement of the reference's serial model (UoB-HPC/CloverLeaf,
+ * A C/C++ restatement (C interface) of the reference's serial model (UoB-HPC/CloverLeaf,
  * src/serial/<kernel>.cpp) and of its step sequencing (driver/hydro.cpp,
  * driver/timestep.cpp, driver/advection.cpp, driver/start.cpp).  Each function
  * cites the reference file:line it follows and keeps the reference's
@@ -120,6 +120,44 @@ void orc_advection(orc_chunk *c, int advect_x);
 int orc_hydro_step(orc_chunk *c, const orc_config *cfg, orc_globals *g);
 /* src/serial/field_summary.cpp:44-104: ideal_gas then the five sums. */
 void orc_field_summary(orc_chunk *c, double out[5]);
+/* One step with a given dt: driver/timestep.cpp:47-69 without calc_dt, then
+ * driver/hydro.cpp:63-89.  Equal to orc_hydro_step whenever calc_dt would not
+ * limit dt (e.g. dtc/u/v/div_safe large, dtinit = dtmax = dt). */
+void orc_step_fixed(orc_chunk *c, double dt, int advect_x);
+
+/* ---- tangent-linear model ----------------------------------------------------------------
+ * The same kernels on forward-mode dual numbers: every 2D array element holds a
+ * value v and a directional derivative d.  Branches follow the value; at a tie
+ * fmin/fmax take their first argument and fabs(0) has derivative +1.
+ * Pinned: values bit-identical to the oracle's; derivatives against central
+ * finite differences of the oracle (tests/test_oracle_tl.py). */
+typedef struct orc_dual {
+  double v, d;
+} orc_dual;
+typedef struct orc_tl_chunk {
+  int nx, ny, px;
+  orc_dual *f[ORC_NUM_2D];
+  double *m[ORC_NUM_1D];
+} orc_tl_chunk;
+int orc_tl_alloc(orc_tl_chunk *t, int nx, int ny);
+void orc_tl_free(orc_tl_chunk *t);
+/* values and mesh from c, derivatives zero */
+void orc_tl_load(orc_tl_chunk *t, const orc_chunk *c);
+void orc_tl_ideal_gas(orc_tl_chunk *t, int predict);
+void orc_tl_viscosity(orc_tl_chunk *t);
+void orc_tl_PdV(orc_tl_chunk *t, int predict, double dt);
+void orc_tl_revert(orc_tl_chunk *t);
+void orc_tl_accelerate(orc_tl_chunk *t, double dt);
+void orc_tl_flux_calc(orc_tl_chunk *t, double dt);
+void orc_tl_advec_cell(orc_tl_chunk *t, int sweep_number, int dir);
+void orc_tl_advec_mom(orc_tl_chunk *t, int which_vel, int sweep_number, int direction);
+void orc_tl_reset_field(orc_tl_chunk *t);
+void orc_tl_update_halo(orc_tl_chunk *t, const int fields[15], int depth);
+void orc_tl_field_summary_kernel(orc_tl_chunk *t, orc_dual out[5]);
+void orc_tl_field_summary(orc_tl_chunk *t, orc_dual out[5]);
+void orc_tl_PdV_driver(orc_tl_chunk *t, int predict, double dt);
+void orc_tl_advection(orc_tl_chunk *t, int advect_x);
+void orc_tl_step_fixed(orc_tl_chunk *t, double dt, int advect_x);
 
 #ifdef __cplusplus
 }

@@ -356,3 +356,104 @@ def random_chunk_arrays(nx, ny, seed, positive=("density0", "density1", "energy0
 def golden(name: str):
     import json
     return json.loads((GOLDEN / f"{name}.json").read_text())
+
+
+# --------------------------------------------------------------------------------------------
+# the oracle's tangent-linear model (orc_tl_*): forward-mode dual numbers
+class _OrcDual(C.Structure):
+    _fields_ = [("v", C.c_double), ("d", C.c_double)]
+
+
+class _OrcTLChunk(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("px", C.c_int),
+                ("f", C.POINTER(_OrcDual) * 25), ("m", C.POINTER(C.c_double) * 8)]
+
+
+_TL_SIGS = {
+    "ideal_gas": [C.c_int], "viscosity": [], "PdV": [C.c_int, C.c_double], "revert": [],
+    "accelerate": [C.c_double], "flux_calc": [C.c_double], "advec_cell": [C.c_int, C.c_int],
+    "advec_mom": [C.c_int, C.c_int, C.c_int], "reset_field": [], "PdV_driver": [C.c_int, C.c_double],
+    "advection": [C.c_int], "step_fixed": [C.c_double, C.c_int],
+}
+
+
+def _tl_lib():
+    L = oracle_lib()
+    if not getattr(L, "_tl_ready", False):
+        P = C.POINTER
+        L.orc_tl_alloc.argtypes = [P(_OrcTLChunk), C.c_int, C.c_int]
+        L.orc_tl_free.argtypes = [P(_OrcTLChunk)]
+        L.orc_tl_load.argtypes = [P(_OrcTLChunk), P(_OrcChunk)]
+        L.orc_tl_update_halo.argtypes = [P(_OrcTLChunk), P(C.c_int), C.c_int]
+        L.orc_tl_field_summary.argtypes = [P(_OrcTLChunk), P(_OrcDual)]
+        L.orc_tl_field_summary_kernel.argtypes = [P(_OrcTLChunk), P(_OrcDual)]
+        L.orc_step_fixed.argtypes = [P(_OrcChunk), C.c_double, C.c_int]
+        for k, args in _TL_SIGS.items():
+            getattr(L, f"orc_tl_{k}").argtypes = [P(_OrcTLChunk)] + args
+        L._tl_ready = True
+    return L
+
+
+class TLOracle:
+    """A chunk of the oracle's tangent-linear model.  Array `name` is exposed as a
+    (ny+5, nx+5, 2) view: [..., 0] the value, [..., 1] the directional derivative."""
+
+    def __init__(self, nx, ny):
+        self.L = _tl_lib()
+        self.nx, self.ny = nx, ny
+        self.c = _OrcTLChunk()
+        if self.L.orc_tl_alloc(C.byref(self.c), nx, ny) != 0:
+            raise MemoryError("tangent-linear oracle allocation failed")
+
+    @classmethod
+    def from_oracle(cls, o: "Oracle") -> "TLOracle":
+        t = cls(o.nx, o.ny)
+        t.L.orc_tl_load(C.byref(t.c), C.byref(o.c))
+        return t
+
+    def __del__(self):
+        try:
+            self.L.orc_tl_free(C.byref(self.c))
+        except Exception:
+            pass
+
+    def a(self, name) -> np.ndarray:
+        n = (self.ny + 5) * (self.nx + 5)
+        return np.ctypeslib.as_array(C.cast(self.c.f[FID[name]], C.POINTER(C.c_double)),
+                                     shape=(2 * n,)).reshape(self.ny + 5, self.nx + 5, 2)
+
+    def val(self, name) -> np.ndarray:
+        return self.a(name)[..., 0]
+
+    def der(self, name) -> np.ndarray:
+        return self.a(name)[..., 1]
+
+    def m(self, name) -> np.ndarray:
+        n = self.nx + 5 if name in ("cellx", "vertexx", "celldx", "vertexdx") else self.ny + 5
+        return np.ctypeslib.as_array(self.c.m[MID[name]], shape=(n,))
+
+    def run(self, kernel, *args):
+        getattr(self.L, f"orc_tl_{kernel}")(C.byref(self.c), *args)
+
+    def update_halo(self, names, depth):
+        self.L.orc_tl_update_halo(C.byref(self.c), mask(names), depth)
+
+    def summary(self, with_ideal_gas=True):
+        out = (_OrcDual * 5)()
+        fn = self.L.orc_tl_field_summary if with_ideal_gas else self.L.orc_tl_field_summary_kernel
+        fn(C.byref(self.c), out)
+        return [(x.v, x.d) for x in out]
+
+
+def oracle_on_arrays(nx, ny, f2d, m1d) -> "Oracle":
+    """An oracle chunk holding the given arrays (for per-kernel checks)."""
+    o = Oracle(Deck(x_cells=nx, y_cells=ny, states=[State(density=1, energy=1)]))
+    for n in FIELDS:
+        o.f(n)[...] = f2d[n]
+    for n in MESH:
+        o.m(n)[...] = m1d[n]
+    return o
+
+
+def oracle_step_fixed(o: "Oracle", dt: float, advect_x: int):
+    _tl_lib().orc_step_fixed(C.byref(o.c), dt, advect_x)
