@@ -153,6 +153,24 @@ SIGNATURES = [
     ("clv_batch_run", _I, [_P, _I, _I, _PI]),
     ("clv_batch_states", _I, [_P, C.POINTER(HydroState)]),
     ("clv_batch_dt_history", _I, [_P, _I, _PD, _I, _PI]),
+    ("clv_adj_ideal_gas", _I, [_P, _P, _I]),
+    ("clv_adj_viscosity", _I, [_P, _P]),
+    ("clv_adj_PdV", _I, [_P, _P, _I, _D]),
+    ("clv_adj_revert", _I, [_P, _P]),
+    ("clv_adj_accelerate", _I, [_P, _P, _D]),
+    ("clv_adj_flux_calc", _I, [_P, _P, _D]),
+    ("clv_adj_advec_cell", _I, [_P, _P, _I, _I]),
+    ("clv_adj_advec_mom", _I, [_P, _P, _I, _I]),
+    ("clv_adj_reset_field", _I, [_P, _P]),
+    ("clv_adj_update_halo", _I, [_P, _PI, _I]),
+    ("clv_adj_kinetic_energy", _I, [_P, _P, _D]),
+    ("clv_adjoint_create", _I, [_P, _D, _I, _I, C.POINTER(_P)]),
+    ("clv_adjoint_destroy", _I, [_P]),
+    ("clv_adjoint_bytes", C.c_size_t, [_P]),
+    ("clv_adjoint_set_initial", _I, [_P, _P, _P, C.c_int64]),
+    ("clv_adjoint_forward", _I, [_P, _PD]),
+    ("clv_adjoint_backward", _I, [_P, _D, _P, _P, C.c_int64]),
+    ("clv_adjoint_chunks", _I, [_P, C.POINTER(_P), C.POINTER(_P)]),
 ]
 
 
@@ -446,3 +464,73 @@ class Batch:
         _check(self.L.clv_batch_dt_history(self.h, int(member), a.ctypes.data_as(_PD), int(max_steps), C.byref(k)),
                "clv_batch_dt_history")
         return a[:k.value].copy()
+
+
+def adjoint_kernel(name: str, primal: Chunk, adjoint: Chunk, *args):
+    """Run the adjoint of one reference kernel (clv_adj_<name>): primal holds the
+    forward kernel's input state, adjoint the output adjoints (in) / input
+    adjoints (out)."""
+    L = lib()
+    if name == "update_halo":
+        fields, depth = args
+        _check(L.clv_adj_update_halo(adjoint.h, _mask(fields), int(depth)), "clv_adj_update_halo")
+        return
+    _check(getattr(L, f"clv_adj_{name}")(primal.h, adjoint.h, *args), f"clv_adj_{name}")
+
+
+class Adjoint:
+    """Checkpointed gradient of the final kinetic energy of a fixed-dt run with
+    respect to the initial density0 / energy0 interior fields (clv_adjoint_*)."""
+
+    def __init__(self, initial: Chunk, dt: float, steps: int, checkpoint_every: int = 20):
+        self.L = lib()
+        self.nx, self.ny = initial.nx, initial.ny
+        h = C.c_void_p()
+        _check(self.L.clv_adjoint_create(initial.h, float(dt), int(steps), int(checkpoint_every), C.byref(h)), "clv_adjoint_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.clv_adjoint_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def bytes(self) -> int:
+        return int(self.L.clv_adjoint_bytes(self.h))
+
+    def set_initial(self, density0, energy0, pitch=None):
+        """density0 / energy0: (ny, nx) float64 numpy arrays, or device pointers (ints) with a row pitch."""
+        if isinstance(density0, np.ndarray):
+            d = np.ascontiguousarray(density0, dtype=np.float64)
+            e = np.ascontiguousarray(energy0, dtype=np.float64)
+            assert d.shape == e.shape == (self.ny, self.nx)
+            _check(self.L.clv_adjoint_set_initial(self.h, d.ctypes.data, e.ctypes.data, self.nx), "clv_adjoint_set_initial")
+        else:
+            _check(self.L.clv_adjoint_set_initial(self.h, int(density0), int(energy0), int(pitch)), "clv_adjoint_set_initial")
+
+    def forward(self) -> list:
+        out = (C.c_double * 5)()
+        _check(self.L.clv_adjoint_forward(self.h, out), "clv_adjoint_forward")
+        return list(out)
+
+    def backward(self, ke_bar: float = 1.0, grad_density0=None, grad_energy0=None, pitch=None):
+        """Host numpy gradients by default; or device pointers (ints) with a row pitch."""
+        if grad_density0 is None and grad_energy0 is None:
+            gd = np.empty((self.ny, self.nx))
+            ge = np.empty((self.ny, self.nx))
+            _check(self.L.clv_adjoint_backward(self.h, float(ke_bar), gd.ctypes.data, ge.ctypes.data, self.nx), "clv_adjoint_backward")
+            return gd, ge
+        _check(self.L.clv_adjoint_backward(self.h, float(ke_bar), int(grad_density0 or 0), int(grad_energy0 or 0), int(pitch)),
+               "clv_adjoint_backward")
+        return None

@@ -286,6 +286,57 @@ clv_status clv_batch_states(clv_batch *b, clv_hydro_state *out);
 /* dt of steps 1 .. *count of member `member` (at most max values). */
 clv_status clv_batch_dt_history(clv_batch *b, int member, double *dt, int max, int *count);
 
+/* ---- adjoint form ----------------------------------------------------------------------- */
+/* No counterpart in the reference (it has no derivatives).  Reverse-mode
+ * adjoints of the reference kernels, for gradients of a fixed-dt run (dt given,
+ * calc_dt not run).  p holds the forward kernel's INPUT state, a (same size and
+ * pitch) the adjoints: on entry the adjoints of the kernel's outputs (plus what
+ * has accumulated on its inputs), on return the inputs' adjoints accumulated and
+ * the overwritten outputs' adjoints cleared.  Piecewise definitions (upwind
+ * choice, van Leer limiters, viscosity switch, fmin/fmax, fabs) are
+ * differentiated on the branch the forward took.  Geometry is constant;
+ * soundspeed (read only by calc_dt) carries no adjoint; a's work arrays are
+ * scratch.  Each replaces no reference entry point; the forward kernel each
+ * transposes is cited. */
+clv_status clv_adj_ideal_gas(clv_chunk *p, clv_chunk *a, int predict);          /* src/serial/ideal_gas.cpp:28-48 */
+clv_status clv_adj_viscosity(clv_chunk *p, clv_chunk *a);                       /* src/serial/viscosity.cpp:30-68 */
+clv_status clv_adj_PdV(clv_chunk *p, clv_chunk *a, int predict, double dt);     /* src/serial/PdV.cpp:36-87 */
+clv_status clv_adj_revert(clv_chunk *p, clv_chunk *a);                          /* src/serial/revert.cpp:29-41 */
+clv_status clv_adj_accelerate(clv_chunk *p, clv_chunk *a, double dt);           /* src/serial/accelerate.cpp:28-56 */
+clv_status clv_adj_flux_calc(clv_chunk *p, clv_chunk *a, double dt);            /* src/serial/flux_calc.cpp:27-43 */
+clv_status clv_adj_advec_cell(clv_chunk *p, clv_chunk *a, int sweep_number, int dir); /* src/serial/advec_cell.cpp:28-220 */
+/* both velocities of one sweep (the yvel pass reuses the xvel pass's node
+ * arrays, src/serial/advec_mom.cpp:81-103): advec_mom(xvel) then advec_mom(yvel) */
+clv_status clv_adj_advec_mom(clv_chunk *p, clv_chunk *a, int sweep_number, int dir); /* src/serial/advec_mom.cpp:30-225 */
+clv_status clv_adj_reset_field(clv_chunk *p, clv_chunk *a);                     /* src/serial/reset_field.cpp:28-51 */
+clv_status clv_adj_update_halo(clv_chunk *a, const int fields[CLV_NUM_HALO_FIELDS], int depth); /* src/serial/update_halo.cpp:33-684 */
+/* ke of field_summary (src/serial/field_summary.cpp:64-90), times ke_bar */
+clv_status clv_adj_kinetic_energy(clv_chunk *p, clv_chunk *a, double ke_bar);
+
+/* Checkpointed gradient of the final kinetic energy of a fixed-dt run with
+ * respect to the initial density0 and energy0 fields (interior cells).  The
+ * forward pass stores the state every checkpoint_every steps; the backward
+ * pass re-runs each segment with an undo log and walks it backwards.
+ * Memory: two chunks, (steps / checkpoint_every + 1) x 4 arrays, and
+ * checkpoint_every x 27 arrays. */
+typedef struct clv_adjoint clv_adjoint;
+/* initial: a started chunk (clv_hydro_start) holding the initial state; copied. */
+clv_status clv_adjoint_create(const clv_chunk *initial, double dt, int steps, int checkpoint_every, clv_adjoint **out);
+clv_status clv_adjoint_destroy(clv_adjoint *g);
+size_t clv_adjoint_bytes(const clv_adjoint *g);
+/* Replace the initial interior density0 / energy0 (y_cells rows of pitch doubles,
+ * host or device memory); the start-up halo is re-applied. */
+clv_status clv_adjoint_set_initial(clv_adjoint *g, const double *density0, const double *energy0, int64_t pitch);
+/* Run the steps; summary (may be NULL) receives the final field_summary
+ * {vol, mass, ie, ke, press}. */
+clv_status clv_adjoint_forward(clv_adjoint *g, double summary[5]);
+/* After clv_adjoint_forward: ke_bar * d ke / d density0 and d energy0 of the
+ * initial interior cells into (y_cells rows of pitch doubles, host or device;
+ * either may be NULL). */
+clv_status clv_adjoint_backward(clv_adjoint *g, double ke_bar, double *grad_density0, double *grad_energy0, int64_t pitch);
+/* the primal and adjoint chunks (for inspection; owned by g) */
+clv_status clv_adjoint_chunks(clv_adjoint *g, clv_chunk **primal, clv_chunk **adjoint);
+
 /* ---- measurement ------------------------------------------------------------------------ */
 /* Per-kernel HIP-event timing on the chunk's stream: when enabled, every launch
  * is bracketed by events and clv_profile_read() returns, for each kernel name,
