@@ -27,6 +27,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
@@ -205,6 +207,84 @@ def batch_leg(clv, n=64, seed=BATCH_SEED, reps=3):
     }
 
 
+def adjoint_leg(clv, directions=32, eps=1e-8):
+    """The adjoint workload (cloverleaf_amd.ADJOINT_WORKLOAD: two-state problem on
+    1920^2 with a seeded 1e-3 density perturbation, 200 fixed-dt steps, checkpoints
+    every 20).  Times the forward pass, the checkpointed backward pass (gradient of
+    the final kinetic energy w.r.t. the initial density and energy) and the
+    product's fused forward of the same run; checks the gradient against central
+    differences of GPU forward runs along `directions` seeded smooth directions."""
+    w = clv.ADJOINT_WORKLOAD
+    nx, ny, steps, dt = w["nx"], w["ny"], w["steps"], w["dt"]
+    cfg = clv.parse_deck(clv.bm_deck(nx, ny))
+    ch = clv.Chunk(nx, ny)
+    ch.start(cfg)
+    rho, e = clv.adjoint_initial(ch.download("density0")[2:ny + 2, 2:nx + 2], ch.download("energy0")[2:ny + 2, 2:nx + 2],
+                                 w["amplitude"], w["seed"])
+    adj = clv.Adjoint(ch, dt, steps, w["checkpoint_every"])
+    adj.set_initial(rho, e)
+    adj.forward()  # warm-up
+    t0 = time.perf_counter()
+    s = adj.forward()
+    fwd_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gr, ge = adj.backward(1.0)
+    bwd_s = time.perf_counter() - t0
+    print(f"# bench: adjoint forward {fwd_s:.3f} s, backward {bwd_s:.3f} s", file=sys.stderr, flush=True)
+    # the product's fused forward of the same fixed-dt run: dt = dtinit = dtmax and calc_dt never limiting
+    fcfg = clv.parse_deck(clv.bm_deck(nx, ny))
+    fcfg.dtinit = fcfg.dtmax = dt
+    fcfg.dtc_safe = fcfg.dtu_safe = fcfg.dtv_safe = fcfg.dtdiv_safe = 1e30
+    fcfg.end_step = steps
+    with clv.Chunk(nx, ny) as fc:
+        st, _ = fc.start(fcfg)
+        for name, a in (("density0", rho), ("energy0", e)):
+            full = fc.download(name)
+            full[2:ny + 2, 2:nx + 2] = a
+            fc.upload(name, full)
+        fc.update_halo(["density0", "energy0"], 2)
+        fc.synchronize()
+        t0 = time.perf_counter()
+        while not st.complete:
+            fc.step(fcfg, st, clv.PLAN_FUSED)
+        fc.synchronize()
+        fused_s = time.perf_counter() - t0
+        ke_fused = fc.summary()[3]
+    # central differences along seeded smooth directions.  eps = 1e-8: over 200 steps
+    # on 1920^2 a step of 1e-7 or more moves some van Leer limiter / upwind switch
+    # across its kink, and the difference quotient then carries an O(1e-5) error
+    # that does not shrink with eps (at 10-40 steps, 1e-6 and 1e-7 agree to 1e-8)
+    rel = []
+    for k in range(directions):
+        dr, de = clv.smooth_field(nx, ny, 300 + k), clv.smooth_field(nx, ny, 400 + k)
+        an = float(np.sum(gr * dr) + np.sum(ge * de))
+        adj.set_initial(rho + eps * dr, e + eps * de)
+        kp = adj.forward()[3]
+        adj.set_initial(rho - eps * dr, e - eps * de)
+        km = adj.forward()[3]
+        fd = (kp - km) / (2 * eps)
+        rel.append(abs(an - fd) / abs(fd))
+    adj.close()
+    rel = np.array(rel)
+    return {
+        "workload": f"two-state problem (InputDecks/clover_bm_short.in) on {nx}x{ny}, initial density x (1 + "
+                    f"{w['amplitude']} s), s = seeded smooth modes; {steps} steps at fixed dt {dt} (0.004 diverges at "
+                    f"this resolution); J = final kinetic energy; checkpoints every {w['checkpoint_every']} steps",
+        "ke": s[3],
+        "ke_fused_forward": ke_fused,
+        "forward_s": fwd_s,
+        "backward_s": bwd_s,
+        "fused_forward_s": fused_s,
+        "adjoint_cost_vs_forward": bwd_s / fwd_s,
+        "adjoint_cost_vs_fused_forward": bwd_s / fused_s,
+        "fd_directions": directions,
+        "fd_eps": eps,
+        "fd_max_rel_diff": float(rel.max()),
+        "fd_median_rel_diff": float(np.median(rel)),
+        "fd_within_1e-6": int(np.sum(rel <= 1e-6)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +296,8 @@ def main():
     ap.add_argument("--no-probe", action="store_true", help="skip the HBM copy-bandwidth probe")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-problem batched workload")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-problem batched workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload")
     ap.add_argument("--probe-only", action="store_true",
                     help="only run the copy kernel (4 GiB, 20 launches): the counter calibration run")
     ap.add_argument("--cpu-steps", type=int, default=120)
@@ -233,6 +315,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batched": batch_leg(clv)}))
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(clv)}))
         return
     deck_path = REPO / "tests" / "golden" / "decks" / f"{args.deck}.in"
     deck_text = deck_path.read_text()
@@ -315,6 +400,9 @@ def main():
     if not args.no_batch:
         print("# bench: batched workload ...", file=sys.stderr, flush=True)
         out["batched"] = batch_leg(clv)
+    if not args.no_adjoint:
+        print("# bench: adjoint workload ...", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_leg(clv)
     if not args.no_cpu:
         print("# bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(deck_text, args.cpu_steps)

@@ -488,6 +488,9 @@ class Adjoint:
         h = C.c_void_p()
         _check(self.L.clv_adjoint_create(initial.h, float(dt), int(steps), int(checkpoint_every), C.byref(h)), "clv_adjoint_create")
         self.h = h
+        # bumped by every set_initial / forward: a caller that holds on to a forward's
+        # checkpoints (the autograd binding) can tell whether they are still the ones stored
+        self.generation = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -511,6 +514,7 @@ class Adjoint:
 
     def set_initial(self, density0, energy0, pitch=None):
         """density0 / energy0: (ny, nx) float64 numpy arrays, or device pointers (ints) with a row pitch."""
+        self.generation += 1
         if isinstance(density0, np.ndarray):
             d = np.ascontiguousarray(density0, dtype=np.float64)
             e = np.ascontiguousarray(energy0, dtype=np.float64)
@@ -521,6 +525,7 @@ class Adjoint:
 
     def forward(self) -> list:
         out = (C.c_double * 5)()
+        self.generation += 1
         _check(self.L.clv_adjoint_forward(self.h, out), "clv_adjoint_forward")
         return list(out)
 
@@ -534,3 +539,39 @@ class Adjoint:
         _check(self.L.clv_adjoint_backward(self.h, float(ke_bar), int(grad_density0 or 0), int(grad_energy0 or 0), int(pitch)),
                "clv_adjoint_backward")
         return None
+
+
+# ---- the adjoint workload -------------------------------------------------------------------
+# The two-state problem of InputDecks/clover_bm_short.in on a 1920^2 grid, its
+# initial density scaled by (1 + 1e-3 s(x, y)) with s a seeded sum of low sine
+# modes (max |s| = 1), advanced 200 steps with a fixed dt; the gradient of the
+# final kinetic energy with respect to the initial density and energy fields.
+# dt = 0.003: a fixed 0.004 drives this problem unstable by step 5 at 1920^2
+# (its adaptive dt runs 0.0019 ... 0.0031); 0.003 stays stable for 200 steps.
+ADJOINT_WORKLOAD = dict(nx=1920, ny=1920, steps=200, dt=0.003, checkpoint_every=20, amplitude=1e-3, seed=4)
+
+
+def smooth_field(nx: int, ny: int, seed: int, modes: int = 4) -> np.ndarray:
+    """A seeded sum of low sine modes on the interior cells, (ny, nx), max |value| = 1."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    p = np.zeros((ny, nx))
+    for _ in range(modes):
+        kx, ky = rng.integers(1, 5, 2)
+        p += rng.uniform(-1, 1) * np.outer(np.sin(np.pi * ky * y), np.sin(np.pi * kx * x))
+    return p / np.abs(p).max()
+
+
+def bm_deck(nx: int, ny: int) -> str:
+    """InputDecks/clover_bm_short.in's two-state problem on an nx x ny grid."""
+    return (f"*clover\n state 1 density=0.2 energy=1.0\n"
+            f" state 2 density=1.0 energy=2.5 geometry=rectangle xmin=0.0 xmax=5.0 ymin=0.0 ymax=2.0\n"
+            f" x_cells={nx}\n y_cells={ny}\n xmin=0.0\n ymin=0.0\n xmax=10.0\n ymax=10.0\n"
+            f" initial_timestep=0.04\n timestep_rise=1.5\n max_timestep=0.04\n end_step=87\n*endclover\n")
+
+
+def adjoint_initial(density0: np.ndarray, energy0: np.ndarray, amplitude: float, seed: int):
+    """The workload's initial interior fields from the unperturbed ones."""
+    ny, nx = density0.shape
+    return density0 * (1.0 + amplitude * smooth_field(nx, ny, seed)), energy0.copy()

@@ -208,3 +208,33 @@ def test_checkpoint_spacing_does_not_change_the_gradient(gpu):
             grads.append(g.backward(1.0))
     for gr, ge in grads[1:]:
         assert np.array_equal(gr, grads[0][0]) and np.array_equal(ge, grads[0][1])
+
+
+# ---- the adjoint workload against the oracle's tangent-linear sweep -------------------------
+GOLDEN_C4 = clv.PKG.parent / "tests" / "golden" / "adjoint_c4.json"
+
+
+@pytest.mark.skipif(not GOLDEN_C4.exists(), reason="adjoint golden not generated")
+def test_adjoint_workload_vs_tangent_linear_golden(gpu):
+    """1920^2, 200 fixed-dt steps, checkpoints every 20: the gradient along the
+    golden's directions equals the oracle's forward-mode derivative."""
+    import json
+    g = json.loads(GOLDEN_C4.read_text())
+    w = g["workload"]
+    assert w == clv.ADJOINT_WORKLOAD
+    nx, ny = w["nx"], w["ny"]
+    ch = clv.Chunk(nx, ny)
+    ch.start(clv.parse_deck(clv.bm_deck(nx, ny)))
+    rho, e = clv.adjoint_initial(ch.download("density0")[2:ny + 2, 2:nx + 2], ch.download("energy0")[2:ny + 2, 2:nx + 2],
+                                 w["amplitude"], w["seed"])
+    print("# adjoint workload: forward", flush=True)
+    with clv.Adjoint(ch, w["dt"], w["steps"], w["checkpoint_every"]) as adj:
+        adj.set_initial(rho, e)
+        s = adj.forward()
+        assert abs(s[3] - g["ke"]) <= 1e-12 * abs(g["ke"])
+        print("# adjoint workload: backward", flush=True)
+        gr, ge = adj.backward(1.0)
+    for rec in g["directions"]:
+        dr, de = clv.smooth_field(nx, ny, rec["seed_density"]), clv.smooth_field(nx, ny, rec["seed_energy"])
+        an = float(np.sum(gr * dr) + np.sum(ge * de))
+        assert abs(an - rec["dke"]) <= 1e-9 * abs(rec["dke"]), (rec, an)
