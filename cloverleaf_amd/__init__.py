@@ -530,7 +530,9 @@ class Adjoint:
         return list(out)
 
     def backward(self, ke_bar: float = 1.0, grad_density0=None, grad_energy0=None, pitch=None):
-        """Host numpy gradients by default; or device pointers (ints) with a row pitch."""
+        """Host numpy gradients by default; or device pointers (ints) with a row pitch.
+        The backward pass consumes the forward run: another backward needs another forward."""
+        self.generation += 1
         if grad_density0 is None and grad_energy0 is None:
             gd = np.empty((self.ny, self.nx))
             ge = np.empty((self.ny, self.nx))

@@ -1,6 +1,6 @@
-// Reverse-mode (adjoint) kernels of the fixed-dt hydro step, one per reference
-// kernel, for the gradient of a scalar of the final state with respect to the
-// initial fields.
+// Reverse-mode (adjoint) kernels of the fixed-dt hydro step, one group per
+// reference kernel, for the gradient of a scalar of the final state with respect
+// to the initial fields.
 //
 // Conventions.  p is the primal chunk holding the forward kernel's INPUT state;
 // a is a second chunk of the same size holding adjoints.  On entry a holds the
@@ -8,27 +8,31 @@
 // inputs); on exit the outputs' adjoints are consumed (zeroed where the kernel
 // overwrote them) and the inputs' adjoints are accumulated.  Geometry (volume,
 // areas, mesh) is constant; soundspeed, which only calc_dt reads, carries no
-// adjoint; work arrays of the adjoint chunk are scratch.
+// adjoint.  The work arrays of BOTH chunks are scratch: the primal's hold the
+// forward's intermediate values (face fluxes, node masses), the adjoint's the
+// per-element shares below.
 //
-// Every kernel is a gather: a thread owns one input element and sums, over the
-// few output elements that read it, d(output)/d(input) * adjoint(output) --
-// deterministic, no atomics.  The local derivative d(output)/d(input) comes from
-// evaluating that output's reference expression on dual numbers with the input
-// seeded (several inputs of one element at once with an N-component dual), so
-// every branch (upwind choice, van Leer limiter, viscosity switch, fmin/fmax,
-// fabs) is differentiated on its active piece exactly as the forward took it.
-// Ties follow the oracle's tangent-linear model: fmin/fmax return their first
-// argument, fabs(0) has derivative +1.
+// Structure.  Every element (cell, node or face) the forward kernel computes is
+// differentiated once.  Its nonlinear core is evaluated on N-component dual
+// numbers seeded at the core's inputs, on the same values in the same order as
+// the forward, so every branch (upwind choice, van Leer limiter, viscosity
+// switch, fmin/fmax, fabs) is differentiated on the piece the forward took; the
+// element's output adjoints times those partials are stored per element and
+// role.  A gather kernel then adds, for every input, the shares of the elements
+// that read it, in a fixed order -- deterministic, no atomics.  Linear stages
+// (sums of velocities, pressure differences, the total volume flux, node
+// masses) are transposed by hand.  Ties follow the oracle's tangent-linear
+// model: fmin/fmax return their first argument, fabs(0) has derivative +1.
 #include "clv_internal.h"
 
 namespace clv {
 namespace {
 
-// ---- N-component dual numbers ------------------------------------------------------------
+// ---- N-component dual numbers (N = 0: values only) ---------------------------------------
 template <int N>
 struct Dn {
   double v;
-  double d[N];
+  double d[N > 0 ? N : 1];
 };
 template <int N>
 __device__ __forceinline__ Dn<N> cst(double v) {
@@ -71,12 +75,16 @@ __device__ __forceinline__ Dn<N> operator*(Dn<N> a, Dn<N> b) {
   for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
   return r;
 }
+// the value is the correctly rounded quotient; the derivative uses one reciprocal
 template <int N>
 __device__ __forceinline__ Dn<N> operator/(Dn<N> a, Dn<N> b) {
   Dn<N> r;
   r.v = a.v / b.v;
+  if constexpr (N > 0) {
+    const double ib = 1.0 / b.v;
 #pragma unroll
-  for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+  }
   return r;
 }
 template <int N>
@@ -112,7 +120,16 @@ __device__ __forceinline__ Dn<N> operator*(double a, Dn<N> b) {
   return r;
 }
 template <int N>
-__device__ __forceinline__ Dn<N> operator/(Dn<N> a, double b) { return a / cst<N>(b); }
+__device__ __forceinline__ Dn<N> operator/(Dn<N> a, double b) {
+  Dn<N> r;
+  r.v = a.v / b;
+  if constexpr (N > 0) {
+    const double ib = 1.0 / b;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * ib;
+  }
+  return r;
+}
 template <int N>
 __device__ __forceinline__ Dn<N> operator/(double a, Dn<N> b) { return cst<N>(a) / b; }
 template <int N>
@@ -137,8 +154,11 @@ template <int N>
 __device__ __forceinline__ Dn<N> sqrt(Dn<N> a) {
   Dn<N> r;
   r.v = ::sqrt(a.v);
+  if constexpr (N > 0) {
+    const double h = r.v > 0.0 ? 0.5 / r.v : 0.0;
 #pragma unroll
-  for (int k = 0; k < N; ++k) r.d[k] = r.v > 0.0 ? a.d[k] / (2.0 * r.v) : 0.0;
+    for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * h;
+  }
   return r;
 }
 
@@ -146,17 +166,20 @@ __device__ __forceinline__ Dn<N> sqrt(Dn<N> a) {
 #define AA(fid, i, j) a.f[fid][(int64_t)(j) * a.pitch + (i)]
 #define PM(mid, i) p.m[mid][i]
 
-// ---- element functions: the reference expressions (oracle order) on dual numbers ----------
-// viscosity of one cell, src/serial/viscosity.cpp:43-66
+__device__ __forceinline__ bool in(int i, int lo, int hi) { return i >= lo && i <= hi; }
+
+// ---- element cores: the reference expressions (oracle order) on dual numbers --------------
+// viscosity of one cell, src/serial/viscosity.cpp:43-66, from its linear pre-stages:
+// ugrad = (u10 + u11) - (u00 + u01), vgrad = (v01 + v11) - (v00 + v10),
+// su = u01 + u11 - u00 - u10, sv = v10 + v11 - v00 - v01, dpx = p(i+1) - p(i-1),
+// dpy = p(j+1) - p(j-1)
 template <int N>
-__device__ Dn<N> el_viscosity(Dn<N> u00, Dn<N> u10, Dn<N> u01, Dn<N> u11, Dn<N> v00, Dn<N> v10, Dn<N> v01, Dn<N> v11, Dn<N> pl,
-                              Dn<N> pr, Dn<N> pb, Dn<N> pt, Dn<N> rho, double dxi, double dxi1, double dyj, double dyj2) {
-  Dn<N> ugrad = (u10 + u11) - (u00 + u01);
-  Dn<N> vgrad = (v01 + v11) - (v00 + v10);
+__device__ Dn<N> el_visc_core(Dn<N> ugrad, Dn<N> vgrad, Dn<N> su, Dn<N> sv, Dn<N> dpx, Dn<N> dpy, Dn<N> rho, double dxi, double dxi1,
+                              double dyj, double dyj2) {
   Dn<N> div = (dxi * (ugrad) + dyj * (vgrad));
-  Dn<N> strain2 = 0.5 * (u01 + u11 - u00 - u10) / dyj + 0.5 * (v10 + v11 - v00 - v01) / dxi;
-  Dn<N> pgradx = (pr - pl) / (dxi + dxi1);
-  Dn<N> pgrady = (pt - pb) / (dyj + dyj2);
+  Dn<N> strain2 = 0.5 * su / dyj + 0.5 * sv / dxi;
+  Dn<N> pgradx = dpx / (dxi + dxi1);
+  Dn<N> pgrady = dpy / (dyj + dyj2);
   Dn<N> pgradx2 = pgradx * pgradx;
   Dn<N> pgrady2 = pgrady * pgrady;
   Dn<N> limiter = ((0.5 * (ugrad) / dxi) * pgradx2 + (0.5 * (vgrad) / dyj) * pgrady2 + strain2 * pgradx * pgrady) /
@@ -176,25 +199,9 @@ __device__ Dn<N> el_viscosity(Dn<N> u00, Dn<N> u10, Dn<N> u01, Dn<N> u11, Dn<N> 
   return 2.0 * rho * grad2 * limiter * limiter;
 }
 
-// PdV of one cell, src/serial/PdV.cpp:49-63 (predict) / :72-83 (correct)
+// PdV of one cell after its total volume flux, src/serial/PdV.cpp:61-63 / :81-83
 template <int N>
-__device__ void el_pdv(bool predict, double dt, double xa0, double xa1, double ya0, double ya1, Dn<N> u00, Dn<N> u01, Dn<N> u10,
-                       Dn<N> u11, Dn<N> v00, Dn<N> v10, Dn<N> v01, Dn<N> v11, Dn<N> w00, Dn<N> w01, Dn<N> w10, Dn<N> w11,
-                       Dn<N> z00, Dn<N> z10, Dn<N> z01, Dn<N> z11, double vol, Dn<N> rho0, Dn<N> e0, Dn<N> pr, Dn<N> q,
-                       Dn<N> &rho1, Dn<N> &e1) {
-  Dn<N> left_flux, right_flux, bottom_flux, top_flux;
-  if (predict) {
-    left_flux = (xa0 * (u00 + u01 + u00 + u01)) * 0.25 * dt * 0.5;
-    right_flux = (xa1 * (u10 + u11 + u10 + u11)) * 0.25 * dt * 0.5;
-    bottom_flux = (ya0 * (v00 + v10 + v00 + v10)) * 0.25 * dt * 0.5;
-    top_flux = (ya1 * (v01 + v11 + v01 + v11)) * 0.25 * dt * 0.5;
-  } else {
-    left_flux = (xa0 * (u00 + u01 + w00 + w01)) * 0.25 * dt;
-    right_flux = (xa1 * (u10 + u11 + w10 + w11)) * 0.25 * dt;
-    bottom_flux = (ya0 * (v00 + v10 + z00 + z10)) * 0.25 * dt;
-    top_flux = (ya1 * (v01 + v11 + z01 + z11)) * 0.25 * dt;
-  }
-  Dn<N> total_flux = right_flux - left_flux + top_flux - bottom_flux;
+__device__ void el_pdv_core(double vol, Dn<N> rho0, Dn<N> e0, Dn<N> pr, Dn<N> q, Dn<N> total_flux, Dn<N> &rho1, Dn<N> &e1) {
   Dn<N> volume_change_s = vol / (vol + total_flux);
   double recip_volume = 1.0 / vol;
   Dn<N> energy_change = (pr / rho0 + q / rho0) * total_flux * recip_volume;
@@ -202,17 +209,26 @@ __device__ void el_pdv(bool predict, double dt, double xa0, double xa1, double y
   rho1 = rho0 * volume_change_s;
 }
 
-// accelerate of one node, src/serial/accelerate.cpp:43-53; cells (i-1,j-1)=mm,
-// (i-1,j)=m0, (i,j)=00, (i,j-1)=0m
-template <int N>
-__device__ void el_accel(double halfdt, Dn<N> rmm, Dn<N> rm0, Dn<N> r00, Dn<N> r0m, double vmm, double vm0, double v00, double v0m,
-                         double xa00, double xa0m, double ya00, double yam0, Dn<N> p00, Dn<N> pm0, Dn<N> p0m, Dn<N> pmm, Dn<N> q00,
-                         Dn<N> qm0, Dn<N> q0m, Dn<N> qmm, Dn<N> u0, Dn<N> w0, Dn<N> &u1, Dn<N> &w1) {
-  Dn<N> stepbymass_s = halfdt / ((rmm * vmm + rm0 * vm0 + r00 * v00 + r0m * v0m) * 0.25);
-  u1 = u0 - stepbymass_s * (xa00 * (p00 - pm0) + xa0m * (p0m - pmm));
-  w1 = w0 - stepbymass_s * (ya00 * (p00 - p0m) + yam0 * (pm0 - pmm));
-  u1 = u1 - stepbymass_s * (xa00 * (q00 - qm0) + xa0m * (q0m - qmm));
-  w1 = w1 - stepbymass_s * (ya00 * (q00 - q0m) + yam0 * (qm0 - qmm));
+// total volume flux of cell (i, j), src/serial/PdV.cpp:49-60 (predict) / :72-80 (correct)
+__device__ __forceinline__ double pdv_total_flux(const Dev &p, bool predict, double dt, int i, int j) {
+  const double xa0 = PA(CLV_XAREA, i, j), xa1 = PA(CLV_XAREA, i + 1, j), ya0 = PA(CLV_YAREA, i, j), ya1 = PA(CLV_YAREA, i, j + 1);
+  const double u00 = PA(CLV_XVEL0, i, j), u01 = PA(CLV_XVEL0, i, j + 1), u10 = PA(CLV_XVEL0, i + 1, j), u11 = PA(CLV_XVEL0, i + 1, j + 1);
+  const double v00 = PA(CLV_YVEL0, i, j), v10 = PA(CLV_YVEL0, i + 1, j), v01 = PA(CLV_YVEL0, i, j + 1), v11 = PA(CLV_YVEL0, i + 1, j + 1);
+  double left_flux, right_flux, bottom_flux, top_flux;
+  if (predict) {
+    left_flux = (xa0 * (u00 + u01 + u00 + u01)) * 0.25 * dt * 0.5;
+    right_flux = (xa1 * (u10 + u11 + u10 + u11)) * 0.25 * dt * 0.5;
+    bottom_flux = (ya0 * (v00 + v10 + v00 + v10)) * 0.25 * dt * 0.5;
+    top_flux = (ya1 * (v01 + v11 + v01 + v11)) * 0.25 * dt * 0.5;
+  } else {
+    const double w00 = PA(CLV_XVEL1, i, j), w01 = PA(CLV_XVEL1, i, j + 1), w10 = PA(CLV_XVEL1, i + 1, j), w11 = PA(CLV_XVEL1, i + 1, j + 1);
+    const double z00 = PA(CLV_YVEL1, i, j), z10 = PA(CLV_YVEL1, i + 1, j), z01 = PA(CLV_YVEL1, i, j + 1), z11 = PA(CLV_YVEL1, i + 1, j + 1);
+    left_flux = (xa0 * (u00 + u01 + w00 + w01)) * 0.25 * dt;
+    right_flux = (xa1 * (u10 + u11 + w10 + w11)) * 0.25 * dt;
+    bottom_flux = (ya0 * (v00 + v10 + z00 + z10)) * 0.25 * dt;
+    top_flux = (ya1 * (v01 + v11 + z01 + z11)) * 0.25 * dt;
+  }
+  return right_flux - left_flux + top_flux - bottom_flux;
 }
 
 // van Leer limiter of advec_cell, src/serial/advec_cell.cpp:88-104
@@ -243,18 +259,6 @@ __device__ void el_face(Dn<N> vf, Dn<N> pre_dn, Dn<N> r_uw, Dn<N> r_dn, Dn<N> r_
   ef = mf * (e_dn + limiter);
 }
 
-// cell update of advec_cell, src/serial/advec_cell.cpp:120-125 / :211-216
-template <int N>
-__device__ void el_cell_update(Dn<N> rho, Dn<N> e, Dn<N> pre, Dn<N> mf_l, Dn<N> mf_r, Dn<N> ef_l, Dn<N> ef_r, Dn<N> vf_l, Dn<N> vf_r,
-                               Dn<N> &rho_new, Dn<N> &e_new) {
-  Dn<N> pre_mass_s = rho * pre;
-  Dn<N> post_mass_s = pre_mass_s + mf_l - mf_r;
-  Dn<N> post_ener_s = (e * pre_mass_s + ef_l - ef_r) / post_mass_s;
-  Dn<N> advec_vol_s = pre + vf_l - vf_r;
-  rho_new = post_mass_s / advec_vol_s;
-  e_new = post_ener_s;
-}
-
 // momentum flux of one node face, src/serial/advec_mom.cpp:114-139 / :186-211
 template <int N>
 __device__ Dn<N> el_momflux(Dn<N> flux, Dn<N> mass_pre_donor, double width, double width_dif, Dn<N> v_uw, Dn<N> v_dn, Dn<N> v_dw) {
@@ -273,13 +277,153 @@ __device__ Dn<N> el_momflux(Dn<N> flux, Dn<N> mass_pre_donor, double width, doub
   return advec_vel_s * flux;
 }
 
-// velocity update of advec_mom, src/serial/advec_mom.cpp:142-150 / :214-222
-template <int N>
-__device__ Dn<N> el_vel_update(Dn<N> vel, Dn<N> mpre, Dn<N> mf_l, Dn<N> mf_r, Dn<N> mpost) {
-  return (vel * mpre + mf_l - mf_r) / mpost;
+// ---- reverse sweeps of the limited fluxes ------------------------------------------------
+// The same values in the same order as the forward; the branch each fmin / fabs /
+// limiter switch took is replayed and differentiated (fmin(a, b) is b only when
+// b < a; d|x|/dx = -1 for x < 0, else +1).
+__device__ __forceinline__ double sgn_of(double x) { return x < 0.0 ? -1.0 : 1.0; }
+
+struct VlBar {
+  double sigma, s3, s4, uw, dn, dw;
+};
+// adjoint of el_vl for the output adjoint lb
+__device__ __forceinline__ VlBar vl_bar(double sigma, double sigma3, double sigma4, double uw, double dn, double dw, double lb) {
+  const double one_by_six = 1.0 / 6.0;
+  VlBar r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double diffuw = dn - uw, diffdw = dw - dn;
+  const double wind = diffdw <= 0.0 ? -1.0 : 1.0;
+  if (!(diffuw * diffdw > 0.0)) return r;
+  const double au = ::fabs(diffuw), ad = ::fabs(diffdw);
+  const double c = one_by_six * (sigma3 * au + sigma4 * ad);
+  const double m1 = ad < au ? ad : au;
+  const double m = c < m1 ? c : m1;
+  r.sigma = -wind * m * lb;
+  const double mb = (1.0 - sigma) * wind * lb;
+  double aub = 0.0, adb = 0.0;
+  if (c < m1) {
+    r.s3 = mb * one_by_six * au;
+    r.s4 = mb * one_by_six * ad;
+    aub = mb * one_by_six * sigma3;
+    adb = mb * one_by_six * sigma4;
+  } else if (ad < au) {
+    adb = mb;
+  } else {
+    aub = mb;
+  }
+  const double dub = aub * sgn_of(diffuw), ddb = adb * sgn_of(diffdw);
+  r.uw = -dub;
+  r.dn = dub - ddb;
+  r.dw = ddb;
+  return r;
 }
 
-__device__ __forceinline__ bool in(int i, int lo, int hi) { return i >= lo && i <= hi; }
+// adjoint of el_face: g = d(mf, ef)/d(vf, pre_dn, r_uw, r_dn, r_dw, e_uw, e_dn, e_dw)^T (mfb, efb)
+__device__ void face_bar(double vf, double pre, double r_uw, double r_dn, double r_dw, double e_uw, double e_dn, double e_dw, double vd_face,
+                         double vd_dif, double mfb, double efb, double g[8]) {
+  const double one_by_six = 1.0 / 6.0;
+  // forward values
+  const double sigmat = ::fabs(vf) / pre;
+  const double ratio = vd_face / vd_dif;
+  const double sigma3 = (1.0 + sigmat) * ratio;
+  const double sigma4 = 2.0 - sigmat;
+  double lim1 = 0.0;
+  {
+    const double diffuw = r_dn - r_uw, diffdw = r_dw - r_dn;
+    const double wind = diffdw <= 0.0 ? -1.0 : 1.0;
+    if (diffuw * diffdw > 0.0) {
+      const double au = ::fabs(diffuw), ad = ::fabs(diffdw);
+      const double m1 = ad < au ? ad : au, c = one_by_six * (sigma3 * au + sigma4 * ad);
+      lim1 = (1.0 - sigmat) * wind * (c < m1 ? c : m1);
+    }
+  }
+  const double mf = vf * (r_dn + lim1);
+  const double den = r_dn * pre;
+  const double sigmam = ::fabs(mf) / den;
+  double lim2 = 0.0;
+  {
+    const double diffuw = e_dn - e_uw, diffdw = e_dw - e_dn;
+    const double wind = diffdw <= 0.0 ? -1.0 : 1.0;
+    if (diffuw * diffdw > 0.0) {
+      const double au = ::fabs(diffuw), ad = ::fabs(diffdw);
+      const double m1 = ad < au ? ad : au, c = one_by_six * (sigma3 * au + sigma4 * ad);
+      lim2 = (1.0 - sigmam) * wind * (c < m1 ? c : m1);
+    }
+  }
+  // reverse: ef = mf (e_dn + lim2)
+  double mf_b = mfb + efb * (e_dn + lim2);
+  const double lim2_b = efb * mf;
+  const VlBar b2 = vl_bar(sigmam, sigma3, sigma4, e_uw, e_dn, e_dw, lim2_b);
+  g[5] = b2.uw;
+  g[6] = efb * mf + b2.dn;
+  g[7] = b2.dw;
+  double s3b = b2.s3, s4b = b2.s4;
+  // sigmam = |mf| / (r_dn pre)
+  mf_b += b2.sigma * sgn_of(mf) / den;
+  const double den_b = -b2.sigma * sigmam / den;
+  double rdn_b = den_b * pre, pre_b = den_b * r_dn;
+  // mf = vf (r_dn + lim1)
+  double vf_b = mf_b * (r_dn + lim1);
+  rdn_b += mf_b * vf;
+  const VlBar b1 = vl_bar(sigmat, sigma3, sigma4, r_uw, r_dn, r_dw, mf_b * vf);
+  s3b += b1.s3;
+  s4b += b1.s4;
+  // sigmat feeds the first limiter's sigma, sigma3 and sigma4
+  const double st_b = b1.sigma + s3b * ratio - s4b;
+  vf_b += st_b * sgn_of(vf) / pre;
+  pre_b -= st_b * sigmat / pre;
+  g[0] = vf_b;
+  g[1] = pre_b;
+  g[2] = b1.uw;
+  g[3] = rdn_b + b1.dn;
+  g[4] = b1.dw;
+}
+
+// adjoint of el_momflux: g = d(flux)/d(node_flux, mass_pre_donor, v_uw, v_dn, v_dw) * gb
+__device__ void momflux_bar(double flux, double mpre, double width, double width_dif, double v_uw, double v_dn, double v_dw, double gb,
+                            double g[5]) {
+  const double sigma = ::fabs(flux) / mpre;
+  const double vdiffuw = v_dn - v_uw, vdiffdw = v_dw - v_dn;
+  double limiter = 0.0, wind = 1.0, auw = 0.0, adw = 0.0, x = 0.0, m1 = 0.0;
+  const bool lim = vdiffuw * vdiffdw > 0.0;
+  if (lim) {
+    auw = ::fabs(vdiffuw);
+    adw = ::fabs(vdiffdw);
+    if (vdiffdw <= 0.0) wind = -1.0;
+    x = width * ((2.0 - sigma) * adw / width + (1.0 + sigma) * auw / width_dif) / 6.0;
+    m1 = auw < x ? auw : x;
+    limiter = wind * (adw < m1 ? adw : m1);
+  }
+  const double advec_vel_s = v_dn + (1.0 - sigma) * limiter;
+  // reverse: flux_out = advec_vel_s * flux
+  const double av_b = gb * flux;
+  double flux_b = gb * advec_vel_s;
+  double vdn_b = av_b, vuw_b = 0.0, vdw_b = 0.0;
+  double sigma_b = -av_b * limiter;
+  if (lim) {
+    const double mb = wind * av_b * (1.0 - sigma);
+    double auw_b = 0.0, adw_b = 0.0;
+    if (adw < m1) {
+      adw_b = mb;
+    } else if (auw < x) {
+      auw_b = mb;
+    } else {
+      const double yb = mb * width / 6.0;
+      sigma_b += yb * (-adw / width + auw / width_dif);
+      adw_b = yb * (2.0 - sigma) / width;
+      auw_b = yb * (1.0 + sigma) / width_dif;
+    }
+    const double duw_b = auw_b * sgn_of(vdiffuw), ddw_b = adw_b * sgn_of(vdiffdw);
+    vuw_b -= duw_b;
+    vdn_b += duw_b - ddw_b;
+    vdw_b += ddw_b;
+  }
+  flux_b += sigma_b * sgn_of(flux) / mpre;
+  g[0] = flux_b;
+  g[1] = -sigma_b * sigma / mpre;
+  g[2] = vuw_b;
+  g[3] = vdn_b;
+  g[4] = vdw_b;
+}
 
 // ---- ideal_gas, src/serial/ideal_gas.cpp:38-46: p = 0.4 rho e on the interior ---------------
 __global__ void adj_ideal_gas_k(Dev p, Dev a, int rho_f, int e_f) {
@@ -350,146 +494,143 @@ __global__ void adj_halo_bt_k(Dev a, AdjHaloList list, int depth) {
   }
 }
 
-// ---- viscosity --------------------------------------------------------------------------
-__device__ Dn<2> visc_at(const Dev &p, int ci, int cj, int ni, int nj, int which, int pi, int pj) {
-  // which: 0 = seed u0,v0 of node (ni,nj); 1 = seed p(pi,pj) (component 0) and rho0 (component 1, own cell)
-  auto U = [&](int i, int j) { return which == 0 && i == ni && j == nj ? var<2>(PA(CLV_XVEL0, i, j), 1u) : cst<2>(PA(CLV_XVEL0, i, j)); };
-  auto V = [&](int i, int j) { return which == 0 && i == ni && j == nj ? var<2>(PA(CLV_YVEL0, i, j), 2u) : cst<2>(PA(CLV_YVEL0, i, j)); };
-  auto Pp = [&](int i, int j) { return which == 1 && i == pi && j == pj ? var<2>(PA(CLV_PRESSURE, i, j), 1u) : cst<2>(PA(CLV_PRESSURE, i, j)); };
-  const Dn<2> rho = which == 1 && ci == pi && cj == pj ? var<2>(PA(CLV_DENSITY0, ci, cj), 2u) : cst<2>(PA(CLV_DENSITY0, ci, cj));
-  return el_viscosity<2>(U(ci, cj), U(ci + 1, cj), U(ci, cj + 1), U(ci + 1, cj + 1), V(ci, cj), V(ci + 1, cj), V(ci, cj + 1),
-                         V(ci + 1, cj + 1), Pp(ci - 1, cj), Pp(ci + 1, cj), Pp(ci, cj - 1), Pp(ci, cj + 1), rho, PM(CLV_CELLDX, ci),
-                         PM(CLV_CELLDX, ci + 1), PM(CLV_CELLDY, cj), PM(CLV_CELLDY, cj + 2));
+// ---- viscosity ----------------------------------------------------------------------------
+// V1, cells [2, nx+1]^2: the core's input adjoints, q_bar * dq/d(ugrad, vgrad, su,
+// sv, dpx, dpy) -> a.work1..6; the density share directly; consumes q_bar.
+__global__ void adj_visc_cell_k(Dev p, Dev a) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > p.nx + 1 || j > p.ny + 1) return;
+  const double qb = AA(CLV_VISCOSITY, i, j);
+  AA(CLV_VISCOSITY, i, j) = 0.0;
+  double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (qb != 0.0) {
+    const double u00 = PA(CLV_XVEL0, i, j), u10 = PA(CLV_XVEL0, i + 1, j), u01 = PA(CLV_XVEL0, i, j + 1), u11 = PA(CLV_XVEL0, i + 1, j + 1);
+    const double v00 = PA(CLV_YVEL0, i, j), v10 = PA(CLV_YVEL0, i + 1, j), v01 = PA(CLV_YVEL0, i, j + 1), v11 = PA(CLV_YVEL0, i + 1, j + 1);
+    const double ugrad = (u10 + u11) - (u00 + u01), vgrad = (v01 + v11) - (v00 + v10);
+    const double su = u01 + u11 - u00 - u10, sv = v10 + v11 - v00 - v01;
+    const double dpx = PA(CLV_PRESSURE, i + 1, j) - PA(CLV_PRESSURE, i - 1, j);
+    const double dpy = PA(CLV_PRESSURE, i, j + 1) - PA(CLV_PRESSURE, i, j - 1);
+    const Dn<7> q = el_visc_core<7>(var<7>(ugrad, 1u), var<7>(vgrad, 2u), var<7>(su, 4u), var<7>(sv, 8u), var<7>(dpx, 16u), var<7>(dpy, 32u),
+                                    var<7>(PA(CLV_DENSITY0, i, j), 64u), PM(CLV_CELLDX, i), PM(CLV_CELLDX, i + 1), PM(CLV_CELLDY, j),
+                                    PM(CLV_CELLDY, j + 2));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g[k] = q.d[k] * qb;
+    AA(CLV_DENSITY0, i, j) += q.d[6] * qb;
+  }
+  AA(CLV_WORK1, i, j) = g[0];
+  AA(CLV_WORK2, i, j) = g[1];
+  AA(CLV_WORK3, i, j) = g[2];
+  AA(CLV_WORK4, i, j) = g[3];
+  AA(CLV_WORK5, i, j) = g[4];
+  AA(CLV_WORK6, i, j) = g[5];
 }
 
-// site (i, j) in [1, nx+2] x [1, ny+2]: node (i, j) velocities, cell (i, j) pressure and density
-__global__ void adj_viscosity_k(Dev p, Dev a) {
+// V2, sites (i, j) in [1, nx+2]^2: node (i, j) velocities from its four cells,
+// cell (i, j) pressure from the four cells whose pressure gradient reads it
+__global__ void adj_visc_gather_k(Dev a) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int nx = p.nx, ny = p.ny;
+  const int nx = a.nx, ny = a.ny;
   if (i > nx + 2 || j > ny + 2) return;
-  // node (i, j): cells (i-1..i, j-1..j)
+  auto W = [&](int fid, int ci, int cj) { return in(ci, 2, nx + 1) && in(cj, 2, ny + 1) ? AA(fid, ci, cj) : 0.0; };
   if (i >= 2 && j >= 2) {
-    double ub = 0.0, vb = 0.0;
-    for (int cj = j - 1; cj <= j; ++cj)
-      for (int ci = i - 1; ci <= i; ++ci) {
-        if (!in(ci, 2, nx + 1) || !in(cj, 2, ny + 1)) continue;
-        const double qb = AA(CLV_VISCOSITY, ci, cj);
-        if (qb == 0.0) continue;
-        const Dn<2> q = visc_at(p, ci, cj, i, j, 0, 0, 0);
-        ub += q.d[0] * qb;
-        vb += q.d[1] * qb;
-      }
-    AA(CLV_XVEL0, i, j) += ub;
-    AA(CLV_YVEL0, i, j) += vb;
+    // node (i, j) is u00/v00 of cell (i, j), u10/v10 of (i-1, j), u01/v01 of (i, j-1), u11/v11 of (i-1, j-1)
+    const double ug = -W(CLV_WORK1, i, j) + W(CLV_WORK1, i - 1, j) - W(CLV_WORK1, i, j - 1) + W(CLV_WORK1, i - 1, j - 1);
+    const double us = -W(CLV_WORK3, i, j) - W(CLV_WORK3, i - 1, j) + W(CLV_WORK3, i, j - 1) + W(CLV_WORK3, i - 1, j - 1);
+    const double vg = -W(CLV_WORK2, i, j) - W(CLV_WORK2, i - 1, j) + W(CLV_WORK2, i, j - 1) + W(CLV_WORK2, i - 1, j - 1);
+    const double vs = -W(CLV_WORK4, i, j) + W(CLV_WORK4, i - 1, j) - W(CLV_WORK4, i, j - 1) + W(CLV_WORK4, i - 1, j - 1);
+    AA(CLV_XVEL0, i, j) += ug + us;
+    AA(CLV_YVEL0, i, j) += vg + vs;
   }
-  // cell (i, j): its pressure is read by cells (i+-1, j), (i, j+-1); its density by itself
-  double pb = 0.0, rb = 0.0;
-  const int nci[5] = {i + 1, i - 1, i, i, i}, ncj[5] = {j, j, j + 1, j - 1, j};
-  for (int k = 0; k < 5; ++k) {
-    const int ci = nci[k], cj = ncj[k];
-    if (!in(ci, 2, nx + 1) || !in(cj, 2, ny + 1)) continue;
-    const double qb = AA(CLV_VISCOSITY, ci, cj);
-    if (qb == 0.0) continue;
-    const Dn<2> q = visc_at(p, ci, cj, 0, 0, 1, i, j);
-    if (k < 4) pb += q.d[0] * qb;
-    else rb += q.d[1] * qb;
-  }
-  AA(CLV_PRESSURE, i, j) += pb;
-  AA(CLV_DENSITY0, i, j) += rb;
+  // dpx(c) = p(c + x) - p(c - x), dpy(c) = p(c + y) - p(c - y)
+  AA(CLV_PRESSURE, i, j) += W(CLV_WORK5, i - 1, j) - W(CLV_WORK5, i + 1, j) + W(CLV_WORK6, i, j - 1) - W(CLV_WORK6, i, j + 1);
 }
 
 // ---- PdV ----------------------------------------------------------------------------------
-// cell (ci, cj) with seeds: mode 0 = node (ni, nj) [u0, v0, u1, v1]; mode 1 = own cell [rho0, e0, p, q]
-__device__ void pdv_at(const Dev &p, bool predict, double dt, int ci, int cj, int mode, int ni, int nj, Dn<4> &r1, Dn<4> &e1) {
-  auto N0 = [&](int fid, int i, int j, unsigned bit) {
-    return mode == 0 && i == ni && j == nj ? var<4>(PA(fid, i, j), bit) : cst<4>(PA(fid, i, j));
-  };
-  const Dn<4> u00 = N0(CLV_XVEL0, ci, cj, 1u), u01 = N0(CLV_XVEL0, ci, cj + 1, 1u), u10 = N0(CLV_XVEL0, ci + 1, cj, 1u),
-              u11 = N0(CLV_XVEL0, ci + 1, cj + 1, 1u);
-  const Dn<4> v00 = N0(CLV_YVEL0, ci, cj, 2u), v10 = N0(CLV_YVEL0, ci + 1, cj, 2u), v01 = N0(CLV_YVEL0, ci, cj + 1, 2u),
-              v11 = N0(CLV_YVEL0, ci + 1, cj + 1, 2u);
-  Dn<4> w00 = cst<4>(0.0), w01 = w00, w10 = w00, w11 = w00, z00 = w00, z10 = w00, z01 = w00, z11 = w00;
-  if (!predict) {
-    w00 = N0(CLV_XVEL1, ci, cj, 4u), w01 = N0(CLV_XVEL1, ci, cj + 1, 4u), w10 = N0(CLV_XVEL1, ci + 1, cj, 4u),
-    w11 = N0(CLV_XVEL1, ci + 1, cj + 1, 4u);
-    z00 = N0(CLV_YVEL1, ci, cj, 8u), z10 = N0(CLV_YVEL1, ci + 1, cj, 8u), z01 = N0(CLV_YVEL1, ci, cj + 1, 8u),
-    z11 = N0(CLV_YVEL1, ci + 1, cj + 1, 8u);
-  }
-  const bool own = mode == 1;
-  const Dn<4> rho0 = own ? var<4>(PA(CLV_DENSITY0, ci, cj), 1u) : cst<4>(PA(CLV_DENSITY0, ci, cj));
-  const Dn<4> e0 = own ? var<4>(PA(CLV_ENERGY0, ci, cj), 2u) : cst<4>(PA(CLV_ENERGY0, ci, cj));
-  const Dn<4> pr = own ? var<4>(PA(CLV_PRESSURE, ci, cj), 4u) : cst<4>(PA(CLV_PRESSURE, ci, cj));
-  const Dn<4> q = own ? var<4>(PA(CLV_VISCOSITY, ci, cj), 8u) : cst<4>(PA(CLV_VISCOSITY, ci, cj));
-  el_pdv<4>(predict, dt, PA(CLV_XAREA, ci, cj), PA(CLV_XAREA, ci + 1, cj), PA(CLV_YAREA, ci, cj), PA(CLV_YAREA, ci, cj + 1), u00, u01,
-            u10, u11, v00, v10, v01, v11, w00, w01, w10, w11, z00, z10, z01, z11, PA(CLV_VOLUME, ci, cj), rho0, e0, pr, q, r1, e1);
+// P1, cells [2, nx+1]^2: own inputs (rho0, e0, p, q) and the total-flux adjoint
+// (-> a.work1); consumes the rho1, e1 adjoints
+__global__ void adj_pdv_cell_k(Dev p, Dev a, int predict, double dt) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > p.nx + 1 || j > p.ny + 1) return;
+  const double rb = AA(CLV_DENSITY1, i, j), eb = AA(CLV_ENERGY1, i, j);
+  const double tf = pdv_total_flux(p, predict != 0, dt, i, j);
+  Dn<5> r1, e1;
+  el_pdv_core<5>(PA(CLV_VOLUME, i, j), var<5>(PA(CLV_DENSITY0, i, j), 1u), var<5>(PA(CLV_ENERGY0, i, j), 2u), var<5>(PA(CLV_PRESSURE, i, j), 4u),
+                 var<5>(PA(CLV_VISCOSITY, i, j), 8u), var<5>(tf, 16u), r1, e1);
+  AA(CLV_DENSITY0, i, j) += r1.d[0] * rb + e1.d[0] * eb;
+  AA(CLV_ENERGY0, i, j) += r1.d[1] * rb + e1.d[1] * eb;
+  AA(CLV_PRESSURE, i, j) += r1.d[2] * rb + e1.d[2] * eb;
+  AA(CLV_VISCOSITY, i, j) += r1.d[3] * rb + e1.d[3] * eb;
+  AA(CLV_WORK1, i, j) = r1.d[4] * rb + e1.d[4] * eb;
+  AA(CLV_DENSITY1, i, j) = 0.0;
+  AA(CLV_ENERGY1, i, j) = 0.0;
 }
 
-// site (i, j) in [2, nx+2] x [2, ny+2]: node velocities (and, inside, the cell's own inputs)
-__global__ void adj_pdv_k(Dev p, Dev a, int predict, double dt) {
+// P2, nodes [2, nx+2]^2.  d(face flux)/d(u of a face node) = area * 0.25 dt in
+// both forms (the predictor counts u0 twice and halves).  Node (i, j) is u00 of
+// cell (i, j) (left face), u01 of (i, j-1) (left), u10 of (i-1, j) (right), u11 of
+// (i-1, j-1) (right); v00 of (i, j) (bottom), v10 of (i-1, j) (bottom), v01 of
+// (i, j-1) (top), v11 of (i-1, j-1) (top).
+__global__ void adj_pdv_node_k(Dev p, Dev a, int predict, double dt) {
   const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = p.nx, ny = p.ny;
   if (i > nx + 2 || j > ny + 2) return;
-  double g[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int cj = j - 1; cj <= j; ++cj)
-    for (int ci = i - 1; ci <= i; ++ci) {
-      if (!in(ci, 2, nx + 1) || !in(cj, 2, ny + 1)) continue;
-      const double rb = AA(CLV_DENSITY1, ci, cj), eb = AA(CLV_ENERGY1, ci, cj);
-      Dn<4> r1, e1;
-      pdv_at(p, predict != 0, dt, ci, cj, 0, i, j, r1, e1);
-      for (int k = 0; k < 4; ++k) g[k] += r1.d[k] * rb + e1.d[k] * eb;
-    }
-  AA(CLV_XVEL0, i, j) += g[0];
-  AA(CLV_YVEL0, i, j) += g[1];
+  auto T = [&](int ci, int cj) { return in(ci, 2, nx + 1) && in(cj, 2, ny + 1) ? AA(CLV_WORK1, ci, cj) : 0.0; };
+  const double k = 0.25 * dt;
+  const double gu = k * (PA(CLV_XAREA, i, j) * (T(i - 1, j) - T(i, j)) + PA(CLV_XAREA, i, j - 1) * (T(i - 1, j - 1) - T(i, j - 1)));
+  const double gv = k * (PA(CLV_YAREA, i, j) * (T(i, j - 1) - T(i, j)) + PA(CLV_YAREA, i - 1, j) * (T(i - 1, j - 1) - T(i - 1, j)));
+  AA(CLV_XVEL0, i, j) += gu;
+  AA(CLV_YVEL0, i, j) += gv;
   if (!predict) {
-    AA(CLV_XVEL1, i, j) += g[2];
-    AA(CLV_YVEL1, i, j) += g[3];
-  }
-  if (i <= nx + 1 && j <= ny + 1) {
-    const double rb = AA(CLV_DENSITY1, i, j), eb = AA(CLV_ENERGY1, i, j);
-    Dn<4> r1, e1;
-    pdv_at(p, predict != 0, dt, i, j, 1, 0, 0, r1, e1);
-    AA(CLV_DENSITY0, i, j) += r1.d[0] * rb + e1.d[0] * eb;
-    AA(CLV_ENERGY0, i, j) += r1.d[1] * rb + e1.d[1] * eb;
-    AA(CLV_PRESSURE, i, j) += r1.d[2] * rb + e1.d[2] * eb;
-    AA(CLV_VISCOSITY, i, j) += r1.d[3] * rb + e1.d[3] * eb;
+    AA(CLV_XVEL1, i, j) += gu;
+    AA(CLV_YVEL1, i, j) += gv;
   }
 }
 
-// ---- accelerate ---------------------------------------------------------------------------
-// node (ni, nj) with the inputs of cell (ci, cj) seeded [rho0, p, q]
-__device__ void accel_at(const Dev &p, double halfdt, int ni, int nj, int ci, int cj, Dn<3> &u1, Dn<3> &v1) {
-  auto C3 = [&](int fid, int i, int j, unsigned bit) { return i == ci && j == cj ? var<3>(PA(fid, i, j), bit) : cst<3>(PA(fid, i, j)); };
-  const int i = ni, j = nj;
-  el_accel<3>(halfdt, C3(CLV_DENSITY0, i - 1, j - 1, 1u), C3(CLV_DENSITY0, i - 1, j, 1u), C3(CLV_DENSITY0, i, j, 1u),
-              C3(CLV_DENSITY0, i, j - 1, 1u), PA(CLV_VOLUME, i - 1, j - 1), PA(CLV_VOLUME, i - 1, j), PA(CLV_VOLUME, i, j),
-              PA(CLV_VOLUME, i, j - 1), PA(CLV_XAREA, i, j), PA(CLV_XAREA, i, j - 1), PA(CLV_YAREA, i, j), PA(CLV_YAREA, i - 1, j),
-              C3(CLV_PRESSURE, i, j, 2u), C3(CLV_PRESSURE, i - 1, j, 2u), C3(CLV_PRESSURE, i, j - 1, 2u), C3(CLV_PRESSURE, i - 1, j - 1, 2u),
-              C3(CLV_VISCOSITY, i, j, 4u), C3(CLV_VISCOSITY, i - 1, j, 4u), C3(CLV_VISCOSITY, i, j - 1, 4u),
-              C3(CLV_VISCOSITY, i - 1, j - 1, 4u), cst<3>(PA(CLV_XVEL0, i, j)), cst<3>(PA(CLV_YVEL0, i, j)), u1, v1);
+// ---- accelerate, src/serial/accelerate.cpp:43-53 ------------------------------------------
+// A1, nodes [2, nx+2]^2: u1 = u0 - s (Pu + Qu), v1 = v0 - s (Pv + Qv) with
+// s = halfdt / (0.25 M), M = sum rho vol over the node's cells.  u0 += u1_bar;
+// a = -s u1_bar, b = -s v1_bar (pressure/viscosity shares) -> a.work1/2; the
+// mass share R = s_bar ds/dM -> a.work3; consumes u1_bar, v1_bar.
+__global__ void adj_accel_node_k(Dev p, Dev a, double dt) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > p.nx + 2 || j > p.ny + 2) return;
+  const double halfdt = 0.5 * dt;
+  const double ub = AA(CLV_XVEL1, i, j), vb = AA(CLV_YVEL1, i, j);
+  AA(CLV_XVEL0, i, j) += ub;
+  AA(CLV_YVEL0, i, j) += vb;
+  AA(CLV_XVEL1, i, j) = 0.0;
+  AA(CLV_YVEL1, i, j) = 0.0;
+  const double m = (PA(CLV_DENSITY0, i - 1, j - 1) * PA(CLV_VOLUME, i - 1, j - 1) + PA(CLV_DENSITY0, i - 1, j) * PA(CLV_VOLUME, i - 1, j) +
+                    PA(CLV_DENSITY0, i, j) * PA(CLV_VOLUME, i, j) + PA(CLV_DENSITY0, i, j - 1) * PA(CLV_VOLUME, i, j - 1)) *
+                   0.25;
+  const double s = halfdt / m;
+  auto D = [&](int fid) {  // x and y differences of a cell field around the node
+    const double f00 = PA(fid, i, j), fm0 = PA(fid, i - 1, j), f0m = PA(fid, i, j - 1), fmm = PA(fid, i - 1, j - 1);
+    return double2{PA(CLV_XAREA, i, j) * (f00 - fm0) + PA(CLV_XAREA, i, j - 1) * (f0m - fmm),
+                   PA(CLV_YAREA, i, j) * (f00 - f0m) + PA(CLV_YAREA, i - 1, j) * (fm0 - fmm)};
+  };
+  const double2 dp = D(CLV_PRESSURE), dq = D(CLV_VISCOSITY);
+  const double sb = -(ub * (dp.x + dq.x) + vb * (dp.y + dq.y));
+  AA(CLV_WORK1, i, j) = -s * ub;
+  AA(CLV_WORK2, i, j) = -s * vb;
+  AA(CLV_WORK3, i, j) = sb * (-s / m) * 0.25;  // ds/d(rho_c) = -s / m * 0.25 vol_c
 }
 
-// site (i, j) in [1, nx+2] x [1, ny+2]: cell (i, j) inputs from nodes (i..i+1, j..j+1); node (i, j) u0, v0
-__global__ void adj_accelerate_k(Dev p, Dev a, double dt) {
+// A2, cells [1, nx+2]^2: cell c is 00 of node c, m0 of node c+x, 0m of node c+y,
+// mm of node c+x+y
+__global__ void adj_accel_cell_k(Dev p, Dev a) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x, j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = p.nx, ny = p.ny;
   if (i > nx + 2 || j > ny + 2) return;
-  const double halfdt = 0.5 * dt;
-  double g[3] = {0.0, 0.0, 0.0};
-  for (int nj = j; nj <= j + 1; ++nj)
-    for (int ni = i; ni <= i + 1; ++ni) {
-      if (!in(ni, 2, nx + 2) || !in(nj, 2, ny + 2)) continue;
-      const double ub = AA(CLV_XVEL1, ni, nj), vb = AA(CLV_YVEL1, ni, nj);
-      Dn<3> u1, v1;
-      accel_at(p, halfdt, ni, nj, i, j, u1, v1);
-      for (int k = 0; k < 3; ++k) g[k] += u1.d[k] * ub + v1.d[k] * vb;
-    }
-  AA(CLV_DENSITY0, i, j) += g[0];
-  AA(CLV_PRESSURE, i, j) += g[1];
-  AA(CLV_VISCOSITY, i, j) += g[2];
-  if (i >= 2 && j >= 2) {  // d u1 / d u0 = 1
-    AA(CLV_XVEL0, i, j) += AA(CLV_XVEL1, i, j);
-    AA(CLV_YVEL0, i, j) += AA(CLV_YVEL1, i, j);
-  }
+  auto W = [&](int fid, int ni, int nj) { return in(ni, 2, nx + 2) && in(nj, 2, ny + 2) ? AA(fid, ni, nj) : 0.0; };
+  const double r = W(CLV_WORK3, i, j) + W(CLV_WORK3, i + 1, j) + W(CLV_WORK3, i, j + 1) + W(CLV_WORK3, i + 1, j + 1);
+  AA(CLV_DENSITY0, i, j) += PA(CLV_VOLUME, i, j) * r;
+  const double xa0 = PA(CLV_XAREA, i, j), xa1 = PA(CLV_XAREA, i + 1, j), ya0 = PA(CLV_YAREA, i, j), ya1 = PA(CLV_YAREA, i, j + 1);
+  const double g = W(CLV_WORK1, i, j) * xa0 + W(CLV_WORK2, i, j) * ya0 - W(CLV_WORK1, i + 1, j) * xa1 + W(CLV_WORK2, i + 1, j) * ya0 +
+                   W(CLV_WORK1, i, j + 1) * xa0 - W(CLV_WORK2, i, j + 1) * ya1 - W(CLV_WORK1, i + 1, j + 1) * xa1 -
+                   W(CLV_WORK2, i + 1, j + 1) * ya1;
+  AA(CLV_PRESSURE, i, j) += g;
+  AA(CLV_VISCOSITY, i, j) += g;
 }
 
 // ---- flux_calc (linear), src/serial/flux_calc.cpp:34-41 ------------------------------------
@@ -535,6 +676,13 @@ struct Sweep {
   int dir;
   __device__ int64_t at(const Dev &d, int s, int t) const { return dir == 1 ? (int64_t)t * d.pitch + s : (int64_t)s * d.pitch + t; }
 };
+// (s, t) of this thread from offsets (s0, t0): lanes run along x -- s for dir x, t for
+// dir y -- so the sweeps of both directions read whole rows; the grid is sgrid()'s
+__device__ __forceinline__ void sweep_pos(int dir, int s0, int t0, int &s, int &t) {
+  const int gx = blockIdx.x * blockDim.x + threadIdx.x, gy = blockIdx.y * blockDim.y + threadIdx.y;
+  s = s0 + (dir == 1 ? gx : gy);
+  t = t0 + (dir == 1 ? gy : gx);
+}
 #define PS(fid, s, t) p.f[fid][sw.at(p, s, t)]
 #define AS(fid, s, t) a.f[fid][sw.at(a, s, t)]
 
@@ -554,151 +702,160 @@ __device__ __forceinline__ double pre_at(const Dev &p, int sweep, const Sweep &s
   return sw.dir == 1 ? cell_pre_vol(p, sweep, 1, s, t) : cell_pre_vol(p, sweep, 2, t, s);
 }
 
-// face f (along the sweep) of line t: roles and the flux with the inputs of cell c seeded
-// [rho (bit 1), e (bit 2), pre (bit 4, donor only), vf (bit 8, when seed_vf)]
-__device__ void face_at(const Dev &p, int sweep, const Sweep &sw, int f, int t, int c, bool seed_vf, Dn<4> &mf, Dn<4> &ef) {
-  const int vf_id = sw.dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y;
-  const int smax = sw.dir == 1 ? p.nx : p.ny;
-  const double vfv = PS(vf_id, f, t);
+// the cells face f reads: upwind, donor, downwind (src/serial/advec_cell.cpp:74-83, the
+// upwind index clamped at smax+2), and the mesh index of the limiter's width ratio
+struct FaceRoles {
   int uw, dn, dw, dif;
-  if (vfv > 0.0) {
-    uw = f - 2, dn = f - 1, dw = f, dif = dn;
-  } else {
-    uw = min(f + 1, smax + 2), dn = f, dw = f - 1, dif = uw;
-  }
-  auto R = [&](int s) { return s == c ? var<4>(PS(CLV_DENSITY1, s, t), 1u) : cst<4>(PS(CLV_DENSITY1, s, t)); };
-  auto E = [&](int s) { return s == c ? var<4>(PS(CLV_ENERGY1, s, t), 2u) : cst<4>(PS(CLV_ENERGY1, s, t)); };
-  const double prev = pre_at(p, sweep, sw, dn, t);
-  const Dn<4> pre_dn = dn == c ? var<4>(prev, 4u) : cst<4>(prev);
-  const Dn<4> vf = seed_vf ? var<4>(vfv, 8u) : cst<4>(vfv);
-  const int vd = sw.dir == 1 ? CLV_VERTEXDX : CLV_VERTEXDY;
-  el_face<4>(vf, pre_dn, R(uw), R(dn), R(dw), E(uw), E(dn), E(dw), PM(vd, f), PM(vd, dif), mf, ef);
+};
+__device__ __forceinline__ FaceRoles face_roles(double vf, int f, int smax) {
+  if (vf > 0.0) return FaceRoles{f - 2, f - 1, f, f - 1};
+  const int uw = min(f + 1, smax + 2);
+  return FaceRoles{uw, f, f - 1, uw};
 }
 
-// mass and energy flux values of face f
-__device__ void face_val(const Dev &p, int sweep, const Sweep &sw, int f, int t, double &mf, double &ef) {
-  Dn<4> m, e;
-  face_at(p, sweep, sw, f, t, -100, false, m, e);
-  mf = m.v;
-  ef = e.v;
-}
-
-// cell update of cell c on line t, seeded [rho, e, pre, mf_l, mf_r, ef_l, ef_r, vf_l, vf_r] selection:
-// which = 0: own [rho(1), e(2), pre(4)]; which = 1: its left face [mf_l(1), ef_l(2), vf_l(4)];
-// which = 2: its right face [mf_r(1), ef_r(2), vf_r(4)]
-__device__ void update_at(const Dev &p, int sweep, const Sweep &sw, int c, int t, int which, Dn<3> &rn, Dn<3> &en) {
-  const int vf_id = sw.dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y;
-  double mfl, efl, mfr, efr;
-  face_val(p, sweep, sw, c, t, mfl, efl);
-  face_val(p, sweep, sw, c + 1, t, mfr, efr);
-  const double pre = pre_at(p, sweep, sw, c, t);
-  auto S = [&](double v, int w, unsigned bit) { return which == w ? var<3>(v, bit) : cst<3>(v); };
-  el_cell_update<3>(S(PS(CLV_DENSITY1, c, t), 0, 1u), S(PS(CLV_ENERGY1, c, t), 0, 2u), S(pre, 0, 4u), S(mfl, 1, 1u), S(mfr, 2, 1u),
-                    S(efl, 1, 2u), S(efr, 2, 2u), S(PS(vf_id, c, t), 1, 4u), S(PS(vf_id, c + 1, t), 2, 4u), rn, en);
-}
-
-// K1: faces f in [2, smax+2] of lines t in [2, tmax+1]: adjoints of mf, ef (total,
-// into work1 / work2) and of vf from the cell updates (into work3); faces
-// smax+3 carry only the external mass-flux adjoint.
-__global__ void adj_cell_faces_k(Dev p, Dev a, int sweep, int dir) {
+// C0 (primal), positions s in [0, smax+3] of lines [2, tmax+1]: pre_vol(s) -> p.work1;
+// for faces s >= 2 the forward's mass and energy flux -> p.work2, p.work3
+__global__ void adj_cell_values_k(Dev p, int sweep, int dir) {
   const Sweep sw{dir};
   const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  const int f = 2 + blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int s, t;
+  sweep_pos(dir, 0, 2, s, t);
+  if (s > smax + 3 || t > tmax + 1) return;
+  PS(CLV_WORK1, s, t) = pre_at(p, sweep, sw, s, t);
+  if (s < 2) return;
+  const int vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y, vd = dir == 1 ? CLV_VERTEXDX : CLV_VERTEXDY;
+  const double vfv = PS(vf_id, s, t);
+  const FaceRoles r = face_roles(vfv, s, smax);
+  Dn<0> mf, ef;
+  el_face<0>(cst<0>(vfv), cst<0>(pre_at(p, sweep, sw, r.dn, t)), cst<0>(PS(CLV_DENSITY1, r.uw, t)), cst<0>(PS(CLV_DENSITY1, r.dn, t)),
+             cst<0>(PS(CLV_DENSITY1, r.dw, t)), cst<0>(PS(CLV_ENERGY1, r.uw, t)), cst<0>(PS(CLV_ENERGY1, r.dn, t)),
+             cst<0>(PS(CLV_ENERGY1, r.dw, t)), PM(vd, s), PM(vd, r.dif), mf, ef);
+  PS(CLV_WORK2, s, t) = mf.v;
+  PS(CLV_WORK3, s, t) = ef.v;
+}
+
+// the update of cell c, src/serial/advec_cell.cpp:120-125 / :211-216:
+// rho' = (rho pre + mf_l - mf_r) / (pre + vf_l - vf_r),
+// e' = (e rho pre + ef_l - ef_r) / (rho pre + mf_l - mf_r).
+struct CellUpd {
+  double rho, e, pre, post_mass, advec_vol, rho_new, e_new;
+};
+__device__ __forceinline__ CellUpd cell_update(const Dev &p, const Sweep &sw, int vf_id, int c, int t) {
+  CellUpd u;
+  u.rho = PS(CLV_DENSITY1, c, t);
+  u.e = PS(CLV_ENERGY1, c, t);
+  u.pre = PS(CLV_WORK1, c, t);
+  const double pre_mass_s = u.rho * u.pre;
+  u.post_mass = pre_mass_s + PS(CLV_WORK2, c, t) - PS(CLV_WORK2, c + 1, t);
+  u.e_new = (u.e * pre_mass_s + PS(CLV_WORK3, c, t) - PS(CLV_WORK3, c + 1, t)) / u.post_mass;
+  u.advec_vol = u.pre + PS(vf_id, c, t) - PS(vf_id, c + 1, t);
+  u.rho_new = u.post_mass / u.advec_vol;
+  return u;
+}
+
+// C1, faces f in [2, smax+3] of lines [2, tmax+1]: the face's total mass / energy flux
+// adjoints (external + the updates of cells f and f-1), then the face differentiated
+// once (face_bar): shares of its vf (+ the updates' vf terms) -> a.work3, donor pre_vol -> a.work1,
+// rho of upwind / donor / downwind -> a.work2 / a.work4 / a.work5, e of them ->
+// a.work6 / a.work7 / p.work4.  Consumes the mass-flux adjoint.
+__global__ void adj_cell_face_k(Dev p, Dev a, int sweep, int dir) {
+  const Sweep sw{dir};
+  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
+  int f, t;
+  sweep_pos(dir, 2, 2, f, t);
   if (f > smax + 3 || t > tmax + 1) return;
-  const int mf_id = dir == 1 ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y;
+  const int mf_id = dir == 1 ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y, vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y;
+  const int vd = dir == 1 ? CLV_VERTEXDX : CLV_VERTEXDY;
   double mfb = AS(mf_id, f, t), efb = 0.0, vfb = 0.0;
-  // f is the left face of cell f and the right face of cell f - 1
-  if (f <= smax + 1) {
-    Dn<3> rn, en;
-    update_at(p, sweep, sw, f, t, 1, rn, en);
+  AS(mf_id, f, t) = 0.0;
+  // d rho'/d mf_l = 1 / advec_vol, d e'/d mf_l = -e' / post_mass, d e'/d ef_l = 1 / post_mass,
+  // d rho'/d vf_l = -rho' / advec_vol; the right face's are the negatives
+  if (f <= smax + 1) {  // left face of cell f
+    const CellUpd u = cell_update(p, sw, vf_id, f, t);
     const double rb = AS(CLV_DENSITY1, f, t), eb = AS(CLV_ENERGY1, f, t);
-    mfb += rn.d[0] * rb + en.d[0] * eb;
-    efb += rn.d[1] * rb + en.d[1] * eb;
-    vfb += rn.d[2] * rb + en.d[2] * eb;
+    mfb += rb / u.advec_vol - eb * u.e_new / u.post_mass;
+    efb += eb / u.post_mass;
+    vfb -= rb * u.rho_new / u.advec_vol;
   }
-  if (f - 1 >= 2 && f - 1 <= smax + 1) {
-    Dn<3> rn, en;
-    update_at(p, sweep, sw, f - 1, t, 2, rn, en);
+  if (f - 1 >= 2 && f - 1 <= smax + 1) {  // right face of cell f - 1
+    const CellUpd u = cell_update(p, sw, vf_id, f - 1, t);
     const double rb = AS(CLV_DENSITY1, f - 1, t), eb = AS(CLV_ENERGY1, f - 1, t);
-    mfb += rn.d[0] * rb + en.d[0] * eb;
-    efb += rn.d[1] * rb + en.d[1] * eb;
-    vfb += rn.d[2] * rb + en.d[2] * eb;
+    mfb -= rb / u.advec_vol - eb * u.e_new / u.post_mass;
+    efb -= eb / u.post_mass;
+    vfb += rb * u.rho_new / u.advec_vol;
   }
-  AS(CLV_WORK1, f, t) = mfb;
-  AS(CLV_WORK2, f, t) = efb;
-  AS(CLV_WORK3, f, t) = vfb;
+  const double vfv = PS(vf_id, f, t);
+  const FaceRoles r = face_roles(vfv, f, smax);
+  double c[8];
+  face_bar(vfv, PS(CLV_WORK1, r.dn, t), PS(CLV_DENSITY1, r.uw, t), PS(CLV_DENSITY1, r.dn, t), PS(CLV_DENSITY1, r.dw, t), PS(CLV_ENERGY1, r.uw, t),
+           PS(CLV_ENERGY1, r.dn, t), PS(CLV_ENERGY1, r.dw, t), PM(vd, f), PM(vd, r.dif), mfb, efb, c);
+  AS(CLV_WORK3, f, t) = vfb + c[0];
+  AS(CLV_WORK1, f, t) = c[1];
+  AS(CLV_WORK2, f, t) = c[2];
+  AS(CLV_WORK4, f, t) = c[3];
+  AS(CLV_WORK5, f, t) = c[4];
+  AS(CLV_WORK6, f, t) = c[5];
+  AS(CLV_WORK7, f, t) = c[6];
+  PS(CLV_WORK4, f, t) = c[7];
 }
 
-// K2: cells c in [0, smax+3] of lines t in [2, tmax+1]: new adjoints of rho1, e1
-// (own update part for interior cells + every face that reads the cell) into
-// work4 / work5, pre_vol adjoint into work6; the face's own vf term into work3.
-__global__ void adj_cell_cells_k(Dev p, Dev a, int sweep, int dir) {
+// C2, cells c in [0, smax+3] of lines [2, tmax+1]: new rho1, e1 adjoints (own update
+// for interior cells, else passed through, plus the shares of faces [c-1, c+2] that
+// read the cell), written in place; the pre_vol adjoint -> p.work5
+__global__ void adj_cell_cell_k(Dev p, Dev a, int sweep, int dir) {
   const Sweep sw{dir};
   const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int c, t;
+  sweep_pos(dir, 0, 2, c, t);
   if (c > smax + 3 || t > tmax + 1) return;
-  double rb = 0.0, eb = 0.0, pb = 0.0;
-  const bool interior = c >= 2 && c <= smax + 1;
-  if (interior) {
-    Dn<3> rn, en;
-    update_at(p, sweep, sw, c, t, 0, rn, en);
+  const int vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y;
+  double rb, eb, pb = 0.0;
+  if (c >= 2 && c <= smax + 1) {
+    // d rho'/d rho = pre / advec_vol, d rho'/d pre = (rho - rho') / advec_vol,
+    // d e'/d e = rho pre / post_mass, d e'/d rho = (e - e') pre / post_mass,
+    // d e'/d pre = (e - e') rho / post_mass
+    const CellUpd u = cell_update(p, sw, vf_id, c, t);
     const double rnb = AS(CLV_DENSITY1, c, t), enb = AS(CLV_ENERGY1, c, t);
-    rb = rn.d[0] * rnb + en.d[0] * enb;
-    eb = rn.d[1] * rnb + en.d[1] * enb;
-    pb = rn.d[2] * rnb + en.d[2] * enb;
+    const double de = (u.e - u.e_new) / u.post_mass;
+    rb = rnb * u.pre / u.advec_vol + enb * de * u.pre;
+    eb = enb * (u.rho * u.pre) / u.post_mass;
+    pb = rnb * (u.rho - u.rho_new) / u.advec_vol + enb * de * u.rho;
   } else {
     rb = AS(CLV_DENSITY1, c, t);  // ghost cells: the kernel leaves them unchanged
     eb = AS(CLV_ENERGY1, c, t);
   }
-  // faces f in [c - 1, c + 2] (within [2, smax + 3]) read cell c
   for (int f = max(2, c - 1); f <= min(smax + 3, c + 2); ++f) {
-    const double mfb = AS(CLV_WORK1, f, t), efb = AS(CLV_WORK2, f, t);
-    if (mfb == 0.0 && efb == 0.0) continue;
-    Dn<4> mf, ef;
-    face_at(p, sweep, sw, f, t, c, false, mf, ef);
-    rb += mf.d[0] * mfb + ef.d[0] * efb;
-    eb += mf.d[1] * mfb + ef.d[1] * efb;
-    pb += mf.d[2] * mfb + ef.d[2] * efb;
+    const FaceRoles r = face_roles(PS(vf_id, f, t), f, smax);
+    if (r.uw == c) {
+      rb += AS(CLV_WORK2, f, t);
+      eb += AS(CLV_WORK6, f, t);
+    }
+    if (r.dn == c) {
+      rb += AS(CLV_WORK4, f, t);
+      eb += AS(CLV_WORK7, f, t);
+      pb += AS(CLV_WORK1, f, t);
+    }
+    if (r.dw == c) {
+      rb += AS(CLV_WORK5, f, t);
+      eb += PS(CLV_WORK4, f, t);
+    }
   }
-  AS(CLV_WORK4, c, t) = rb;
-  AS(CLV_WORK5, c, t) = eb;
-  AS(CLV_WORK6, c, t) = pb;
+  AS(CLV_DENSITY1, c, t) = rb;
+  AS(CLV_ENERGY1, c, t) = eb;
+  PS(CLV_WORK5, c, t) = pb;
 }
 
-// K2b: faces f in [2, smax+3]: d(mf, ef)/d(vf) of the face itself, added to work3
-__global__ void adj_cell_face_vf_k(Dev p, Dev a, int sweep, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  const int f = 2 + blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (f > smax + 3 || t > tmax + 1) return;
-  const double mfb = AS(CLV_WORK1, f, t), efb = AS(CLV_WORK2, f, t);
-  Dn<4> mf, ef;
-  face_at(p, sweep, sw, f, t, -100, true, mf, ef);
-  AS(CLV_WORK3, f, t) += mf.d[3] * mfb + ef.d[3] * efb;
-}
-
-// K3: install the new rho1, e1 adjoints (cells [0, smax+3] of lines [2, tmax+1]),
-// consume the mass-flux adjoint (faces [2, smax+3])
-__global__ void adj_cell_install_k(Dev a, int dir) {
+// C3: pre_vol of cells [0, smax+3] x lines [2, tmax+1] (its adjoint in p.work5).  vf
+// along the sweep, faces [0, smax+4] of those lines: vf(f) += pb(f-1) - pb(f) + the
+// face's own share (a.work3); sweep 1 also the other direction's flux:
+// vf_t(s, t) += pb(s, t-1) - pb(s, t), lines [1, tmax+2].
+__global__ void adj_cell_vol_k(Dev p, Dev a, int sweep, int dir) {
   const Sweep sw{dir};
   const int smax = dir == 1 ? a.nx : a.ny, tmax = dir == 1 ? a.ny : a.nx;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (c > smax + 3 || t > tmax + 1) return;
-  AS(CLV_DENSITY1, c, t) = AS(CLV_WORK4, c, t);
-  AS(CLV_ENERGY1, c, t) = AS(CLV_WORK5, c, t);
-  if (c >= 2) AS(dir == 1 ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y, c, t) = 0.0;
-}
-
-// K4: pre_vol is computed on every cell [0, smax+3] x [0, tmax+3] and read where
-// work6 (its adjoint, lines [2, tmax+1]) is nonzero.  vf along the sweep: faces
-// [0, smax+4] of those lines: vf(f) += pb(f-1) - pb(f) + work3(f); sweep 1 also
-// the other direction's flux: vf_t(s, t) += pb(s, t-1) - pb(s, t).
-__global__ void adj_cell_vol_k(Dev a, int sweep, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? a.nx : a.ny, tmax = dir == 1 ? a.ny : a.nx;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x, t = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  int s, t;
+  sweep_pos(dir, 0, 1, s, t);
   if (s > smax + 4 || t > tmax + 2) return;
-  auto PB = [&](int ss, int tt) { return ss >= 0 && ss <= smax + 3 && tt >= 2 && tt <= tmax + 1 ? AS(CLV_WORK6, ss, tt) : 0.0; };
+  auto PB = [&](int ss, int tt) { return ss >= 0 && ss <= smax + 3 && tt >= 2 && tt <= tmax + 1 ? PS(CLV_WORK5, ss, tt) : 0.0; };
   const int vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y, vo_id = dir == 1 ? CLV_VOL_FLUX_Y : CLV_VOL_FLUX_X;
   if (t >= 2 && t <= tmax + 1) {
     double g = PB(s - 1, t) - PB(s, t);
@@ -733,129 +890,128 @@ __device__ __forceinline__ double node_mass_post_val(const Dev &p, int mom_sweep
                  PA(CLV_DENSITY1, i - 1, j - 1) * mom_post_vol(p, mom_sweep, i - 1, j - 1) +
                  PA(CLV_DENSITY1, i - 1, j) * mom_post_vol(p, mom_sweep, i - 1, j));
 }
-__device__ __forceinline__ double node_mass_pre_val(const Dev &p, const Sweep &sw, int mom_sweep, int s, int t) {
-  const int i = sw.dir == 1 ? s : t, j = sw.dir == 1 ? t : s;
-  return node_mass_post_val(p, mom_sweep, i, j) - node_flux_val(p, sw, s - 1, t) + node_flux_val(p, sw, s, t);
-}
 
-// momentum flux of node face f (between nodes f and f+1) of line t for velocity vel_id,
-// seeding node c's velocity (bit 1) and node_mass_pre (bit 2), or the face's node flux (bit 4)
-__device__ Dn<3> momflux_at(const Dev &p, const Sweep &sw, int mom_sweep, int vel_id, int f, int t, int c, bool seed_nf) {
-  const double nf = node_flux_val(p, sw, f, t);
-  int uw, dn, dw, dif;
-  if (nf < 0.0) {
-    uw = f + 2, dn = f + 1, dw = f, dif = dn;
-  } else {
-    uw = f - 1, dn = f, dw = f + 1, dif = uw;
-  }
-  const int cd = sw.dir == 1 ? CLV_CELLDX : CLV_CELLDY;
-  auto Vv = [&](int s) { return s == c ? var<3>(PS(vel_id, s, t), 1u) : cst<3>(PS(vel_id, s, t)); };
-  const double mpre = node_mass_pre_val(p, sw, mom_sweep, dn, t);
-  return el_momflux<3>(seed_nf ? var<3>(nf, 4u) : cst<3>(nf), dn == c ? var<3>(mpre, 2u) : cst<3>(mpre), PM(cd, f), PM(cd, dif), Vv(uw),
-                       Vv(dn), Vv(dw));
-}
-
-// M1: nodes s in [2, smax+2], lines t in [2, tmax+2]: for each velocity, the update's
-// derivatives: own velocity -> work1 (u) / work2 (v) (new adjoint), node_mass_pre
-// and node_mass_post adjoints -> work3 / work4; node-face momentum-flux adjoints are
-// gathered in M2 from the two nodes that read each face.
-__global__ void adj_mom_update_k(Dev p, Dev a, int mom_sweep, int dir) {
+// M0 (primal), nodes s in [0, smax+3] of lines [2, tmax+2]: node flux -> p.work1;
+// for s >= 1 node_mass_post -> p.work2, node_mass_pre = post - nf(s-1) + nf(s) -> p.work3
+__global__ void adj_mom_values_k(Dev p, int mom_sweep, int dir) {
   const Sweep sw{dir};
   const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  const int s = 2 + blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (s > smax + 2 || t > tmax + 2) return;
+  int s, t;
+  sweep_pos(dir, 0, 2, s, t);
+  if (s > smax + 3 || t > tmax + 2) return;
+  const double nf = node_flux_val(p, sw, s, t);
+  PS(CLV_WORK1, s, t) = nf;
+  if (s < 1) return;
   const int i = dir == 1 ? s : t, j = dir == 1 ? t : s;
-  const double mpost = node_mass_post_val(p, mom_sweep, i, j);
-  const double mpre = node_mass_pre_val(p, sw, mom_sweep, s, t);
+  const double post = node_mass_post_val(p, mom_sweep, i, j);
+  PS(CLV_WORK2, s, t) = post;
+  PS(CLV_WORK3, s, t) = post - node_flux_val(p, sw, s - 1, t) + nf;
+}
+
+// the nodes node face f (between nodes f and f+1) reads, by the sign of its node flux
+// (src/serial/advec_mom.cpp:114-125)
+__device__ __forceinline__ FaceRoles mom_roles(double nf, int f) {
+  if (nf < 0.0) return FaceRoles{f + 2, f + 1, f, f + 1};
+  return FaceRoles{f - 1, f, f + 1, f - 1};
+}
+// momentum flux of node face f for velocity vel_id from the stored node values
+__device__ __forceinline__ double momflux_value(const Dev &p, const Sweep &sw, int vel_id, int f, int t) {
+  const double nf = PS(CLV_WORK1, f, t);
+  const FaceRoles r = mom_roles(nf, f);
+  const int cd = sw.dir == 1 ? CLV_CELLDX : CLV_CELLDY;
+  return el_momflux<0>(cst<0>(nf), cst<0>(PS(CLV_WORK3, r.dn, t)), PM(cd, f), PM(cd, r.dif), cst<0>(PS(vel_id, r.uw, t)),
+                       cst<0>(PS(vel_id, r.dn, t)), cst<0>(PS(vel_id, r.dw, t)))
+      .v;
+}
+
+// M1, nodes s in [2, smax+2] of lines [2, tmax+2]: vel' = (vel mpre + mf(s-1) - mf(s)) / mpost
+// for both velocities: own-velocity adjoints -> a.work1 (u) / a.work2 (v), node_mass_pre
+// and node_mass_post adjoints -> a.work3 / a.work4
+__global__ void adj_mom_update_k(Dev p, Dev a, int dir) {
+  const Sweep sw{dir};
+  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
+  int s, t;
+  sweep_pos(dir, 2, 2, s, t);
+  if (s > smax + 2 || t > tmax + 2) return;
+  const double mpost = PS(CLV_WORK2, s, t), mpre = PS(CLV_WORK3, s, t);
   double mpreb = 0.0, mpostb = 0.0;
   for (int w = 0; w < 2; ++w) {
     const int vid = w == 0 ? CLV_XVEL1 : CLV_YVEL1;
-    const double vb = AS(vid, s, t);
-    const double mfl = momflux_at(p, sw, mom_sweep, vid, s - 1, t, -100, false).v, mfr = momflux_at(p, sw, mom_sweep, vid, s, t, -100, false).v;
-    // seeds: vel (1), mpre (2), mpost (4)
-    const Dn<3> out = el_vel_update<3>(var<3>(PS(vid, s, t), 1u), var<3>(mpre, 2u), cst<3>(mfl), cst<3>(mfr), var<3>(mpost, 4u));
-    AS(w == 0 ? CLV_WORK1 : CLV_WORK2, s, t) = out.d[0] * vb;
-    mpreb += out.d[1] * vb;
-    mpostb += out.d[2] * vb;
+    const double vb = AS(vid, s, t), vel = PS(vid, s, t);
+    const double vnew = (vel * mpre + momflux_value(p, sw, vid, s - 1, t) - momflux_value(p, sw, vid, s, t)) / mpost;
+    AS(w == 0 ? CLV_WORK1 : CLV_WORK2, s, t) = vb * mpre / mpost;
+    mpreb += vb * vel / mpost;
+    mpostb -= vb * vnew / mpost;
   }
   AS(CLV_WORK3, s, t) = mpreb;
   AS(CLV_WORK4, s, t) = mpostb;
 }
 
-// M2: node faces f in [1, smax+2] of lines [2, tmax+2]: momentum-flux adjoint of
-// each velocity from the updates of nodes f (as its right flux) and f + 1 (left).
-// d(update)/d(mf_l) = 1 / mpost, d/d(mf_r) = -1 / mpost.  -> work5 (u), work6 (v)
-__global__ void adj_mom_flux_k(Dev p, Dev a, int mom_sweep, int dir) {
+// MF, node faces f in [1, smax+2] of lines [2, tmax+2], one velocity: the face's
+// momentum-flux adjoint from the updates of nodes f (its right flux, -1 / mpost) and
+// f+1 (left, +1 / mpost), then the face differentiated once (momflux_bar): node-flux share ->
+// a.work5, donor node_mass_pre share -> p.work5 (first velocity writes, second adds);
+// velocity shares of upwind / donor / downwind -> a.work6 / a.work7 / p.work4
+template <int VEL, bool FIRST>
+__global__ void adj_mom_face_k(Dev p, Dev a, int dir) {
   const Sweep sw{dir};
   const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  const int f = 1 + blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int f, t;
+  sweep_pos(dir, 1, 2, f, t);
   if (f > smax + 2 || t > tmax + 2) return;
-  for (int w = 0; w < 2; ++w) {
-    const int vid = w == 0 ? CLV_XVEL1 : CLV_YVEL1;
-    double g = 0.0;
-    if (f >= 2) {  // right flux of node f
-      const int i = dir == 1 ? f : t, j = dir == 1 ? t : f;
-      g -= AS(vid, f, t) / node_mass_post_val(p, mom_sweep, i, j);
-    }
-    if (f + 1 <= smax + 2) {  // left flux of node f + 1
-      const int i = dir == 1 ? f + 1 : t, j = dir == 1 ? t : f + 1;
-      g += AS(vid, f + 1, t) / node_mass_post_val(p, mom_sweep, i, j);
-    }
-    AS(w == 0 ? CLV_WORK5 : CLV_WORK6, f, t) = g;
+  double fb = 0.0;
+  if (f >= 2) fb -= AS(VEL, f, t) / PS(CLV_WORK2, f, t);
+  if (f + 1 <= smax + 2) fb += AS(VEL, f + 1, t) / PS(CLV_WORK2, f + 1, t);
+  const double nf = PS(CLV_WORK1, f, t);
+  const FaceRoles r = mom_roles(nf, f);
+  const int cd = dir == 1 ? CLV_CELLDX : CLV_CELLDY;
+  double g[5];
+  momflux_bar(nf, PS(CLV_WORK3, r.dn, t), PM(cd, f), PM(cd, r.dif), PS(VEL, r.uw, t), PS(VEL, r.dn, t), PS(VEL, r.dw, t), fb, g);
+  if (FIRST) {
+    AS(CLV_WORK5, f, t) = g[0];
+    PS(CLV_WORK5, f, t) = g[1];
+  } else {
+    AS(CLV_WORK5, f, t) += g[0];
+    PS(CLV_WORK5, f, t) += g[1];
   }
+  AS(CLV_WORK6, f, t) = g[2];
+  AS(CLV_WORK7, f, t) = g[3];
+  PS(CLV_WORK4, f, t) = g[4];
 }
 
-// M3: nodes s in [0, smax+4] of lines [2, tmax+2]: through the momentum fluxes of
-// faces f in [s-2, s+1] (within [1, smax+2]) the node's velocity (both) and its
-// node_mass_pre; writes the velocities' new adjoints (work1/2 + flux part), adds to
-// work3 (mpre adjoint); the node-flux adjoint of the face itself -> work7.
-__global__ void adj_mom_nodes_k(Dev p, Dev a, int mom_sweep, int dir) {
+// MN, nodes s in [0, smax+4] of lines [2, tmax+2], one velocity: new adjoint (own
+// update share for updated nodes, else passed through, plus the shares of faces
+// [s-2, s+1] that read the node), installed in place.  After the second velocity,
+// the node_mass_pre adjoint (a.work3) gains the donor shares of faces s-1 and s.
+template <int VEL, bool LAST>
+__global__ void adj_mom_node_k(Dev p, Dev a, int dir) {
   const Sweep sw{dir};
   const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  int s, t;
+  sweep_pos(dir, 0, 2, s, t);
   if (s > smax + 4 || t > tmax + 2) return;
   const bool updated = s >= 2 && s <= smax + 2;
-  double gv[2] = {0.0, 0.0}, gm = 0.0, gnf = 0.0;
-  for (int w = 0; w < 2; ++w) {
-    const int vid = w == 0 ? CLV_XVEL1 : CLV_YVEL1, fb_id = w == 0 ? CLV_WORK5 : CLV_WORK6;
-    for (int f = max(1, s - 2); f <= min(smax + 2, s + 1); ++f) {
-      const double fb = AS(fb_id, f, t);
-      if (fb == 0.0) continue;
-      const Dn<3> mf = momflux_at(p, sw, mom_sweep, vid, f, t, s, false);
-      gv[w] += mf.d[0] * fb;
-      gm += mf.d[1] * fb;
+  double g = updated ? AS(VEL == CLV_XVEL1 ? CLV_WORK1 : CLV_WORK2, s, t) : AS(VEL, s, t);
+  double gm = 0.0;
+  for (int f = max(1, s - 2); f <= min(smax + 2, s + 1); ++f) {
+    const FaceRoles r = mom_roles(PS(CLV_WORK1, f, t), f);
+    if (r.uw == s) g += AS(CLV_WORK6, f, t);
+    if (r.dn == s) {
+      g += AS(CLV_WORK7, f, t);
+      if (LAST) gm += PS(CLV_WORK5, f, t);
     }
-    if (s >= 1 && s <= smax + 2) {  // the face s itself: its node flux
-      const double fb = AS(fb_id, s, t);
-      const Dn<3> mf = momflux_at(p, sw, mom_sweep, vid, s, t, -100, true);
-      gnf += mf.d[2] * fb;
-    }
+    if (r.dw == s) g += PS(CLV_WORK4, f, t);
   }
-  const double u_old = updated ? AS(CLV_WORK1, s, t) : AS(CLV_XVEL1, s, t);
-  const double v_old = updated ? AS(CLV_WORK2, s, t) : AS(CLV_YVEL1, s, t);
-  // staged in work1/work2 (velocities) until M4 installs them
-  AS(CLV_WORK1, s, t) = u_old + gv[0];
-  AS(CLV_WORK2, s, t) = v_old + gv[1];
-  AS(CLV_WORK3, s, t) = (updated ? AS(CLV_WORK3, s, t) : 0.0) + gm;
-  AS(CLV_WORK7, s, t) = gnf;
-}
-
-// M4: install velocities (nodes [0, smax+4] of lines [2, tmax+2])
-__global__ void adj_mom_install_k(Dev a, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? a.nx : a.ny, tmax = dir == 1 ? a.ny : a.nx;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x, t = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (s > smax + 4 || t > tmax + 2) return;
-  AS(CLV_XVEL1, s, t) = AS(CLV_WORK1, s, t);
-  AS(CLV_YVEL1, s, t) = AS(CLV_WORK2, s, t);
+  AS(VEL, s, t) = g;
+  if (LAST) AS(CLV_WORK3, s, t) = (updated ? AS(CLV_WORK3, s, t) : 0.0) + gm;
 }
 
 // M5: node masses.  node_mass_pre(s) = post(s) - nf(s-1) + nf(s) (nodes s in [1, smax+3])
 // and post = 0.25 sum rho1 * post_vol over the node's 4 cells.  Per cell (i, j) in
 // [0, nx+3] x [0, ny+3]: rho1 and post_vol adjoints from the nodes (i..i+1, j..j+1).
 // mpost adjoint of a node = work4 + work3 (mpre's post term); nodes with a mass but
-// no update have only the mpre part.  The post_vol adjoint of the cell goes to work5
-// (free after M3) and M5b gathers it onto the volume fluxes.
+// no update have only the mpre part.  The post_vol adjoint of the cell goes to
+// a.work6 and M5b gathers it onto the volume fluxes.
 __device__ __forceinline__ double mpost_bar(const Dev &a, const Sweep &sw, int smax, int tmax, int i, int j) {
   // node (i, j) in storage indices -> (s, t)
   const int s = sw.dir == 1 ? i : j, t = sw.dir == 1 ? j : i;
@@ -874,14 +1030,14 @@ __global__ void adj_mom_mass_k(Dev p, Dev a, int mom_sweep, int dir) {
     for (int ni = i; ni <= i + 1; ++ni) g += mpost_bar(a, sw, smax, tmax, ni, nj);
   g *= 0.25;
   AA(CLV_DENSITY1, i, j) += mom_post_vol(p, mom_sweep, i, j) * g;
-  AA(CLV_WORK5, i, j) = PA(CLV_DENSITY1, i, j) * g;  // post_vol adjoint of cell (i, j)
+  AA(CLV_WORK6, i, j) = PA(CLV_DENSITY1, i, j) * g;  // post_vol adjoint of cell (i, j)
 }
 
 // M5b: post_vol = vol + vfy(j+1) - vfy(j) (mom_sweep 1) or vol + vfx(i+1) - vfx(i)
 // (mom_sweep 2) on cells [0, nx+3] x [0, ny+3]: gather onto the fluxes
 __global__ void adj_mom_postvol_k(Dev a, int mom_sweep) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-  auto PV = [&](int ci, int cj) { return ci >= 0 && ci <= a.nx + 3 && cj >= 0 && cj <= a.ny + 3 ? AA(CLV_WORK5, ci, cj) : 0.0; };
+  auto PV = [&](int ci, int cj) { return ci >= 0 && ci <= a.nx + 3 && cj >= 0 && cj <= a.ny + 3 ? AA(CLV_WORK6, ci, cj) : 0.0; };
   if (mom_sweep == 1) {
     if (i > a.nx + 3 || j > a.ny + 4) return;
     AA(CLV_VOL_FLUX_Y, i, j) += PV(i, j - 1) - PV(i, j);
@@ -892,13 +1048,13 @@ __global__ void adj_mom_postvol_k(Dev a, int mom_sweep) {
 }
 
 // M6: node fluxes.  nf(s) enters mpre(s) (+1), mpre(s+1) (-1) and the momentum flux
-// of face s (work7).  nf(s, t) = 0.25 (mf of 4 faces) -> mass-flux adjoints.
+// of face s (a.work5).  nf(s, t) = 0.25 (mf of 4 faces) -> mass-flux adjoints.
 __device__ __forceinline__ double nf_bar(const Dev &a, const Sweep &sw, int smax, int tmax, int s, int t) {
   if (t < 2 || t > tmax + 2 || s < 0 || s > smax + 3) return 0.0;
   double g = 0.0;
   if (s >= 1) g += a.f[CLV_WORK3][sw.at(a, s, t)];                     // mpre(s) = ... + nf(s)
   if (s + 1 <= smax + 3) g -= a.f[CLV_WORK3][sw.at(a, s + 1, t)];      // mpre(s+1) = ... - nf(s)
-  if (s >= 1 && s <= smax + 2) g += a.f[CLV_WORK7][sw.at(a, s, t)];   // momentum flux of face s
+  if (s >= 1 && s <= smax + 2) g += a.f[CLV_WORK5][sw.at(a, s, t)];   // momentum flux of face s
   return g;
 }
 __global__ void adj_mom_nodeflux_k(Dev a, int dir) {
@@ -946,13 +1102,10 @@ __global__ void adj_ke_k(Dev p, Dev a, double keb) {
 // ---- launchers ----------------------------------------------------------------------------
 static dim3 grid2(int ni, int nj) { return dim3((ni + 63) / 64, (nj + 3) / 4, 1); }
 static const dim3 kBlk(64, 4, 1);
+// grid of a sweep kernel over ns positions along the sweep and nt lines (sweep_pos)
+static dim3 sgrid(int dir, int ns, int nt) { return dir == 1 ? grid2(ns, nt) : grid2(nt, ns); }
 
 static clv_status check(const char *what) { return hip_check(hipGetLastError(), what); }
-
-static clv_status zero(clv_chunk *a, int fid, int i0, int i1, int j0, int j1) {
-  hipLaunchKernelGGL(adj_zero_k, grid2(i1 - i0 + 1, j1 - j0 + 1), kBlk, 0, a->stream, a->d.f[fid], a->d.pitch, i0, i1, j0, j1);
-  return check("adjoint zero");
-}
 
 clv_status adj_ideal_gas(clv_chunk *p, clv_chunk *a, int predict) {
   const int nx = p->d.nx, ny = p->d.ny;
@@ -974,19 +1127,16 @@ clv_status adj_update_halo(clv_chunk *a, const int fields[CLV_NUM_HALO_FIELDS], 
 
 clv_status adj_viscosity(clv_chunk *p, clv_chunk *a) {
   const int nx = p->d.nx, ny = p->d.ny;
-  hipLaunchKernelGGL(adj_viscosity_k, grid2(nx + 2, ny + 2), kBlk, 0, a->stream, p->d, a->d);
-  clv_status st = check("adj viscosity");
-  if (st != CLV_OK) return st;
-  return zero(a, CLV_VISCOSITY, 2, nx + 1, 2, ny + 1);
+  hipLaunchKernelGGL(adj_visc_cell_k, grid2(nx, ny), kBlk, 0, a->stream, p->d, a->d);
+  hipLaunchKernelGGL(adj_visc_gather_k, grid2(nx + 2, ny + 2), kBlk, 0, a->stream, a->d);
+  return check("adj viscosity");
 }
 
 clv_status adj_PdV(clv_chunk *p, clv_chunk *a, int predict, double dt) {
   const int nx = p->d.nx, ny = p->d.ny;
-  hipLaunchKernelGGL(adj_pdv_k, grid2(nx + 1, ny + 1), kBlk, 0, a->stream, p->d, a->d, predict, dt);
-  clv_status st = check("adj PdV");
-  if (st != CLV_OK) return st;
-  if ((st = zero(a, CLV_DENSITY1, 2, nx + 1, 2, ny + 1)) != CLV_OK) return st;
-  return zero(a, CLV_ENERGY1, 2, nx + 1, 2, ny + 1);
+  hipLaunchKernelGGL(adj_pdv_cell_k, grid2(nx, ny), kBlk, 0, a->stream, p->d, a->d, predict, dt);
+  hipLaunchKernelGGL(adj_pdv_node_k, grid2(nx + 1, ny + 1), kBlk, 0, a->stream, p->d, a->d, predict, dt);
+  return check("adj PdV");
 }
 
 clv_status adj_revert(clv_chunk *p, clv_chunk *a) {
@@ -1000,11 +1150,9 @@ clv_status adj_revert(clv_chunk *p, clv_chunk *a) {
 
 clv_status adj_accelerate(clv_chunk *p, clv_chunk *a, double dt) {
   const int nx = p->d.nx, ny = p->d.ny;
-  hipLaunchKernelGGL(adj_accelerate_k, grid2(nx + 2, ny + 2), kBlk, 0, a->stream, p->d, a->d, dt);
-  clv_status st = check("adj accelerate");
-  if (st != CLV_OK) return st;
-  if ((st = zero(a, CLV_XVEL1, 2, nx + 2, 2, ny + 2)) != CLV_OK) return st;
-  return zero(a, CLV_YVEL1, 2, nx + 2, 2, ny + 2);
+  hipLaunchKernelGGL(adj_accel_node_k, grid2(nx + 1, ny + 1), kBlk, 0, a->stream, p->d, a->d, dt);
+  hipLaunchKernelGGL(adj_accel_cell_k, grid2(nx + 2, ny + 2), kBlk, 0, a->stream, p->d, a->d);
+  return check("adj accelerate");
 }
 
 clv_status adj_flux_calc(clv_chunk *p, clv_chunk *a, double dt) {
@@ -1012,29 +1160,31 @@ clv_status adj_flux_calc(clv_chunk *p, clv_chunk *a, double dt) {
   hipLaunchKernelGGL(adj_flux_calc_k, grid2(nx + 2, ny + 2), kBlk, 0, a->stream, p->d, a->d, dt);
   clv_status st = check("adj flux_calc");
   if (st != CLV_OK) return st;
-  if ((st = zero(a, CLV_VOL_FLUX_X, 2, nx + 2, 2, ny + 2)) != CLV_OK) return st;
-  return zero(a, CLV_VOL_FLUX_Y, 2, nx + 2, 2, ny + 2);
+  // the forward writes both fluxes over [2, nx+2] x [2, ny+2] (src/serial/flux_calc.cpp:34-41)
+  hipLaunchKernelGGL(adj_zero_k, grid2(nx + 1, ny + 1), kBlk, 0, a->stream, a->d.f[CLV_VOL_FLUX_X], a->d.pitch, 2, nx + 2, 2, ny + 2);
+  hipLaunchKernelGGL(adj_zero_k, grid2(nx + 1, ny + 1), kBlk, 0, a->stream, a->d.f[CLV_VOL_FLUX_Y], a->d.pitch, 2, nx + 2, 2, ny + 2);
+  return check("adj flux_calc zero");
 }
 
 clv_status adj_advec_cell(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
   const int smax = dir == 1 ? p->d.nx : p->d.ny, tmax = dir == 1 ? p->d.ny : p->d.nx;
-  // work arrays of the adjoint chunk are scratch: clear the ones read before written
-  const dim3 gf = grid2(smax + 2, tmax), gc = grid2(smax + 4, tmax), gv = grid2(smax + 5, tmax + 2);
-  hipLaunchKernelGGL(adj_cell_faces_k, gf, kBlk, 0, a->stream, p->d, a->d, sweep, dir);
-  hipLaunchKernelGGL(adj_cell_cells_k, gc, kBlk, 0, a->stream, p->d, a->d, sweep, dir);
-  hipLaunchKernelGGL(adj_cell_face_vf_k, gf, kBlk, 0, a->stream, p->d, a->d, sweep, dir);
-  hipLaunchKernelGGL(adj_cell_install_k, gc, kBlk, 0, a->stream, a->d, dir);
-  hipLaunchKernelGGL(adj_cell_vol_k, gv, kBlk, 0, a->stream, a->d, sweep, dir);
+  hipLaunchKernelGGL(adj_cell_values_k, sgrid(dir, smax + 4, tmax), kBlk, 0, a->stream, p->d, sweep, dir);
+  hipLaunchKernelGGL(adj_cell_face_k, sgrid(dir, smax + 2, tmax), kBlk, 0, a->stream, p->d, a->d, sweep, dir);
+  hipLaunchKernelGGL(adj_cell_cell_k, sgrid(dir, smax + 4, tmax), kBlk, 0, a->stream, p->d, a->d, sweep, dir);
+  hipLaunchKernelGGL(adj_cell_vol_k, sgrid(dir, smax + 5, tmax + 2), kBlk, 0, a->stream, p->d, a->d, sweep, dir);
   return check("adj advec_cell");
 }
 
 clv_status adj_advec_mom(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
   const int smax = dir == 1 ? p->d.nx : p->d.ny, tmax = dir == 1 ? p->d.ny : p->d.nx;
   const int mom_sweep = dir + 2 * (sweep - 1);
-  hipLaunchKernelGGL(adj_mom_update_k, grid2(smax + 1, tmax + 1), kBlk, 0, a->stream, p->d, a->d, mom_sweep, dir);
-  hipLaunchKernelGGL(adj_mom_flux_k, grid2(smax + 2, tmax + 1), kBlk, 0, a->stream, p->d, a->d, mom_sweep, dir);
-  hipLaunchKernelGGL(adj_mom_nodes_k, grid2(smax + 5, tmax + 1), kBlk, 0, a->stream, p->d, a->d, mom_sweep, dir);
-  hipLaunchKernelGGL(adj_mom_install_k, grid2(smax + 5, tmax + 1), kBlk, 0, a->stream, a->d, dir);
+  const dim3 gn = sgrid(dir, smax + 5, tmax + 1), gf = sgrid(dir, smax + 2, tmax + 1);
+  hipLaunchKernelGGL(adj_mom_values_k, sgrid(dir, smax + 4, tmax + 1), kBlk, 0, a->stream, p->d, mom_sweep, dir);
+  hipLaunchKernelGGL(adj_mom_update_k, sgrid(dir, smax + 1, tmax + 1), kBlk, 0, a->stream, p->d, a->d, dir);
+  hipLaunchKernelGGL((adj_mom_face_k<CLV_XVEL1, true>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
+  hipLaunchKernelGGL((adj_mom_node_k<CLV_XVEL1, false>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
+  hipLaunchKernelGGL((adj_mom_face_k<CLV_YVEL1, false>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
+  hipLaunchKernelGGL((adj_mom_node_k<CLV_YVEL1, true>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
   hipLaunchKernelGGL(adj_mom_mass_k, grid2(p->d.nx + 4, p->d.ny + 4), kBlk, 0, a->stream, p->d, a->d, mom_sweep, dir);
   if (mom_sweep <= 2) hipLaunchKernelGGL(adj_mom_postvol_k, grid2(p->d.nx + 5, p->d.ny + 5), kBlk, 0, a->stream, a->d, mom_sweep);
   hipLaunchKernelGGL(adj_mom_nodeflux_k, grid2(p->d.nx + 5, p->d.ny + 5), kBlk, 0, a->stream, a->d, dir);

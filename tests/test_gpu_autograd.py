@@ -68,6 +68,11 @@ def test_autograd_backward_after_another_forward(gpu):
         rho2 = torch.tensor(rho0 * 1.01, device="cuda", requires_grad=True)
         ke2 = kinetic_energy(adj, rho2, torch.tensor(e0, device="cuda"))
         assert float(ke2.detach()) != float(ke1.detach())
+        ke1.backward(retain_graph=True)
+        assert np.array_equal(rho.grad.cpu().numpy(), gr) and np.array_equal(e.grad.cpu().numpy(), ge)
+        # a second backward through the same graph: the first consumed the forward run
+        rho.grad = None
+        e.grad = None
         ke1.backward()
         assert np.array_equal(rho.grad.cpu().numpy(), gr) and np.array_equal(e.grad.cpu().numpy(), ge)
 
