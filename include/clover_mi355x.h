@@ -295,9 +295,10 @@ clv_status clv_batch_dt_history(clv_batch *b, int member, double *dt, int max, i
  * the overwritten outputs' adjoints cleared.  Piecewise definitions (upwind
  * choice, van Leer limiters, viscosity switch, fmin/fmax, fabs) are
  * differentiated on the branch the forward took.  Geometry is constant;
- * soundspeed (read only by calc_dt) carries no adjoint; a's work arrays are
- * scratch.  Each replaces no reference entry point; the forward kernel each
- * transposes is cited. */
+ * soundspeed (read only by calc_dt) carries no adjoint; the work arrays of both
+ * p and a are scratch (p's receive the forward's intermediate values).  Each
+ * replaces no reference entry point; the forward kernel each transposes is
+ * cited. */
 clv_status clv_adj_ideal_gas(clv_chunk *p, clv_chunk *a, int predict);          /* src/serial/ideal_gas.cpp:28-48 */
 clv_status clv_adj_viscosity(clv_chunk *p, clv_chunk *a);                       /* src/serial/viscosity.cpp:30-68 */
 clv_status clv_adj_PdV(clv_chunk *p, clv_chunk *a, int predict, double dt);     /* src/serial/PdV.cpp:36-87 */
