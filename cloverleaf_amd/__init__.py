@@ -129,6 +129,14 @@ SIGNATURES = [
     ("clv_advec_mom", _I, [_P, _I, _I, _I]),
     ("clv_reset_field", _I, [_P]),
     ("clv_update_halo", _I, [_P, _PI, _I]),
+    ("clv_initialise_tile", _I, [_P, C.POINTER(Config), _I, _I]),
+    ("clv_update_halo_faces", _I, [_P, _PI, _I, _PI]),
+    ("clv_update_tile_halo", _I, [_P, _P, _I, _PI, _I]),
+    ("clv_pack_message", _I, [_P, _P, _I, _I, _I, _P, C.c_size_t, C.c_int64]),
+    ("clv_unpack_message", _I, [_P, _P, _I, _I, _I, _P, C.c_size_t, C.c_int64]),
+    ("clv_message_length", C.c_size_t, [_PI, _I, _I]),
+    ("clv_pack_fields", _I, [_P, _I, _PI, _I, _I, C.c_int64, _P, C.c_size_t]),
+    ("clv_unpack_fields", _I, [_P, _I, _PI, _I, _I, C.c_int64, _P, C.c_size_t]),
     ("clv_field_summary", _I, [_P, _PD]),
     ("clv_chunk_set_summary_order", _I, [_P, _I]),
     ("clv_deck_parse", _I, [C.c_char_p, C.POINTER(Config)]),
@@ -210,6 +218,47 @@ def parse_deck(text: str) -> Config:
     cfg = Config()
     _check(lib().clv_deck_parse(text.encode(), C.byref(cfg)), "clv_deck_parse")
     return cfg
+
+
+FACE_LEFT, FACE_RIGHT, FACE_BOTTOM, FACE_TOP = 0, 1, 2, 3
+CELL_DATA, VERTEX_DATA, X_FACE_DATA, Y_FACE_DATA = 1, 2, 3, 4
+
+
+def message_length(fields, depth: int, chunk_cells: int) -> int:
+    """Doubles in one face message of `fields` (clv_message_length)."""
+    return int(lib().clv_message_length(_mask(fields), depth, chunk_cells))
+
+
+class DeviceBuffer:
+    """n doubles of device memory (clv_device_alloc), e.g. an exchange message."""
+
+    def __init__(self, n: int):
+        self.L = lib()
+        self.n = int(n)
+        p = C.c_void_p()
+        _check(self.L.clv_device_alloc(self.n * 8, C.byref(p)), "clv_device_alloc")
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        _check(self.L.clv_copy_to_device(self.ptr, a.ctypes.data, self.n * 8), "clv_copy_to_device")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n)
+        _check(self.L.clv_copy_to_host(out.ctypes.data, self.ptr, self.n * 8), "clv_copy_to_host")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.L.clv_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _mask(names) -> "C.Array":
@@ -322,6 +371,36 @@ class Chunk:
 
     def update_halo(self, fields, depth):
         _check(self.L.clv_update_halo(self.h, _mask(fields), depth), "update_halo")
+
+    # ---- decomposed domains (faces: FACE_LEFT, FACE_RIGHT, FACE_BOTTOM, FACE_TOP) ----
+    def initialise_tile(self, cfg: Config, left: int, bottom: int):
+        """initialise_chunk for a tile whose first cell is (left, bottom) of the grid."""
+        _check(self.L.clv_initialise_tile(self.h, C.byref(cfg), int(left), int(bottom)), "initialise_tile")
+
+    def update_halo_faces(self, fields, depth, external):
+        """The reflective halo on the faces with external[face] true only."""
+        ext = (C.c_int * 4)(*[1 if e else 0 for e in external])
+        _check(self.L.clv_update_halo_faces(self.h, _mask(fields), depth, ext), "update_halo_faces")
+
+    def update_tile_halo(self, neighbour: "Chunk", face: int, fields, depth):
+        """This tile's ghosts on `face` from the neighbouring tile's interior."""
+        _check(self.L.clv_update_tile_halo(self.h, neighbour.h, int(face), _mask(fields), depth), "update_tile_halo")
+
+    def pack_message(self, field_ptr: int, field_type: int, face: int, depth: int, buf: "DeviceBuffer", offset: int, unpack=False):
+        """One array's exchange message (field_ptr: a device array of this chunk's layout)."""
+        fn = self.L.clv_unpack_message if unpack else self.L.clv_pack_message
+        _check(fn(self.h, field_ptr, field_type, face, depth, buf.ptr, buf.n, int(offset)), "pack_message")
+
+    def pack_fields(self, face: int, fields, depth: int, chunk_cells: int, tile_offset: int, buf: "DeviceBuffer", unpack=False):
+        """Every requested field of this tile into (or out of) one face message."""
+        fn = self.L.clv_unpack_fields if unpack else self.L.clv_pack_fields
+        _check(fn(self.h, int(face), _mask(fields), depth, int(chunk_cells), int(tile_offset), buf.ptr, buf.n), "pack_fields")
+
+    def field_ptr(self, name: str) -> int:
+        """Device address of element (0, 0) of a 2D field."""
+        v = ChunkView()
+        _check(self.L.clv_chunk_get_view(self.h, C.byref(v)), "clv_chunk_get_view")
+        return int(v.field[FID[name]])
 
     def field_summary(self):
         out = (C.c_double * 5)()

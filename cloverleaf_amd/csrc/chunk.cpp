@@ -299,7 +299,7 @@ clv_status clv_build_field(clv_chunk *c) {
 clv_status clv_initialise_chunk(clv_chunk *c, const clv_config *cfg) {
   CHECK_CHUNK(c);
   if (!cfg || cfg->x_cells != c->d.nx || cfg->y_cells != c->d.ny) return set_error(CLV_ERR_ARG, "clv_initialise_chunk: config does not match chunk");
-  return k_initialise_chunk(c, cfg);
+  return k_initialise_chunk(c, cfg, 1, 1);
 }
 
 clv_status clv_generate_chunk(clv_chunk *c, const clv_config *cfg) {
@@ -390,6 +390,98 @@ clv_status clv_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], 
   CHECK_CHUNK(c);
   if (!fields || (depth != 1 && depth != 2)) return set_error(CLV_ERR_ARG, "clv_update_halo: fields required and depth must be 1 or 2");
   return k_update_halo(c, fields, depth);
+}
+
+clv_status clv_initialise_tile(clv_chunk *c, const clv_config *cfg, int t_left, int t_bottom) {
+  CHECK_CHUNK(c);
+  if (!cfg) return set_error(CLV_ERR_ARG, "clv_initialise_tile: null config");
+  if (t_left < 1 || t_bottom < 1 || t_left + c->d.nx - 1 > cfg->x_cells || t_bottom + c->d.ny - 1 > cfg->y_cells)
+    return set_error(CLV_ERR_ARG, "clv_initialise_tile: the tile does not lie inside the grid");
+  return k_initialise_chunk(c, cfg, t_left, t_bottom);
+}
+
+static bool faces_ok(int face) { return face >= CLV_FACE_LEFT && face <= CLV_FACE_TOP; }
+
+clv_status clv_update_halo_faces(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth, const int external[4]) {
+  CHECK_CHUNK(c);
+  if (!fields || !external || (depth != 1 && depth != 2))
+    return set_error(CLV_ERR_ARG, "clv_update_halo_faces: fields and external required, depth must be 1 or 2");
+  return k_update_halo_faces(c, fields, depth, external);
+}
+
+clv_status clv_update_tile_halo(clv_chunk *tile, clv_chunk *neighbour, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth) {
+  CHECK_CHUNK(tile);
+  CHECK_CHUNK(neighbour);
+  if (!fields || !faces_ok(face) || (depth != 1 && depth != 2))
+    return set_error(CLV_ERR_ARG, "clv_update_tile_halo: fields required, face CLV_FACE_*, depth 1 or 2");
+  if (tile == neighbour) return set_error(CLV_ERR_ARG, "clv_update_tile_halo: a tile is not its own neighbour");
+  if (face <= CLV_FACE_RIGHT ? neighbour->d.ny != tile->d.ny : neighbour->d.nx != tile->d.nx)
+    return set_error(CLV_ERR_ARG, "clv_update_tile_halo: neighbours across a left/right face share y_cells, across bottom/top x_cells");
+  if (neighbour->device != tile->device) return set_error(CLV_ERR_ARG, "clv_update_tile_halo: tiles on different devices");
+  // the neighbour's interior must be final before the copy on the tile's stream
+  if (neighbour->stream != tile->stream) {
+    clv_status st = hip_check(hipStreamSynchronize(neighbour->stream), "clv_update_tile_halo: neighbour sync");
+    if (st != CLV_OK) return st;
+  }
+  return k_tile_halo(tile, neighbour, face, fields, depth);
+}
+
+// one past the largest message element a face's message of one field touches
+static int64_t message_extent(const clv_chunk *c, int face, int x_inc, int y_inc, int depth) {
+  const int64_t len = face <= CLV_FACE_RIGHT ? c->d.ny + y_inc : c->d.nx + x_inc;
+  return (len + depth + 1) * depth + depth;
+}
+
+static clv_status pack_array(clv_chunk *c, double *field, int field_type, int face, int depth, double *buffer, size_t buffer_len,
+                             int64_t offset, int unpack, const char *who) {
+  CHECK_CHUNK(c);
+  if (!field || !buffer || !faces_ok(face) || (depth != 1 && depth != 2) || field_type < CLV_CELL_DATA || field_type > CLV_Y_FACE_DATA)
+    return set_error(CLV_ERR_ARG, std::string(who) + ": field and buffer required, face CLV_FACE_*, depth 1 or 2, field_type CLV_*_DATA");
+  const int x_inc = (field_type == CLV_VERTEX_DATA || field_type == CLV_X_FACE_DATA) ? 1 : 0;
+  const int y_inc = (field_type == CLV_VERTEX_DATA || field_type == CLV_Y_FACE_DATA) ? 1 : 0;
+  if (offset < 0 || offset + message_extent(c, face, x_inc, y_inc, depth) > (int64_t)buffer_len)
+    return set_error(CLV_ERR_ARG, std::string(who) + ": the message does not fit in buffer_len");
+  return k_pack_array(c, field, field_type, face, depth, buffer, offset, unpack);
+}
+
+clv_status clv_pack_message(clv_chunk *c, const double *field, int field_type, int face, int depth, double *buffer, size_t buffer_len,
+                            int64_t offset) {
+  return pack_array(c, const_cast<double *>(field), field_type, face, depth, buffer, buffer_len, offset, 0, "clv_pack_message");
+}
+
+clv_status clv_unpack_message(clv_chunk *c, double *field, int field_type, int face, int depth, const double *buffer, size_t buffer_len,
+                              int64_t offset) {
+  return pack_array(c, field, field_type, face, depth, const_cast<double *>(buffer), buffer_len, offset, 1, "clv_unpack_message");
+}
+
+size_t clv_message_length(const int fields[CLV_NUM_HALO_FIELDS], int depth, int chunk_cells) {
+  if (!fields || depth < 1 || chunk_cells < 1) return 0;
+  size_t n = 0;
+  for (int f = 0; f < CLV_NUM_HALO_FIELDS; ++f) n += fields[f] == 1;
+  return n * (size_t)depth * (size_t)(chunk_cells + 5);
+}
+
+static clv_status pack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth, int chunk_cells, int64_t tile_offset,
+                              double *buffer, size_t buffer_len, int unpack, const char *who) {
+  CHECK_CHUNK(c);
+  if (!fields || !buffer || !faces_ok(face) || (depth != 1 && depth != 2))
+    return set_error(CLV_ERR_ARG, std::string(who) + ": fields and buffer required, face CLV_FACE_*, depth 1 or 2");
+  const int cells = face <= CLV_FACE_RIGHT ? c->d.ny : c->d.nx;
+  if (chunk_cells < cells || tile_offset < 0 || tile_offset + (int64_t)cells * depth > (int64_t)chunk_cells * depth)
+    return set_error(CLV_ERR_ARG, std::string(who) + ": the tile does not lie inside the chunk's face");
+  if (clv_message_length(fields, depth, chunk_cells) > buffer_len)
+    return set_error(CLV_ERR_ARG, std::string(who) + ": buffer_len is shorter than clv_message_length");
+  return k_pack_fields(c, face, fields, depth, (int64_t)depth * (chunk_cells + 5), tile_offset, buffer, unpack);
+}
+
+clv_status clv_pack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth, int chunk_cells, int64_t tile_offset,
+                           double *buffer, size_t buffer_len) {
+  return pack_fields(c, face, fields, depth, chunk_cells, tile_offset, buffer, buffer_len, 0, "clv_pack_fields");
+}
+
+clv_status clv_unpack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth, int chunk_cells, int64_t tile_offset,
+                             const double *buffer, size_t buffer_len) {
+  return pack_fields(c, face, fields, depth, chunk_cells, tile_offset, const_cast<double *>(buffer), buffer_len, 1, "clv_unpack_fields");
 }
 
 clv_status clv_field_summary(clv_chunk *c, double out[5]) {

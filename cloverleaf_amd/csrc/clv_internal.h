@@ -62,7 +62,7 @@ void before_launch(clv_chunk *c, const char *name);
 
 // Kernel wrappers (defined in the .hip files), each returning CLV_OK or an error.
 clv_status k_fill_zero(clv_chunk *c);
-clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg);
+clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg, int t_left, int t_bottom);
 clv_status k_generate_chunk(clv_chunk *c, const clv_config *cfg);
 clv_status k_ideal_gas(clv_chunk *c, int predict);
 clv_status k_viscosity(clv_chunk *c);
@@ -74,6 +74,11 @@ clv_status k_advec_cell(clv_chunk *c, int sweep_number, int dir);
 clv_status k_advec_mom(clv_chunk *c, int which_vel, int direction, int sweep_number);
 clv_status k_reset_field(clv_chunk *c);
 clv_status k_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth);
+clv_status k_update_halo_faces(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth, const int external[4]);
+clv_status k_tile_halo(clv_chunk *c, const clv_chunk *n, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth);
+clv_status k_pack_array(clv_chunk *c, double *a, int field_type, int face, int depth, double *buf, int64_t offset, int unpack);
+clv_status k_pack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth, int64_t stride, int64_t tile_offset,
+                         double *buf, int unpack);
 clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, double *dt_min_val);
 clv_status k_field_summary(clv_chunk *c, double out[5]);
 

@@ -185,7 +185,7 @@ clv_status clv_hydro_start(clv_chunk *c, const clv_config *cfg, clv_hydro_state 
   if (!c || !cfg || !st) return set_error(CLV_ERR_ARG, "clv_hydro_start: null argument");
   if (cfg->x_cells != c->d.nx || cfg->y_cells != c->d.ny) return set_error(CLV_ERR_ARG, "clv_hydro_start: config does not match chunk");
   TRY(k_fill_zero(c));
-  TRY(k_initialise_chunk(c, cfg));
+  TRY(k_initialise_chunk(c, cfg, 1, 1));
   TRY(k_generate_chunk(c, cfg));
   TRY(k_ideal_gas(c, 0));
   int fields[CLV_NUM_HALO_FIELDS];

@@ -158,6 +158,88 @@ __global__ __launch_bounds__(256) void update_halo_pass_k(Dev d, HaloList list, 
   }
 }
 
+// ---- decomposed halos -------------------------------------------------------------------
+// Tile-to-tile ghost copies, src/serial/update_tile_halo_kernel.cpp:30-804: the
+// ghosts of d on `face` (0 left, 1 right, 2 bottom, 3 top) receive the neighbour
+// n's interior layers next to the shared face; node and face data normal to it
+// skip the shared boundary node (xext / yext), which both tiles compute.  One
+// thread per row (left/right, rows 2-depth .. ny+yext+depth+1, corner ghosts
+// included) or column (bottom/top, 2-depth .. nx+xext+depth+1); one field per
+// blockIdx.y.  d and n are distinct chunks of equal height (left/right) or
+// width (bottom/top).
+__global__ __launch_bounds__(256) void tile_halo_k(Dev d, Dev n, HaloList list, int face, int depth) {
+  const int fid = list.ids[blockIdx.y];
+  const HaloRule r = kRules[fid];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double *a = d.f[fid];
+  const double *b = n.f[fid];
+  if (face <= 1) {
+    const int y = 2 - depth + t;
+    if (y > d.ny + r.yext + depth + 1) return;
+    for (int j = 0; j < depth; ++j) {
+      if (face == 0) a[(int64_t)y * d.pitch + (1 - j)] = b[(int64_t)y * n.pitch + (n.nx + 1 - j)];
+      else a[(int64_t)y * d.pitch + (d.nx + 2 + r.xext + j)] = b[(int64_t)y * n.pitch + (2 + r.xext + j)];
+    }
+  } else {
+    const int x = 2 - depth + t;
+    if (x > d.nx + r.xext + depth + 1) return;
+    for (int k = 0; k < depth; ++k) {
+      if (face == 2) a[(int64_t)(1 - k) * d.pitch + x] = b[(int64_t)(n.ny + 1 - k) * n.pitch + x];
+      else a[(int64_t)(d.ny + 2 + r.yext + k) * d.pitch + x] = b[(int64_t)(2 + r.yext + k) * n.pitch + x];
+    }
+  }
+}
+
+// One field's exchange message, src/serial/pack_kernel.cpp:27-303: left/right
+// messages hold element offset + j + k*depth for rows k in 2-depth ..
+// ny+y_inc+depth+1 and layers j < depth, bottom/top messages offset + k + j*depth
+// for columns j in 2-depth .. nx+x_inc+depth+1.  Packing reads the interior layers
+// next to the face (left and bottom skip the shared boundary node of node / face
+// data), unpacking writes the ghost layers.  One thread per row / column.
+__device__ __forceinline__ void pack_line(double *a, int64_t pitch, int nx, int ny, int x_inc, int y_inc, int face, int depth, double *buf,
+                                          int64_t offset, bool unpack, int t) {
+  if (face <= 1) {
+    const int k = 2 - depth + t;
+    if (k > ny + y_inc + depth + 1) return;
+    for (int j = 0; j < depth; ++j) {
+      const int64_t index = offset + j + (int64_t)k * depth;
+      const int x = face == 0 ? (unpack ? 1 - j : 2 + x_inc + j) : (unpack ? nx + x_inc + 2 + j : nx + 1 - j);
+      double &v = a[(int64_t)k * pitch + x];
+      if (unpack) v = buf[index];
+      else buf[index] = v;
+    }
+  } else {
+    const int j = 2 - depth + t;
+    if (j > nx + x_inc + depth + 1) return;
+    for (int k = 0; k < depth; ++k) {
+      const int64_t index = offset + k + (int64_t)j * depth;
+      const int y = face == 2 ? (unpack ? 1 - k : 2 + y_inc + k) : (unpack ? ny + y_inc + 2 + k : ny + 1 - k);
+      double &v = a[(int64_t)y * pitch + j];
+      if (unpack) v = buf[index];
+      else buf[index] = v;
+    }
+  }
+}
+
+// one array of the chunk's layout, field_type 1 cell, 2 vertex, 3 x face, 4 y face
+__global__ __launch_bounds__(256) void pack_array_k(double *a, int64_t pitch, int nx, int ny, int x_inc, int y_inc, int face, int depth,
+                                                    double *buf, int64_t offset, int unpack) {
+  pack_line(a, pitch, nx, ny, x_inc, y_inc, face, depth, buf, offset, unpack != 0, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// every requested field of a chunk, one per blockIdx.y, each at its own offset
+struct PackList {
+  int n;
+  int ids[CLV_NUM_HALO_FIELDS];
+  int64_t off[CLV_NUM_HALO_FIELDS];
+};
+__global__ __launch_bounds__(256) void pack_fields_k(Dev d, PackList list, int face, int depth, double *buf, int unpack) {
+  const int fid = list.ids[blockIdx.y];
+  const HaloRule r = kRules[fid];
+  pack_line(d.f[fid], d.pitch, d.nx, d.ny, r.xext, r.yext, face, depth, buf, list.off[blockIdx.y], unpack != 0,
+            blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ---- reductions -------------------------------------------------------------------------
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_BLOCKS = 2048;
@@ -476,6 +558,58 @@ clv_status k_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], in
   return CLV_OK;
 }
 
+static HaloList halo_list(const int fields[CLV_NUM_HALO_FIELDS]) {
+  HaloList list{};
+  for (int f = 0; f < CLV_NUM_HALO_FIELDS; ++f)
+    if (fields[f] == 1) list.ids[list.n++] = f;
+  return list;
+}
+
+// faces not flagged external keep the ghosts their tile / chunk exchange wrote, so
+// the passes run in the reference's order over the flagged faces only
+clv_status k_update_halo_faces(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth, const int external[4]) {
+  if (external[0] && external[1] && external[2] && external[3]) return k_update_halo(c, fields, depth);
+  const Dev &d = c->d;
+  const HaloList list = halo_list(fields);
+  if (list.n == 0) return CLV_OK;
+  const int work = (d.nx > d.ny ? d.nx : d.ny) + 2 * depth + 2;
+  const int pass_face[4] = {2, 3, 0, 1};  // passes bottom, top, left, right
+  for (int pass = 0; pass < 4; ++pass)
+    if (external[pass_face[pass]])
+      LAUNCH_G("update_halo", update_halo_pass_k, dim3((work + 255) / 256, list.n, 1), dim3(256, 1, 1), d, list, depth, pass);
+  return CLV_OK;
+}
+
+clv_status k_tile_halo(clv_chunk *c, const clv_chunk *n, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth) {
+  const HaloList list = halo_list(fields);
+  if (list.n == 0) return CLV_OK;
+  const int work = (face <= 1 ? c->d.ny : c->d.nx) + 2 * depth + 2;
+  LAUNCH_G("update_tile_halo", tile_halo_k, dim3((work + 255) / 256, list.n, 1), dim3(256, 1, 1), c->d, n->d, list, face, depth);
+  return CLV_OK;
+}
+
+clv_status k_pack_array(clv_chunk *c, double *a, int field_type, int face, int depth, double *buf, int64_t offset, int unpack) {
+  const int x_inc = (field_type == 2 || field_type == 3) ? 1 : 0, y_inc = (field_type == 2 || field_type == 4) ? 1 : 0;
+  const int work = (face <= 1 ? c->d.ny : c->d.nx) + 2 * depth + 2;
+  LAUNCH_G("pack_message", pack_array_k, dim3((work + 255) / 256, 1, 1), dim3(256, 1, 1), a, c->d.pitch, c->d.nx, c->d.ny, x_inc, y_inc,
+           face, depth, buf, offset, unpack);
+  return CLV_OK;
+}
+
+clv_status k_pack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth, int64_t stride, int64_t tile_offset,
+                         double *buf, int unpack) {
+  PackList list{};
+  for (int f = 0; f < CLV_NUM_HALO_FIELDS; ++f)
+    if (fields[f] == 1) {
+      list.off[list.n] = (int64_t)list.n * stride + tile_offset;
+      list.ids[list.n++] = f;
+    }
+  if (list.n == 0) return CLV_OK;
+  const int work = (face <= 1 ? c->d.ny : c->d.nx) + 2 * depth + 2;
+  LAUNCH_G("pack_message", pack_fields_k, dim3((work + 255) / 256, list.n, 1), dim3(256, 1, 1), c->d, list, face, depth, buf, unpack);
+  return CLV_OK;
+}
+
 // every problem of a batch has x_cells, y_cells >= 4: the single-pass form holds
 hipError_t batch_launch_halo(hipStream_t s, int nprob, const Dev *tab, const BatchState *ps, const int *ids, int nids, int depth) {
   HaloList list{};
@@ -565,12 +699,13 @@ clv_status k_fill_zero(clv_chunk *c) {
   return CLV_OK;
 }
 
-clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg) {
+// a tile whose first cell is (t_left, t_bottom) of the grid, src/serial/initialise_chunk.cpp:32-37
+clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg, int t_left, int t_bottom) {
   const Dev &d = c->d;
   double dx = (cfg->xmax - cfg->xmin) / (double)(cfg->x_cells);
   double dy = (cfg->ymax - cfg->ymin) / (double)(cfg->y_cells);
-  double xmin = cfg->xmin + dx * (double)(1 - 1);
-  double ymin = cfg->ymin + dy * (double)(1 - 1);
+  double xmin = cfg->xmin + dx * (double)(t_left - 1);
+  double ymin = cfg->ymin + dy * (double)(t_bottom - 1);
   const int n = (d.nx > d.ny ? d.nx : d.ny) + 5;
   LAUNCH_G("initialise_chunk", init_mesh_k, dim3((n + 255) / 256, 1, 1), dim3(256, 1, 1), d, xmin, ymin, dx, dy);
   const dim3 g((unsigned)((d.nx + 4 + 63) / 64), (unsigned)((d.ny + 4 + 3) / 4), 1);

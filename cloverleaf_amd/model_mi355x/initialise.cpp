@@ -1,5 +1,6 @@
 // create_context / report_context (driver/initialise.h:160-162) and the tile ->
-// chunk binding of the MI355X model.
+// chunk binding of the MI355X model: each tile of each chunk is wrapped once as a
+// clover_mi355x chunk of the tile's size.
 #include "initialise.h"
 
 #include <cstdio>
@@ -35,8 +36,38 @@ namespace mi355x {
 static std::map<const void *, clv_chunk *> g_chunks;
 
 [[noreturn]] void unsupported(const char *what) {
-  std::fprintf(stderr, "mi355x model: %s needs more than one chunk or tile, which this model does not implement\n", what);
+  std::fprintf(stderr, "mi355x model: %s is not implemented by this model\n", what);
   std::abort();
+}
+
+namespace {
+clover::Buffer2D<double> *const *fields_of(field_type &f, clover::Buffer2D<double> *out[CLV_NUM_FIELDS]) {
+  clover::Buffer2D<double> *b2[CLV_NUM_FIELDS] = {
+      &f.density0,   &f.density1,    &f.energy0,     &f.energy1,     &f.pressure,    &f.viscosity,   &f.soundspeed,
+      &f.xvel0,      &f.xvel1,       &f.yvel0,       &f.yvel1,       &f.vol_flux_x,  &f.vol_flux_y,  &f.mass_flux_x,
+      &f.mass_flux_y, &f.work_array1, &f.work_array2, &f.work_array3, &f.work_array4, &f.work_array5, &f.work_array6,
+      &f.work_array7, &f.volume,      &f.xarea,       &f.yarea};
+  for (int k = 0; k < CLV_NUM_FIELDS; ++k) out[k] = b2[k];
+  return out;
+}
+}  // namespace
+
+clv_chunk *chunk_of(global_variables &globals, const double *field) {
+  for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {
+    clover::Buffer2D<double> *b2[CLV_NUM_FIELDS];
+    fields_of(globals.chunk.tiles[tile].field, b2);
+    for (int k = 0; k < CLV_NUM_FIELDS; ++k)
+      if (b2[k]->data == field) return chunk(globals, tile);
+  }
+  std::fprintf(stderr, "mi355x model: a halo or message buffer that belongs to no tile\n");
+  std::abort();
+}
+
+void external_faces(const global_variables &globals, int tile, int external[4]) {
+  const tile_info &t = globals.chunk.tiles[tile].info;
+  const chunk_type &c = globals.chunk;
+  const bool at[4] = {t.t_left == c.left, t.t_right == c.right, t.t_bottom == c.bottom, t.t_top == c.top};
+  for (int face = 0; face < 4; ++face) external[face] = at[face] && c.chunk_neighbours[face] == external_face;
 }
 
 clv_chunk *chunk(global_variables &globals, int tile) {
@@ -44,17 +75,13 @@ clv_chunk *chunk(global_variables &globals, int tile) {
   field_type &f = t.field;
   auto it = g_chunks.find(f.density0.data);
   if (it != g_chunks.end()) return it->second;
-  if (globals.config.tiles_per_chunk != 1 || t.info.t_xmin != 1 || t.info.t_ymin != 1 || t.info.t_left != 1 || t.info.t_bottom != 1)
-    unsupported("a tiled or decomposed chunk");
+  if (t.info.t_xmin != 1 || t.info.t_ymin != 1) unsupported("a tile whose local indices do not start at 1");
   clv_chunk_view v{};
   v.x_cells = t.info.t_xmax - t.info.t_xmin + 1;
   v.y_cells = t.info.t_ymax - t.info.t_ymin + 1;
   v.pitch = (int64_t)f.density0.pitch;
-  clover::Buffer2D<double> *b2[CLV_NUM_FIELDS] = {
-      &f.density0,   &f.density1,    &f.energy0,     &f.energy1,     &f.pressure,    &f.viscosity,   &f.soundspeed,
-      &f.xvel0,      &f.xvel1,       &f.yvel0,       &f.yvel1,       &f.vol_flux_x,  &f.vol_flux_y,  &f.mass_flux_x,
-      &f.mass_flux_y, &f.work_array1, &f.work_array2, &f.work_array3, &f.work_array4, &f.work_array5, &f.work_array6,
-      &f.work_array7, &f.volume,      &f.xarea,       &f.yarea};
+  clover::Buffer2D<double> *b2[CLV_NUM_FIELDS];
+  fields_of(f, b2);
   for (int k = 0; k < CLV_NUM_FIELDS; ++k) {
     if ((int64_t)b2[k]->pitch != v.pitch) {
       std::fprintf(stderr, "mi355x model: field %d has pitch %zu, density0 %zu\n", k, b2[k]->pitch, f.density0.pitch);

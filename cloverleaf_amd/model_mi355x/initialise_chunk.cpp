@@ -38,7 +38,10 @@ clv_config config_of(const global_variables &globals) {
 }
 }  // namespace mi355x
 
+// src/serial/initialise_chunk.cpp:32-37: the tile's mesh starts at its first cell
+// (t_left, t_bottom) of the grid
 void initialise_chunk(const int tile, global_variables &globals) {
   clv_config c = mi355x::config_of(globals);
-  mi355x::check(clv_initialise_chunk(mi355x::chunk(globals, tile), &c), "initialise_chunk");
+  const tile_info &t = globals.chunk.tiles[tile].info;
+  mi355x::check(clv_initialise_tile(mi355x::chunk(globals, tile), &c, t.t_left, t.t_bottom), "initialise_chunk");
 }

@@ -8,8 +8,15 @@
 namespace mi355x {
 
 // The chunk over tile `tile`'s buffers, wrapped on first use (clv_chunk_wrap).
-// One chunk and one tile only: every halo is the external reflective halo.
 clv_chunk *chunk(global_variables &globals, int tile);
+// The chunk of the tile one of whose 2D buffers is `field` (aborts if none is).
+clv_chunk *chunk_of(global_variables &globals, const double *field);
+// external[face] (face: left, right, bottom, top) = the tile's face lies on the
+// chunk's face and the chunk has no neighbour there -- the faces the reflective
+// halo updates (src/serial/update_halo.cpp:41-71).  Taken from the tile's extent:
+// the driver's tile_neighbours array keeps bottom and top in one slot
+// (driver/definitions.h:58).
+void external_faces(const global_variables &globals, int tile, int external[4]);
 // When the reference profiler is on, wait for the tile's queued kernels so the
 // driver's host timers measure them (as src/hip/update_halo.cpp:635-661 does).
 void sync_if_profiling(global_variables &globals, int tile);

@@ -1,7 +1,7 @@
 // void update_halo(global_variables&, int fields[NUM_FIELDS], int depth)   driver/update_halo.h:24
-// The order of src/serial/update_halo.cpp:690-718: tile halo (a no-op for one
-// tile, driver/update_tile_halo.cpp:33-36), chunk exchange (a no-op for one
-// chunk), then the external reflective halo on the GPU in one launch.
+// The order of src/serial/update_halo.cpp:690-718: the tile-to-tile copies
+// (driver/update_tile_halo.cpp), the chunk exchange (clover_exchange), then the
+// reflective halo on each tile's external faces (one launch when all four are).
 #include "update_halo.h"
 
 #include "comms_kernel.h"
@@ -26,7 +26,9 @@ void update_halo(global_variables &globals, int fields[NUM_FIELDS], const int de
   if (nb[chunk_left] == external_face || nb[chunk_right] == external_face || nb[chunk_bottom] == external_face ||
       nb[chunk_top] == external_face) {
     for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {
-      mi355x::check(clv_update_halo(mi355x::chunk(globals, tile), fields, depth), "update_halo");
+      int external[4];
+      mi355x::external_faces(globals, tile, external);
+      mi355x::check(clv_update_halo_faces(mi355x::chunk(globals, tile), fields, depth, external), "update_halo");
       mi355x::sync_if_profiling(globals, tile);
     }
   }

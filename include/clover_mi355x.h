@@ -194,6 +194,53 @@ clv_status clv_reset_field(clv_chunk *c);
  * The reflective self halo of a single chunk and tile (every face external);
  * fields[f] == 1 selects field f (0..14), depth is 1 or 2. */
 clv_status clv_update_halo(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth);
+
+/* ---- decomposed domains: tiles of a chunk, chunks of a mesh ----------------------------- */
+/* Faces, in the order of the reference's chunk_neighbour_type (driver/definitions.h:57),
+ * and the data types of exchange messages (driver/definitions.h:80). */
+enum { CLV_FACE_LEFT = 0, CLV_FACE_RIGHT = 1, CLV_FACE_BOTTOM = 2, CLV_FACE_TOP = 3 };
+enum { CLV_CELL_DATA = 1, CLV_VERTEX_DATA = 2, CLV_X_FACE_DATA = 3, CLV_Y_FACE_DATA = 4 };
+/* void initialise_chunk(int tile, global_variables&)   driver/initialise_chunk.h:24
+ * for a tile whose first cell is (t_left, t_bottom) of the grid
+ * (src/serial/initialise_chunk.cpp:32-37); the chunk is the tile's size. */
+clv_status clv_initialise_tile(clv_chunk *c, const clv_config *cfg, int t_left, int t_bottom);
+/* update_halo_kernel of a tile whose faces are not all external
+ *                                       src/serial/update_halo.cpp:33-684, :41-71
+ * external[face] != 0 reflects that face; the other faces keep the ghosts their
+ * tile or chunk exchange wrote (corners included). */
+clv_status clv_update_halo_faces(clv_chunk *c, const int fields[CLV_NUM_HALO_FIELDS], int depth, const int external[4]);
+/* void update_tile_halo_{l,r,b,t}_kernel(...)   driver/update_tile_halo_kernel.h:25-86
+ * The ghosts of `tile` on `face` from the neighbouring tile's interior
+ * (src/serial/update_tile_halo_kernel.cpp:30-804).  Neighbours across a
+ * left/right face have equal y_cells, across a bottom/top face equal x_cells;
+ * both chunks on one device.  Ordered on the tile's stream (the neighbour's
+ * stream is synchronised first when it differs).  The reference's own driver
+ * runs every tile's bottom/top copies before any left/right copy
+ * (driver/update_tile_halo.cpp:26-91). */
+clv_status clv_update_tile_halo(clv_chunk *tile, clv_chunk *neighbour, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth);
+/* void clover_pack_message_{left,right,bottom,top}(...) / clover_unpack_message_*(...)
+ *                                                        driver/pack_kernel.h:23-47
+ * One exchange message of one array (src/serial/pack_kernel.cpp:27-303).  field:
+ * a device array of c's layout (pitch, y_cells + 5 rows); buffer: device memory of
+ * buffer_len doubles; the message starts at element offset.  Left/right
+ * messages hold offset + j + k*depth (rows k, layers j), bottom/top
+ * offset + k + j*depth. */
+clv_status clv_pack_message(clv_chunk *c, const double *field, int field_type, int face, int depth, double *buffer, size_t buffer_len,
+                            int64_t offset);
+clv_status clv_unpack_message(clv_chunk *c, double *field, int field_type, int face, int depth, const double *buffer, size_t buffer_len,
+                              int64_t offset);
+/* void clover_pack_{left,right,bottom,top}(...) / clover_unpack_*(...)   driver/comms.cpp:282-930
+ * Every requested field of tile c into one face message: the k-th requested
+ * field (in field order) at k * depth * (chunk_cells + 5) + tile_offset, where
+ * chunk_cells is the chunk's cell count along the face (y_cells for left/right)
+ * and tile_offset = (the tile's first cell along the face - the chunk's) * depth
+ * (src/serial/comms_kernel.cpp:64-73, driver/comms.cpp:286).  buffer_len must be
+ * at least clv_message_length. */
+size_t clv_message_length(const int fields[CLV_NUM_HALO_FIELDS], int depth, int chunk_cells);
+clv_status clv_pack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth, int chunk_cells, int64_t tile_offset,
+                           double *buffer, size_t buffer_len);
+clv_status clv_unpack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_FIELDS], int depth, int chunk_cells, int64_t tile_offset,
+                             const double *buffer, size_t buffer_len);
 /* The sums of void field_summary(global_variables&, parallel_&)     driver/field_summary.h:25
  * (src/serial/field_summary.cpp:60-92) over the current fields, without the
  * ideal_gas call that precedes them in the reference; out = {vol, mass, ie, ke,

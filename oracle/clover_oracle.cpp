@@ -671,23 +671,111 @@ static const orc_halo_rule ORC_HALO_RULES[15] = {
     /* mass_flux_y*/ {3, 3, 1, 3, 2, 0, 0, 1, -1.0, 1.0},
 };
 
+/* ext[face] (face: 0 left, 1 right, 2 bottom, 3 top, the order of the
+ * reference's chunk_neighbour_type, driver/definitions.h:57): whether that face
+ * is reflected.  The reference reflects a face of a tile only where both the
+ * chunk's and the tile's neighbour on it are external
+ * (src/serial/update_halo.cpp:41-71); faces shared with another tile or chunk
+ * keep the ghosts their exchange wrote, corners included. */
 template <class T>
-static void k_update_halo(View<T> *c, const int fields[15], int depth) {
+static void k_update_halo_ext(View<T> *c, const int fields[15], int depth, const int ext[4]) {
   const int px = c->px, x_min = 1, y_min = 1, x_max = c->nx, y_max = c->ny;
   for (int f = 0; f < 15; ++f) {
     if (fields[f] != 1) continue;
     const orc_halo_rule *r = &ORC_HALO_RULES[f];
     T *a = c->f[f];
     /* bottom then top: j over x_min-depth .. x_max(+1)+depth */
-    for (int j = x_min - depth + 1; j < x_max + r->xext + depth + 2; j++)
-      for (int k = 0; k < depth; ++k) AT(a, j, 1 - k) = r->sy * AT(a, j, r->bsrc + k);
-    for (int j = x_min - depth + 1; j < x_max + r->xext + depth + 2; j++)
-      for (int k = 0; k < depth; ++k) AT(a, j, y_max + r->tdst + k) = r->sy * AT(a, j, y_max + r->tsrc - k);
+    if (ext[2])
+      for (int j = x_min - depth + 1; j < x_max + r->xext + depth + 2; j++)
+        for (int k = 0; k < depth; ++k) AT(a, j, 1 - k) = r->sy * AT(a, j, r->bsrc + k);
+    if (ext[3])
+      for (int j = x_min - depth + 1; j < x_max + r->xext + depth + 2; j++)
+        for (int k = 0; k < depth; ++k) AT(a, j, y_max + r->tdst + k) = r->sy * AT(a, j, y_max + r->tsrc - k);
     /* left then right: k over y_min-depth .. y_max(+1)+depth */
-    for (int k = y_min - depth + 1; k < y_max + r->yext + depth + 2; k++)
-      for (int j = 0; j < depth; ++j) AT(a, 1 - j, k) = r->sx * AT(a, r->lsrc + j, k);
-    for (int k = y_min - depth + 1; k < y_max + r->yext + depth + 2; k++)
-      for (int j = 0; j < depth; ++j) AT(a, x_max + r->rdst + j, k) = r->sx * AT(a, x_max + r->rsrc - j, k);
+    if (ext[0])
+      for (int k = y_min - depth + 1; k < y_max + r->yext + depth + 2; k++)
+        for (int j = 0; j < depth; ++j) AT(a, 1 - j, k) = r->sx * AT(a, r->lsrc + j, k);
+    if (ext[1])
+      for (int k = y_min - depth + 1; k < y_max + r->yext + depth + 2; k++)
+        for (int j = 0; j < depth; ++j) AT(a, x_max + r->rdst + j, k) = r->sx * AT(a, x_max + r->rsrc - j, k);
+  }
+}
+
+template <class T>
+static void k_update_halo(View<T> *c, const int fields[15], int depth) {
+  static const int all[4] = {1, 1, 1, 1};
+  k_update_halo_ext(c, fields, depth, all);
+}
+
+/* Ghost layers of tile c on one face copied from the neighbouring tile n,
+ * src/serial/update_tile_halo_kernel.cpp:30-804 (the l, r, t and b kernels).
+ * The ghosts receive the neighbour's interior layers next to the shared face;
+ * node and face fields normal to it skip the shared boundary node, which both
+ * tiles compute (xext / yext of ORC_HALO_RULES).  Left/right copy rows
+ * 2-depth .. ny+yext+depth+1 (the corner ghosts the bottom/top copies wrote
+ * included), bottom/top columns 2-depth .. nx+xext+depth+1. */
+template <class T>
+static void k_tile_halo(View<T> *c, View<T> *n, int face, const int fields[15], int depth) {
+  const size_t pc = c->px, pn = n->px;
+  const int nx = c->nx, ny = c->ny;
+  for (int f = 0; f < 15; ++f) {
+    if (fields[f] != 1) continue;
+    const orc_halo_rule *r = &ORC_HALO_RULES[f];
+    T *a = c->f[f];
+    const T *b = n->f[f];
+    if (face == 0)
+      for (int k = 2 - depth; k < ny + r->yext + depth + 2; k++)
+        for (int j = 0; j < depth; ++j) a[(size_t)k * pc + (1 - j)] = b[(size_t)k * pn + (n->nx + 1 - j)];
+    if (face == 1)
+      for (int k = 2 - depth; k < ny + r->yext + depth + 2; k++)
+        for (int j = 0; j < depth; ++j) a[(size_t)k * pc + (nx + 2 + r->xext + j)] = b[(size_t)k * pn + (2 + r->xext + j)];
+    if (face == 2)
+      for (int k = 0; k < depth; ++k)
+        for (int j = 2 - depth; j < nx + r->xext + depth + 2; j++) a[(size_t)(1 - k) * pc + j] = b[(size_t)(n->ny + 1 - k) * pn + j];
+    if (face == 3)
+      for (int k = 0; k < depth; ++k)
+        for (int j = 2 - depth; j < nx + r->xext + depth + 2; j++)
+          a[(size_t)(ny + 2 + r->yext + k) * pc + j] = b[(size_t)(2 + r->yext + k) * pn + j];
+  }
+}
+
+/* One field's exchange message for one face, src/serial/pack_kernel.cpp:27-303
+ * (clover_pack_message_* / clover_unpack_message_*).  field_type 1 cell,
+ * 2 vertex, 3 x face, 4 y face (driver/definitions.h:80) sets x_inc / y_inc.
+ * Left/right messages hold element offset + j + k*depth for rows k in
+ * 2-depth .. ny+y_inc+depth+1 and layers j < depth; bottom/top messages
+ * offset + k + j*depth for columns j in 2-depth .. nx+x_inc+depth+1.  Packing
+ * reads the interior layers next to the face (the left and bottom ones skip the
+ * shared boundary node of node/face data); unpacking writes the ghost layers. */
+template <class T>
+static void k_pack(View<T> *c, T *a, int field_type, int face, int depth, T *buf, long offset, int unpack) {
+  const int px = c->px, nx = c->nx, ny = c->ny;
+  const int x_inc = (field_type == 2 || field_type == 3) ? 1 : 0;
+  const int y_inc = (field_type == 2 || field_type == 4) ? 1 : 0;
+  if (face == 0 || face == 1) {
+    for (int k = 2 - depth; k < ny + y_inc + depth + 2; k++)
+      for (int j = 0; j < depth; ++j) {
+        const long index = offset + j + (long)k * depth;
+        if (face == 0) {
+          if (unpack) AT(a, 1 - j, k) = buf[index];
+          else buf[index] = AT(a, 2 + x_inc + j, k);
+        } else {
+          if (unpack) AT(a, nx + x_inc + 2 + j, k) = buf[index];
+          else buf[index] = AT(a, nx + 1 - j, k);
+        }
+      }
+  } else {
+    for (int k = 0; k < depth; ++k)
+      for (int j = 2 - depth; j < nx + x_inc + depth + 2; j++) {
+        const long index = offset + k + (long)j * depth;
+        if (face == 2) {
+          if (unpack) AT(a, j, 1 - k) = buf[index];
+          else buf[index] = AT(a, j, 2 + y_inc + k);
+        } else {
+          if (unpack) AT(a, j, ny + y_inc + 2 + k) = buf[index];
+          else buf[index] = AT(a, j, ny + 1 - k);
+        }
+      }
   }
 }
 
@@ -957,6 +1045,16 @@ void orc_advec_cell(orc_chunk *c, int sweep_number, int dir) { k_advec_cell(V(c)
 void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction) { k_advec_mom(V(c), which_vel, sweep_number, direction); }
 void orc_reset_field(orc_chunk *c) { k_reset_field(V(c)); }
 void orc_update_halo(orc_chunk *c, const int fields[15], int depth) { k_update_halo(V(c), fields, depth); }
+void orc_update_halo_ext(orc_chunk *c, const int fields[15], int depth, const int ext[4]) { k_update_halo_ext(V(c), fields, depth, ext); }
+void orc_tile_halo(orc_chunk *c, orc_chunk *neighbour, int face, const int fields[15], int depth) {
+  k_tile_halo(V(c), V(neighbour), face, fields, depth);
+}
+void orc_pack_message(orc_chunk *c, double *field, int field_type, int face, int depth, double *buf, long offset) {
+  k_pack(V(c), field, field_type, face, depth, buf, offset, 0);
+}
+void orc_unpack_message(orc_chunk *c, double *field, int field_type, int face, int depth, const double *buf, long offset) {
+  k_pack(V(c), field, field_type, face, depth, const_cast<double *>(buf), offset, 1);
+}
 void orc_field_summary_kernel(orc_chunk *c, double out[5]) { k_field_summary_kernel(V(c), out); }
 void orc_PdV_driver(orc_chunk *c, int predict, double dt) { k_PdV_driver(V(c), predict, dt); }
 void orc_advection(orc_chunk *c, int advect_x) { k_advection(V(c), advect_x); }

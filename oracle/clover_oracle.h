@@ -102,6 +102,15 @@ void orc_advec_cell(orc_chunk *c, int sweep_number, int dir);
 void orc_advec_mom(orc_chunk *c, int which_vel, int sweep_number, int direction);
 void orc_reset_field(orc_chunk *c);
 void orc_update_halo(orc_chunk *c, const int fields[15], int depth);
+/* the reflective halo on the faces with ext[face] != 0 only (face 0 left, 1 right,
+ * 2 bottom, 3 top): a face shared with another tile or chunk is left to its exchange */
+void orc_update_halo_ext(orc_chunk *c, const int fields[15], int depth, const int ext[4]);
+/* ghost layers of c on `face` from the neighbouring tile (update_tile_halo_{l,r,b,t}_kernel) */
+void orc_tile_halo(orc_chunk *c, orc_chunk *neighbour, int face, const int fields[15], int depth);
+/* one field's exchange message (clover_pack_message_* / clover_unpack_message_*):
+ * field is an array of c's layout, field_type 1 cell / 2 vertex / 3 x face / 4 y face */
+void orc_pack_message(orc_chunk *c, double *field, int field_type, int face, int depth, double *buf, long offset);
+void orc_unpack_message(orc_chunk *c, double *field, int field_type, int face, int depth, const double *buf, long offset);
 void orc_field_summary_kernel(orc_chunk *c, double out[5]);
 void orc_build_field(orc_chunk *c);
 void orc_initialise_chunk(orc_chunk *c, const orc_config *cfg);

@@ -39,6 +39,11 @@ void reset_field_kernel(int x_min, int x_max, int y_min, int y_max, B2 &density0
                         B2 &xvel1, B2 &yvel0, B2 &yvel1);
 void update_halo_kernel(int x_min, int x_max, int y_min, int y_max, const std::array<int, 4> &chunk_neighbours,
                         const std::array<int, 4> &tile_neighbours, field_type &field, const int fields[NUM_FIELDS], int depth);
+// tile-to-tile copies, exchange messages and the decompositions: declared by the
+// reference's own headers
+#include "comms.h"
+#include "pack_kernel.h"
+#include "update_tile_halo_kernel.h"
 
 // The driver objects linked in reference the report functions of driver/report.cpp
 // and g_out of driver/clover_leaf.cpp; none is reached by the kernels run here.
@@ -101,7 +106,124 @@ B1 *field1d(field_type &f, int id) {
   return nullptr;
 }
 
+// The tile-halo and pack kernels take a global_variables& that they never read.
+global_variables &unread_globals() {
+  alignas(global_variables) static unsigned char storage[sizeof(global_variables)];
+  return *reinterpret_cast<global_variables *>(storage);
+}
+
+void load(field_type &f, int nx, double **f2d, double **m1d) {
+  const size_t px = nx + 5;
+  for (int a = 0; a < 25; ++a) {
+    B2 &b = *field2d(f, a);
+    for (size_t i = 0; i < b.sizeX; ++i)
+      for (size_t j = 0; j < b.sizeY; ++j) b(i, j) = f2d[a][j * px + i];
+  }
+  for (int a = 0; a < 8; ++a) {
+    B1 &b = *field1d(f, a);
+    for (size_t i = 0; i < b.size; ++i) b[i] = m1d[a][i];
+  }
+}
+void store(field_type &f, int nx, double **f2d) {
+  const size_t px = nx + 5;
+  for (int a = 0; a < 25; ++a) {
+    B2 &b = *field2d(f, a);
+    for (size_t i = 0; i < b.sizeX; ++i)
+      for (size_t j = 0; j < b.sizeY; ++j) f2d[a][j * px + i] = b(i, j);
+  }
+}
+
 } // namespace
+
+// update_tile_halo_{l,r,b,t}_kernel (face 0 left, 1 right, 2 bottom, 3 top) on tile
+// (nx, ny) from its neighbour (nnx, nny); the tile's arrays are updated in place.
+extern "C" int refk_tile_halo(int face, int nx, int ny, double **f2d, double **m1d, int nnx, int nny, double **nf2d, double **nm1d,
+                              const int *fields, int depth) {
+  clover::context ctx{};
+  field_type f(nx + 4, ny + 4, ctx), n(nnx + 4, nny + 4, ctx);
+  load(f, nx, f2d, m1d);
+  load(n, nnx, nf2d, nm1d);
+  global_variables &g = unread_globals();
+#define TILE_ARGS(t) t.density0, t.energy0, t.pressure, t.viscosity, t.soundspeed, t.density1, t.energy1, t.xvel0, t.yvel0, t.xvel1, \
+                     t.yvel1, t.vol_flux_x, t.vol_flux_y, t.mass_flux_x, t.mass_flux_y
+  switch (face) {
+    case 0: update_tile_halo_l_kernel(g, 1, nx, 1, ny, TILE_ARGS(f), 1, nnx, 1, nny, TILE_ARGS(n), fields, depth); break;
+    case 1: update_tile_halo_r_kernel(g, 1, nx, 1, ny, TILE_ARGS(f), 1, nnx, 1, nny, TILE_ARGS(n), fields, depth); break;
+    case 2: update_tile_halo_b_kernel(g, 1, nx, 1, ny, TILE_ARGS(f), 1, nnx, 1, nny, TILE_ARGS(n), fields, depth); break;
+    case 3: update_tile_halo_t_kernel(g, 1, nx, 1, ny, TILE_ARGS(f), 1, nnx, 1, nny, TILE_ARGS(n), fields, depth); break;
+    default: return -1;
+  }
+#undef TILE_ARGS
+  store(f, nx, f2d);
+  return 0;
+}
+
+// clover_decompose (driver/comms.cpp:64-156) as seen by rank `task`: out = {left,
+// right, bottom, top, chunk_neighbours[0..3]} (neighbours are 1-based ranks, -1 external)
+extern "C" int refk_chunk_decompose(int x_cells, int y_cells, int chunks, int task, int *out) {
+  global_config cfg{};
+  cfg.number_of_chunks = chunks;
+  parallel_ par;
+  par.task = task;
+  par.boss = false;
+  int left = 0, right = 0, bottom = 0, top = 0;
+  const std::array<int, 4> nb = clover_decompose(cfg, par, x_cells, y_cells, left, right, bottom, top);
+  const int v[8] = {left, right, bottom, top, nb[0], nb[1], nb[2], nb[3]};
+  for (int i = 0; i < 8; ++i) out[i] = v[i];
+  return 0;
+}
+
+// clover_tile_decompose (driver/comms.cpp:158-253) of a chunk whose first cell is
+// (left, bottom): out[12 t + ...] = {t_left, t_right, t_bottom, t_top,
+// tile_neighbours[0..3], external_tile_mask[0..3]} for each of the `tiles` tiles
+extern "C" int refk_tile_decompose(int x_cells, int y_cells, int tiles, int left, int bottom, int *out) {
+  global_config cfg{};
+  cfg.tiles_per_chunk = tiles;
+  const std::array<int, 4> nb{external_face, external_face, external_face, external_face};
+  chunk_type chunk(nb, 0, 1, 1, x_cells, y_cells, left, left + x_cells - 1, bottom, bottom + y_cells - 1, 0, 0, 0, 0, tiles);
+  global_variables g(cfg, clover::context{}, chunk);
+  const std::vector<tile_info> t = clover_tile_decompose(g, x_cells, y_cells);
+  for (int i = 0; i < tiles; ++i) {
+    const int v[12] = {t[i].t_left, t[i].t_right, t[i].t_bottom, t[i].t_top, t[i].tile_neighbours[0], t[i].tile_neighbours[1],
+                       t[i].tile_neighbours[2], t[i].tile_neighbours[3], t[i].external_tile_mask[0], t[i].external_tile_mask[1],
+                       t[i].external_tile_mask[2], t[i].external_tile_mask[3]};
+    for (int k = 0; k < 12; ++k) out[12 * i + k] = v[k];
+  }
+  return 0;
+}
+
+// clover_pack_message_* (unpack = 0) / clover_unpack_message_* (unpack = 1) of one
+// (nx + 5) x (ny + 5) array for face 0 left, 1 right, 2 bottom, 3 top; buf has buflen
+// elements; both are updated in place.
+extern "C" int refk_pack(int face, int unpack, int nx, int ny, double *field, int field_type_, int depth, double *buf, int buflen,
+                         int offset) {
+  clover::context ctx{};
+  B2 a(ctx, nx + 5, ny + 5);
+  const size_t px = nx + 5;
+  for (size_t i = 0; i < a.sizeX; ++i)
+    for (size_t j = 0; j < a.sizeY; ++j) a(i, j) = field[j * px + i];
+  B1 b(ctx, buflen);
+  for (int i = 0; i < buflen; ++i) b[i] = buf[i];
+  global_variables &g = unread_globals();
+  const int c = cell_data, v = vertex_data, xf = x_face_data, yf = y_face_data;
+  if (!unpack) {
+    if (face == 0) clover_pack_message_left(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else if (face == 1) clover_pack_message_right(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else if (face == 2) clover_pack_message_bottom(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else if (face == 3) clover_pack_message_top(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else return -1;
+  } else {
+    if (face == 0) clover_unpack_message_left(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else if (face == 1) clover_unpack_message_right(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else if (face == 2) clover_unpack_message_bottom(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else if (face == 3) clover_unpack_message_top(g, 1, nx, 1, ny, a, b, c, v, xf, yf, depth, field_type_, offset);
+    else return -1;
+  }
+  for (size_t i = 0; i < a.sizeX; ++i)
+    for (size_t j = 0; j < a.sizeY; ++j) field[j * px + i] = a(i, j);
+  for (int i = 0; i < buflen; ++i) buf[i] = b[i];
+  return 0;
+}
 
 extern "C" int refk_run(const char *kernel, int nx, int ny, double **f2d, double **m1d, const int *ia, const double *da, double *dout,
                         int *iout) {
@@ -152,6 +274,13 @@ extern "C" int refk_run(const char *kernel, int nx, int ny, double **f2d, double
                      ia[1], ia[2]);
   } else if (k == "reset_field") {
     reset_field_kernel(x_min, x_max, y_min, y_max, f.density0, f.density1, f.energy0, f.energy1, f.xvel0, f.xvel1, f.yvel0, f.yvel1);
+  } else if (k == "update_halo_ext") {
+    // ia = {depth, mask[0..14], ext[0..3]}: a face is reflected where the chunk's
+    // neighbour is external (src/serial/update_halo.cpp:41-71); every tile face external
+    std::array<int, 4> chunk_nb{};
+    for (int face = 0; face < 4; ++face) chunk_nb[face] = ia[16 + face] ? external_face : 1;
+    std::array<int, 4> tile_nb{external_tile, external_tile, external_tile, external_tile};
+    update_halo_kernel(x_min, x_max, y_min, y_max, chunk_nb, tile_nb, f, ia + 1, ia[0]);
   } else if (k == "update_halo") {
     // ia = {depth, mask[0..14]}; one chunk, one tile: every neighbour external (driver/comms.cpp:221-237)
     std::array<int, 4> chunk_nb{external_face, external_face, external_face, external_face};

@@ -303,6 +303,10 @@ def refkern_lib():
                                    C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _REFK.refk_run.restype = C.c_int
+        _REFK.refk_tile_halo.restype = C.c_int
+        _REFK.refk_pack.restype = C.c_int
+        _REFK.refk_chunk_decompose.restype = C.c_int
+        _REFK.refk_tile_decompose.restype = C.c_int
     return _REFK
 
 
@@ -322,6 +326,40 @@ def refk_run(kernel: str, nx: int, ny: int, f2d: dict, m1d: dict, ia=(), da=()):
     if rc != 0:
         raise ValueError(f"unknown reference kernel {kernel}")
     return f_out, list(dout), list(iout)
+
+
+def refk_tile_halo(face: int, nx: int, ny: int, f2d: dict, m1d: dict, nnx: int, nny: int, nf2d: dict, nm1d: dict,
+                   fields, depth: int) -> dict:
+    """The reference update_tile_halo_{l,r,b,t}_kernel (face 0 left, 1 right, 2 bottom,
+    3 top) on a copy of tile f2d from neighbour nf2d; returns the tile's arrays."""
+    L = refkern_lib()
+    f_out = {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in f2d.items()}
+    keep = [f_out, {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in m1d.items()},
+            {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in nf2d.items()},
+            {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in nm1d.items()}]
+    P = C.POINTER(C.c_double)
+    fp = (P * 25)(*[keep[0][n].ctypes.data_as(P) for n in FIELDS])
+    mp = (P * 8)(*[keep[1][n].ctypes.data_as(P) for n in MESH])
+    nfp = (P * 25)(*[keep[2][n].ctypes.data_as(P) for n in FIELDS])
+    nmp = (P * 8)(*[keep[3][n].ctypes.data_as(P) for n in MESH])
+    rc = L.refk_tile_halo(face, nx, ny, fp, mp, nnx, nny, nfp, nmp, mask(fields), depth)
+    if rc != 0:
+        raise ValueError("bad face")
+    return f_out
+
+
+def refk_pack(face: int, unpack: int, nx: int, ny: int, field: np.ndarray, field_type: int, depth: int, buf: np.ndarray,
+              offset: int):
+    """The reference clover_pack_message_* / clover_unpack_message_* on copies of one
+    (ny+5, nx+5) array and a message buffer; returns (array, buffer)."""
+    L = refkern_lib()
+    a = np.ascontiguousarray(field, dtype=np.float64).copy()
+    b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+    P = C.POINTER(C.c_double)
+    rc = L.refk_pack(face, unpack, nx, ny, a.ctypes.data_as(P), field_type, depth, b.ctypes.data_as(P), len(b), offset)
+    if rc != 0:
+        raise ValueError("bad face")
+    return a, b
 
 
 def random_chunk_arrays(nx, ny, seed, positive=("density0", "density1", "energy0", "energy1", "volume",
@@ -457,3 +495,75 @@ def oracle_on_arrays(nx, ny, f2d, m1d) -> "Oracle":
 
 def oracle_step_fixed(o: "Oracle", dt: float, advect_x: int):
     _tl_lib().orc_step_fixed(C.byref(o.c), dt, advect_x)
+
+
+class DecomposedOracle:
+    """The oracle on a decomposed domain: one oracle chunk per piece, the reference
+    plan's step (driver/hydro.cpp:57-89) kernel by kernel on every piece, and
+    update_halo as the tile copies (every piece's bottom/top copies, then
+    left/right, driver/update_tile_halo.cpp:26-91) followed by the reflective
+    halo on each piece's external faces (src/serial/update_halo.cpp:690-718).
+    Each piece's kernels clamp at the piece's own edge as the reference's do
+    (e.g. advec_cell's upwind index, src/serial/advec_cell.cpp:77,169), which is
+    why a decomposed run may differ slightly from the undecomposed one."""
+
+    def __init__(self, deck: "Deck", pieces, arrays):
+        """arrays[k] = (fields, mesh) of piece k: (ny+5, nx+5) / 1D numpy arrays."""
+        self.deck, self.pieces = deck, pieces
+        self.o = [oracle_on_arrays(p.nx, p.ny, f, m) for p, (f, m) in zip(pieces, arrays)]
+        self.L = self.o[0].L
+        self.dt = self.dtold = deck.dtinit
+        self.advect_x = 1
+
+    def update_halo(self, names, depth):
+        mk = mask(names)
+        for faces in ((3, 2), (0, 1)):
+            for p, o in zip(self.pieces, self.o):
+                for fc in faces:
+                    if p.neighbours[fc] >= 0:
+                        self.L.orc_tile_halo(C.byref(o.c), C.byref(self.o[p.neighbours[fc]].c), fc, mk, depth)
+        for p, o in zip(self.pieces, self.o):
+            ext = (C.c_int * 4)(*[1 if n < 0 else 0 for n in p.neighbours])
+            self.L.orc_update_halo_ext(C.byref(o.c), mk, depth, ext)
+
+    def _each(self, fn):
+        for o in self.o:
+            fn(C.byref(o.c))
+
+    def step(self):
+        L, d = self.L, self.deck
+        self._each(lambda c: L.orc_ideal_gas(c, 0))
+        self.update_halo(["pressure", "energy0", "density0", "xvel0", "yvel0"], 1)
+        self._each(lambda c: L.orc_viscosity(c))
+        self.update_halo(["viscosity"], 1)
+        dt = G_BIG
+        for o in self.o:
+            v = C.c_double()
+            ints = [C.c_int() for _ in range(4)]
+            L.orc_calc_dt(C.byref(o.c), d.dtmin, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtdiv_safe, C.byref(v),
+                          *[C.byref(x) for x in ints])
+            if v.value <= dt:
+                dt = v.value
+        b = self.dtold * d.dtrise
+        m = b if b < dt else dt
+        dt = d.dtmax if d.dtmax < m else m
+        self.dt = self.dtold = dt
+        self._each(lambda c: L.orc_PdV(c, 1, dt))
+        self._each(lambda c: L.orc_ideal_gas(c, 1))
+        self.update_halo(["pressure"], 1)
+        self._each(lambda c: L.orc_revert(c))
+        self._each(lambda c: L.orc_accelerate(c, dt))
+        self._each(lambda c: L.orc_PdV(c, 0, dt))
+        self._each(lambda c: L.orc_flux_calc(c, dt))
+        d1, d2 = (1, 2) if self.advect_x else (2, 1)
+        cells = ["density1", "energy1", "xvel1", "yvel1", "mass_flux_x", "mass_flux_y"]
+        self.update_halo(["energy1", "density1", "vol_flux_x", "vol_flux_y"], 2)
+        self._each(lambda c: L.orc_advec_cell(c, 1, d1))
+        self.update_halo(cells, 2)
+        self._each(lambda c: (L.orc_advec_mom(c, 1, 1, d1), L.orc_advec_mom(c, 2, 1, d1)))
+        self._each(lambda c: L.orc_advec_cell(c, 2, d2))
+        self.update_halo(cells, 2)
+        self._each(lambda c: (L.orc_advec_mom(c, 1, 2, d2), L.orc_advec_mom(c, 2, 2, d2)))
+        self._each(lambda c: L.orc_reset_field(c))
+        self.advect_x = 1 - self.advect_x
+        return dt

@@ -113,3 +113,66 @@ def test_update_halo_bitwise(nx, ny, depth, fields):
     ref, _, _ = T.refk_run("update_halo", nx, ny, f, m, ia=[depth] + mask)
     orc, _, _ = _oracle_run("update_halo", nx, ny, f, m, ia=[depth] + mask)
     _assert_same("update_halo", nx, ny, ref, orc)
+
+
+# ---- decomposed halos: faces shared with a tile or chunk, tile copies, messages ---------------
+HALO_SETS = [T.FIELDS[:15], ["density0", "energy0", "pressure", "xvel0", "yvel0"], ["vol_flux_x", "vol_flux_y", "mass_flux_x",
+                                                                                     "mass_flux_y"], ["soundspeed"]]
+EXT_PATTERNS = [(1, 1, 1, 1), (0, 1, 1, 1), (1, 0, 1, 1), (1, 1, 0, 1), (1, 1, 1, 0), (0, 0, 1, 1), (1, 1, 0, 0), (0, 1, 0, 1),
+                (1, 0, 1, 0), (0, 0, 0, 0)]
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 8), (13, 7)])
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("ext", EXT_PATTERNS)
+@pytest.mark.parametrize("fields", [0, 1, 2, 3])
+def test_update_halo_external_faces_bitwise(nx, ny, depth, ext, fields):
+    """src/serial/update_halo.cpp:33-684 with only the flagged faces external."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=zlib.crc32(repr((nx, ny, depth, ext, fields)).encode()))
+    names = HALO_SETS[fields]
+    mask = [1 if n in names else 0 for n in T.FIELDS[:15]]
+    ref, _, _ = T.refk_run("update_halo_ext", nx, ny, f, m, ia=[depth] + mask + list(ext))
+    o = T.oracle_on_arrays(nx, ny, f, m)
+    o.L.orc_update_halo_ext(C.byref(o.c), T.mask(names), depth, (C.c_int * 4)(*ext))
+    _assert_same("update_halo_ext", nx, ny, ref, {n: o.f(n) for n in T.FIELDS})
+
+
+@pytest.mark.parametrize("face", [0, 1, 2, 3])
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("fields", [0, 1, 2])
+def test_tile_halo_bitwise(face, depth, fields):
+    """update_tile_halo_{l,r,b,t}_kernel (src/serial/update_tile_halo_kernel.cpp:30-804):
+    the tile and its neighbour differ in size along the face normal."""
+    nx, ny = 11, 9
+    nnx, nny = (7, ny) if face in (0, 1) else (nx, 6)
+    seed = zlib.crc32(repr((face, depth, fields)).encode())
+    f, m = T.random_chunk_arrays(nx, ny, seed=seed)
+    nf, nm = T.random_chunk_arrays(nnx, nny, seed=seed + 1)
+    names = HALO_SETS[fields]
+    ref = T.refk_tile_halo(face, nx, ny, f, m, nnx, nny, nf, nm, names, depth)
+    o = T.oracle_on_arrays(nx, ny, f, m)
+    n = T.oracle_on_arrays(nnx, nny, nf, nm)
+    o.L.orc_tile_halo(C.byref(o.c), C.byref(n.c), face, T.mask(names), depth)
+    _assert_same("tile_halo", nx, ny, ref, {k: o.f(k) for k in T.FIELDS})
+
+
+@pytest.mark.parametrize("face", [0, 1, 2, 3])
+@pytest.mark.parametrize("unpack", [0, 1])
+@pytest.mark.parametrize("field_type", [1, 2, 3, 4])
+@pytest.mark.parametrize("depth", [1, 2])
+def test_pack_message_bitwise(face, unpack, field_type, depth):
+    """clover_pack_message_* / clover_unpack_message_* (src/serial/pack_kernel.cpp:27-303):
+    message layout and the layers read or written, with a nonzero buffer offset."""
+    nx, ny = 10, 7
+    rng = np.random.default_rng(zlib.crc32(repr((face, unpack, field_type, depth)).encode()))
+    field = rng.uniform(-1, 1, (ny + 5, nx + 5))
+    offset = 3 * depth * ((ny if face in (0, 1) else nx) + 5)
+    buf = rng.uniform(-1, 1, offset + 2 * depth * (max(nx, ny) + 5))
+    ref_a, ref_b = T.refk_pack(face, unpack, nx, ny, field, field_type, depth, buf, offset)
+    o = T.oracle_on_arrays(nx, ny, *T.random_chunk_arrays(nx, ny, seed=1))
+    a = field.copy()
+    b = buf.copy()
+    P = C.POINTER(C.c_double)
+    fn = o.L.orc_unpack_message if unpack else o.L.orc_pack_message
+    fn(C.byref(o.c), a.ctypes.data_as(P), field_type, face, depth, b.ctypes.data_as(P), C.c_long(offset))
+    assert np.array_equal(a, ref_a) and np.array_equal(b, ref_b)
