@@ -78,7 +78,11 @@ struct clv_adjoint {
   size_t arr = 0;          // doubles per 2D array (pitch * (ny + 5))
   double *ckpt = nullptr;  // (nckpt) x 4 arrays
   int nckpt = 0;
-  double *log = nullptr;   // log slots
+  double *log = nullptr;   // the undo log's allocation
+  // the slots' arrays (element (0, 0) each; aligned like the chunk arena's arrays).
+  // Restoring a snapshot swaps the primal field's pointer with its slot's, so a
+  // slot may later hold what was a field's buffer and vice versa.
+  std::vector<double *> slot;
   size_t log_slots = 0, log_used = 0;
   std::vector<LogEntry> entries;
   bool forward_done = false;
@@ -89,7 +93,7 @@ namespace {
 
 const int kState[4] = {CLV_DENSITY0, CLV_ENERGY0, CLV_XVEL0, CLV_YVEL0};
 // arrays copied into the undo log per step (forward_step's logged() lists)
-constexpr int kSnapsPerStep = 27;
+constexpr int kSnapsPerStep = 23;
 
 clv_status copy_array(clv_adjoint *g, double *dst, const double *src) {
   return hip_check(hipMemcpyAsync(dst, src, g->arr * sizeof(double), hipMemcpyDeviceToDevice, g->p->stream), "adjoint copy");
@@ -119,7 +123,7 @@ clv_status logged(clv_adjoint *g, bool rec, LogEntry e, std::initializer_list<in
   e.slot = g->log_used;
   for (int f : snap) {
     if (g->log_used >= g->log_slots) return set_error(CLV_ERR_ARG, "adjoint: undo log full");
-    clv_status st = copy_array(g, g->log + g->log_used * g->arr, base(g->p, f));
+    clv_status st = copy_array(g, g->slot[g->log_used], base(g->p, f));
     if (st != CLV_OK) return st;
     e.snap[e.nsnap++] = f;
     ++g->log_used;
@@ -170,9 +174,12 @@ clv_status forward_step(clv_adjoint *g, int n, bool rec) {
   TRY_A(halo_op(g, rec, {CLV_PRESSURE}, 1));
   TRY_A(logged(g, rec, entry(OP_REVERT), {CLV_DENSITY1, CLV_ENERGY1}));
   TRY_A(k_revert(p));
-  TRY_A(logged(g, rec, entry(OP_ACCEL), {CLV_XVEL1, CLV_YVEL1}));
+  // accelerate and the PdV corrector overwrite u1, v1 and rho1, e1, but no kernel
+  // adjoint reads those values before revert (this step) and advec_mom (the
+  // previous step) restore their own inputs: nothing to log
+  TRY_A(logged(g, rec, entry(OP_ACCEL), {}));
   TRY_A(k_accelerate(p, dt));
-  TRY_A(logged(g, rec, entry(OP_PDV, 0), {CLV_DENSITY1, CLV_ENERGY1}));
+  TRY_A(logged(g, rec, entry(OP_PDV, 0), {}));
   TRY_A(k_PdV(p, 0, dt));
   TRY_A(logged(g, rec, entry(OP_FLUX), {CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}));
   TRY_A(k_flux_calc(p, dt));
@@ -296,8 +303,12 @@ clv_status clv_adjoint_create(const clv_chunk *initial, double dt, int steps, in
   if (st == CLV_OK && hipMalloc((void **)&g->ckpt, (size_t)g->nckpt * 4 * g->arr * sizeof(double)) != hipSuccess)
     st = set_error(CLV_ERR_ALLOC, "clv_adjoint_create: checkpoint allocation failed");
   g->log_slots = (size_t)g->every * kSnapsPerStep;
-  if (st == CLV_OK && hipMalloc((void **)&g->log, g->log_slots * g->arr * sizeof(double)) != hipSuccess)
+  // kLeadPad doubles ahead of each slot's element (0, 0), as in the chunk arena, so
+  // that a slot swapped in as a field keeps element (2, j) on a 128-byte line
+  if (st == CLV_OK && hipMalloc((void **)&g->log, (g->log_slots * g->arr + 32) * sizeof(double)) != hipSuccess)
     st = set_error(CLV_ERR_ALLOC, "clv_adjoint_create: undo-log allocation failed");
+  if (st == CLV_OK)
+    for (size_t k = 0; k < g->log_slots; ++k) g->slot.push_back(g->log + kLeadPad + k * g->arr);
   if (st == CLV_OK) st = save_state(g, 0);  // checkpoint 0 = the initial state
   if (st == CLV_OK) st = hip_check(hipStreamSynchronize(g->p->stream), "clv_adjoint_create: sync");
   if (st != CLV_OK) {
@@ -379,8 +390,9 @@ clv_status clv_adjoint_backward(clv_adjoint *g, double ke_bar, double *grad_dens
     for (int n = n0; n < n1 && st == CLV_OK; ++n) st = forward_step(g, n, true);
     if (st == CLV_OK) st = hip_check(hipStreamSynchronize(p->stream), "clv_adjoint_backward: sync");
     for (auto it = g->entries.rbegin(); it != g->entries.rend() && st == CLV_OK; ++it) {
-      for (int q = 0; q < it->nsnap && st == CLV_OK; ++q)
-        st = copy_array(g, base(p, it->snap[q]), g->log + (it->slot + q) * g->arr);
+      // restore what the kernel overwrote: the snapshot becomes the field, the
+      // field's buffer the (consumed) slot
+      for (int q = 0; q < it->nsnap; ++q) std::swap(p->d.f[it->snap[q]], g->slot[it->slot + q]);
       if (st == CLV_OK && p->stream != a->stream) st = hip_check(hipStreamSynchronize(p->stream), "sync");
       if (st == CLV_OK) st = adjoint_of(g, *it);
       if (st == CLV_OK && p->stream != a->stream) st = hip_check(hipStreamSynchronize(a->stream), "sync");

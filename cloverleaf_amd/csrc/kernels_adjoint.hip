@@ -378,9 +378,10 @@ __device__ void face_bar(double vf, double pre, double r_uw, double r_dn, double
   g[4] = b1.dw;
 }
 
-// adjoint of el_momflux: g = d(flux)/d(node_flux, mass_pre_donor, v_uw, v_dn, v_dw) * gb
-__device__ void momflux_bar(double flux, double mpre, double width, double width_dif, double v_uw, double v_dn, double v_dw, double gb,
-                            double g[5]) {
+// adjoint of el_momflux: g = d(flux)/d(node_flux, mass_pre_donor, v_uw, v_dn, v_dw) * gb;
+// returns the momentum flux itself (el_momflux<0>'s value, the same operations)
+__device__ double momflux_bar(double flux, double mpre, double width, double width_dif, double v_uw, double v_dn, double v_dw, double gb,
+                              double g[5]) {
   const double sigma = ::fabs(flux) / mpre;
   const double vdiffuw = v_dn - v_uw, vdiffdw = v_dw - v_dn;
   double limiter = 0.0, wind = 1.0, auw = 0.0, adw = 0.0, x = 0.0, m1 = 0.0;
@@ -423,6 +424,7 @@ __device__ void momflux_bar(double flux, double mpre, double width, double width
   g[2] = vuw_b;
   g[3] = vdn_b;
   g[4] = vdw_b;
+  return advec_vel_s * flux;
 }
 
 // ---- ideal_gas, src/serial/ideal_gas.cpp:38-46: p = 0.4 rho e on the interior ---------------
@@ -914,44 +916,12 @@ __device__ __forceinline__ FaceRoles mom_roles(double nf, int f) {
   if (nf < 0.0) return FaceRoles{f + 2, f + 1, f, f + 1};
   return FaceRoles{f - 1, f, f + 1, f - 1};
 }
-// momentum flux of node face f for velocity vel_id from the stored node values
-__device__ __forceinline__ double momflux_value(const Dev &p, const Sweep &sw, int vel_id, int f, int t) {
-  const double nf = PS(CLV_WORK1, f, t);
-  const FaceRoles r = mom_roles(nf, f);
-  const int cd = sw.dir == 1 ? CLV_CELLDX : CLV_CELLDY;
-  return el_momflux<0>(cst<0>(nf), cst<0>(PS(CLV_WORK3, r.dn, t)), PM(cd, f), PM(cd, r.dif), cst<0>(PS(vel_id, r.uw, t)),
-                       cst<0>(PS(vel_id, r.dn, t)), cst<0>(PS(vel_id, r.dw, t)))
-      .v;
-}
-
-// M1, nodes s in [2, smax+2] of lines [2, tmax+2]: vel' = (vel mpre + mf(s-1) - mf(s)) / mpost
-// for both velocities: own-velocity adjoints -> a.work1 (u) / a.work2 (v), node_mass_pre
-// and node_mass_post adjoints -> a.work3 / a.work4
-__global__ void adj_mom_update_k(Dev p, Dev a, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  int s, t;
-  sweep_pos(dir, 2, 2, s, t);
-  if (s > smax + 2 || t > tmax + 2) return;
-  const double mpost = PS(CLV_WORK2, s, t), mpre = PS(CLV_WORK3, s, t);
-  double mpreb = 0.0, mpostb = 0.0;
-  for (int w = 0; w < 2; ++w) {
-    const int vid = w == 0 ? CLV_XVEL1 : CLV_YVEL1;
-    const double vb = AS(vid, s, t), vel = PS(vid, s, t);
-    const double vnew = (vel * mpre + momflux_value(p, sw, vid, s - 1, t) - momflux_value(p, sw, vid, s, t)) / mpost;
-    AS(w == 0 ? CLV_WORK1 : CLV_WORK2, s, t) = vb * mpre / mpost;
-    mpreb += vb * vel / mpost;
-    mpostb -= vb * vnew / mpost;
-  }
-  AS(CLV_WORK3, s, t) = mpreb;
-  AS(CLV_WORK4, s, t) = mpostb;
-}
-
 // MF, node faces f in [1, smax+2] of lines [2, tmax+2], one velocity: the face's
 // momentum-flux adjoint from the updates of nodes f (its right flux, -1 / mpost) and
 // f+1 (left, +1 / mpost), then the face differentiated once (momflux_bar): node-flux share ->
 // a.work5, donor node_mass_pre share -> p.work5 (first velocity writes, second adds);
-// velocity shares of upwind / donor / downwind -> a.work6 / a.work7 / p.work4
+// velocity shares of upwind / donor / downwind -> a.work6 / a.work7 / p.work4; the
+// momentum flux itself -> a.work1 (read by the node update's adjoint)
 template <int VEL, bool FIRST>
 __global__ void adj_mom_face_k(Dev p, Dev a, int dir) {
   const Sweep sw{dir};
@@ -966,7 +936,8 @@ __global__ void adj_mom_face_k(Dev p, Dev a, int dir) {
   const FaceRoles r = mom_roles(nf, f);
   const int cd = dir == 1 ? CLV_CELLDX : CLV_CELLDY;
   double g[5];
-  momflux_bar(nf, PS(CLV_WORK3, r.dn, t), PM(cd, f), PM(cd, r.dif), PS(VEL, r.uw, t), PS(VEL, r.dn, t), PS(VEL, r.dw, t), fb, g);
+  AS(CLV_WORK1, f, t) =
+      momflux_bar(nf, PS(CLV_WORK3, r.dn, t), PM(cd, f), PM(cd, r.dif), PS(VEL, r.uw, t), PS(VEL, r.dn, t), PS(VEL, r.dw, t), fb, g);
   if (FIRST) {
     AS(CLV_WORK5, f, t) = g[0];
     PS(CLV_WORK5, f, t) = g[1];
@@ -979,10 +950,14 @@ __global__ void adj_mom_face_k(Dev p, Dev a, int dir) {
   PS(CLV_WORK4, f, t) = g[4];
 }
 
-// MN, nodes s in [0, smax+4] of lines [2, tmax+2], one velocity: new adjoint (own
-// update share for updated nodes, else passed through, plus the shares of faces
-// [s-2, s+1] that read the node), installed in place.  After the second velocity,
-// the node_mass_pre adjoint (a.work3) gains the donor shares of faces s-1 and s.
+// MN, nodes s in [0, smax+4] of lines [2, tmax+2], one velocity.  Updated nodes
+// (s in [2, smax+2]) differentiate vel' = (vel mpre + mf(s-1) - mf(s)) / mpost
+// (src/serial/advec_mom.cpp:140-146): own share vb mpre / mpost, and the node_mass_pre /
+// node_mass_post adjoints vb vel / mpost and -vb vel' / mpost, started by the first
+// velocity in a.work3 / a.work4 and completed by the second; other nodes pass their
+// adjoint through.  Then the shares of faces [s-2, s+1] that read the node are added and
+// the new adjoint installed in place.  After the second velocity, the node_mass_pre
+// adjoint (a.work3) gains the donor shares of faces s-1 and s.
 template <int VEL, bool LAST>
 __global__ void adj_mom_node_k(Dev p, Dev a, int dir) {
   const Sweep sw{dir};
@@ -991,7 +966,15 @@ __global__ void adj_mom_node_k(Dev p, Dev a, int dir) {
   sweep_pos(dir, 0, 2, s, t);
   if (s > smax + 4 || t > tmax + 2) return;
   const bool updated = s >= 2 && s <= smax + 2;
-  double g = updated ? AS(VEL == CLV_XVEL1 ? CLV_WORK1 : CLV_WORK2, s, t) : AS(VEL, s, t);
+  double g = AS(VEL, s, t), mpreb = 0.0;
+  if (updated) {
+    const double vb = g, vel = PS(VEL, s, t), mpost = PS(CLV_WORK2, s, t), mpre = PS(CLV_WORK3, s, t);
+    const double vnew = (vel * mpre + AS(CLV_WORK1, s - 1, t) - AS(CLV_WORK1, s, t)) / mpost;
+    g = vb * mpre / mpost;
+    mpreb = (LAST ? AS(CLV_WORK3, s, t) : 0.0) + vb * vel / mpost;
+    AS(CLV_WORK4, s, t) = (LAST ? AS(CLV_WORK4, s, t) : 0.0) - vb * vnew / mpost;
+    if (!LAST) AS(CLV_WORK3, s, t) = mpreb;
+  }
   double gm = 0.0;
   for (int f = max(1, s - 2); f <= min(smax + 2, s + 1); ++f) {
     const FaceRoles r = mom_roles(PS(CLV_WORK1, f, t), f);
@@ -1003,7 +986,7 @@ __global__ void adj_mom_node_k(Dev p, Dev a, int dir) {
     if (r.dw == s) g += PS(CLV_WORK4, f, t);
   }
   AS(VEL, s, t) = g;
-  if (LAST) AS(CLV_WORK3, s, t) = (updated ? AS(CLV_WORK3, s, t) : 0.0) + gm;
+  if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
 }
 
 // M5: node masses.  node_mass_pre(s) = post(s) - nf(s-1) + nf(s) (nodes s in [1, smax+3])
@@ -1180,7 +1163,6 @@ clv_status adj_advec_mom(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
   const int mom_sweep = dir + 2 * (sweep - 1);
   const dim3 gn = sgrid(dir, smax + 5, tmax + 1), gf = sgrid(dir, smax + 2, tmax + 1);
   hipLaunchKernelGGL(adj_mom_values_k, sgrid(dir, smax + 4, tmax + 1), kBlk, 0, a->stream, p->d, mom_sweep, dir);
-  hipLaunchKernelGGL(adj_mom_update_k, sgrid(dir, smax + 1, tmax + 1), kBlk, 0, a->stream, p->d, a->d, dir);
   hipLaunchKernelGGL((adj_mom_face_k<CLV_XVEL1, true>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
   hipLaunchKernelGGL((adj_mom_node_k<CLV_XVEL1, false>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
   hipLaunchKernelGGL((adj_mom_face_k<CLV_YVEL1, false>), gf, kBlk, 0, a->stream, p->d, a->d, dir);

@@ -382,7 +382,9 @@ clv_status clv_adjoint_forward(clv_adjoint *g, double summary[5]);
  * initial interior cells into (y_cells rows of pitch doubles, host or device;
  * either may be NULL). */
 clv_status clv_adjoint_backward(clv_adjoint *g, double ke_bar, double *grad_density0, double *grad_energy0, int64_t pitch);
-/* the primal and adjoint chunks (for inspection; owned by g) */
+/* the primal and adjoint chunks (for inspection; owned by g).  The backward pass
+ * restores logged arrays by exchanging buffers, so a field pointer of the primal
+ * read before clv_adjoint_backward may be stale after it: read it again. */
 clv_status clv_adjoint_chunks(clv_adjoint *g, clv_chunk **primal, clv_chunk **adjoint);
 
 /* ---- measurement ------------------------------------------------------------------------ */
