@@ -21,9 +21,13 @@ path and as one batch, and reports both times.
 kinetic energy of a 200-step fixed-dt run on 1920^2 (cloverleaf_amd.ADJOINT_WORKLOAD)
 against its forward pass, and checks it against central differences.
 
+`decomposed` (unless --no-decomposed) runs clover_bm16_short (3840^2) as one
+chunk, as 4 tiles and as 4 chunks on the one GPU, 50 steps each.
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--plan fused|reference]
                        [--deck NAME] [--no-cpu] [--no-batch] [--batch-only]
-                       [--no-adjoint] [--adjoint-only] [--adjoint-directions D] [--profile-out FILE]
+                       [--no-adjoint] [--adjoint-only] [--adjoint-directions D]
+                       [--no-decomposed] [--decomposed-only] [--profile-out FILE]
 """
 import argparse
 import json
@@ -290,6 +294,53 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
     }
 
 
+def decomposed_leg(clv, deck="clover_bm16_short", pieces=4, steps=50, warmup=2):
+    """The decomposed-domain path on one GPU: the same problem as one chunk, as
+    `pieces` tiles of one chunk (tile-to-tile halo copies) and as `pieces` chunks of
+    a mesh (packed halo messages, the reference's MPI layout), all on the
+    reference-plan kernels that the decomposed driver runs per piece, plus the
+    fused plan on one chunk.  Reports each run's rate, the decomposition's cost
+    against the one-chunk reference plan, and how far its final state moved from
+    the one-chunk run (advec_cell's upwind clamp at a piece's own edge, the
+    reference's behaviour; bit-identical elsewhere)."""
+    from cloverleaf_amd.decompose import Decomposed, chunk_decompose, tile_decompose
+    text = (REPO / "tests" / "golden" / "decks" / f"{deck}.in").read_text()
+    cfg = clv.parse_deck(text)
+    nx, ny = cfg.x_cells, cfg.y_cells
+    cfg.end_step, cfg.end_time = 1 << 30, 1e30
+    out = {"workload": f"{deck}: {nx}x{ny}, {pieces} pieces, steps {warmup + 1}..{warmup + steps}", "steps": steps}
+
+    def timed(step, sync):
+        for _ in range(warmup):
+            step()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        sync()
+        return time.perf_counter() - t0
+
+    summaries = {}
+    for plan_name, plan in (("one_chunk_reference_plan", clv.PLAN_REFERENCE), ("one_chunk_fused_plan", clv.PLAN_FUSED)):
+        with clv.Chunk(nx, ny) as ch:
+            st, _ = ch.start(cfg)
+            wall = timed(lambda: ch.step(cfg, st, plan), ch.synchronize)
+            summaries[plan_name] = ch.summary()
+        out[plan_name] = {"s": wall, "cell_steps_per_s": nx * ny * steps / wall}
+    for mode, split in (("tiles", tile_decompose), ("chunks", chunk_decompose)):
+        with Decomposed(cfg, split(nx, ny, pieces), mode) as d:
+            d.start()
+            wall = timed(d.step, lambda: [c.synchronize() for c in d.chunks])
+            s = d.summary()
+        ref = summaries["one_chunk_reference_plan"]
+        out[mode] = {"s": wall, "cell_steps_per_s": nx * ny * steps / wall,
+                     "time_vs_one_chunk_reference_plan": wall / out["one_chunk_reference_plan"]["s"],
+                     "ke_rel_diff_vs_one_chunk": abs(s[3] - ref[3]) / abs(ref[3]),
+                     "mass_rel_diff_vs_one_chunk": abs(s[1] - ref[1]) / abs(ref[1])}
+        print(f"# bench: decomposed {mode} {wall:.3f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -305,6 +356,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload")
     ap.add_argument("--adjoint-directions", type=int, default=32,
                     help="finite-difference directions of the adjoint check (0: timings only)")
+    ap.add_argument("--no-decomposed", action="store_true", help="skip the decomposed-domain workload")
+    ap.add_argument("--decomposed-only", action="store_true", help="only the decomposed-domain workload")
     ap.add_argument("--probe-only", action="store_true",
                     help="only run the copy kernel (4 GiB, 20 launches): the counter calibration run")
     ap.add_argument("--cpu-steps", type=int, default=120)
@@ -325,6 +378,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(clv, args.adjoint_directions)}))
+        return
+    if args.decomposed_only:
+        print(json.dumps({"decomposed": decomposed_leg(clv)}))
         return
     deck_path = REPO / "tests" / "golden" / "decks" / f"{args.deck}.in"
     deck_text = deck_path.read_text()
@@ -410,6 +466,9 @@ def main():
     if not args.no_adjoint:
         print("# bench: adjoint workload ...", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(clv, args.adjoint_directions)
+    if not args.no_decomposed:
+        print("# bench: decomposed workload ...", file=sys.stderr, flush=True)
+        out["decomposed"] = decomposed_leg(clv)
     if not args.no_cpu:
         print("# bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(deck_text, args.cpu_steps)

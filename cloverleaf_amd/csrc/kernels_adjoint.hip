@@ -318,7 +318,7 @@ __device__ __forceinline__ VlBar vl_bar(double sigma, double sigma3, double sigm
 }
 
 // adjoint of el_face: g = d(mf, ef)/d(vf, pre_dn, r_uw, r_dn, r_dw, e_uw, e_dn, e_dw)^T (mfb, efb)
-__device__ void face_bar(double vf, double pre, double r_uw, double r_dn, double r_dw, double e_uw, double e_dn, double e_dw, double vd_face,
+__device__ __forceinline__ void face_bar(double vf, double pre, double r_uw, double r_dn, double r_dw, double e_uw, double e_dn, double e_dw, double vd_face,
                          double vd_dif, double mfb, double efb, double g[8]) {
   const double one_by_six = 1.0 / 6.0;
   // forward values
@@ -663,6 +663,11 @@ __global__ void adj_zero_k(double *f, int64_t pitch, int i0, int i1, int j0, int
   f[(int64_t)j * pitch + i] = 0.0;
 }
 // dst += src over a range, then src = 0 (revert, reset_field)
+__global__ void adj_copy_k(double *dst, const double *src, size_t n) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
 __global__ void adj_move_k(double *dst, double *src, int64_t pitch, int i0, int i1, int j0, int j1) {
   const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x, j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > i1 || j > j1) return;
@@ -736,115 +741,226 @@ __global__ void adj_cell_values_k(Dev p, int sweep, int dir) {
   PS(CLV_WORK3, s, t) = ef.v;
 }
 
-// the update of cell c, src/serial/advec_cell.cpp:120-125 / :211-216:
+// One cell's update, src/serial/advec_cell.cpp:120-125 / :211-216:
 // rho' = (rho pre + mf_l - mf_r) / (pre + vf_l - vf_r),
-// e' = (e rho pre + ef_l - ef_r) / (rho pre + mf_l - mf_r).
-struct CellUpd {
-  double rho, e, pre, post_mass, advec_vol, rho_new, e_new;
+// e' = (e rho pre + ef_l - ef_r) / (rho pre + mf_l - mf_r),
+// differentiated for the incoming adjoints rnb, enb of rho', e'.  Own shares of rho, e
+// and pre (rb, eb, pb; the adjoints then accumulate the faces' shares), and the terms its
+// faces take: d/d mf_l = M = rnb / advec_vol - enb e' / post_mass, d/d ef_l = E =
+// enb / post_mass, d/d vf_l = -V = -rnb rho' / advec_vol (the right face's: negated).
+// Ghost cells pass their adjoints through; cells outside [0, smax+3] are never read.
+struct CellAdj {
+  double rho, e, pre;  // primal values, read by the faces
+  double rb, eb, pb;
+  double M, E, V;
 };
-__device__ __forceinline__ CellUpd cell_update(const Dev &p, const Sweep &sw, int vf_id, int c, int t) {
-  CellUpd u;
-  u.rho = PS(CLV_DENSITY1, c, t);
-  u.e = PS(CLV_ENERGY1, c, t);
-  u.pre = PS(CLV_WORK1, c, t);
-  const double pre_mass_s = u.rho * u.pre;
-  u.post_mass = pre_mass_s + PS(CLV_WORK2, c, t) - PS(CLV_WORK2, c + 1, t);
-  u.e_new = (u.e * pre_mass_s + PS(CLV_WORK3, c, t) - PS(CLV_WORK3, c + 1, t)) / u.post_mass;
-  u.advec_vol = u.pre + PS(vf_id, c, t) - PS(vf_id, c + 1, t);
-  u.rho_new = u.post_mass / u.advec_vol;
-  return u;
-}
-
-// C1, faces f in [2, smax+3] of lines [2, tmax+1]: the face's total mass / energy flux
-// adjoints (external + the updates of cells f and f-1), then the face differentiated
-// once (face_bar): shares of its vf (+ the updates' vf terms) -> a.work3, donor pre_vol -> a.work1,
-// rho of upwind / donor / downwind -> a.work2 / a.work4 / a.work5, e of them ->
-// a.work6 / a.work7 / p.work4.  Consumes the mass-flux adjoint.
-__global__ void adj_cell_face_k(Dev p, Dev a, int sweep, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  int f, t;
-  sweep_pos(dir, 2, 2, f, t);
-  if (f > smax + 3 || t > tmax + 1) return;
-  const int mf_id = dir == 1 ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y, vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y;
-  const int vd = dir == 1 ? CLV_VERTEXDX : CLV_VERTEXDY;
-  double mfb = AS(mf_id, f, t), efb = 0.0, vfb = 0.0;
-  AS(mf_id, f, t) = 0.0;
-  // d rho'/d mf_l = 1 / advec_vol, d e'/d mf_l = -e' / post_mass, d e'/d ef_l = 1 / post_mass,
-  // d rho'/d vf_l = -rho' / advec_vol; the right face's are the negatives
-  if (f <= smax + 1) {  // left face of cell f
-    const CellUpd u = cell_update(p, sw, vf_id, f, t);
-    const double rb = AS(CLV_DENSITY1, f, t), eb = AS(CLV_ENERGY1, f, t);
-    mfb += rb / u.advec_vol - eb * u.e_new / u.post_mass;
-    efb += eb / u.post_mass;
-    vfb -= rb * u.rho_new / u.advec_vol;
+// the values cell c's update reads (zero outside [0, smax+3]; the fluxes only for
+// interior cells)
+struct CellIn {
+  double rho, e, pre, rnb, enb, mf_l, mf_r, ef_l, ef_r, vf_l, vf_r;
+};
+__device__ __forceinline__ CellIn cell_in(const Dev &p, const Dev &a, const Sweep &sw, int smax, int vf_id, int c, int t) {
+  CellIn v{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (c >= 0 && c <= smax + 3) {
+    v.rho = PS(CLV_DENSITY1, c, t);
+    v.e = PS(CLV_ENERGY1, c, t);
+    v.pre = PS(CLV_WORK1, c, t);
+    v.rnb = AS(CLV_DENSITY1, c, t);
+    v.enb = AS(CLV_ENERGY1, c, t);
+    if (c >= 2 && c <= smax + 1) {
+      v.mf_l = PS(CLV_WORK2, c, t);
+      v.mf_r = PS(CLV_WORK2, c + 1, t);
+      v.ef_l = PS(CLV_WORK3, c, t);
+      v.ef_r = PS(CLV_WORK3, c + 1, t);
+      v.vf_l = PS(vf_id, c, t);
+      v.vf_r = PS(vf_id, c + 1, t);
+    }
   }
-  if (f - 1 >= 2 && f - 1 <= smax + 1) {  // right face of cell f - 1
-    const CellUpd u = cell_update(p, sw, vf_id, f - 1, t);
-    const double rb = AS(CLV_DENSITY1, f - 1, t), eb = AS(CLV_ENERGY1, f - 1, t);
-    mfb -= rb / u.advec_vol - eb * u.e_new / u.post_mass;
-    efb -= eb / u.post_mass;
-    vfb += rb * u.rho_new / u.advec_vol;
-  }
-  const double vfv = PS(vf_id, f, t);
-  const FaceRoles r = face_roles(vfv, f, smax);
-  double c[8];
-  face_bar(vfv, PS(CLV_WORK1, r.dn, t), PS(CLV_DENSITY1, r.uw, t), PS(CLV_DENSITY1, r.dn, t), PS(CLV_DENSITY1, r.dw, t), PS(CLV_ENERGY1, r.uw, t),
-           PS(CLV_ENERGY1, r.dn, t), PS(CLV_ENERGY1, r.dw, t), PM(vd, f), PM(vd, r.dif), mfb, efb, c);
-  AS(CLV_WORK3, f, t) = vfb + c[0];
-  AS(CLV_WORK1, f, t) = c[1];
-  AS(CLV_WORK2, f, t) = c[2];
-  AS(CLV_WORK4, f, t) = c[3];
-  AS(CLV_WORK5, f, t) = c[4];
-  AS(CLV_WORK6, f, t) = c[5];
-  AS(CLV_WORK7, f, t) = c[6];
-  PS(CLV_WORK4, f, t) = c[7];
+  return v;
 }
-
-// C2, cells c in [0, smax+3] of lines [2, tmax+1]: new rho1, e1 adjoints (own update
-// for interior cells, else passed through, plus the shares of faces [c-1, c+2] that
-// read the cell), written in place; the pre_vol adjoint -> p.work5
-__global__ void adj_cell_cell_k(Dev p, Dev a, int sweep, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  int c, t;
-  sweep_pos(dir, 0, 2, c, t);
-  if (c > smax + 3 || t > tmax + 1) return;
-  const int vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y;
-  double rb, eb, pb = 0.0;
-  if (c >= 2 && c <= smax + 1) {
+__device__ __forceinline__ CellAdj cell_adj_of(const CellIn &v, bool interior) {
+  // scalars, and one struct built at the end, so that the cell stays in registers
+  double rb = v.rnb, eb = v.enb, pb = 0.0, M = 0.0, E = 0.0, V = 0.0;
+  if (interior) {
+    const double pre_mass_s = v.rho * v.pre;
+    const double post_mass = pre_mass_s + v.mf_l - v.mf_r;
+    const double e_new = (v.e * pre_mass_s + v.ef_l - v.ef_r) / post_mass;
+    const double advec_vol = v.pre + v.vf_l - v.vf_r;
+    const double rho_new = post_mass / advec_vol;
     // d rho'/d rho = pre / advec_vol, d rho'/d pre = (rho - rho') / advec_vol,
     // d e'/d e = rho pre / post_mass, d e'/d rho = (e - e') pre / post_mass,
     // d e'/d pre = (e - e') rho / post_mass
-    const CellUpd u = cell_update(p, sw, vf_id, c, t);
-    const double rnb = AS(CLV_DENSITY1, c, t), enb = AS(CLV_ENERGY1, c, t);
-    const double de = (u.e - u.e_new) / u.post_mass;
-    rb = rnb * u.pre / u.advec_vol + enb * de * u.pre;
-    eb = enb * (u.rho * u.pre) / u.post_mass;
-    pb = rnb * (u.rho - u.rho_new) / u.advec_vol + enb * de * u.rho;
-  } else {
-    rb = AS(CLV_DENSITY1, c, t);  // ghost cells: the kernel leaves them unchanged
-    eb = AS(CLV_ENERGY1, c, t);
+    const double de = (v.e - e_new) / post_mass;
+    rb = v.rnb * v.pre / advec_vol + v.enb * de * v.pre;
+    eb = v.enb * (v.rho * v.pre) / post_mass;
+    pb = v.rnb * (v.rho - rho_new) / advec_vol + v.enb * de * v.rho;
+    M = v.rnb / advec_vol - v.enb * e_new / post_mass;
+    E = v.enb / post_mass;
+    V = v.rnb * rho_new / advec_vol;
   }
-  for (int f = max(2, c - 1); f <= min(smax + 3, c + 2); ++f) {
-    const FaceRoles r = face_roles(PS(vf_id, f, t), f, smax);
-    if (r.uw == c) {
-      rb += AS(CLV_WORK2, f, t);
-      eb += AS(CLV_WORK6, f, t);
-    }
-    if (r.dn == c) {
-      rb += AS(CLV_WORK4, f, t);
-      eb += AS(CLV_WORK7, f, t);
-      pb += AS(CLV_WORK1, f, t);
-    }
-    if (r.dw == c) {
-      rb += AS(CLV_WORK5, f, t);
-      eb += PS(CLV_WORK4, f, t);
-    }
+  return CellAdj{v.rho, v.e, v.pre, rb, eb, pb, M, E, V};
+}
+__device__ __forceinline__ CellAdj cell_adj(const Dev &p, const Dev &a, const Sweep &sw, int smax, int vf_id, int c, int t) {
+  return cell_adj_of(cell_in(p, a, sw, smax, vf_id, c, t), c >= 2 && c <= smax + 1);
+}
+
+// Face f's adjoint: its total mass / energy flux adjoints (the external mass-flux
+// adjoint mfb0, plus the update terms of its right cell f and left cell f-1 where those
+// are interior), then the face differentiated once (face_bar): g[1] = donor pre share,
+// g[2..4] = rho of upwind / donor / downwind, g[5..7] = e of them.  Returns the face's
+// whole vf share.
+__device__ __forceinline__ double face_adj(double vfv, double mfb0, bool has_r, const CellAdj &cr, bool has_l, double Ml, double El, double Vl,
+                                           double pre_dn, double r_uw, double r_dn, double r_dw, double e_uw, double e_dn, double e_dw,
+                                           double vd_face, double vd_dif, double g[8]) {
+  double mfb = mfb0, efb = 0.0, vfb = 0.0;
+  if (has_r) {
+    mfb += cr.M;
+    efb += cr.E;
+    vfb -= cr.V;
   }
-  AS(CLV_DENSITY1, c, t) = rb;
-  AS(CLV_ENERGY1, c, t) = eb;
-  PS(CLV_WORK5, c, t) = pb;
+  if (has_l) {
+    mfb -= Ml;
+    efb -= El;
+    vfb += Vl;
+  }
+  face_bar(vfv, pre_dn, r_uw, r_dn, r_dw, e_uw, e_dn, e_dw, vd_face, vd_dif, mfb, efb, g);
+  return vfb + g[0];
+}
+// a face's shares into cell w when w is the face's upwind / donor / downwind cell
+__device__ __forceinline__ void take_shares(CellAdj &w, bool uw, bool dn, bool dw, const double g[8]) {
+  if (uw) {
+    w.rb += g[2];
+    w.eb += g[5];
+  }
+  if (dn) {
+    w.rb += g[3];
+    w.eb += g[6];
+    w.pb += g[1];
+  }
+  if (dw) {
+    w.rb += g[4];
+    w.eb += g[7];
+  }
+}
+
+// C1 (x sweep), faces and cells of lines [2, ny+1] in one pass.  One wave per 58 cells of
+// a line (blocks of 64 x 4, one line per wave): lane l sits at position s = s0 - 3 + l.
+// Each lane takes its cell's own update (cell_adj), differentiates its face s (the
+// cells the face reads come from lanes l-2..l+1), and gathers its cell's shares from the
+// faces of lanes l-1..l+2, in face order (upwind, donor, downwind within a face): the
+// order of a gather over faces c-1..c+2.  Lanes 3..60 own their cell and face.  New
+// rho1 / e1 adjoints -> rn / en over the whole storage extent (lines the sweep leaves
+// alone and column nx+4 pass through), pre adjoint -> p.work5 (cells [0, nx+3]), each
+// face's vf share -> a.work3 (faces [2, nx+3]).  The mass-flux adjoint it consumes is
+// cleared by C3.
+constexpr int kCellXOwn = 58;
+constexpr int kCellYSeg = 16;  // cells per thread of adj_cell_y_k
+__global__ __launch_bounds__(256) void adj_cell_x_k(Dev p, Dev a, double *rn, double *en) {
+  const Sweep sw{1};
+  const int smax = p.nx, tmax = p.ny;
+  const int lane = threadIdx.x, t = blockIdx.y * blockDim.y + threadIdx.y;
+  const int s0 = blockIdx.x * kCellXOwn, s = s0 - 3 + lane;
+  if (t > tmax + 4) return;  // whole waves
+  const bool owned = lane >= 3 && lane < 3 + kCellXOwn && s <= smax + 4;
+  const int64_t o = sw.at(a, s, t);
+  if (owned && (t < 2 || t > tmax + 1 || s == smax + 4)) {
+    rn[o] = AS(CLV_DENSITY1, s, t);
+    en[o] = AS(CLV_ENERGY1, s, t);
+  }
+  if (t < 2 || t > tmax + 1) return;  // whole waves
+  const CellAdj u = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_X, s, t);
+  const bool fval = s >= 2 && s <= smax + 3 && lane >= 2 && lane <= 62;
+  const double vfv = fval ? PS(CLV_VOL_FLUX_X, s, t) : 0.0;
+  const FaceRoles r = face_roles(vfv, s, smax);
+  const int lu = lane + r.uw - s, ld = lane + r.dn - s, lw = lane + r.dw - s;
+  const double pre_dn = __shfl(u.pre, ld), r_uw = __shfl(u.rho, lu), r_dn = __shfl(u.rho, ld), r_dw = __shfl(u.rho, lw);
+  const double e_uw = __shfl(u.e, lu), e_dn = __shfl(u.e, ld), e_dw = __shfl(u.e, lw);
+  const double Ml = __shfl(u.M, lane - 1), El = __shfl(u.E, lane - 1), Vl = __shfl(u.V, lane - 1);
+  double g[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (fval) {
+    const double vfs = face_adj(vfv, AS(CLV_MASS_FLUX_X, s, t), s <= smax + 1, u, s - 1 >= 2 && s - 1 <= smax + 1, Ml, El, Vl, pre_dn, r_uw, r_dn,
+                                r_dw, e_uw, e_dn, e_dw, PM(CLV_VERTEXDX, s), PM(CLV_VERTEXDX, r.dif), g);
+    if (owned) AS(CLV_WORK3, s, t) = vfs;
+  }
+  // role cells as offsets + 2 from the face (uw in [-2, 1], dn and dw in [-1, 0]); bit 6: a face
+  const int roles = fval ? (64 | (r.uw - s + 2) | ((r.dn - s + 2) << 2) | ((r.dw - s + 2) << 4)) : 0;
+  CellAdj w = u;
+#pragma unroll
+  for (int j = -1; j <= 2; ++j) {
+    const int src = lane + j, rr = __shfl(roles, src);
+    double h[8];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) h[q] = __shfl(g[q], src);
+    const int want = 2 - j;  // this cell seen from face s + j
+    if (rr & 64) take_shares(w, (rr & 3) == want, ((rr >> 2) & 3) == want, ((rr >> 4) & 3) == want, h);
+  }
+  if (owned && s <= smax + 3) {
+    rn[o] = w.rb;
+    en[o] = w.eb;
+    PS(CLV_WORK5, s, t) = w.pb;
+  }
+}
+
+// C1 (y sweep), the same pass with lanes along x: each thread walks the cells of one
+// segment [c0, c0 + SEG - 1] of its line (positions along y) in order, holding the
+// adjoints of the four cells f-2..f+1 that face f reads in registers; cell c is complete
+// after face c+2.
+// Faces from c0-1 on are differentiated (the first few again by the segment before), each
+// face's vf share written by the segment that owns position f.  Outputs as adj_cell_x_k.
+template <int SEG>
+__global__ __launch_bounds__(256) void adj_cell_y_k(Dev p, Dev a, double *rn, double *en) {
+  const Sweep sw{2};
+  const int smax = p.ny, tmax = p.nx;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int c0 = k * SEG;
+  if (t > tmax + 4 || c0 > smax + 4) return;
+  const int c1 = min(c0 + SEG - 1, smax + 4);
+  if (t < 2 || t > tmax + 1 || c1 == smax + 4) {
+    for (int c = (t < 2 || t > tmax + 1) ? c0 : smax + 4; c <= c1; ++c) {
+      rn[sw.at(a, c, t)] = AS(CLV_DENSITY1, c, t);
+      en[sw.at(a, c, t)] = AS(CLV_ENERGY1, c, t);
+    }
+    if (t < 2 || t > tmax + 1) return;
+  }
+  const int cl = min(c1, smax + 3);
+  if (c0 > cl) return;
+  auto put = [&](const CellAdj &w, int c) {
+    rn[sw.at(a, c, t)] = w.rb;
+    en[sw.at(a, c, t)] = w.eb;
+    PS(CLV_WORK5, c, t) = w.pb;
+  };
+  const int fs = max(2, c0 - 1), fe = min(smax + 3, cl + 2);
+  CellAdj w0 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs - 2, t), w1 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs - 1, t);
+  CellAdj w2 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs, t), w3 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs + 1, t);
+  for (int f = fs; f <= fe; ++f) {
+    // everything this step reads, before its stores (which the compiler cannot order
+    // the loads past): the next cell's inputs first
+    const CellIn nin = cell_in(p, a, sw, smax, CLV_VOL_FLUX_Y, f + 2, t);
+    const double vfv = PS(CLV_VOL_FLUX_Y, f, t);
+    const FaceRoles r = face_roles(vfv, f, smax);
+    // role cells as window slots: slot q holds cell f - 2 + q.  The primal values are
+    // read again (cache hits): selecting them out of the window would put it in memory.
+    const int qu = r.uw - f + 2, qd = r.dn - f + 2, qw = r.dw - f + 2;
+    double g[8];
+    const double vfs =
+        face_adj(vfv, AS(CLV_MASS_FLUX_Y, f, t), f <= smax + 1, w2, f - 1 >= 2 && f - 1 <= smax + 1, w1.M, w1.E, w1.V, PS(CLV_WORK1, r.dn, t),
+                 PS(CLV_DENSITY1, r.uw, t), PS(CLV_DENSITY1, r.dn, t), PS(CLV_DENSITY1, r.dw, t), PS(CLV_ENERGY1, r.uw, t),
+                 PS(CLV_ENERGY1, r.dn, t), PS(CLV_ENERGY1, r.dw, t), PM(CLV_VERTEXDY, f), PM(CLV_VERTEXDY, r.dif), g);
+    if (f >= c0 && f <= c1) AS(CLV_WORK3, f, t) = vfs;
+    take_shares(w0, qu == 0, false, false, g);
+    take_shares(w1, qu == 1, qd == 1, qw == 1, g);
+    take_shares(w2, qu == 2, qd == 2, qw == 2, g);
+    take_shares(w3, qu == 3, false, false, g);
+    if (f - 2 >= c0) put(w0, f - 2);
+    w0 = w1;
+    w1 = w2;
+    w2 = w3;
+    w3 = cell_adj_of(nin, f + 2 >= 2 && f + 2 <= smax + 1);
+  }
+  // the window now holds cells fe-1..fe+2; those up to cl are complete
+  if (fe - 1 >= c0 && fe - 1 <= cl) put(w0, fe - 1);
+  if (fe >= c0 && fe <= cl) put(w1, fe);
+  if (fe + 1 >= c0 && fe + 1 <= cl) put(w2, fe + 1);
 }
 
 // C3: pre_vol of cells [0, smax+3] x lines [2, tmax+1] (its adjoint in p.work5).  vf
@@ -859,9 +975,13 @@ __global__ void adj_cell_vol_k(Dev p, Dev a, int sweep, int dir) {
   if (s > smax + 4 || t > tmax + 2) return;
   auto PB = [&](int ss, int tt) { return ss >= 0 && ss <= smax + 3 && tt >= 2 && tt <= tmax + 1 ? PS(CLV_WORK5, ss, tt) : 0.0; };
   const int vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y, vo_id = dir == 1 ? CLV_VOL_FLUX_Y : CLV_VOL_FLUX_X;
+  const int mf_id = dir == 1 ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y;
   if (t >= 2 && t <= tmax + 1) {
     double g = PB(s - 1, t) - PB(s, t);
-    if (s >= 2 && s <= smax + 3) g += AS(CLV_WORK3, s, t);
+    if (s >= 2 && s <= smax + 3) {
+      g += AS(CLV_WORK3, s, t);
+      AS(mf_id, s, t) = 0.0;  // consumed by C1
+    }
     AS(vf_id, s, t) += g;
   }
   if (sweep == 1 && s <= smax + 3) AS(vo_id, s, t) += PB(s, t - 1) - PB(s, t);
@@ -1152,8 +1272,23 @@ clv_status adj_flux_calc(clv_chunk *p, clv_chunk *a, double dt) {
 clv_status adj_advec_cell(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
   const int smax = dir == 1 ? p->d.nx : p->d.ny, tmax = dir == 1 ? p->d.ny : p->d.nx;
   hipLaunchKernelGGL(adj_cell_values_k, sgrid(dir, smax + 4, tmax), kBlk, 0, a->stream, p->d, sweep, dir);
-  hipLaunchKernelGGL(adj_cell_face_k, sgrid(dir, smax + 2, tmax), kBlk, 0, a->stream, p->d, a->d, sweep, dir);
-  hipLaunchKernelGGL(adj_cell_cell_k, sgrid(dir, smax + 4, tmax), kBlk, 0, a->stream, p->d, a->d, sweep, dir);
+  // the new rho1 / e1 adjoints go to a.work1 / a.work2 (the pass reads the old ones
+  // around each cell), then replace the old: by buffer exchange where the chunk owns its
+  // arrays, else copied back
+  double *rn = a->d.f[CLV_WORK1], *en = a->d.f[CLV_WORK2];
+  if (dir == 1)
+    hipLaunchKernelGGL(adj_cell_x_k, dim3((smax + 5 + kCellXOwn - 1) / kCellXOwn, (tmax + 5 + 3) / 4), kBlk, 0, a->stream, p->d, a->d, rn, en);
+  else
+    hipLaunchKernelGGL(adj_cell_y_k<kCellYSeg>, dim3((tmax + 5 + 63) / 64, ((smax + 5 + kCellYSeg - 1) / kCellYSeg + 3) / 4), kBlk, 0, a->stream,
+                       p->d, a->d, rn, en);
+  if (a->owns) {
+    std::swap(a->d.f[CLV_DENSITY1], a->d.f[CLV_WORK1]);
+    std::swap(a->d.f[CLV_ENERGY1], a->d.f[CLV_WORK2]);
+  } else {
+    const size_t n = (size_t)a->d.pitch * (size_t)(a->d.ny + 5);
+    hipLaunchKernelGGL(adj_copy_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, a->stream, a->d.f[CLV_DENSITY1], rn, n);
+    hipLaunchKernelGGL(adj_copy_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, a->stream, a->d.f[CLV_ENERGY1], en, n);
+  }
   hipLaunchKernelGGL(adj_cell_vol_k, sgrid(dir, smax + 5, tmax + 2), kBlk, 0, a->stream, p->d, a->d, sweep, dir);
   return check("adj advec_cell");
 }
