@@ -1109,6 +1109,89 @@ __global__ void adj_mom_node_k(Dev p, Dev a, int dir) {
   if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
 }
 
+// MX (x sweep), one velocity: faces and nodes of lines [2, ny+2] in one pass, one wave
+// per 58 nodes of a line (blocks of 64 x 4, one line per wave), lane l at node
+// s = s0 - 3 + l.  Each lane loads its node, differentiates its face s (momflux_bar;
+// the nodes the face reads come from lanes l-1..l+2, the update adjoints of nodes s and
+// s+1 from lanes l and l+1), then takes its node's own update (MN's) and gathers the
+// velocity shares of faces s-2..s+1 (lanes l-2..l+1) in face order.  Lanes 3..60 own
+// their node and face.  Outputs as MF + MN, except the new velocity adjoint -> vn over
+// the whole storage extent (lines the sweep leaves alone pass through).
+constexpr int kMomXOwn = 58;
+template <int VEL, bool LAST>
+__global__ __launch_bounds__(256) void adj_mom_x_k(Dev p, Dev a, double *vn) {
+  const Sweep sw{1};
+  const int smax = p.nx, tmax = p.ny;
+  const int lane = threadIdx.x, t = blockIdx.y * blockDim.y + threadIdx.y;
+  const int s = blockIdx.x * kMomXOwn - 3 + lane;
+  if (t > tmax + 4) return;  // whole waves
+  const bool owned = lane >= 3 && lane < 3 + kMomXOwn && s <= smax + 4;
+  const int64_t o = sw.at(a, s, t);
+  if (t < 2 || t > tmax + 2) {  // whole waves
+    if (owned) vn[o] = AS(VEL, s, t);
+    return;
+  }
+  const bool node = s >= 0 && s <= smax + 4, upd = s >= 2 && s <= smax + 2;
+  const double vb = node ? AS(VEL, s, t) : 0.0, vel = node ? PS(VEL, s, t) : 0.0;
+  const double mpost = upd ? PS(CLV_WORK2, s, t) : 1.0;
+  const double mpre = s >= 1 && s <= smax + 3 ? PS(CLV_WORK3, s, t) : 0.0;
+  const double q = upd ? vb / mpost : 0.0;
+  // face s
+  const bool fval = s >= 1 && s <= smax + 2 && lane >= 1 && lane <= 61;
+  const double nf = fval ? PS(CLV_WORK1, s, t) : 0.0;
+  const FaceRoles r = mom_roles(nf, s);
+  const int lu = lane + r.uw - s, ld = lane + r.dn - s, lw = lane + r.dw - s;
+  const double v_uw = __shfl(vel, lu), v_dn = __shfl(vel, ld), v_dw = __shfl(vel, lw), mpre_dn = __shfl(mpre, ld);
+  const double q_r = __shfl(q, lane + 1);
+  double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double flux = 0.0, dsum = 0.0;
+  if (fval) {
+    double fb = 0.0;
+    if (s >= 2) fb -= q;
+    if (s + 1 <= smax + 2) fb += q_r;
+    flux = momflux_bar(nf, mpre_dn, PM(CLV_CELLDX, s), PM(CLV_CELLDX, r.dif), v_uw, v_dn, v_dw, fb, g);
+    if (LAST) dsum = PS(CLV_WORK5, s, t) + g[1];  // both velocities' donor shares of face s
+    if (owned) {
+      if (LAST) {
+        AS(CLV_WORK5, s, t) += g[0];
+      } else {
+        AS(CLV_WORK5, s, t) = g[0];
+        PS(CLV_WORK5, s, t) = g[1];
+      }
+    }
+  }
+  const double flux_l = __shfl(flux, lane - 1);
+  double gv = vb, mpreb = 0.0;
+  if (upd && owned) {
+    const double vnew = (vel * mpre + flux_l - flux) / mpost;
+    gv = vb * mpre / mpost;
+    mpreb = (LAST ? AS(CLV_WORK3, s, t) : 0.0) + vb * vel / mpost;
+    AS(CLV_WORK4, s, t) = (LAST ? AS(CLV_WORK4, s, t) : 0.0) - vb * vnew / mpost;
+    if (!LAST) AS(CLV_WORK3, s, t) = mpreb;
+  }
+  // role nodes as offsets from the face: uw + 1 in {0, 3}, dn and dw in {0, 1}; bit 6: a face
+  const int roles = fval ? (64 | (r.uw - s + 1) | ((r.dn - s) << 2) | ((r.dw - s) << 4)) : 0;
+  double gm = 0.0;
+#pragma unroll
+  for (int j = -2; j <= 1; ++j) {  // face s + j
+    const int src = lane + j, rr = __shfl(roles, src);
+    const double g2 = __shfl(g[2], src), g3 = __shfl(g[3], src), g4 = __shfl(g[4], src);
+    const double ds = LAST ? __shfl(dsum, src) : 0.0;
+    if (rr & 64) {
+      if ((rr & 3) == 1 - j) gv += g2;
+      if (((rr >> 2) & 3) == -j) {
+        gv += g3;
+        if (LAST) gm += ds;
+      }
+      if (((rr >> 4) & 3) == -j) gv += g4;
+    }
+  }
+  if (owned) {
+    vn[o] = gv;
+    if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
+  }
+}
+
 // M5: node masses.  node_mass_pre(s) = post(s) - nf(s-1) + nf(s) (nodes s in [1, smax+3])
 // and post = 0.25 sum rho1 * post_vol over the node's 4 cells.  Per cell (i, j) in
 // [0, nx+3] x [0, ny+3]: rho1 and post_vol adjoints from the nodes (i..i+1, j..j+1).
@@ -1298,10 +1381,30 @@ clv_status adj_advec_mom(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
   const int mom_sweep = dir + 2 * (sweep - 1);
   const dim3 gn = sgrid(dir, smax + 5, tmax + 1), gf = sgrid(dir, smax + 2, tmax + 1);
   hipLaunchKernelGGL(adj_mom_values_k, sgrid(dir, smax + 4, tmax + 1), kBlk, 0, a->stream, p->d, mom_sweep, dir);
-  hipLaunchKernelGGL((adj_mom_face_k<CLV_XVEL1, true>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
-  hipLaunchKernelGGL((adj_mom_node_k<CLV_XVEL1, false>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
-  hipLaunchKernelGGL((adj_mom_face_k<CLV_YVEL1, false>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
-  hipLaunchKernelGGL((adj_mom_node_k<CLV_YVEL1, true>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
+  if (dir == 1) {
+    // each velocity's new adjoint goes to a.work1, then replaces the old (buffer
+    // exchange where the chunk owns its arrays, else copied back)
+    const dim3 gx((smax + 5 + kMomXOwn - 1) / kMomXOwn, (tmax + 5 + 3) / 4);
+    for (int w = 0; w < 2; ++w) {
+      const int vid = w == 0 ? CLV_XVEL1 : CLV_YVEL1;
+      double *vn = a->d.f[CLV_WORK1];
+      if (w == 0)
+        hipLaunchKernelGGL((adj_mom_x_k<CLV_XVEL1, false>), gx, kBlk, 0, a->stream, p->d, a->d, vn);
+      else
+        hipLaunchKernelGGL((adj_mom_x_k<CLV_YVEL1, true>), gx, kBlk, 0, a->stream, p->d, a->d, vn);
+      if (a->owns) {
+        std::swap(a->d.f[vid], a->d.f[CLV_WORK1]);
+      } else {
+        const size_t n = (size_t)a->d.pitch * (size_t)(a->d.ny + 5);
+        hipLaunchKernelGGL(adj_copy_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, a->stream, a->d.f[vid], vn, n);
+      }
+    }
+  } else {
+    hipLaunchKernelGGL((adj_mom_face_k<CLV_XVEL1, true>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
+    hipLaunchKernelGGL((adj_mom_node_k<CLV_XVEL1, false>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
+    hipLaunchKernelGGL((adj_mom_face_k<CLV_YVEL1, false>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
+    hipLaunchKernelGGL((adj_mom_node_k<CLV_YVEL1, true>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
+  }
   hipLaunchKernelGGL(adj_mom_mass_k, grid2(p->d.nx + 4, p->d.ny + 4), kBlk, 0, a->stream, p->d, a->d, mom_sweep, dir);
   if (mom_sweep <= 2) hipLaunchKernelGGL(adj_mom_postvol_k, grid2(p->d.nx + 5, p->d.ny + 5), kBlk, 0, a->stream, a->d, mom_sweep);
   hipLaunchKernelGGL(adj_mom_nodeflux_k, grid2(p->d.nx + 5, p->d.ny + 5), kBlk, 0, a->stream, a->d, dir);
