@@ -1036,87 +1036,23 @@ __device__ __forceinline__ FaceRoles mom_roles(double nf, int f) {
   if (nf < 0.0) return FaceRoles{f + 2, f + 1, f, f + 1};
   return FaceRoles{f - 1, f, f + 1, f - 1};
 }
-// MF, node faces f in [1, smax+2] of lines [2, tmax+2], one velocity: the face's
-// momentum-flux adjoint from the updates of nodes f (its right flux, -1 / mpost) and
-// f+1 (left, +1 / mpost), then the face differentiated once (momflux_bar): node-flux share ->
-// a.work5, donor node_mass_pre share -> p.work5 (first velocity writes, second adds);
-// velocity shares of upwind / donor / downwind -> a.work6 / a.work7 / p.work4; the
-// momentum flux itself -> a.work1 (read by the node update's adjoint)
-template <int VEL, bool FIRST>
-__global__ void adj_mom_face_k(Dev p, Dev a, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  int f, t;
-  sweep_pos(dir, 1, 2, f, t);
-  if (f > smax + 2 || t > tmax + 2) return;
-  double fb = 0.0;
-  if (f >= 2) fb -= AS(VEL, f, t) / PS(CLV_WORK2, f, t);
-  if (f + 1 <= smax + 2) fb += AS(VEL, f + 1, t) / PS(CLV_WORK2, f + 1, t);
-  const double nf = PS(CLV_WORK1, f, t);
-  const FaceRoles r = mom_roles(nf, f);
-  const int cd = dir == 1 ? CLV_CELLDX : CLV_CELLDY;
-  double g[5];
-  AS(CLV_WORK1, f, t) =
-      momflux_bar(nf, PS(CLV_WORK3, r.dn, t), PM(cd, f), PM(cd, r.dif), PS(VEL, r.uw, t), PS(VEL, r.dn, t), PS(VEL, r.dw, t), fb, g);
-  if (FIRST) {
-    AS(CLV_WORK5, f, t) = g[0];
-    PS(CLV_WORK5, f, t) = g[1];
-  } else {
-    AS(CLV_WORK5, f, t) += g[0];
-    PS(CLV_WORK5, f, t) += g[1];
-  }
-  AS(CLV_WORK6, f, t) = g[2];
-  AS(CLV_WORK7, f, t) = g[3];
-  PS(CLV_WORK4, f, t) = g[4];
-}
-
-// MN, nodes s in [0, smax+4] of lines [2, tmax+2], one velocity.  Updated nodes
-// (s in [2, smax+2]) differentiate vel' = (vel mpre + mf(s-1) - mf(s)) / mpost
-// (src/serial/advec_mom.cpp:140-146): own share vb mpre / mpost, and the node_mass_pre /
-// node_mass_post adjoints vb vel / mpost and -vb vel' / mpost, started by the first
-// velocity in a.work3 / a.work4 and completed by the second; other nodes pass their
-// adjoint through.  Then the shares of faces [s-2, s+1] that read the node are added and
-// the new adjoint installed in place.  After the second velocity, the node_mass_pre
-// adjoint (a.work3) gains the donor shares of faces s-1 and s.
-template <int VEL, bool LAST>
-__global__ void adj_mom_node_k(Dev p, Dev a, int dir) {
-  const Sweep sw{dir};
-  const int smax = dir == 1 ? p.nx : p.ny, tmax = dir == 1 ? p.ny : p.nx;
-  int s, t;
-  sweep_pos(dir, 0, 2, s, t);
-  if (s > smax + 4 || t > tmax + 2) return;
-  const bool updated = s >= 2 && s <= smax + 2;
-  double g = AS(VEL, s, t), mpreb = 0.0;
-  if (updated) {
-    const double vb = g, vel = PS(VEL, s, t), mpost = PS(CLV_WORK2, s, t), mpre = PS(CLV_WORK3, s, t);
-    const double vnew = (vel * mpre + AS(CLV_WORK1, s - 1, t) - AS(CLV_WORK1, s, t)) / mpost;
-    g = vb * mpre / mpost;
-    mpreb = (LAST ? AS(CLV_WORK3, s, t) : 0.0) + vb * vel / mpost;
-    AS(CLV_WORK4, s, t) = (LAST ? AS(CLV_WORK4, s, t) : 0.0) - vb * vnew / mpost;
-    if (!LAST) AS(CLV_WORK3, s, t) = mpreb;
-  }
-  double gm = 0.0;
-  for (int f = max(1, s - 2); f <= min(smax + 2, s + 1); ++f) {
-    const FaceRoles r = mom_roles(PS(CLV_WORK1, f, t), f);
-    if (r.uw == s) g += AS(CLV_WORK6, f, t);
-    if (r.dn == s) {
-      g += AS(CLV_WORK7, f, t);
-      if (LAST) gm += PS(CLV_WORK5, f, t);
-    }
-    if (r.dw == s) g += PS(CLV_WORK4, f, t);
-  }
-  AS(VEL, s, t) = g;
-  if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
-}
-
-// MX (x sweep), one velocity: faces and nodes of lines [2, ny+2] in one pass, one wave
-// per 58 nodes of a line (blocks of 64 x 4, one line per wave), lane l at node
-// s = s0 - 3 + l.  Each lane loads its node, differentiates its face s (momflux_bar;
-// the nodes the face reads come from lanes l-1..l+2, the update adjoints of nodes s and
-// s+1 from lanes l and l+1), then takes its node's own update (MN's) and gathers the
-// velocity shares of faces s-2..s+1 (lanes l-2..l+1) in face order.  Lanes 3..60 own
-// their node and face.  Outputs as MF + MN, except the new velocity adjoint -> vn over
-// the whole storage extent (lines the sweep leaves alone pass through).
+// MX (x sweep), one velocity of advec_mom (src/serial/advec_mom.cpp:114-146), faces and
+// nodes of lines [2, ny+2] in one pass.  One wave per 58 nodes of a line (blocks of
+// 64 x 4, one line per wave), lane l at node s = s0 - 3 + l; lanes 3..60 own their
+// node and face.
+// - Face s (momflux_bar): its momentum-flux adjoint comes from the updates of nodes s
+//   (its right flux, -1 / mpost) and s+1 (its left flux, +1 / mpost); the nodes it reads
+//   come from lanes l-1..l+2.  Node-flux share -> a.work5 (the first velocity writes,
+//   the second adds); the first velocity's donor node_mass_pre share -> p.work5.
+// - Node s: updated nodes (s in [2, nx+2]) differentiate
+//   vel' = (vel mpre + mf(s-1) - mf(s)) / mpost: own share vb mpre / mpost, and the
+//   node_mass_pre / node_mass_post adjoints vb vel / mpost and -vb vel' / mpost, started by
+//   the first velocity in a.work3 / a.work4 and completed by the second; other nodes
+//   pass their adjoint through.  Then the velocity shares of faces s-2..s+1 (lanes
+//   l-2..l+1), in face order.  After the second velocity node_mass_pre's adjoint
+//   (a.work3) gains the donor shares of faces s-1 and s, both velocities' summed per face.
+// The new velocity adjoint -> vn over the whole storage extent (lines the sweep leaves
+// alone pass through), for the caller to swap in.
 constexpr int kMomXOwn = 58;
 template <int VEL, bool LAST>
 __global__ __launch_bounds__(256) void adj_mom_x_k(Dev p, Dev a, double *vn) {
@@ -1189,6 +1125,143 @@ __global__ __launch_bounds__(256) void adj_mom_x_k(Dev p, Dev a, double *vn) {
   if (owned) {
     vn[o] = gv;
     if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
+  }
+}
+
+// MY (y sweep), one velocity, the same pass with lanes along x: each thread walks the
+// nodes of one segment [c0, c0 + SEG - 1] of its line in order.  It keeps the four
+// nodes s..s+3 that face s+1 reads and the results of faces s-2..s+1 in registers and
+// loads one node (and one face's node flux) ahead, so each step reads one new node;
+// face s+1 is differentiated at step s and node s then completed.  Outputs as adj_mom_x_k.
+struct MomNode {
+  double vel, vb, mpost, mpre, q, w3, w4;
+};
+// node n's values (its mass adjoints, LAST, only where the segment owns it)
+template <int VEL, bool LAST>
+__device__ __forceinline__ MomNode mom_node_in(const Dev &p, const Dev &a, const Sweep &sw, int smax, int n, int t, bool own) {
+  double vel = 0.0, vb = 0.0, mpost = 1.0, mpre = 0.0, q = 0.0, w3 = 0.0, w4 = 0.0;
+  if (n >= 0 && n <= smax + 4) {
+    vel = PS(VEL, n, t);
+    vb = AS(VEL, n, t);
+  }
+  if (n >= 1 && n <= smax + 3) mpre = PS(CLV_WORK3, n, t);
+  if (n >= 2 && n <= smax + 2) {
+    mpost = PS(CLV_WORK2, n, t);
+    q = vb / mpost;
+    if (LAST && own) {
+      w3 = AS(CLV_WORK3, n, t);
+      w4 = AS(CLV_WORK4, n, t);
+    }
+  }
+  return MomNode{vel, vb, mpost, mpre, q, w3, w4};
+}
+struct MomFace {
+  double flux, g2, g3, g4, ds;
+  int roles;  // as adj_mom_x_k's
+};
+// face f from its node flux nf, the nodes f-1..f+2 (n0..n3) and, LAST, the face's
+// first-velocity donor share pw5 and node-flux share aw5; writes the face's shares
+// where the segment owns it
+template <int VEL, bool LAST>
+__device__ __forceinline__ MomFace mom_face_y(const Dev &p, const Dev &a, const Sweep &sw, int smax, int f, int t, bool own, double nf,
+                                              const MomNode &n0, const MomNode &n1, const MomNode &n2, const MomNode &n3, double pw5,
+                                              double aw5) {
+  double flux = 0.0, g2 = 0.0, g3 = 0.0, g4 = 0.0, ds = 0.0;
+  int roles = 0;
+  if (f >= 1 && f <= smax + 2) {
+    const FaceRoles r = mom_roles(nf, f);
+    const bool neg = nf < 0.0;  // uw = f+2, dn = f+1, dw = f; else uw = f-1, dn = f, dw = f+1
+    double fb = 0.0;
+    if (f >= 2) fb -= n1.q;
+    if (f + 1 <= smax + 2) fb += n2.q;
+    double g[5];
+    flux = momflux_bar(nf, neg ? n2.mpre : n1.mpre, PM(CLV_CELLDY, f), PM(CLV_CELLDY, r.dif), neg ? n3.vel : n0.vel, neg ? n2.vel : n1.vel,
+                       neg ? n1.vel : n2.vel, fb, g);
+    g2 = g[2];
+    g3 = g[3];
+    g4 = g[4];
+    if (LAST) ds = pw5 + g[1];
+    if (own) {
+      if (LAST) {
+        AS(CLV_WORK5, f, t) = aw5 + g[0];
+      } else {
+        AS(CLV_WORK5, f, t) = g[0];
+        PS(CLV_WORK5, f, t) = g[1];
+      }
+    }
+    roles = 64 | (r.uw - f + 1) | ((r.dn - f) << 2) | ((r.dw - f) << 4);
+  }
+  return MomFace{flux, g2, g3, g4, ds, roles};
+}
+// node s's share from face s + j
+template <bool LAST>
+__device__ __forceinline__ void mom_take(double &gv, double &gm, const MomFace &m, int j) {
+  if (m.roles & 64) {
+    if ((m.roles & 3) == 1 - j) gv += m.g2;
+    if (((m.roles >> 2) & 3) == -j) {
+      gv += m.g3;
+      if (LAST) gm += m.ds;
+    }
+    if (((m.roles >> 4) & 3) == -j) gv += m.g4;
+  }
+}
+constexpr int kMomYSeg = 16;
+template <int VEL, bool LAST>
+__global__ __launch_bounds__(256) void adj_mom_y_k(Dev p, Dev a, double *vn) {
+  const Sweep sw{2};
+  const int smax = p.ny, tmax = p.nx;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  const int c0 = k * kMomYSeg;
+  if (t > tmax + 4 || c0 > smax + 4) return;
+  const int c1 = min(c0 + kMomYSeg - 1, smax + 4);
+  if (t < 2 || t > tmax + 2) {
+    for (int s = c0; s <= c1; ++s) vn[sw.at(a, s, t)] = AS(VEL, s, t);
+    return;
+  }
+  auto nf_at = [&](int f) { return f >= 1 && f <= smax + 2 ? PS(CLV_WORK1, f, t) : 0.0; };
+  auto pw5_at = [&](int f) { return LAST && f >= 1 && f <= smax + 2 ? PS(CLV_WORK5, f, t) : 0.0; };
+  auto aw5_at = [&](int f) { return LAST && f >= c0 && f <= c1 && f >= 1 && f <= smax + 2 ? AS(CLV_WORK5, f, t) : 0.0; };
+  // nodes c0-3..c0+3 for the first faces c0-2..c0 and the window
+  auto node = [&](int n) { return mom_node_in<VEL, LAST>(p, a, sw, smax, n, t, n >= c0 && n <= c1); };
+  const MomNode m0 = node(c0 - 3), m1 = node(c0 - 2), m2 = node(c0 - 1);
+  MomNode n0 = node(c0), n1 = node(c0 + 1), n2 = node(c0 + 2), n3 = node(c0 + 3);
+  MomFace F0 = mom_face_y<VEL, LAST>(p, a, sw, smax, c0 - 2, t, false, nf_at(c0 - 2), m0, m1, m2, n0, pw5_at(c0 - 2), 0.0);
+  MomFace F1 = mom_face_y<VEL, LAST>(p, a, sw, smax, c0 - 1, t, false, nf_at(c0 - 1), m1, m2, n0, n1, pw5_at(c0 - 1), 0.0);
+  MomFace F2 = mom_face_y<VEL, LAST>(p, a, sw, smax, c0, t, true, nf_at(c0), m2, n0, n1, n2, pw5_at(c0), aw5_at(c0));
+  double nf = nf_at(c0 + 1), pw5 = pw5_at(c0 + 1), aw5 = aw5_at(c0 + 1);
+  for (int s = c0; s <= c1; ++s) {
+    // window: nodes n0..n3 = s..s+3; faces F0..F2 = s-2..s; the next node and face
+    // values load first, ahead of this step's stores
+    const MomNode n4 = node(s + 4);
+    const double nf_n = nf_at(s + 2), pw5_n = pw5_at(s + 2), aw5_n = aw5_at(s + 2);
+    const MomFace F3 = mom_face_y<VEL, LAST>(p, a, sw, smax, s + 1, t, s + 1 <= c1, nf, n0, n1, n2, n3, pw5, aw5);
+    // node s
+    const bool upd = s >= 2 && s <= smax + 2;
+    double gv = n0.vb, mpreb = 0.0;
+    if (upd) {
+      const double vnew = (n0.vel * n0.mpre + F1.flux - F2.flux) / n0.mpost;
+      gv = n0.vb * n0.mpre / n0.mpost;
+      mpreb = (LAST ? n0.w3 : 0.0) + n0.vb * n0.vel / n0.mpost;
+      AS(CLV_WORK4, s, t) = (LAST ? n0.w4 : 0.0) - n0.vb * vnew / n0.mpost;
+      if (!LAST) AS(CLV_WORK3, s, t) = mpreb;
+    }
+    double gm = 0.0;
+    mom_take<LAST>(gv, gm, F0, -2);
+    mom_take<LAST>(gv, gm, F1, -1);
+    mom_take<LAST>(gv, gm, F2, 0);
+    mom_take<LAST>(gv, gm, F3, 1);
+    vn[sw.at(a, s, t)] = gv;
+    if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
+    n0 = n1;
+    n1 = n2;
+    n2 = n3;
+    n3 = n4;
+    F0 = F1;
+    F1 = F2;
+    F2 = F3;
+    nf = nf_n;
+    pw5 = pw5_n;
+    aw5 = aw5_n;
   }
 }
 
@@ -1379,19 +1452,23 @@ clv_status adj_advec_cell(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
 clv_status adj_advec_mom(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
   const int smax = dir == 1 ? p->d.nx : p->d.ny, tmax = dir == 1 ? p->d.ny : p->d.nx;
   const int mom_sweep = dir + 2 * (sweep - 1);
-  const dim3 gn = sgrid(dir, smax + 5, tmax + 1), gf = sgrid(dir, smax + 2, tmax + 1);
   hipLaunchKernelGGL(adj_mom_values_k, sgrid(dir, smax + 4, tmax + 1), kBlk, 0, a->stream, p->d, mom_sweep, dir);
-  if (dir == 1) {
+  {
     // each velocity's new adjoint goes to a.work1, then replaces the old (buffer
     // exchange where the chunk owns its arrays, else copied back)
     const dim3 gx((smax + 5 + kMomXOwn - 1) / kMomXOwn, (tmax + 5 + 3) / 4);
+    const dim3 gy((tmax + 5 + 63) / 64, ((smax + 5 + kMomYSeg - 1) / kMomYSeg + 3) / 4);
     for (int w = 0; w < 2; ++w) {
       const int vid = w == 0 ? CLV_XVEL1 : CLV_YVEL1;
       double *vn = a->d.f[CLV_WORK1];
-      if (w == 0)
+      if (dir == 1 && w == 0)
         hipLaunchKernelGGL((adj_mom_x_k<CLV_XVEL1, false>), gx, kBlk, 0, a->stream, p->d, a->d, vn);
-      else
+      else if (dir == 1)
         hipLaunchKernelGGL((adj_mom_x_k<CLV_YVEL1, true>), gx, kBlk, 0, a->stream, p->d, a->d, vn);
+      else if (w == 0)
+        hipLaunchKernelGGL((adj_mom_y_k<CLV_XVEL1, false>), gy, kBlk, 0, a->stream, p->d, a->d, vn);
+      else
+        hipLaunchKernelGGL((adj_mom_y_k<CLV_YVEL1, true>), gy, kBlk, 0, a->stream, p->d, a->d, vn);
       if (a->owns) {
         std::swap(a->d.f[vid], a->d.f[CLV_WORK1]);
       } else {
@@ -1399,11 +1476,6 @@ clv_status adj_advec_mom(clv_chunk *p, clv_chunk *a, int sweep, int dir) {
         hipLaunchKernelGGL(adj_copy_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, a->stream, a->d.f[vid], vn, n);
       }
     }
-  } else {
-    hipLaunchKernelGGL((adj_mom_face_k<CLV_XVEL1, true>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
-    hipLaunchKernelGGL((adj_mom_node_k<CLV_XVEL1, false>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
-    hipLaunchKernelGGL((adj_mom_face_k<CLV_YVEL1, false>), gf, kBlk, 0, a->stream, p->d, a->d, dir);
-    hipLaunchKernelGGL((adj_mom_node_k<CLV_YVEL1, true>), gn, kBlk, 0, a->stream, p->d, a->d, dir);
   }
   hipLaunchKernelGGL(adj_mom_mass_k, grid2(p->d.nx + 4, p->d.ny + 4), kBlk, 0, a->stream, p->d, a->d, mom_sweep, dir);
   if (mom_sweep <= 2) hipLaunchKernelGGL(adj_mom_postvol_k, grid2(p->d.nx + 5, p->d.ny + 5), kBlk, 0, a->stream, a->d, mom_sweep);
