@@ -932,20 +932,26 @@ __global__ __launch_bounds__(256) void adj_cell_y_k(Dev p, Dev a, double *rn, do
   const int fs = max(2, c0 - 1), fe = min(smax + 3, cl + 2);
   CellAdj w0 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs - 2, t), w1 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs - 1, t);
   CellAdj w2 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs, t), w3 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs + 1, t);
+  // face values one face ahead, vertexdy of f-1..f+1
+  double vfv = PS(CLV_VOL_FLUX_Y, fs, t), mfb0 = AS(CLV_MASS_FLUX_Y, fs, t);
+  double vdm = PM(CLV_VERTEXDY, fs - 1), vd0 = PM(CLV_VERTEXDY, fs), vdp = PM(CLV_VERTEXDY, fs + 1);
   for (int f = fs; f <= fe; ++f) {
-    // everything this step reads, before its stores (which the compiler cannot order
-    // the loads past): the next cell's inputs first
+    // everything the next step reads, before this step's stores (which the compiler
+    // cannot order loads past)
     const CellIn nin = cell_in(p, a, sw, smax, CLV_VOL_FLUX_Y, f + 2, t);
-    const double vfv = PS(CLV_VOL_FLUX_Y, f, t);
+    const bool more = f + 1 <= smax + 3;
+    const double vf_n = more ? PS(CLV_VOL_FLUX_Y, f + 1, t) : 0.0, mf_n = more ? AS(CLV_MASS_FLUX_Y, f + 1, t) : 0.0;
+    const double vd_n = f + 2 <= smax + 4 ? PM(CLV_VERTEXDY, f + 2) : 0.0;
     const FaceRoles r = face_roles(vfv, f, smax);
-    // role cells as window slots: slot q holds cell f - 2 + q.  The primal values are
-    // read again (cache hits): selecting them out of the window would put it in memory.
+    // role cells as window slots: slot q holds cell f - 2 + q
     const int qu = r.uw - f + 2, qd = r.dn - f + 2, qw = r.dw - f + 2;
+    const double r_uw = qu == 0 ? w0.rho : qu == 1 ? w1.rho : qu == 2 ? w2.rho : w3.rho;
+    const double e_uw = qu == 0 ? w0.e : qu == 1 ? w1.e : qu == 2 ? w2.e : w3.e;
     double g[8];
-    const double vfs =
-        face_adj(vfv, AS(CLV_MASS_FLUX_Y, f, t), f <= smax + 1, w2, f - 1 >= 2 && f - 1 <= smax + 1, w1.M, w1.E, w1.V, PS(CLV_WORK1, r.dn, t),
-                 PS(CLV_DENSITY1, r.uw, t), PS(CLV_DENSITY1, r.dn, t), PS(CLV_DENSITY1, r.dw, t), PS(CLV_ENERGY1, r.uw, t),
-                 PS(CLV_ENERGY1, r.dn, t), PS(CLV_ENERGY1, r.dw, t), PM(CLV_VERTEXDY, f), PM(CLV_VERTEXDY, r.dif), g);
+    const double vd_dif = r.dif == f - 1 ? vdm : r.dif == f ? vd0 : vdp;  // dif in [f-1, f+1]
+    const double vfs = face_adj(vfv, mfb0, f <= smax + 1, w2, f - 1 >= 2 && f - 1 <= smax + 1, w1.M, w1.E, w1.V, qd == 1 ? w1.pre : w2.pre,
+                                r_uw, qd == 1 ? w1.rho : w2.rho, qw == 1 ? w1.rho : w2.rho, e_uw, qd == 1 ? w1.e : w2.e,
+                                qw == 1 ? w1.e : w2.e, vd0, vd_dif, g);
     if (f >= c0 && f <= c1) AS(CLV_WORK3, f, t) = vfs;
     take_shares(w0, qu == 0, false, false, g);
     take_shares(w1, qu == 1, qd == 1, qw == 1, g);
@@ -956,6 +962,11 @@ __global__ __launch_bounds__(256) void adj_cell_y_k(Dev p, Dev a, double *rn, do
     w1 = w2;
     w2 = w3;
     w3 = cell_adj_of(nin, f + 2 >= 2 && f + 2 <= smax + 1);
+    vfv = vf_n;
+    mfb0 = mf_n;
+    vdm = vd0;
+    vd0 = vdp;
+    vdp = vd_n;
   }
   // the window now holds cells fe-1..fe+2; those up to cl are complete
   if (fe - 1 >= c0 && fe - 1 <= cl) put(w0, fe - 1);
