@@ -13,6 +13,9 @@
 //   mom_x_k / mom_y_k     advec_mom for xvel1 AND yvel1 (the yvel pass reuses the
 //               xvel pass's node arrays, src/serial/advec_mom.cpp:81-103, so both
 //               velocities share one set)           writes u1, v1
+//   cm_x_k      cell_x + mom_x of one x sweep in one pass (the plain plan's x
+//               sweeps; the batched form keeps the two)
+//                                                   writes rho1, e1, mass_flux_x, u1, v1
 //   reset_field: pointer swaps 1 <-> 0.
 //
 // Every value a kernel writes is computed by the reference expression on the
@@ -663,6 +666,127 @@ __global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, int seg, doub
   mom_x_tile<MOM_SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, u_out, v_out);
 }
 
+// x direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
+// with mom_sweep 1, cell sweep 2 with mom_sweep 3), so the advected density and the
+// mass fluxes reach advec_mom in registers.  advec_mom reads them through the
+// reflective halo the driver updates in between (driver/advection.cpp:42-46); here
+// the ghost values are taken from the cells and faces they mirror
+// (src/serial/update_halo.cpp: density1 ghost col 1 - k <- 2 + k, nx + 2 + k <- nx + 1 - k,
+// rows alike; mass_flux_x col 1 - k <- -(3 + k), nx + 3 + k <- -(nx + 1 - k), row 1 <- 3,
+// row ny + 2 <- ny), columns by lane shuffles, rows by computing the mirrored row
+// (cell rows are independent in an x sweep).  Lanes: column x0 + L, x0 = 57 * strip - 2;
+// cells, faces and nodes owned on lanes 4..60.  Cell values hold on lanes 2..61,
+// face fluxes on 2..62, and advec_mom's stencil narrows that to 4..60.  March: cell
+// row r (r from j0 - 1), then node row r from cell rows r - 1 and r.
+template <int SWEEP, class V, class O>
+__device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
+  constexpr int MOM_SWEEP = SWEEP == 1 ? 1 : 3;
+  const int lane = threadIdx.x & 63;
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  const int nx = d.nx, ny = d.ny;
+  const int i = 57 * strip - 2 + lane;
+  const int ic = i < 0 ? 0 : (i < nx + 4 ? i : nx + 4);
+  const bool own = lane >= 4 && lane <= 60;
+  const bool own_cell = own && i <= nx + 1, own_face = own && i <= nx + 2;
+  // halo mirrors of the columns: cells, then x faces (negated)
+  const int lane_c = lane + (mirror_cell(i, nx) - i);
+  const int fm = i <= 1 ? 4 - i : (i >= nx + 3 ? 2 * nx + 4 - i : i);
+  const int lane_f = lane + (fm - i);
+  const double sgn_f = fm == i ? 1.0 : -1.0;
+  const double vd_face = M(CLV_VERTEXDX, ic);
+  const double vd_m1 = up1(vd_face), vd_e = dn1(vd_face);
+  const double vd_p1 = i + 1 <= nx + 2 ? vd_e : vd_face;
+  const double width = M(CLV_CELLDX, ic);
+  const double w_m1 = up1(width), w_p1 = dn1(width);
+  // advec_cell of cell row `row` at this lane (cell_x_tile's arithmetic); returns the
+  // new density and energy, the face mass flux and the loads advec_mom's mass reuses
+  struct CellRow {
+    double rho, e, mf, vol, vfy0, vfy1;
+  };
+  auto cell_row = [&](int row) {
+    const double rho = AT(CLV_DENSITY1, ic, row), e = AT(CLV_ENERGY1, ic, row);
+    const double vol = AT(CLV_VOLUME, ic, row), vfx = AT(CLV_VOL_FLUX_X, ic, row);
+    const double vfx_e = dn1(vfx);
+    double pre, y0 = 0.0, y1 = 0.0;
+    if (SWEEP == 1) {
+      y0 = AT(CLV_VOL_FLUX_Y, ic, row);
+      y1 = AT(CLV_VOL_FLUX_Y, ic, row + 1);
+      pre = vol + (vfx_e - vfx + y1 - y0);  // src/serial/advec_cell.cpp:47
+    } else {
+      pre = vol + vfx_e - vfx;  // :57
+    }
+    const double rho_m1 = up1(rho), rho_m2 = up2(rho), e_m1 = up1(e), e_m2 = up2(e);
+    const double rho_e = dn1(rho), e_e = dn1(e);
+    const double rho_p1 = i + 1 <= nx + 2 ? rho_e : rho, e_p1 = i + 1 <= nx + 2 ? e_e : e;  // :77
+    const double pre_m1 = up1(pre);
+    double mf, ef;
+    cell_face_flux(vfx, pre_m1, pre, rho_m2, rho_m1, rho, rho_p1, e_m2, e_m1, e, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
+    const double mf_e = dn1(mf), ef_e = dn1(ef);
+    // :120-125
+    double pre_mass_s = rho * pre;
+    double post_mass_s = pre_mass_s + mf - mf_e;
+    double post_ener_s = (e * pre_mass_s + ef - ef_e) / post_mass_s;
+    double advec_vol_s = pre + vfx - vfx_e;
+    return CellRow{post_mass_s / advec_vol_s, post_ener_s, mf, vol, y0, y1};
+  };
+  double mp_prev = 0.0, mfx_prev = 0.0;
+  for (int r = j0 - 1; r < j1; ++r) {
+    // cell row r, or the rows its ghosts mirror (density: mirror_cell; mass flux: 1 <- 3,
+    // ny + 2 <- ny)
+    const int rc = mirror_cell(r, ny), rf = r == 1 ? 3 : (r == ny + 2 ? ny : r);
+    const CellRow cr = cell_row(rc);
+    const double mf = rf == rc ? cr.mf : cell_row(rf).mf;
+    if (r >= j0 && r <= ny + 1) {
+      if (own_cell) {
+        rho_out[(int64_t)r * d.pitch + i] = cr.rho;
+        e_out[(int64_t)r * d.pitch + i] = cr.e;
+      }
+      if (own_face) AT(CLV_MASS_FLUX_X, i, r) = cr.mf;
+    }
+    // advec_mom's view of the cell row: ghost columns mirrored
+    const double rho_h = __shfl(cr.rho, lane_c), mfx = sgn_f * __shfl(mf, lane_f);
+    // density1 * post_vol of cell (i, r) (src/serial/advec_mom.cpp:47 / :65, :98-99)
+    double post;
+    if (MOM_SWEEP == 1) post = rc == r ? cr.vol + cr.vfy1 - cr.vfy0 : AT(CLV_VOLUME, ic, r) + AT(CLV_VOL_FLUX_Y, ic, r + 1) - AT(CLV_VOL_FLUX_Y, ic, r);
+    else post = rc == r ? cr.vol : AT(CLV_VOLUME, ic, r);
+    const double mp = rho_h * post;
+    if (r >= j0) {
+      // node row r: mom_x_tile's arithmetic on the values in registers
+      const double node_flux = 0.25 * (mfx_prev + mfx + dn1(mfx_prev) + dn1(mfx));
+      const double node_mass_post = 0.25 * (mp_prev + mp + up1(mp_prev) + up1(mp));
+      const double node_mass_pre = node_mass_post - up1(node_flux) + node_flux;
+      const double pre_e = dn1(node_mass_pre);
+      const double u = AT(CLV_XVEL1, ic, r), v = AT(CLV_YVEL1, ic, r);
+      const double u_w = up1(u), u_e = dn1(u), u_ee = dn2(u);
+      const double v_w = up1(v), v_e = dn1(v), v_ee = dn2(v);
+      double mu, mv;
+      if (node_flux < 0.0) {
+        mu = mom_flux_of(node_flux, pre_e, width, w_p1, u_ee, u_e, u);
+        mv = mom_flux_of(node_flux, pre_e, width, w_p1, v_ee, v_e, v);
+      } else {
+        mu = mom_flux_of(node_flux, node_mass_pre, width, w_m1, u_w, u, u_e);
+        mv = mom_flux_of(node_flux, node_mass_pre, width, w_m1, v_w, v, v_e);
+      }
+      const double mu_w = up1(mu), mv_w = up1(mv);
+      if (own_face) {  // nodes [2, nx + 2]
+        u_out[(int64_t)r * d.pitch + i] = (u * node_mass_pre + mu_w - mu) / node_mass_post;
+        v_out[(int64_t)r * d.pitch + i] = (v * node_mass_pre + mv_w - mv) / node_mass_post;
+      }
+    }
+    mp_prev = mp;
+    mfx_prev = mfx;
+  }
+}
+
+template <int SWEEP>
+__global__ __launch_bounds__(256) void cm_x_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + seg * by;
+  cm_x_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+}
+
 // y direction (mom_sweep 2 = y first, 4 = y second).  Lanes: node column
 // x0 + L, x0 = 2 + 64 * strip; the cells and faces at column i - 1 are loaded
 // directly, so every lane is independent.  Iteration k computes node_flux and
@@ -939,14 +1063,14 @@ hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, Batch
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // Rows per wave segment of each plain launch, per tile family (timestep,
-// lagrange, cell x, cell y, mom x, mom y).  Longer segments re-read fewer window
+// lagrange, cell x, cell y, mom x, mom y, cell + mom x).  Longer segments re-read fewer window
 // rows at segment edges; shorter ones give more, shorter workgroups (a smaller
 // tail, a tighter working set per XCD L2).  At 15360^2 (ms per launch over
 // segments of 8 ... 256 rows) the kernels with x shuffles are fastest at 8-16
 // rows and the per-lane y sweeps at 128.  On a smaller grid a family's segment
 // is halved until it has 4 workgroups per CU (or 8 rows).
-enum { SF_TIMESTEP, SF_LAGRANGE, SF_CELL_X, SF_CELL_Y, SF_MOM_X, SF_MOM_Y };
-constexpr int kSegLarge[6] = {16, 16, 8, 128, 8, 128};
+enum { SF_TIMESTEP, SF_LAGRANGE, SF_CELL_X, SF_CELL_Y, SF_MOM_X, SF_MOM_Y, SF_CM_X };
+constexpr int kSegLarge[7] = {16, 16, 8, 128, 8, 128, 8};
 constexpr int kSegMin = 8;
 
 static void choose_segments(clv_chunk *c) {
@@ -954,9 +1078,10 @@ static void choose_segments(clv_chunk *c) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
   const int nx = c->d.nx, ny = c->d.ny;
-  const int strips[6] = {ceil_div(nx, 62), ceil_div(nx, 61), ceil_div(nx, 60), ceil_div(nx, 64), ceil_div(nx + 1, 60), ceil_div(nx + 1, 64)};
-  const int rows[6] = {ny, ny, ny, ny, ny + 1, ny + 1};
-  for (int f = 0; f < 6; ++f) {
+  const int strips[7] = {ceil_div(nx, 62),     ceil_div(nx, 61),     ceil_div(nx, 60),    ceil_div(nx, 64),
+                         ceil_div(nx + 1, 60), ceil_div(nx + 1, 64), ceil_div(nx + 1, 57)};
+  const int rows[7] = {ny, ny, ny, ny, ny + 1, ny + 1, ny + 1};
+  for (int f = 0; f < 7; ++f) {
     int seg = kSegLarge[f];
     while (seg > kSegMin && ceil_div(strips[f], WAVES) * ceil_div(rows[f], seg) < 4 * cus) seg /= 2;
     c->seg[f] = seg;
@@ -1066,18 +1191,46 @@ static clv_status mom_sweep(clv_chunk *c, int sweep, int dir) {
   return CLV_OK;
 }
 
+// advec_cell and advec_mom of one x sweep in one launch (cm_x_k); the halo update the
+// driver runs between the two (driver/advection.cpp:42-46) is split: the velocities'
+// before (the cell sweep does not change them), the rest after (the mom sweep does not
+// change those), and cm_x_k reads the ghosts it needs by mirror
+static clv_status cm_x_sweep(clv_chunk *c, int sweep) {
+  const Dev &d = c->d;
+  const int ns = ceil_div(d.nx + 1, 57), seg = c->seg[SF_CM_X];
+  const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
+  double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2], *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
+  if (sweep == 1) LAUNCH_F("f_cm_x1", cm_x_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+  else LAUNCH_F("f_cm_x2", cm_x_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+  swap_fields(c, CLV_DENSITY1, CLV_WORK1);
+  swap_fields(c, CLV_ENERGY1, CLV_WORK2);
+  swap_fields(c, CLV_XVEL1, CLV_WORK3);
+  swap_fields(c, CLV_YVEL1, CLV_WORK4);
+  return halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2);
+}
+
 // driver/advection.cpp:29-105 followed by reset_field (src/serial/reset_field.cpp)
 clv_status fused_advection(clv_chunk *c, int advect_x) {
   clv_status st = require_owning(c);
   if (st != CLV_OK) return st;
   const int dir1 = advect_x ? CLV_XDIR : CLV_YDIR, dir2 = advect_x ? CLV_YDIR : CLV_XDIR;
-  if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}, 2)) != CLV_OK) return st;
-  if ((st = cell_sweep(c, 1, dir1)) != CLV_OK) return st;
-  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
-  if ((st = mom_sweep(c, 1, dir1)) != CLV_OK) return st;
-  if ((st = cell_sweep(c, 2, dir2)) != CLV_OK) return st;
-  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
-  if ((st = mom_sweep(c, 2, dir2)) != CLV_OK) return st;
+  if (dir1 == CLV_XDIR) {
+    if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
+    if ((st = cm_x_sweep(c, 1)) != CLV_OK) return st;
+  } else {
+    if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}, 2)) != CLV_OK) return st;
+    if ((st = cell_sweep(c, 1, dir1)) != CLV_OK) return st;
+    if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
+    if ((st = mom_sweep(c, 1, dir1)) != CLV_OK) return st;
+  }
+  if (dir2 == CLV_XDIR) {
+    if ((st = halo(c, {CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
+    if ((st = cm_x_sweep(c, 2)) != CLV_OK) return st;
+  } else {
+    if ((st = cell_sweep(c, 2, dir2)) != CLV_OK) return st;
+    if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
+    if ((st = mom_sweep(c, 2, dir2)) != CLV_OK) return st;
+  }
   // reset_field: the end-of-step fields become the start-of-step fields
   swap_fields(c, CLV_DENSITY0, CLV_DENSITY1);
   swap_fields(c, CLV_ENERGY0, CLV_ENERGY1);
