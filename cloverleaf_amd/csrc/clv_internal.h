@@ -49,7 +49,7 @@ struct clv_chunk {
   double *result_host = nullptr;
   clv::Profiler *prof = nullptr;
   int summary_serial = 0;  // CLV_SUMMARY_SERIAL: reproduce the serial model's summation order
-  int seg[7] = {0};        // rows per segment of the fused plan's tile families (chosen on first use)
+  int seg[8] = {0};        // rows per segment of the fused plan's tile families (chosen on first use)
 };
 
 namespace clv {

@@ -875,6 +875,187 @@ __global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, int seg, doub
   mom_y_tile<MOM_SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, u_out, v_out);
 }
 
+// y direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
+// with mom_sweep 2, cell sweep 2 with mom_sweep 4).  Lanes: column 62 * strip + L;
+// nodes, cells and faces are owned on lanes 2..63, and lanes 0 and 1 supply the two
+// columns left of them.  Each lane walks its column: the cell part (cell_y_tile's
+// arithmetic) runs four faces ahead of the mom part (mom_y_tile's), and keeps the
+// advected density and the mass fluxes of the last rows in registers.  advec_mom reads
+// them through the halo update between the two sweeps (driver/advection.cpp:42-46,
+// src/serial/update_halo.cpp): density1 rows 1 - k <- 2 + k, ny + 2 + k <- ny + 1 - k;
+// mass_flux_y rows 1 - k <- -(3 + k), ny + 3 + k <- -(ny + 1 - k); density1 columns
+// 1 <- 2, nx + 2 <- nx + 1; mass_flux_y columns 1 <- 3, nx + 2 <- nx.  Here the ghost
+// rows are read from the rows they mirror (kept in the registers) and the ghost
+// columns from lane shuffles.
+// History slots at mom row k: slot q holds row k - 3 + q (mass flux rows k-3..k+4,
+// density rows k-3..k+3).
+__device__ __forceinline__ int cm_y_rho_slot(int k, int ny) { return mirror_cell(k, ny) - k + 3; }
+__device__ __forceinline__ int cm_y_mf_slot(int x, int k, int ny, double &sgn) {
+  // mass flux row x (k or k + 1) through the mirror
+  int src = x;
+  if (x < 2) src = 4 - x;
+  else if (x > ny + 2) src = 2 * ny + 4 - x;
+  sgn = src == x ? 1.0 : -1.0;
+  return src - k + 3;
+}
+// the history is held in named registers (an array indexed by a run-time slot would
+// live in scratch memory): density rows in r0..r6, mass flux rows in m0..m7
+struct CmHist {
+  double m0, m1, m2, m3, m4, m5, m6, m7, r0, r1, r2, r3, r4, r5, r6;
+  __device__ void push(double mf, double rho) {
+    m0 = m1; m1 = m2; m2 = m3; m3 = m4; m4 = m5; m5 = m6; m6 = m7; m7 = mf;
+    r0 = r1; r1 = r2; r2 = r3; r3 = r4; r4 = r5; r5 = r6; r6 = rho;
+  }
+  __device__ double mf(int q) const {
+    return q == 0 ? m0 : q == 1 ? m1 : q == 2 ? m2 : q == 3 ? m3 : q == 4 ? m4 : q == 5 ? m5 : q == 6 ? m6 : m7;
+  }
+  __device__ double rho(int q) const { return q == 0 ? r0 : q == 2 ? r2 : q == 4 ? r4 : q == 6 ? r6 : r3; }
+};
+template <int SWEEP, class V, class O>
+__device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
+  constexpr int MOM_SWEEP = SWEEP == 1 ? 2 : 4;
+  const int lane = threadIdx.x & 63;
+  const int strip = bx * WAVES + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  const int nx = d.nx, ny = d.ny;
+  const int i = 62 * strip + lane;
+  const int ic = i < nx + 3 ? i : nx + 3;
+  const bool own = lane >= 2;
+  const bool own_cell = own && i <= nx + 1, own_node = own && i <= nx + 2;
+  const int lane_c = lane + (i == 1 ? 1 : (i == nx + 2 ? -1 : 0));
+  const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
+  auto pre_of = [&](int row) {
+    const double vol = AT(CLV_VOLUME, ic, row);
+    if (SWEEP == 1)  // src/serial/advec_cell.cpp:139
+      return vol + (AT(CLV_VOL_FLUX_Y, ic, row + 1) - AT(CLV_VOL_FLUX_Y, ic, row) + AT(CLV_VOL_FLUX_X, ic + 1, row) - AT(CLV_VOL_FLUX_X, ic, row));
+    return vol + AT(CLV_VOL_FLUX_Y, ic, row + 1) - AT(CLV_VOL_FLUX_Y, ic, row);  // :149
+  };
+  auto post_of = [&](int c) {
+    if (MOM_SWEEP == 2) return AT(CLV_VOLUME, ic, c) + AT(CLV_VOL_FLUX_X, ic + 1, c) - AT(CLV_VOL_FLUX_X, ic, c);  // src/serial/advec_mom.cpp:56
+    return AT(CLV_VOLUME, ic, c);                                                                                 // :74
+  };
+  // cell part: faces Fs .. Fe (cell F - 1 is updated at face F)
+  const int Fs = j0 - 2 > 2 ? j0 - 2 : 2, Fe = min(j1 + 1, ny + 2);
+  double rho_m2 = AT(CLV_DENSITY1, ic, Fs - 2), rho_m1 = AT(CLV_DENSITY1, ic, Fs - 1), rho_0 = AT(CLV_DENSITY1, ic, Fs);
+  double e_m2 = AT(CLV_ENERGY1, ic, Fs - 2), e_m1 = AT(CLV_ENERGY1, ic, Fs - 1), e_0 = AT(CLV_ENERGY1, ic, Fs);
+  double pre_m1 = pre_of(Fs - 1);
+  double vd_m1 = M(CLV_VERTEXDY, Fs - 1);
+  double mf_prev = 0.0, ef_prev = 0.0, vf_prev = 0.0;
+  CmHist h{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  // mom part state (mom_y_tile's)
+  double nf_prev = 0.0, mo_prev = 0.0, mw_prev = 0.0, pre_prev = 0.0, post_prev = 0.0, mu_prev = 0.0, mv_prev = 0.0;
+  double u_m2 = 0.0, v_m2 = 0.0, u_m1 = 0.0, v_m1 = 0.0, u_0 = 0.0, v_0 = 0.0;
+  // node row k's flux and masses from the history (src/serial/advec_mom.cpp:160, :170-171)
+  auto mom_row = [&](int k, double &nf, double &mo, double &mw) {
+    double s0, s1;
+    const int q0 = cm_y_mf_slot(k, k, ny, s0), q1 = cm_y_mf_slot(k + 1, k, ny, s1);
+    const double m0 = s0 * h.mf(q0), m1 = s1 * h.mf(q1);
+    nf = 0.25 * (up1(m0) + m0 + up1(m1) + m1);
+    mo = h.rho(cm_y_rho_slot(k, ny)) * post_of(k);
+    mw = up1(mo);
+  };
+  for (int F = Fs; F <= j1 + 4; ++F) {
+    double mf_h = 0.0, rho_h = 0.0;
+    if (F <= Fe) {
+      const double pre_0 = pre_of(F);
+      const double vf = AT(CLV_VOL_FLUX_Y, ic, F);
+      const double vd_face = M(CLV_VERTEXDY, F);
+      const int rp = F + 1 <= ny + 2 ? F + 1 : ny + 2;  // src/serial/advec_cell.cpp:169
+      const double rho_p1 = AT(CLV_DENSITY1, ic, rp), e_p1 = AT(CLV_ENERGY1, ic, rp);
+      const double vd_p1 = M(CLV_VERTEXDY, rp);
+      double mf, ef;
+      cell_face_flux(vf, pre_m1, pre_0, rho_m2, rho_m1, rho_0, rho_p1, e_m2, e_m1, e_0, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
+      if (own_cell && F >= j0 && F < j1) AT(CLV_MASS_FLUX_Y, i, F) = mf;
+      mf_h = mf;
+      if (F > Fs) {
+        // cell F - 1 from faces F - 1 and F, src/serial/advec_cell.cpp:211-216
+        const double pre = pre_m1, rho = rho_m1, e = e_m1;
+        double pre_mass_s = rho * pre;
+        double post_mass_s = pre_mass_s + mf_prev - mf;
+        double post_ener_s = (e * pre_mass_s + ef_prev - ef) / post_mass_s;
+        double advec_vol_s = pre + vf_prev - vf;
+        rho_h = post_mass_s / advec_vol_s;
+        const int c = F - 1;
+        if (own_cell && c >= j0 && c < j1 && c <= ny + 1) {
+          rho_out[(int64_t)c * d.pitch + i] = rho_h;
+          e_out[(int64_t)c * d.pitch + i] = post_ener_s;
+        }
+      }
+      mf_prev = mf;
+      ef_prev = ef;
+      vf_prev = vf;
+      rho_m2 = rho_m1;
+      rho_m1 = rho_0;
+      e_m2 = e_m1;
+      e_m1 = e_0;
+      if (F + 1 <= Fe) {
+        rho_0 = AT(CLV_DENSITY1, ic, F + 1);
+        e_0 = AT(CLV_ENERGY1, ic, F + 1);
+      }
+      pre_m1 = pre_0;
+      vd_m1 = vd_face;
+    }
+    // history: mass flux row F and density row F - 1, ghost columns mirrored
+    h.push(__shfl(mf_h, lane_f), __shfl(rho_h, lane_c));
+    // mom part at node row k = F - 4 (mom_y_tile's iteration k)
+    const int k = F - 4;
+    if (k == j0 - 2) {
+      mom_row(k, nf_prev, mo_prev, mw_prev);
+      u_m1 = AT(CLV_XVEL1, ic, k);
+      v_m1 = AT(CLV_YVEL1, ic, k);
+      u_0 = AT(CLV_XVEL1, ic, k + 1);
+      v_0 = AT(CLV_YVEL1, ic, k + 1);
+    } else if (k > j0 - 2) {
+      double nf, mo, mw;
+      mom_row(k, nf, mo, mw);
+      const double nm_post = 0.25 * (mo_prev + mo + mw_prev + mw);  // :170-171
+      const double nm_pre = nm_post - nf_prev + nf;                 // :172
+      const double u_p1 = AT(CLV_XVEL1, ic, k + 1), v_p1 = AT(CLV_YVEL1, ic, k + 1);
+      if (k >= j0) {
+        // momentum flux of face f = k - 1
+        const int f = k - 1;
+        const double wd = M(CLV_CELLDY, f);
+        double mu, mv;
+        if (nf_prev < 0.0) {  // upwind f+2, donor f+1, downwind f, dif = donor (:186-191)
+          const double wdif = M(CLV_CELLDY, f + 1);
+          mu = mom_flux_of(nf_prev, nm_pre, wd, wdif, u_p1, u_0, u_m1);
+          mv = mom_flux_of(nf_prev, nm_pre, wd, wdif, v_p1, v_0, v_m1);
+        } else {  // upwind f-1, donor f, downwind f+1, dif = upwind (:192-196)
+          const double wdif = M(CLV_CELLDY, f - 1);
+          mu = mom_flux_of(nf_prev, pre_prev, wd, wdif, u_m2, u_m1, u_0);
+          mv = mom_flux_of(nf_prev, pre_prev, wd, wdif, v_m2, v_m1, v_0);
+        }
+        if (k >= j0 + 1 && own_node) {
+          // update node row f from faces f - 1 and f (:221)
+          u_out[(int64_t)f * d.pitch + i] = (u_m1 * pre_prev + mu_prev - mu) / post_prev;
+          v_out[(int64_t)f * d.pitch + i] = (v_m1 * pre_prev + mv_prev - mv) / post_prev;
+        }
+        mu_prev = mu;
+        mv_prev = mv;
+      }
+      nf_prev = nf;
+      mo_prev = mo;
+      mw_prev = mw;
+      pre_prev = nm_pre;
+      post_prev = nm_post;
+      u_m2 = u_m1;
+      u_m1 = u_0;
+      u_0 = u_p1;
+      v_m2 = v_m1;
+      v_m1 = v_0;
+      v_0 = v_p1;
+    }
+  }
+}
+
+template <int SWEEP>
+__global__ __launch_bounds__(256) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
+  int bx, by;
+  xcd_tile(bx, by);
+  const int j0 = 2 + seg * by;
+  cm_y_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+}
+
 // ---- batched form: persistent launches over the tiles of many chunks ----------------------
 // A launch walks a list of BatchItems (problem, strip block, row segment).  Its
 // blocks are dealt round-robin to the 8 XCDs (block b on XCD b % 8), so the
@@ -1063,14 +1244,14 @@ hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, Batch
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // Rows per wave segment of each plain launch, per tile family (timestep,
-// lagrange, cell x, cell y, mom x, mom y, cell + mom x).  Longer segments re-read fewer window
+// lagrange, cell x, cell y, mom x, mom y, cell + mom x, cell + mom y).  Longer segments re-read fewer window
 // rows at segment edges; shorter ones give more, shorter workgroups (a smaller
 // tail, a tighter working set per XCD L2).  At 15360^2 (ms per launch over
 // segments of 8 ... 256 rows) the kernels with x shuffles are fastest at 8-16
 // rows and the per-lane y sweeps at 128.  On a smaller grid a family's segment
 // is halved until it has 4 workgroups per CU (or 8 rows).
-enum { SF_TIMESTEP, SF_LAGRANGE, SF_CELL_X, SF_CELL_Y, SF_MOM_X, SF_MOM_Y, SF_CM_X };
-constexpr int kSegLarge[7] = {16, 16, 8, 128, 8, 128, 8};
+enum { SF_TIMESTEP, SF_LAGRANGE, SF_CELL_X, SF_CELL_Y, SF_MOM_X, SF_MOM_Y, SF_CM_X, SF_CM_Y };
+constexpr int kSegLarge[8] = {16, 16, 8, 128, 8, 128, 8, 128};
 constexpr int kSegMin = 8;
 
 static void choose_segments(clv_chunk *c) {
@@ -1078,10 +1259,10 @@ static void choose_segments(clv_chunk *c) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
   const int nx = c->d.nx, ny = c->d.ny;
-  const int strips[7] = {ceil_div(nx, 62),     ceil_div(nx, 61),     ceil_div(nx, 60),    ceil_div(nx, 64),
-                         ceil_div(nx + 1, 60), ceil_div(nx + 1, 64), ceil_div(nx + 1, 57)};
-  const int rows[7] = {ny, ny, ny, ny, ny + 1, ny + 1, ny + 1};
-  for (int f = 0; f < 7; ++f) {
+  const int strips[8] = {ceil_div(nx, 62),     ceil_div(nx, 61),     ceil_div(nx, 60),     ceil_div(nx, 64),
+                         ceil_div(nx + 1, 60), ceil_div(nx + 1, 64), ceil_div(nx + 1, 57), ceil_div(nx + 1, 62)};
+  const int rows[8] = {ny, ny, ny, ny, ny + 1, ny + 1, ny + 1, ny + 1};
+  for (int f = 0; f < 8; ++f) {
     int seg = kSegLarge[f];
     while (seg > kSegMin && ceil_div(strips[f], WAVES) * ceil_div(rows[f], seg) < 4 * cus) seg /= 2;
     c->seg[f] = seg;
@@ -1191,17 +1372,24 @@ static clv_status mom_sweep(clv_chunk *c, int sweep, int dir) {
   return CLV_OK;
 }
 
-// advec_cell and advec_mom of one x sweep in one launch (cm_x_k); the halo update the
-// driver runs between the two (driver/advection.cpp:42-46) is split: the velocities'
-// before (the cell sweep does not change them), the rest after (the mom sweep does not
-// change those), and cm_x_k reads the ghosts it needs by mirror
-static clv_status cm_x_sweep(clv_chunk *c, int sweep) {
+// advec_cell and advec_mom of one sweep in one launch (cm_x_k, cm_y_k); the halo
+// update the driver runs between the two (driver/advection.cpp:42-46) is split: the
+// velocities' before (the cell sweep does not change them), the rest after (the mom
+// sweep does not change those), and the kernels read the ghosts they need by mirror
+static clv_status cm_sweep(clv_chunk *c, int sweep, int dir) {
   const Dev &d = c->d;
-  const int ns = ceil_div(d.nx + 1, 57), seg = c->seg[SF_CM_X];
-  const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
   double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2], *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
-  if (sweep == 1) LAUNCH_F("f_cm_x1", cm_x_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
-  else LAUNCH_F("f_cm_x2", cm_x_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+  if (dir == CLV_XDIR) {
+    const int ns = ceil_div(d.nx + 1, 57), seg = c->seg[SF_CM_X];
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
+    if (sweep == 1) LAUNCH_F("f_cm_x1", cm_x_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+    else LAUNCH_F("f_cm_x2", cm_x_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+  } else {
+    const int ns = ceil_div(d.nx + 1, 62), seg = c->seg[SF_CM_Y];
+    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
+    if (sweep == 1) LAUNCH_F("f_cm_y1", cm_y_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+    else LAUNCH_F("f_cm_y2", cm_y_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+  }
   swap_fields(c, CLV_DENSITY1, CLV_WORK1);
   swap_fields(c, CLV_ENERGY1, CLV_WORK2);
   swap_fields(c, CLV_XVEL1, CLV_WORK3);
@@ -1214,23 +1402,10 @@ clv_status fused_advection(clv_chunk *c, int advect_x) {
   clv_status st = require_owning(c);
   if (st != CLV_OK) return st;
   const int dir1 = advect_x ? CLV_XDIR : CLV_YDIR, dir2 = advect_x ? CLV_YDIR : CLV_XDIR;
-  if (dir1 == CLV_XDIR) {
-    if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
-    if ((st = cm_x_sweep(c, 1)) != CLV_OK) return st;
-  } else {
-    if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}, 2)) != CLV_OK) return st;
-    if ((st = cell_sweep(c, 1, dir1)) != CLV_OK) return st;
-    if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
-    if ((st = mom_sweep(c, 1, dir1)) != CLV_OK) return st;
-  }
-  if (dir2 == CLV_XDIR) {
-    if ((st = halo(c, {CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
-    if ((st = cm_x_sweep(c, 2)) != CLV_OK) return st;
-  } else {
-    if ((st = cell_sweep(c, 2, dir2)) != CLV_OK) return st;
-    if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
-    if ((st = mom_sweep(c, 2, dir2)) != CLV_OK) return st;
-  }
+  if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
+  if ((st = cm_sweep(c, 1, dir1)) != CLV_OK) return st;
+  if ((st = halo(c, {CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
+  if ((st = cm_sweep(c, 2, dir2)) != CLV_OK) return st;
   // reset_field: the end-of-step fields become the start-of-step fields
   swap_fields(c, CLV_DENSITY0, CLV_DENSITY1);
   swap_fields(c, CLV_ENERGY0, CLV_ENERGY1);
