@@ -924,41 +924,53 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   const bool own_cell = own && i <= nx + 1, own_node = own && i <= nx + 2;
   const int lane_c = lane + (i == 1 ? 1 : (i == nx + 2 ? -1 : 0));
   const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
-  auto pre_of = [&](int row) {
-    const double vol = AT(CLV_VOLUME, ic, row);
-    if (SWEEP == 1)  // src/serial/advec_cell.cpp:139
-      return vol + (AT(CLV_VOL_FLUX_Y, ic, row + 1) - AT(CLV_VOL_FLUX_Y, ic, row) + AT(CLV_VOL_FLUX_X, ic + 1, row) - AT(CLV_VOL_FLUX_X, ic, row));
-    return vol + AT(CLV_VOL_FLUX_Y, ic, row + 1) - AT(CLV_VOL_FLUX_Y, ic, row);  // :149
+  // pre_vol of cell row `row` (src/serial/advec_cell.cpp:139 / :149) from its bottom and
+  // top vol_flux_y, and advec_mom's post_vol (src/serial/advec_mom.cpp:56 / :74)
+  auto pre_of = [&](int row, double vfy0, double vfy1, double vol, double vfx0, double vfx1) {
+    if (SWEEP == 1) return vol + (vfy1 - vfy0 + vfx1 - vfx0);
+    return vol + vfy1 - vfy0;
   };
+  auto post_from = [&](double vol, double vfx0, double vfx1) { return MOM_SWEEP == 2 ? vol + vfx1 - vfx0 : vol; };
   auto post_of = [&](int c) {
-    if (MOM_SWEEP == 2) return AT(CLV_VOLUME, ic, c) + AT(CLV_VOL_FLUX_X, ic + 1, c) - AT(CLV_VOL_FLUX_X, ic, c);  // src/serial/advec_mom.cpp:56
-    return AT(CLV_VOLUME, ic, c);                                                                                 // :74
+    return post_from(AT(CLV_VOLUME, ic, c), MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic, c) : 0.0, MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic + 1, c) : 0.0);
+  };
+  auto pre_load = [&](int row, double vfy0, double &vfy1, double &post) {
+    const double vol = AT(CLV_VOLUME, ic, row);
+    vfy1 = AT(CLV_VOL_FLUX_Y, ic, row + 1);
+    const double vfx0 = SWEEP == 1 ? AT(CLV_VOL_FLUX_X, ic, row) : 0.0, vfx1 = SWEEP == 1 ? AT(CLV_VOL_FLUX_X, ic + 1, row) : 0.0;
+    post = post_from(vol, vfx0, vfx1);
+    return pre_of(row, vfy0, vfy1, vol, vfx0, vfx1);
   };
   // cell part: faces Fs .. Fe (cell F - 1 is updated at face F)
   const int Fs = j0 - 2 > 2 ? j0 - 2 : 2, Fe = min(j1 + 1, ny + 2);
   double rho_m2 = AT(CLV_DENSITY1, ic, Fs - 2), rho_m1 = AT(CLV_DENSITY1, ic, Fs - 1), rho_0 = AT(CLV_DENSITY1, ic, Fs);
   double e_m2 = AT(CLV_ENERGY1, ic, Fs - 2), e_m1 = AT(CLV_ENERGY1, ic, Fs - 1), e_0 = AT(CLV_ENERGY1, ic, Fs);
-  double pre_m1 = pre_of(Fs - 1);
+  double vfy_c = AT(CLV_VOL_FLUX_Y, ic, Fs - 1), post_unused;  // vol_flux_y of the face below the cell in hand
+  double pre_m1 = pre_load(Fs - 1, vfy_c, vfy_c, post_unused);
+  // post_vol of the last five cell rows: row F - 4 + q in p[q]
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0;
   double vd_m1 = M(CLV_VERTEXDY, Fs - 1);
   double mf_prev = 0.0, ef_prev = 0.0, vf_prev = 0.0;
   CmHist h{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   // mom part state (mom_y_tile's)
   double nf_prev = 0.0, mo_prev = 0.0, mw_prev = 0.0, pre_prev = 0.0, post_prev = 0.0, mu_prev = 0.0, mv_prev = 0.0;
   double u_m2 = 0.0, v_m2 = 0.0, u_m1 = 0.0, v_m1 = 0.0, u_0 = 0.0, v_0 = 0.0;
-  // node row k's flux and masses from the history (src/serial/advec_mom.cpp:160, :170-171)
+  // node row k's flux and masses from the history (src/serial/advec_mom.cpp:160, :170-171);
+  // post_vol of row k from the history where the cell part passed that row, else loaded
   auto mom_row = [&](int k, double &nf, double &mo, double &mw) {
     double s0, s1;
     const int q0 = cm_y_mf_slot(k, k, ny, s0), q1 = cm_y_mf_slot(k + 1, k, ny, s1);
     const double m0 = s0 * h.mf(q0), m1 = s1 * h.mf(q1);
     nf = 0.25 * (up1(m0) + m0 + up1(m1) + m1);
-    mo = h.rho(cm_y_rho_slot(k, ny)) * post_of(k);
+    const double post = k >= Fs && k <= Fe ? p0 : post_of(k);
+    mo = h.rho(cm_y_rho_slot(k, ny)) * post;
     mw = up1(mo);
   };
   for (int F = Fs; F <= j1 + 4; ++F) {
-    double mf_h = 0.0, rho_h = 0.0;
+    double mf_h = 0.0, rho_h = 0.0, post_h = 0.0;
     if (F <= Fe) {
-      const double pre_0 = pre_of(F);
-      const double vf = AT(CLV_VOL_FLUX_Y, ic, F);
+      const double vf = vfy_c;
+      const double pre_0 = pre_load(F, vf, vfy_c, post_h);
       const double vd_face = M(CLV_VERTEXDY, F);
       const int rp = F + 1 <= ny + 2 ? F + 1 : ny + 2;  // src/serial/advec_cell.cpp:169
       const double rho_p1 = AT(CLV_DENSITY1, ic, rp), e_p1 = AT(CLV_ENERGY1, ic, rp);
@@ -988,15 +1000,18 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       rho_m1 = rho_0;
       e_m2 = e_m1;
       e_m1 = e_0;
-      if (F + 1 <= Fe) {
-        rho_0 = AT(CLV_DENSITY1, ic, F + 1);
-        e_0 = AT(CLV_ENERGY1, ic, F + 1);
-      }
+      rho_0 = rho_p1;  // row F + 1 (unclamped while F + 1 <= Fe <= ny + 2)
+      e_0 = e_p1;
       pre_m1 = pre_0;
       vd_m1 = vd_face;
     }
     // history: mass flux row F and density row F - 1, ghost columns mirrored
     h.push(__shfl(mf_h, lane_f), __shfl(rho_h, lane_c));
+    p0 = p1;
+    p1 = p2;
+    p2 = p3;
+    p3 = p4;
+    p4 = post_h;
     // mom part at node row k = F - 4 (mom_y_tile's iteration k)
     const int k = F - 4;
     if (k == j0 - 2) {
