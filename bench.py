@@ -95,12 +95,13 @@ ALG_TERMS = {
     "f_mom_x2": [(7, "N")],                    # vol, mfx, rho1, u1, v1 -> u1, v1
     "f_mom_y2": [(7, "N")],
     "f_cm_x1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
+    "f_cm_y1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
     "f_cm_x2": [(11, "C")],                    # vol, vfx, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
+    "f_cm_y2": [(11, "C")],                    # vol, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
 }
 # compulsory traffic of one fused step, array sweeps: timestep 8 + lagrange 14 + the
-# x pass (12 as the first sweep, 11 as the second) + the y cell and mom sweeps (8 + 8
-# first, 7 + 7 second); steps alternate the order: (48 + 49) / 2
-B_FUSED_PER_CELL_STEP = 48.5 * 8.0
+# first advection pass 12 + the second 11
+B_FUSED_PER_CELL_STEP = 45 * 8.0
 B_ALG_PER_CELL_STEP = 1440.0  # 180 array sweeps of 8 B per interior cell-step at reference kernel granularity
 
 
@@ -452,7 +453,7 @@ def main():
                    "x_cells": nx, "y_cells": ny, "plan": args.plan, "parallelism": "1 GPU, 1 chunk"},
         "gcell_steps_per_s": value / 1e9,
         "effective_hbm_frac_at_1440B": value * B_ALG_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
-        "compulsory_hbm_frac_fused_388B": value * B_FUSED_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9) if args.plan == "fused" else None,
+        "compulsory_hbm_frac_fused_360B": value * B_FUSED_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9) if args.plan == "fused" else None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_us": avg_s * 1e6, "share_of_kernel_time": dom_ms / total_kernel_ms},
