@@ -74,7 +74,7 @@ class BatchFamily(C.Structure):
     _fields_ = [("seg", C.c_int), ("blocks_per_cu", C.c_int), ("order", C.c_int)]
 
 
-BATCH_FAMILIES = ["timestep", "lagrange", "cell_x", "cell_y", "mom_x", "mom_y"]
+BATCH_FAMILIES = ["timestep", "lagrange", "cm_x", "cm_y"]
 ORDER_GIVEN, ORDER_LARGEST_FIRST = 0, 1
 
 

@@ -6,21 +6,19 @@
 //   batch_dt_k        (one block per member) -> dt, time, step, complete on the device
 //   batch_halo_k      viscosity, depth 1
 //   batch_lagrange_k
-//   batch_halo_k      energy1, density1, vol_flux_x/y, depth 2
-//   batch_advec_k     cell sweep 1, dir1
+//   batch_halo_k      energy1, density1, vol_flux_x/y, xvel1, yvel1, depth 2
+//   batch_advec_k     advection pass 1 (advec_cell + advec_mom), dir1
 //   batch_halo_k      density1, energy1, xvel1, yvel1, mass_flux_x/y, depth 2
-//   batch_advec_k     mom sweep 1, dir1
-//   batch_advec_k     cell sweep 2, dir2
-//   batch_halo_k      as above
-//   batch_advec_k     mom sweep 2, dir2
+//   batch_advec_k     advection pass 2, dir2
+//   batch_halo_k      density1, energy1, mass_flux_x/y, depth 2
 //
-// with no host synchronisation: dt never leaves the device, so the eleven
+// with no host synchronisation: dt never leaves the device, so the nine
 // launches are captured once as a hipGraph and replayed per step.
 //
 // Buffer renaming.  The fused step double-buffers the advected fields and ends
 // with reset_field as a rename (kernels_fused.hip).  Over one step the renames
 // compose to the 0 <-> 1 swap of density, energy, xvel and yvel, so the views a
-// step's kernels see depend only on the parity of the step: two tables of four
+// step's kernels see depend only on the parity of the step: two tables of three
 // stage views per member ([parity][stage][member]) cover every step, and a graph
 // per parity is replayed alternately.  A member that completes stops being
 // stepped; its host view is reconciled from the number of steps it ran.
@@ -39,7 +37,7 @@ using namespace clv;
 
 namespace {
 
-constexpr int kStages = 4;  // views before cell sweep 1, mom sweep 1, cell sweep 2, mom sweep 2
+constexpr int kStages = 3;  // views before advection pass 1, before pass 2, after pass 2
 constexpr int kMinSeg = 8, kMaxSeg = 4096;
 constexpr int kCheckEvery = 16;  // batch steps between completion checks
 
@@ -51,12 +49,9 @@ Dev after_reset(Dev d) {
   swap_in(d, CLV_YVEL0, CLV_YVEL1);
   return d;
 }
-Dev after_cell(Dev d) {
+Dev after_pass(Dev d) {
   swap_in(d, CLV_DENSITY1, CLV_WORK1);
   swap_in(d, CLV_ENERGY1, CLV_WORK2);
-  return d;
-}
-Dev after_mom(Dev d) {
   swap_in(d, CLV_XVEL1, CLV_WORK3);
   swap_in(d, CLV_YVEL1, CLV_WORK4);
   return d;
@@ -64,23 +59,22 @@ Dev after_mom(Dev d) {
 bool same_view(const Dev &a, const Dev &b) { return std::memcmp(&a, &b, sizeof(Dev)) == 0; }
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
-// host mirror of the device strips_of (owned columns per wave: 62, 61, 60, 64)
+// host mirror of the device strips_of (owned columns per wave: 62, 61, 57, 62)
 int strips(int fam, int nx) {
   switch (fam) {
     case CLV_BF_TIMESTEP: return ceil_div(nx, 62);
     case CLV_BF_LAGRANGE: return ceil_div(nx, 61);
-    case CLV_BF_CELL_X: return ceil_div(nx, 60);
-    case CLV_BF_CELL_Y: return ceil_div(nx, 64);
-    case CLV_BF_MOM_X: return ceil_div(nx + 1, 60);
-    default: return ceil_div(nx + 1, 64);
+    case CLV_BF_CM_X: return ceil_div(nx + 1, 57);
+    default: return ceil_div(nx + 1, 62);
   }
 }
-int tile_rows(int fam, int ny) { return (fam == CLV_BF_MOM_X || fam == CLV_BF_MOM_Y) ? ny + 1 : ny; }
+// the advection passes' tiles cover node rows
+int tile_rows(int fam, int ny) { return (fam == CLV_BF_CM_X || fam == CLV_BF_CM_Y) ? ny + 1 : ny; }
 constexpr int kWaves = 4;  // strips per 256-thread block
 
 // the kernel each family is timed with by the autotuner
 int probe_kind(int fam) {
-  static const int k[CLV_BF_COUNT] = {BK_TIMESTEP, BK_LAGRANGE, BK_CELL_X1, BK_CELL_Y1, BK_MOM_X1, BK_MOM_Y2};
+  static const int k[CLV_BF_COUNT] = {BK_TIMESTEP, BK_LAGRANGE, BK_CM_X1, BK_CM_Y1};
   return k[fam];
 }
 
@@ -242,22 +236,17 @@ clv_status emit_step(clv_batch *b, int q) {
   do {                                \
     if ((st = (x)) != CLV_OK) return st; \
   } while (0)
-  const Dev *s0 = b->tab(q, 0), *s1 = b->tab(q, 1), *s2 = b->tab(q, 2), *s3 = b->tab(q, 3);
+  const Dev *s0 = b->tab(q, 0), *s1 = b->tab(q, 1), *s2 = b->tab(q, 2);
+  const int fam1 = x_first ? CLV_BF_CM_X : CLV_BF_CM_Y, fam2 = x_first ? CLV_BF_CM_Y : CLV_BF_CM_X;
   TRY_B(launch_family(b, BK_TIMESTEP, CLV_BF_TIMESTEP, s0));
   TRY_B(hip_check(batch_launch_dt(b->stream, b->n, b->pc, b->ps), "batch dt launch"));
   TRY_B(launch_halo(b, s0, {CLV_VISCOSITY}, 1));
   TRY_B(launch_family(b, BK_LAGRANGE, CLV_BF_LAGRANGE, s0));
-  TRY_B(launch_halo(b, s0, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}, 2));
-  if (x_first) TRY_B(launch_family(b, BK_CELL_X1, CLV_BF_CELL_X, s0));
-  else TRY_B(launch_family(b, BK_CELL_Y1, CLV_BF_CELL_Y, s0));
+  TRY_B(launch_halo(b, s0, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2));
+  TRY_B(launch_family(b, x_first ? BK_CM_X1 : BK_CM_Y1, fam1, s0));
   TRY_B(launch_halo(b, s1, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2));
-  if (x_first) TRY_B(launch_family(b, BK_MOM_X1, CLV_BF_MOM_X, s1));  // mom_sweep = dir + 2 (sweep - 1)
-  else TRY_B(launch_family(b, BK_MOM_Y2, CLV_BF_MOM_Y, s1));
-  if (x_first) TRY_B(launch_family(b, BK_CELL_Y2, CLV_BF_CELL_Y, s2));
-  else TRY_B(launch_family(b, BK_CELL_X2, CLV_BF_CELL_X, s2));
-  TRY_B(launch_halo(b, s3, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2));
-  if (x_first) TRY_B(launch_family(b, BK_MOM_Y4, CLV_BF_MOM_Y, s3));
-  else TRY_B(launch_family(b, BK_MOM_X3, CLV_BF_MOM_X, s3));
+  TRY_B(launch_family(b, x_first ? BK_CM_Y2 : BK_CM_X2, fam2, s1));
+  TRY_B(launch_halo(b, s2, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2));
 #undef TRY_B
   return CLV_OK;
 }
@@ -374,10 +363,9 @@ clv_status clv_batch_create(clv_chunk *const *chunks, const clv_config *cfgs, co
     Dev s = b->d0[p];
     for (int q = 0; q < 2; ++q) {
       tabs[((size_t)q * kStages + 0) * n + p] = s;
-      tabs[((size_t)q * kStages + 1) * n + p] = after_cell(s);
-      tabs[((size_t)q * kStages + 2) * n + p] = after_mom(after_cell(s));
-      tabs[((size_t)q * kStages + 3) * n + p] = after_cell(after_mom(after_cell(s)));
-      s = after_reset(after_mom(after_cell(after_mom(after_cell(s)))));
+      tabs[((size_t)q * kStages + 1) * n + p] = after_pass(s);
+      tabs[((size_t)q * kStages + 2) * n + p] = after_pass(after_pass(s));
+      s = after_reset(after_pass(after_pass(s)));
     }
   }
   size_t nparts = 0;
@@ -414,7 +402,7 @@ clv_status clv_batch_create(clv_chunk *const *chunks, const clv_config *cfgs, co
     return fail(st);
   // the plain launches' segments (measured at 15360^2), one block per tile
   clv_batch_schedule s{};
-  static const int seg0[CLV_BF_COUNT] = {16, 16, 8, 128, 8, 128};
+  static const int seg0[CLV_BF_COUNT] = {16, 16, 8, 128};
   for (int f = 0; f < CLV_BF_COUNT; ++f) s.fam[f] = clv_batch_family{seg0[f], 0, CLV_BATCH_ORDER_GIVEN};
   if ((st = apply_schedule(b, s)) != CLV_OK) return fail(st);
   *out = b;

@@ -98,13 +98,9 @@ inline void dt_location(int nx, int *control, int *jldt, int *kldt) {
 }
 
 // ---- batched form (kernels_fused.hip, kernels_misc.hip, batch.cpp) ------------------------
-// Tile kernels of a batched step, one per fused kernel and sweep variant.
-enum BatchKind {
-  BK_TIMESTEP, BK_LAGRANGE,
-  BK_CELL_X1, BK_CELL_X2, BK_CELL_Y1, BK_CELL_Y2,
-  BK_MOM_X1, BK_MOM_X3, BK_MOM_Y2, BK_MOM_Y4,
-  BK_COUNT
-};
+// Tile kernels of a batched step, one per fused kernel and sweep variant (the
+// advection passes: cell + mom of sweep 1 / 2 in x / y).
+enum BatchKind { BK_TIMESTEP, BK_LAGRANGE, BK_CM_X1, BK_CM_X2, BK_CM_Y1, BK_CM_Y2, BK_COUNT };
 // One tile of a persistent launch: strip block bx of problem p, rows
 // [2 + k * seg, 2 + (k + 1) * seg) clipped to the problem.
 struct BatchItem {

@@ -1,6 +1,6 @@
 // Fused sweeps for CLV_PLAN_FUSED.
 //
-// One step of driver/hydro.cpp:57-89 in six single-pass kernels:
+// One step of driver/hydro.cpp:57-89 in four single-pass kernels:
 //
 //   timestep_k  ideal_gas(rho0,e0) + viscosity + calc_dt partial minimum
 //               (driver/timestep.cpp:47-78)        writes q (p stays in registers)
@@ -8,14 +8,11 @@
 //   lagrange_k  PdV predictor + ideal_gas + pressure halo + revert, accelerate,
 //               PdV corrector, flux_calc (driver/hydro.cpp:63-75)
 //                                                   writes u1, v1, rho1, e1, vol_flux_x/y
-//   cell_x_k / cell_y_k   advec_cell, one direction (src/serial/advec_cell.cpp)
-//                                                   writes rho1, e1, mass_flux_d
-//   mom_x_k / mom_y_k     advec_mom for xvel1 AND yvel1 (the yvel pass reuses the
-//               xvel pass's node arrays, src/serial/advec_mom.cpp:81-103, so both
-//               velocities share one set)           writes u1, v1
-//   cm_x_k      cell_x + mom_x of one x sweep in one pass (the plain plan's x
-//               sweeps; the batched form keeps the two)
-//                                                   writes rho1, e1, mass_flux_x, u1, v1
+//   cm_x_k / cm_y_k  one advection sweep: advec_cell (src/serial/advec_cell.cpp),
+//               then advec_mom for xvel1 AND yvel1 (the yvel pass reuses the xvel
+//               pass's node arrays, src/serial/advec_mom.cpp:81-103, so both
+//               velocities share one set); twice per step, x and y in the
+//               driver's alternating order   writes rho1, e1, mass_flux_d, u1, v1
 //   reset_field: pointer swaps 1 <-> 0.
 //
 // Every value a kernel writes is computed by the reference expression on the
@@ -456,135 +453,6 @@ __device__ __forceinline__ void cell_face_flux(double vf, double pre_m1, double 
   ef = mf * (e_d + limiter);
 }
 
-// x direction.  Lanes: column x0 + L, x0 = 60 * strip; faces valid on lanes 2..62,
-// cells updated on lanes 2..61.  Rows independent (no y stencil besides the own
-// cell's top face of vol_flux_y in sweep 1).
-template <int SWEEP, class V, class O>
-__device__ __forceinline__ void cell_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out) {
-  const int lane = threadIdx.x & 63;
-  const int strip = bx * WAVES + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
-  const int nx = d.nx;
-  const int x0 = 60 * strip;
-  const int i = x0 + lane;
-  const int ic = i < nx + 4 ? i : nx + 4;
-  const bool own_cell = lane >= 2 && lane <= 61 && i <= nx + 1;
-  const bool own_face = (lane >= 2 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
-  const double vd_face = M(CLV_VERTEXDX, ic);
-  // shuffles run on every lane; the clamp selects afterwards
-  const double vd_m1 = up1(vd_face), vd_e = dn1(vd_face);
-  const double vd_p1 = i + 1 <= nx + 2 ? vd_e : vd_face;
-  double vfy_r = SWEEP == 1 ? AT(CLV_VOL_FLUX_Y, ic, j0) : 0.0;
-  for (int r = j0; r < j1; ++r) {
-    const double rho = AT(CLV_DENSITY1, ic, r), e = AT(CLV_ENERGY1, ic, r);
-    const double vol = AT(CLV_VOLUME, ic, r), vfx = AT(CLV_VOL_FLUX_X, ic, r);
-    const double vfx_e = dn1(vfx);
-    double pre;
-    if (SWEEP == 1) {
-      const double vfy_r1 = AT(CLV_VOL_FLUX_Y, ic, r + 1);
-      pre = vol + (vfx_e - vfx + vfy_r1 - vfy_r);  // src/serial/advec_cell.cpp:47
-      vfy_r = vfy_r1;
-    } else {
-      pre = vol + vfx_e - vfx;  // :57
-    }
-    const double rho_m1 = up1(rho), rho_m2 = up2(rho), e_m1 = up1(e), e_m2 = up2(e);
-    const double rho_e = dn1(rho), e_e = dn1(e);
-    // upwind = min(i + 1, x_max + 2) (:77): at face nx + 2 the upwind cell is the donor
-    const double rho_p1 = i + 1 <= nx + 2 ? rho_e : rho, e_p1 = i + 1 <= nx + 2 ? e_e : e;
-    const double pre_m1 = up1(pre);
-    double mf, ef;
-    cell_face_flux(vfx, pre_m1, pre, rho_m2, rho_m1, rho, rho_p1, e_m2, e_m1, e, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
-    const double mf_e = dn1(mf), ef_e = dn1(ef);
-    if (own_face) AT(CLV_MASS_FLUX_X, i, r) = mf;
-    // src/serial/advec_cell.cpp:120-125
-    double pre_mass_s = rho * pre;
-    double post_mass_s = pre_mass_s + mf - mf_e;
-    double post_ener_s = (e * pre_mass_s + ef - ef_e) / post_mass_s;
-    double advec_vol_s = pre + vfx - vfx_e;
-    if (own_cell) {
-      rho_out[(int64_t)r * d.pitch + i] = post_mass_s / advec_vol_s;
-      e_out[(int64_t)r * d.pitch + i] = post_ener_s;
-    }
-  }
-}
-
-template <int SWEEP>
-__global__ __launch_bounds__(256) void cell_x_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out) {
-  int bx, by;
-  xcd_tile(bx, by);
-  const int j0 = 2 + seg * by;
-  cell_x_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 2), nstrips, rho_out, e_out);
-}
-
-// y direction.  Lanes: column x0 + L, x0 = 2 + 64 * strip (no x stencil besides
-// vol_flux_x at i + 1 in sweep 1, loaded directly).  March faces r = j0 .. j1:
-// face r needs cells r-2 .. r+1; cell r - 1 is updated from faces r - 1 and r.
-template <int SWEEP, class V, class O>
-__device__ __forceinline__ void cell_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out) {
-  const int lane = threadIdx.x & 63;
-  const int strip = bx * WAVES + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
-  const int nx = d.nx, ny = d.ny;
-  const int i = 2 + 64 * strip + lane;
-  if (i > nx + 1) return;  // per-lane: no cross-lane traffic in this kernel
-  auto pre_of = [&](int row) {
-    const double vol = AT(CLV_VOLUME, i, row);
-    if (SWEEP == 1)  // src/serial/advec_cell.cpp:139
-      return vol + (AT(CLV_VOL_FLUX_Y, i, row + 1) - AT(CLV_VOL_FLUX_Y, i, row) + AT(CLV_VOL_FLUX_X, i + 1, row) - AT(CLV_VOL_FLUX_X, i, row));
-    return vol + AT(CLV_VOL_FLUX_Y, i, row + 1) - AT(CLV_VOL_FLUX_Y, i, row);  // :149
-  };
-  // window rows r-2, r-1, r (r = face index), pre at r-1
-  double rho_m2 = AT(CLV_DENSITY1, i, j0 - 2), rho_m1 = AT(CLV_DENSITY1, i, j0 - 1), rho_0 = AT(CLV_DENSITY1, i, j0);
-  double e_m2 = AT(CLV_ENERGY1, i, j0 - 2), e_m1 = AT(CLV_ENERGY1, i, j0 - 1), e_0 = AT(CLV_ENERGY1, i, j0);
-  double pre_m1 = pre_of(j0 - 1);
-  double vd_m1 = M(CLV_VERTEXDY, j0 - 1);
-  double mf_prev = 0.0, ef_prev = 0.0, vf_prev = 0.0;
-  for (int r = j0; r <= j1; ++r) {
-    const double pre_0 = pre_of(r);
-    const double vf = AT(CLV_VOL_FLUX_Y, i, r);
-    const double vd_face = M(CLV_VERTEXDY, r);
-    // upwind = min(j + 1, y_max + 2) (:169)
-    const int rp = r + 1 <= ny + 2 ? r + 1 : ny + 2;
-    const double rho_p1 = AT(CLV_DENSITY1, i, rp), e_p1 = AT(CLV_ENERGY1, i, rp);
-    const double vd_p1 = M(CLV_VERTEXDY, rp);
-    double mf, ef;
-    cell_face_flux(vf, pre_m1, pre_0, rho_m2, rho_m1, rho_0, rho_p1, e_m2, e_m1, e_0, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
-    if (r < j1 || r == ny + 2) AT(CLV_MASS_FLUX_Y, i, r) = mf;
-    if (r > j0) {
-      // update cell r - 1 from faces r - 1 and r, src/serial/advec_cell.cpp:211-216
-      const double pre = pre_m1, rho = rho_m1, e = e_m1;
-      double pre_mass_s = rho * pre;
-      double post_mass_s = pre_mass_s + mf_prev - mf;
-      double post_ener_s = (e * pre_mass_s + ef_prev - ef) / post_mass_s;
-      double advec_vol_s = pre + vf_prev - vf;
-      rho_out[(int64_t)(r - 1) * d.pitch + i] = post_mass_s / advec_vol_s;
-      e_out[(int64_t)(r - 1) * d.pitch + i] = post_ener_s;
-    }
-    mf_prev = mf;
-    ef_prev = ef;
-    vf_prev = vf;
-    rho_m2 = rho_m1;
-    rho_m1 = rho_0;
-    e_m2 = e_m1;
-    e_m1 = e_0;
-    // next face r + 1 needs cell r + 1 (unclamped: rows <= ny + 3 exist)
-    if (r + 1 <= j1) {
-      rho_0 = AT(CLV_DENSITY1, i, r + 1);
-      e_0 = AT(CLV_ENERGY1, i, r + 1);
-    }
-    pre_m1 = pre_0;
-    vd_m1 = vd_face;
-  }
-}
-
-template <int SWEEP>
-__global__ __launch_bounds__(256) void cell_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out) {
-  int bx, by;
-  xcd_tile(bx, by);
-  const int j0 = 2 + seg * by;
-  cell_y_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 2), nstrips, rho_out, e_out);
-}
-
 // ---- advec_mom --------------------------------------------------------------------------
 // Limited momentum flux, src/serial/advec_mom.cpp:114-139 (x) / :186-211 (y).
 __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor, double width, double width_dif, double v_uw,
@@ -602,68 +470,6 @@ __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor
   }
   double advec_vel_s = v_dn + (1.0 - sigma) * limiter;
   return advec_vel_s * flux;
-}
-
-// x direction (mom_sweep 1 = x first, 3 = x second).  Lanes: node column x0 + L,
-// x0 = 60 * strip; node velocities updated on lanes 2..61.
-// node rows [j0, j1)
-template <int MOM_SWEEP, class V, class O>
-__device__ __forceinline__ void mom_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O u_out, O v_out) {
-  const int lane = threadIdx.x & 63;
-  const int strip = bx * WAVES + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
-  const int nx = d.nx;
-  const int x0 = 60 * strip;
-  const int i = x0 + lane;
-  const int ic = i < nx + 4 ? i : nx + 4;
-  const bool own = lane >= 2 && lane <= 61 && i <= nx + 2;
-  const double width = M(CLV_CELLDX, ic);
-  const double w_m1 = up1(width), w_p1 = dn1(width);
-  // cell row c products density1 * post_vol (src/serial/advec_mom.cpp:98-99)
-  auto mass_of = [&](int c) {
-    double post;
-    if (MOM_SWEEP == 1) post = AT(CLV_VOLUME, ic, c) + AT(CLV_VOL_FLUX_Y, ic, c + 1) - AT(CLV_VOL_FLUX_Y, ic, c);  // :47
-    else post = AT(CLV_VOLUME, ic, c);                                                                               // :65
-    return AT(CLV_DENSITY1, ic, c) * post;
-  };
-  double mp_prev = mass_of(j0 - 1);
-  double mfx_prev = AT(CLV_MASS_FLUX_X, ic, j0 - 1);
-  for (int r = j0; r < j1; ++r) {
-    const double mp = mass_of(r);
-    const double mfx = AT(CLV_MASS_FLUX_X, ic, r);
-    // node_flux (:88), node_mass_post / pre (:98-100)
-    const double node_flux = 0.25 * (mfx_prev + mfx + dn1(mfx_prev) + dn1(mfx));
-    const double node_mass_post = 0.25 * (mp_prev + mp + up1(mp_prev) + up1(mp));
-    const double node_mass_pre = node_mass_post - up1(node_flux) + node_flux;
-    const double pre_e = dn1(node_mass_pre);
-    const double u = AT(CLV_XVEL1, ic, r), v = AT(CLV_YVEL1, ic, r);
-    const double u_w = up1(u), u_e = dn1(u), u_ee = dn2(u);
-    const double v_w = up1(v), v_e = dn1(v), v_ee = dn2(v);
-    double mu, mv;
-    if (node_flux < 0.0) {  // upwind i+2, donor i+1, downwind i, dif = donor
-      mu = mom_flux_of(node_flux, pre_e, width, w_p1, u_ee, u_e, u);
-      mv = mom_flux_of(node_flux, pre_e, width, w_p1, v_ee, v_e, v);
-    } else {  // upwind i-1, donor i, downwind i+1, dif = upwind
-      mu = mom_flux_of(node_flux, node_mass_pre, width, w_m1, u_w, u, u_e);
-      mv = mom_flux_of(node_flux, node_mass_pre, width, w_m1, v_w, v, v_e);
-    }
-    const double mu_w = up1(mu), mv_w = up1(mv);
-    // :149
-    if (own) {
-      u_out[(int64_t)r * d.pitch + i] = (u * node_mass_pre + mu_w - mu) / node_mass_post;
-      v_out[(int64_t)r * d.pitch + i] = (v * node_mass_pre + mv_w - mv) / node_mass_post;
-    }
-    mp_prev = mp;
-    mfx_prev = mfx;
-  }
-}
-
-template <int MOM_SWEEP>
-__global__ __launch_bounds__(256) void mom_x_k(Dev d, int nstrips, int seg, double *u_out, double *v_out) {
-  int bx, by;
-  xcd_tile(bx, by);
-  const int j0 = 2 + seg * by;
-  mom_x_tile<MOM_SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, u_out, v_out);
 }
 
 // x direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
@@ -699,8 +505,9 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
   const double vd_p1 = i + 1 <= nx + 2 ? vd_e : vd_face;
   const double width = M(CLV_CELLDX, ic);
   const double w_m1 = up1(width), w_p1 = dn1(width);
-  // advec_cell of cell row `row` at this lane (cell_x_tile's arithmetic); returns the
-  // new density and energy, the face mass flux and the loads advec_mom's mass reuses
+  // advec_cell of cell row `row` at this lane (src/serial/advec_cell.cpp:46-57, 71-125);
+  // returns the new density and energy, the face mass flux and the loads advec_mom's
+  // mass reuses
   struct CellRow {
     double rho, e, mf, vol, vfy0, vfy1;
   };
@@ -752,7 +559,10 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
     else post = rc == r ? cr.vol : AT(CLV_VOLUME, ic, r);
     const double mp = rho_h * post;
     if (r >= j0) {
-      // node row r: mom_x_tile's arithmetic on the values in registers
+      // node row r (src/serial/advec_mom.cpp:88-149) on the values in registers:
+      // node_flux, node_mass_post / pre, the limited momentum flux of node face i
+      // (upwind i+2, donor i+1, downwind i when node_flux < 0, else upwind i-1, donor i,
+      // downwind i+1) and the velocity update
       const double node_flux = 0.25 * (mfx_prev + mfx + dn1(mfx_prev) + dn1(mfx));
       const double node_mass_post = 0.25 * (mp_prev + mp + up1(mp_prev) + up1(mp));
       const double node_mass_pre = node_mass_post - up1(node_flux) + node_flux;
@@ -787,99 +597,14 @@ __global__ __launch_bounds__(256) void cm_x_k(Dev d, int nstrips, int seg, doubl
   cm_x_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
-// y direction (mom_sweep 2 = y first, 4 = y second).  Lanes: node column
-// x0 + L, x0 = 2 + 64 * strip; the cells and faces at column i - 1 are loaded
-// directly, so every lane is independent.  Iteration k computes node_flux and
-// the node masses of node row k, the momentum flux of face f = k - 1 and the
-// update of node row k - 1 (src/serial/advec_mom.cpp:158-222):
-//   nf(n)      = 0.25 (mfy(i-1,n) + mfy(i,n) + mfy(i-1,n+1) + mfy(i,n+1))
-//   nm_post(n) = 0.25 (mo(n-1) + mo(n) + mw(n-1) + mw(n)), mo/mw = density1 * post_vol
-//                at cell (i, .) / (i - 1, .)
-//   nm_pre(n)  = nm_post(n) - nf(n-1) + nf(n)
-//   mom_flux(f) from nf(f), nm_pre(donor), vel(f-1 .. f+2)
-//   vel(n)     = (vel(n) nm_pre(n) + mom_flux(n-1) - mom_flux(n)) / nm_post(n)
-// node rows updated [j0, j1)
-template <int MOM_SWEEP, class V, class O>
-__device__ __forceinline__ void mom_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O u_out, O v_out) {
-  const int lane = threadIdx.x & 63;
-  const int strip = bx * WAVES + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
-  const int nx = d.nx;
-  const int i = 2 + 64 * strip + lane;
-  if (i > nx + 2) return;  // per-lane kernel
-  auto post_of = [&](int col, int c) {
-    if (MOM_SWEEP == 2) return AT(CLV_VOLUME, col, c) + AT(CLV_VOL_FLUX_X, col + 1, c) - AT(CLV_VOL_FLUX_X, col, c);  // :56
-    return AT(CLV_VOLUME, col, c);                                                                                    // :74
-  };
-  auto node_flux_of = [&](int n) {  // :160
-    return 0.25 * (AT(CLV_MASS_FLUX_Y, i - 1, n) + AT(CLV_MASS_FLUX_Y, i, n) + AT(CLV_MASS_FLUX_Y, i - 1, n + 1) + AT(CLV_MASS_FLUX_Y, i, n + 1));
-  };
-  // state entering iteration k: nf(k-1), mo/mw(k-1), nm_pre(k-1), nm_post(k-1),
-  // mom_flux(k-2), velocities of rows k-2, k-1, k
-  int k = j0 - 1;
-  double nf_prev = node_flux_of(k - 1);
-  double mo_prev = AT(CLV_DENSITY1, i, k - 1) * post_of(i, k - 1);
-  double mw_prev = AT(CLV_DENSITY1, i - 1, k - 1) * post_of(i - 1, k - 1);
-  double pre_prev = 0.0, post_prev = 0.0, mu_prev = 0.0, mv_prev = 0.0;
-  double u_m2 = 0.0, v_m2 = 0.0;
-  double u_m1 = AT(CLV_XVEL1, i, k - 1), v_m1 = AT(CLV_YVEL1, i, k - 1);
-  double u_0 = AT(CLV_XVEL1, i, k), v_0 = AT(CLV_YVEL1, i, k);
-  for (; k <= j1; ++k) {
-    const double nf = node_flux_of(k);
-    const double mo = AT(CLV_DENSITY1, i, k) * post_of(i, k);
-    const double mw = AT(CLV_DENSITY1, i - 1, k) * post_of(i - 1, k);
-    const double nm_post = 0.25 * (mo_prev + mo + mw_prev + mw);  // :170-171
-    const double nm_pre = nm_post - nf_prev + nf;                 // :172
-    const double u_p1 = AT(CLV_XVEL1, i, k + 1), v_p1 = AT(CLV_YVEL1, i, k + 1);
-    if (k >= j0) {
-      // momentum flux of face f = k - 1: vel(f-1) = row k-2 .. vel(f+2) = row k+1
-      const int f = k - 1;
-      const double wd = M(CLV_CELLDY, f);
-      double mu, mv;
-      if (nf_prev < 0.0) {  // upwind f+2, donor f+1, downwind f, dif = donor (:186-191)
-        const double wdif = M(CLV_CELLDY, f + 1);
-        mu = mom_flux_of(nf_prev, nm_pre, wd, wdif, u_p1, u_0, u_m1);
-        mv = mom_flux_of(nf_prev, nm_pre, wd, wdif, v_p1, v_0, v_m1);
-      } else {  // upwind f-1, donor f, downwind f+1, dif = upwind (:192-196)
-        const double wdif = M(CLV_CELLDY, f - 1);
-        mu = mom_flux_of(nf_prev, pre_prev, wd, wdif, u_m2, u_m1, u_0);
-        mv = mom_flux_of(nf_prev, pre_prev, wd, wdif, v_m2, v_m1, v_0);
-      }
-      if (k >= j0 + 1) {
-        // update node row f = k - 1 from faces f - 1 and f (:221)
-        u_out[(int64_t)f * d.pitch + i] = (u_m1 * pre_prev + mu_prev - mu) / post_prev;
-        v_out[(int64_t)f * d.pitch + i] = (v_m1 * pre_prev + mv_prev - mv) / post_prev;
-      }
-      mu_prev = mu;
-      mv_prev = mv;
-    }
-    nf_prev = nf;
-    mo_prev = mo;
-    mw_prev = mw;
-    pre_prev = nm_pre;
-    post_prev = nm_post;
-    u_m2 = u_m1;
-    u_m1 = u_0;
-    u_0 = u_p1;
-    v_m2 = v_m1;
-    v_m1 = v_0;
-    v_0 = v_p1;
-  }
-}
-
-template <int MOM_SWEEP>
-__global__ __launch_bounds__(256) void mom_y_k(Dev d, int nstrips, int seg, double *u_out, double *v_out) {
-  int bx, by;
-  xcd_tile(bx, by);
-  const int j0 = 2 + seg * by;
-  mom_y_tile<MOM_SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, u_out, v_out);
-}
-
 // y direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
 // with mom_sweep 2, cell sweep 2 with mom_sweep 4).  Lanes: column 62 * strip + L;
 // nodes, cells and faces are owned on lanes 2..63, and lanes 0 and 1 supply the two
-// columns left of them.  Each lane walks its column: the cell part (cell_y_tile's
-// arithmetic) runs four faces ahead of the mom part (mom_y_tile's), and keeps the
+// columns left of them.  Each lane walks its column: the cell part (face r needs cells
+// r-2 .. r+1; cell r - 1 is updated from faces r - 1 and r) runs four faces ahead of
+// the mom part (node row k: node_flux(k) = 0.25 (mfy(i-1,k) + mfy(i,k) + mfy(i-1,k+1) +
+// mfy(i,k+1)), the masses of cells (i-1 .. i, k-1 .. k), the flux of face k - 1 and the
+// update of node row k - 1; src/serial/advec_mom.cpp:158-222), and keeps the
 // advected density and the mass fluxes of the last rows in registers.  advec_mom reads
 // them through the halo update between the two sweeps (driver/advection.cpp:42-46,
 // src/serial/update_halo.cpp): density1 rows 1 - k <- 2 + k, ny + 2 + k <- ny + 1 - k;
@@ -952,7 +677,8 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   double vd_m1 = M(CLV_VERTEXDY, Fs - 1);
   double mf_prev = 0.0, ef_prev = 0.0, vf_prev = 0.0;
   CmHist h{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  // mom part state (mom_y_tile's)
+  // mom part state: node flux and masses of the previous row, the previous face's
+  // momentum fluxes, velocities of rows k-2 .. k
   double nf_prev = 0.0, mo_prev = 0.0, mw_prev = 0.0, pre_prev = 0.0, post_prev = 0.0, mu_prev = 0.0, mv_prev = 0.0;
   double u_m2 = 0.0, v_m2 = 0.0, u_m1 = 0.0, v_m1 = 0.0, u_0 = 0.0, v_0 = 0.0;
   // node row k's flux and masses from the history (src/serial/advec_mom.cpp:160, :170-171);
@@ -1012,7 +738,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
     p2 = p3;
     p3 = p4;
     p4 = post_h;
-    // mom part at node row k = F - 4 (mom_y_tile's iteration k)
+    // mom part at node row k = F - 4
     const int k = F - 4;
     if (k == j0 - 2) {
       mom_row(k, nf_prev, mo_prev, mw_prev);
@@ -1089,15 +815,13 @@ __device__ __forceinline__ ItemRange batch_items(int n) {
   return {lo + k, hi, nbx};
 }
 
-// strips of each tile family (owned columns per wave: 62, 61, 60, 64)
+// strips of each tile family (owned columns per wave: 62, 61, 57, 62)
 __device__ __forceinline__ int strips_of(int kind, int nx) {
   switch (kind) {
     case BK_TIMESTEP: return (nx + 61) / 62;
     case BK_LAGRANGE: return (nx + 60) / 61;
-    case BK_CELL_X1: case BK_CELL_X2: return (nx + 59) / 60;
-    case BK_CELL_Y1: case BK_CELL_Y2: return (nx + 63) / 64;
-    case BK_MOM_X1: case BK_MOM_X3: return (nx + 1 + 59) / 60;
-    default: return (nx + 1 + 63) / 64;  // mom y
+    case BK_CM_X1: case BK_CM_X2: return (nx + 1 + 56) / 57;
+    default: return (nx + 1 + 61) / 62;  // cm y
   }
 }
 
@@ -1168,16 +892,11 @@ __global__ __launch_bounds__(256) void batch_advec_k(const Dev *__restrict__ tab
     if (!ps[it.p].run) continue;
     const GDev d = global_view(tab[it.p]);
     const int ns = strips_of(KIND, d.nx);
-    const int j0 = 2 + (int)it.k * it.seg;
-    const int jc1 = min(j0 + (int)it.seg, d.ny + 2), jn1 = min(j0 + (int)it.seg, d.ny + 3);
-    if constexpr (KIND == BK_CELL_X1) cell_x_tile<1>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
-    if constexpr (KIND == BK_CELL_X2) cell_x_tile<2>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
-    if constexpr (KIND == BK_CELL_Y1) cell_y_tile<1>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
-    if constexpr (KIND == BK_CELL_Y2) cell_y_tile<2>(d, it.bx, j0, jc1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2]);
-    if constexpr (KIND == BK_MOM_X1) mom_x_tile<1>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_MOM_X3) mom_x_tile<3>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_MOM_Y2) mom_y_tile<2>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_MOM_Y4) mom_y_tile<4>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    const int j0 = 2 + (int)it.k * it.seg, jn1 = min(j0 + (int)it.seg, d.ny + 3);
+    if constexpr (KIND == BK_CM_X1) cm_x_tile<1>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_X2) cm_x_tile<2>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y1) cm_y_tile<1>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y2) cm_y_tile<2>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
   }
 }
 
@@ -1229,14 +948,10 @@ hipError_t batch_launch_tiles(int kind, hipStream_t s, int grid, const Dev *tab,
   switch (kind) {
     case BK_TIMESTEP: hipLaunchKernelGGL(batch_timestep_k, g, blk, 0, s, tab, pc, ps, items, nitems); break;
     case BK_LAGRANGE: hipLaunchKernelGGL(batch_lagrange_k, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CELL_X1: hipLaunchKernelGGL(batch_advec_k<BK_CELL_X1>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CELL_X2: hipLaunchKernelGGL(batch_advec_k<BK_CELL_X2>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CELL_Y1: hipLaunchKernelGGL(batch_advec_k<BK_CELL_Y1>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CELL_Y2: hipLaunchKernelGGL(batch_advec_k<BK_CELL_Y2>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_MOM_X1: hipLaunchKernelGGL(batch_advec_k<BK_MOM_X1>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_MOM_X3: hipLaunchKernelGGL(batch_advec_k<BK_MOM_X3>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_MOM_Y2: hipLaunchKernelGGL(batch_advec_k<BK_MOM_Y2>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_MOM_Y4: hipLaunchKernelGGL(batch_advec_k<BK_MOM_Y4>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_X1: hipLaunchKernelGGL(batch_advec_k<BK_CM_X1>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_X2: hipLaunchKernelGGL(batch_advec_k<BK_CM_X2>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_Y1: hipLaunchKernelGGL(batch_advec_k<BK_CM_Y1>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_Y2: hipLaunchKernelGGL(batch_advec_k<BK_CM_Y2>, g, blk, 0, s, tab, ps, items, nitems); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1348,45 +1063,6 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
   return CLV_OK;
 }
 
-static clv_status cell_sweep(clv_chunk *c, int sweep, int dir) {
-  const Dev &d = c->d;
-  double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2];
-  if (dir == CLV_XDIR) {
-    const int ns = ceil_div(d.nx, 60), seg = c->seg[SF_CELL_X];
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, seg), 1);
-    if (sweep == 1) LAUNCH_F("f_cell_x1", cell_x_k<1>, g, d, ns, seg, rho_out, e_out);
-    else LAUNCH_F("f_cell_x2", cell_x_k<2>, g, d, ns, seg, rho_out, e_out);
-  } else {
-    const int ns = ceil_div(d.nx, 64), seg = c->seg[SF_CELL_Y];
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny, seg), 1);
-    if (sweep == 1) LAUNCH_F("f_cell_y1", cell_y_k<1>, g, d, ns, seg, rho_out, e_out);
-    else LAUNCH_F("f_cell_y2", cell_y_k<2>, g, d, ns, seg, rho_out, e_out);
-  }
-  swap_fields(c, CLV_DENSITY1, CLV_WORK1);
-  swap_fields(c, CLV_ENERGY1, CLV_WORK2);
-  return CLV_OK;
-}
-
-static clv_status mom_sweep(clv_chunk *c, int sweep, int dir) {
-  const Dev &d = c->d;
-  double *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
-  const int mom = dir + 2 * (sweep - 1);  // src/serial/advec_mom.cpp:37
-  if (dir == CLV_XDIR) {
-    const int ns = ceil_div(d.nx + 1, 60), seg = c->seg[SF_MOM_X];
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
-    if (mom == 1) LAUNCH_F("f_mom_x1", mom_x_k<1>, g, d, ns, seg, u_out, v_out);
-    else LAUNCH_F("f_mom_x2", mom_x_k<3>, g, d, ns, seg, u_out, v_out);
-  } else {
-    const int ns = ceil_div(d.nx + 1, 64), seg = c->seg[SF_MOM_Y];
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
-    if (mom == 2) LAUNCH_F("f_mom_y1", mom_y_k<2>, g, d, ns, seg, u_out, v_out);
-    else LAUNCH_F("f_mom_y2", mom_y_k<4>, g, d, ns, seg, u_out, v_out);
-  }
-  swap_fields(c, CLV_XVEL1, CLV_WORK3);
-  swap_fields(c, CLV_YVEL1, CLV_WORK4);
-  return CLV_OK;
-}
-
 // advec_cell and advec_mom of one sweep in one launch (cm_x_k, cm_y_k); the halo
 // update the driver runs between the two (driver/advection.cpp:42-46) is split: the
 // velocities' before (the cell sweep does not change them), the rest after (the mom
@@ -1409,7 +1085,7 @@ static clv_status cm_sweep(clv_chunk *c, int sweep, int dir) {
   swap_fields(c, CLV_ENERGY1, CLV_WORK2);
   swap_fields(c, CLV_XVEL1, CLV_WORK3);
   swap_fields(c, CLV_YVEL1, CLV_WORK4);
-  return halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2);
+  return CLV_OK;
 }
 
 // driver/advection.cpp:29-105 followed by reset_field (src/serial/reset_field.cpp)
@@ -1419,8 +1095,10 @@ clv_status fused_advection(clv_chunk *c, int advect_x) {
   const int dir1 = advect_x ? CLV_XDIR : CLV_YDIR, dir2 = advect_x ? CLV_YDIR : CLV_XDIR;
   if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
   if ((st = cm_sweep(c, 1, dir1)) != CLV_OK) return st;
-  if ((st = halo(c, {CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
+  // the first pass's halo after it and the second pass's velocity halo before it
+  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
   if ((st = cm_sweep(c, 2, dir2)) != CLV_OK) return st;
+  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
   // reset_field: the end-of-step fields become the start-of-step fields
   swap_fields(c, CLV_DENSITY0, CLV_DENSITY1);
   swap_fields(c, CLV_ENERGY0, CLV_ENERGY1);

@@ -284,11 +284,9 @@ clv_status clv_hydro_summary(clv_chunk *c, double out[5]);
  * fields and dt sequence are bit-identical to stepping it alone. */
 typedef struct clv_batch clv_batch;
 
-/* Tile families of a batched step (one per fused kernel; sweep variants share). */
-enum {
-  CLV_BF_TIMESTEP = 0, CLV_BF_LAGRANGE, CLV_BF_CELL_X, CLV_BF_CELL_Y, CLV_BF_MOM_X, CLV_BF_MOM_Y,
-  CLV_BF_COUNT
-};
+/* Tile families of a batched step (one per fused kernel; the advection passes
+ * advec_cell + advec_mom in x and in y; sweep variants share). */
+enum { CLV_BF_TIMESTEP = 0, CLV_BF_LAGRANGE, CLV_BF_CM_X, CLV_BF_CM_Y, CLV_BF_COUNT };
 enum { CLV_BATCH_ORDER_GIVEN = 0, CLV_BATCH_ORDER_LARGEST_FIRST = 1 };
 /* Work schedule of one family: rows per tile (8 ... 4096), persistent grid in
  * blocks per CU (0: one block per tile) and the order of members in the tile list. */
