@@ -649,6 +649,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   const bool own_cell = own && i <= nx + 1, own_node = own && i <= nx + 2;
   const int lane_c = lane + (i == 1 ? 1 : (i == nx + 2 ? -1 : 0));
   const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
+  const int Fs = j0 - 2 > 2 ? j0 - 2 : 2, Fe = min(j1 + 1, ny + 2);  // faces the cell part walks
   // pre_vol of cell row `row` (src/serial/advec_cell.cpp:139 / :149) from its bottom and
   // top vol_flux_y, and advec_mom's post_vol (src/serial/advec_mom.cpp:56 / :74)
   auto pre_of = [&](int row, double vfy0, double vfy1, double vol, double vfx0, double vfx1) {
@@ -666,8 +667,29 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
     post = post_from(vol, vfx0, vfx1);
     return pre_of(row, vfy0, vfy1, vol, vfx0, vfx1);
   };
+  // the loads of face F's step, issued one step ahead (before the step in hand's
+  // stores, which the compiler cannot move loads past): the walk is latency-bound
+  struct CellLd {
+    double vol, vfy1, vfx0, vfx1, vd, rho_p1, e_p1, vd_p1;
+  };
+  auto cell_ld = [&](int F) {
+    double vol = 0.0, vfy1 = 0.0, vfx0 = 0.0, vfx1 = 0.0, vd = 0.0, rho_p1 = 0.0, e_p1 = 0.0, vd_p1 = 0.0;
+    if (F <= Fe) {
+      const int rp = F + 1 <= ny + 2 ? F + 1 : ny + 2;  // src/serial/advec_cell.cpp:169
+      vol = AT(CLV_VOLUME, ic, F);
+      vfy1 = AT(CLV_VOL_FLUX_Y, ic, F + 1);
+      if (SWEEP == 1) {
+        vfx0 = AT(CLV_VOL_FLUX_X, ic, F);
+        vfx1 = AT(CLV_VOL_FLUX_X, ic + 1, F);
+      }
+      vd = M(CLV_VERTEXDY, F);
+      rho_p1 = AT(CLV_DENSITY1, ic, rp);
+      e_p1 = AT(CLV_ENERGY1, ic, rp);
+      vd_p1 = M(CLV_VERTEXDY, rp);
+    }
+    return CellLd{vol, vfy1, vfx0, vfx1, vd, rho_p1, e_p1, vd_p1};
+  };
   // cell part: faces Fs .. Fe (cell F - 1 is updated at face F)
-  const int Fs = j0 - 2 > 2 ? j0 - 2 : 2, Fe = min(j1 + 1, ny + 2);
   double rho_m2 = AT(CLV_DENSITY1, ic, Fs - 2), rho_m1 = AT(CLV_DENSITY1, ic, Fs - 1), rho_0 = AT(CLV_DENSITY1, ic, Fs);
   double e_m2 = AT(CLV_ENERGY1, ic, Fs - 2), e_m1 = AT(CLV_ENERGY1, ic, Fs - 1), e_0 = AT(CLV_ENERGY1, ic, Fs);
   double vfy_c = AT(CLV_VOL_FLUX_Y, ic, Fs - 1), post_unused;  // vol_flux_y of the face below the cell in hand
@@ -692,15 +714,23 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
     mo = h.rho(cm_y_rho_slot(k, ny)) * post;
     mw = up1(mo);
   };
+  CellLd cur = cell_ld(Fs);
+  double u_ahead = 0.0, v_ahead = 0.0;  // velocities of row k + 1 for the next mom step
   for (int F = Fs; F <= j1 + 4; ++F) {
+    const CellLd nxt = cell_ld(F + 1);
+    // the next mom step (k + 1 = F - 3) reads velocity row F - 2
+    double u_n = 0.0, v_n = 0.0;
+    if (F - 3 >= j0 - 1 && F - 3 <= j1) {
+      u_n = AT(CLV_XVEL1, ic, F - 2);
+      v_n = AT(CLV_YVEL1, ic, F - 2);
+    }
     double mf_h = 0.0, rho_h = 0.0, post_h = 0.0;
     if (F <= Fe) {
       const double vf = vfy_c;
-      const double pre_0 = pre_load(F, vf, vfy_c, post_h);
-      const double vd_face = M(CLV_VERTEXDY, F);
-      const int rp = F + 1 <= ny + 2 ? F + 1 : ny + 2;  // src/serial/advec_cell.cpp:169
-      const double rho_p1 = AT(CLV_DENSITY1, ic, rp), e_p1 = AT(CLV_ENERGY1, ic, rp);
-      const double vd_p1 = M(CLV_VERTEXDY, rp);
+      vfy_c = cur.vfy1;
+      post_h = post_from(cur.vol, cur.vfx0, cur.vfx1);
+      const double pre_0 = pre_of(F, vf, cur.vfy1, cur.vol, cur.vfx0, cur.vfx1);
+      const double vd_face = cur.vd, rho_p1 = cur.rho_p1, e_p1 = cur.e_p1, vd_p1 = cur.vd_p1;
       double mf, ef;
       cell_face_flux(vf, pre_m1, pre_0, rho_m2, rho_m1, rho_0, rho_p1, e_m2, e_m1, e_0, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
       if (own_cell && F >= j0 && F < j1) AT(CLV_MASS_FLUX_Y, i, F) = mf;
@@ -751,7 +781,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       mom_row(k, nf, mo, mw);
       const double nm_post = 0.25 * (mo_prev + mo + mw_prev + mw);  // :170-171
       const double nm_pre = nm_post - nf_prev + nf;                 // :172
-      const double u_p1 = AT(CLV_XVEL1, ic, k + 1), v_p1 = AT(CLV_YVEL1, ic, k + 1);
+      const double u_p1 = u_ahead, v_p1 = v_ahead;  // row k + 1
       if (k >= j0) {
         // momentum flux of face f = k - 1
         const int f = k - 1;
@@ -786,6 +816,9 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       v_m1 = v_0;
       v_0 = v_p1;
     }
+    cur = nxt;
+    u_ahead = u_n;
+    v_ahead = v_n;
   }
 }
 
