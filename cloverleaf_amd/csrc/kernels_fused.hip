@@ -81,14 +81,20 @@ __device__ __forceinline__ int mirror_cell(int i, int n) { return i < 2 ? 3 - i 
 // strips and row segments -- which re-read each other's edge columns and rows --
 // in one L2.  The map is a bijection on [0, n).
 constexpr int NUM_XCD = 8;
+template <bool Y_FASTEST = false>
 __device__ __forceinline__ void xcd_tile(int &bx, int &by) {
   const int n = gridDim.x * gridDim.y;
   const int b = blockIdx.y * gridDim.x + blockIdx.x;
   const int xcd = b % NUM_XCD, k = b / NUM_XCD;
   const int per = n / NUM_XCD, rem = n % NUM_XCD;
   const int t = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
-  bx = t % gridDim.x;
-  by = t / gridDim.x;
+  if (Y_FASTEST) {
+    by = t % gridDim.y;
+    bx = t / gridDim.y;
+  } else {
+    bx = t % gridDim.x;
+    by = t / gridDim.x;
+  }
 }
 
 // ---- timestep: ideal_gas + viscosity + calc_dt ----------------------------------------
