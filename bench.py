@@ -9,7 +9,10 @@ time (driver/hydro.cpp:213-217).
 
 Prints ONE JSON line.  `roofline` is the dominant kernel (largest share of the
 timed region by HIP events on the chunk's stream): achieved = its algorithmic
-bytes per launch / its mean launch duration, against 8 TB/s.  `cpu_baseline`
+bytes per launch / its mean launch duration, against 8 TB/s.  Unless --no-probe,
+`pattern_ceiling` times each fused kernel's access pattern without arithmetic
+(clv_march_probe: the same march, arrays read and written), the ceiling of that
+kernel on this GPU.  `cpu_baseline`
 times the OpenMP CPU oracle (oracle/liboracle.so) on a bounded sample of the
 same problem.
 
@@ -103,6 +106,10 @@ ALG_TERMS = {
 # first advection pass 12 + the second 11
 B_FUSED_PER_CELL_STEP = 45 * 8.0
 B_ALG_PER_CELL_STEP = 1440.0  # 180 array sweeps of 8 B per interior cell-step at reference kernel granularity
+
+
+# arrays read and written per interior cell by each fused kernel (the march probe's shape)
+PATTERN = {"f_timestep": (7, 1), "f_lagrange": (8, 6), "f_cm_x1": (7, 5), "f_cm_y1": (7, 5), "f_cm_x2": (6, 5), "f_cm_y2": (6, 5)}
 
 
 def alg_bytes(name, nx, ny):
@@ -465,6 +472,19 @@ def main():
         out["hbm_copy_GBps_measured"] = clv.bandwidth_probe(4 << 30, 20)
         if achieved:
             out["roofline"]["frac_of_measured_copy"] = achieved / out["hbm_copy_GBps_measured"]
+        if args.plan == "fused":
+            # the ceiling of each fused kernel's access pattern: the same march over the
+            # same grid with its arrays read and written and no arithmetic
+            ceil = {}
+            for name, (r, w) in PATTERN.items():
+                if name in per_kernel:
+                    probe_ms = clv.march_probe(nx, ny, r, w, 16, 10)
+                    ceil[name] = {"arrays_read": r, "arrays_written": w, "probe_ms": round(probe_ms, 3),
+                                  "kernel_ms": round(per_kernel[name]["avg_us"] / 1e3, 3),
+                                  "kernel_over_probe": round(per_kernel[name]["avg_us"] / 1e3 / probe_ms, 3)}
+            out["pattern_ceiling"] = ceil
+            if dom in ceil:
+                out["roofline"]["frac_of_pattern_ceiling"] = ceil[dom]["probe_ms"] / ceil[dom]["kernel_ms"]
     if not args.no_batch:
         print("# bench: batched workload ...", file=sys.stderr, flush=True)
         out["batched"] = batch_leg(clv)

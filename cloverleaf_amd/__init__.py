@@ -148,6 +148,7 @@ SIGNATURES = [
     ("clv_profile_read", _I, [_P, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), _PD, _I]),
     ("clv_profile_reset", _I, [_P]),
     ("clv_bandwidth_probe", _I, [C.c_size_t, _I, _PD]),
+    ("clv_march_probe", _I, [_I, _I, _I, _I, _I, _I, _PD]),
     ("clv_device_alloc", _I, [C.c_size_t, C.POINTER(_P)]),
     ("clv_device_free", _I, [_P]),
     ("clv_copy_to_host", _I, [_P, _P, C.c_size_t]),
@@ -186,6 +187,15 @@ def bandwidth_probe(nbytes: int = 4 << 30, iters: int = 20) -> float:
     """Copy bandwidth (GB/s, read + write) of the current GPU's HBM."""
     out = C.c_double()
     _check(lib().clv_bandwidth_probe(int(nbytes), int(iters), C.byref(out)), "clv_bandwidth_probe")
+    return out.value
+
+
+def march_probe(nx: int, ny: int, reads: int, writes: int, seg: int = 16, iters: int = 10) -> float:
+    """ms per launch of the fused kernels' march over an nx x ny grid reading `reads`
+    and writing `writes` arrays, no arithmetic: the ceiling of a fused kernel with
+    those arrays."""
+    out = C.c_double()
+    _check(lib().clv_march_probe(int(nx), int(ny), int(reads), int(writes), int(seg), int(iters), C.byref(out)), "clv_march_probe")
     return out.value
 
 

@@ -59,13 +59,13 @@ Dev after_pass(Dev d) {
 bool same_view(const Dev &a, const Dev &b) { return std::memcmp(&a, &b, sizeof(Dev)) == 0; }
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
-// host mirror of the device strips_of (owned columns per wave: 62, 61, 57, 62)
+// host mirror of the device strips_of
 int strips(int fam, int nx) {
   switch (fam) {
-    case CLV_BF_TIMESTEP: return ceil_div(nx, 62);
-    case CLV_BF_LAGRANGE: return ceil_div(nx, 61);
-    case CLV_BF_CM_X: return ceil_div(nx + 1, 57);
-    default: return ceil_div(nx + 1, 62);
+    case CLV_BF_TIMESTEP: return ceil_div(nx, kOwnTimestep);
+    case CLV_BF_LAGRANGE: return ceil_div(nx, kOwnLagrange);
+    case CLV_BF_CM_X: return ceil_div(nx + 1, kOwnCmX);
+    default: return ceil_div(nx + 1, kOwnCmY);
   }
 }
 // the advection passes' tiles cover node rows

@@ -97,6 +97,10 @@ inline void dt_location(int nx, int *control, int *jldt, int *kldt) {
   *kldt = (int)(1.f + (jk_control / nx));
 }
 
+// Columns each wave of a fused kernel owns (computes and stores).  A wave's 64
+// lanes hold its owned columns plus the neighbours their stencils need.
+constexpr int kOwnTimestep = 62, kOwnLagrange = 60, kOwnCmX = 56, kOwnCmY = 62;
+
 // ---- batched form (kernels_fused.hip, kernels_misc.hip, batch.cpp) ------------------------
 // Tile kernels of a batched step, one per fused kernel and sweep variant (the
 // advection passes: cell + mom of sweep 1 / 2 in x / y).

@@ -97,6 +97,32 @@ __device__ __forceinline__ void xcd_tile(int &bx, int &by) {
   }
 }
 
+// ---- block-wide row stores ---------------------------------------------------------------
+// A 128-byte line written partly by one wave and partly by another costs HBM time
+// (a march that writes 6 arrays with line-misaligned waves runs 14% longer than
+// the same march aligned, at 15360^2; misaligned reads cost nothing).  So a
+// block's four strips of S owned columns each own the consecutive columns
+// [X, X + 4 S), X = 2 + 4 S bx, 4 S a multiple of 16: with element (2, j) on a
+// line boundary these are whole lines.  The waves put their owned values of a
+// row at LDS slot column - X; after a barrier thread t stores slot t to column
+// X + t, every store instruction covering whole lines.  With EXTRA_NODE (lagrange:
+// strips cover the cells, the last strip's lane S + 1 owns the boundary node column
+// nx + 2) that column can land on slot 4 S.
+template <int S, bool EXTRA_NODE = false>
+struct BlockRows {
+  static_assert((WAVES * S) % 16 == 0, "a block's columns must be whole 128-byte lines");
+  static constexpr int kSlots = WAVES * 64;
+  int x_first, nx;
+  __device__ BlockRows(int bx, int nx_) : x_first(2 + WAVES * S * bx), nx(nx_) {}
+  __device__ int pos(int i) const { return i - x_first; }  // owned lanes: [0, 4 S]
+  __device__ int column(int t) const { return x_first + t; }
+  // slot t holds an owned cell / node (x face) value of the row
+  __device__ bool cell(int t) const { return t < WAVES * S && x_first + t <= nx + 1; }
+  __device__ bool node(int t) const {
+    return (t < WAVES * S && x_first + t <= nx + 2) || (EXTRA_NODE && t == WAVES * S && x_first + t == nx + 2);
+  }
+};
+
 // ---- timestep: ideal_gas + viscosity + calc_dt ----------------------------------------
 // Lanes: column x0 + L, x0 = 1 + 62 * strip; lanes 1..62 own cells (p of the
 // neighbouring columns comes from lanes 0 and 63).
@@ -124,10 +150,10 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;  // whole wave
-  const int x0 = 1 + 62 * strip;
+  const int x0 = 1 + kOwnTimestep * strip;
   const int i = x0 + lane;
   const int ic = i < d.nx + 4 ? i : d.nx + 4;  // clamped load column (lanes past the chunk)
-  const bool own = lane >= 1 && lane <= 62 && i <= d.nx + 1;
+  const bool own = lane >= 1 && lane <= kOwnTimestep && i <= d.nx + 1;
 
   const double cdx = M(CLV_CELLDX, ic), cdx1 = M(CLV_CELLDX, ic + 1 <= d.nx + 4 ? ic + 1 : ic);
   // rho0/e0 of ghost cells through their mirrors (the halo of driver/timestep.cpp:53-60)
@@ -252,11 +278,12 @@ __global__ __launch_bounds__(1024) void reduce_min_big_k(const double *partials,
 }
 
 // ---- lagrange: PdV predictor chain, accelerate, PdV corrector, flux_calc ----------------
-// Lanes: column x0 + L, x0 = 1 + 61 * strip.  Cells are owned by lanes 1..61;
-// nodes by lanes 1..61 plus the boundary node column nx + 2 (lane <= 62).
-// March: cell/node row r from j0 - 1 (or 2) to j1; at row r the predictor
-// pressure p'(r), the node velocities of node row r, then the corrector of cell
-// row r - 1 and the fluxes.
+// Lanes: column x0 + L, x0 = 1 + kOwnLagrange * strip.  Cells are owned by lanes
+// 1..kOwnLagrange; nodes by the same lanes plus the boundary node column nx + 2
+// (lane kOwnLagrange + 1).  March: cell/node row r from j0 - 1 (or 2) to j1; at
+// row r the predictor pressure p'(r), the node velocities of node row r, then the
+// corrector of cell row r - 1 and the fluxes.  The block's results go out through
+// LDS (BlockRows): its four strips own the columns [X, X + 240), whole lines.
 struct PredIn {
   double xa0, xa1, ya0, ya1, u00, u01, u10, u11, v00, v01, v10, v11, vol, rho0, e0, p, q;
 };
@@ -282,13 +309,18 @@ template <class V>
 __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc1, double dt, int nstrips) {
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
+  // a strip past the chunk runs the march (every wave meets the block's barriers)
+  // with clamped loads and owns nothing
+  const bool live = strip < nstrips;
   const int nx = d.nx, ny = d.ny;
-  const int x0 = 1 + 61 * strip;
+  const int x0 = 1 + kOwnLagrange * strip;
   const int i = x0 + lane;
   const int ic = i < nx + 4 ? i : nx + 4;
-  const bool own_cell = lane >= 1 && lane <= 61 && i <= nx + 1;
-  const bool own_node = (lane >= 1 && lane <= 61 && i <= nx + 2) || (lane == 62 && i == nx + 2);
+  const bool own_cell = live && lane >= 1 && lane <= kOwnLagrange && i <= nx + 1;
+  const bool own_node = live && ((lane >= 1 && lane <= kOwnLagrange && i <= nx + 2) || (lane == kOwnLagrange + 1 && i == nx + 2));
+  __shared__ double stage[2][6][BlockRows<kOwnLagrange, true>::kSlots];
+  const BlockRows<kOwnLagrange, true> out(bx, nx);
+  const int pos = out.pos(i);
   const int jn1 = jc1 == ny + 2 ? ny + 3 : jc1;    // owned node rows [j0, jn1)
   const double halfdt = 0.5 * dt;                  // src/serial/accelerate.cpp:33
 
@@ -373,23 +405,41 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
         double volume_change_s = vol_prev / (vol_prev + total_flux);
         double recip_volume = 1.0 / vol_prev;
         double energy_change = (pp_prev / rho0_prev + q0_prev / rho0_prev) * total_flux * recip_volume;
-        if (own_cell && c < jc1) {
-          AT(CLV_ENERGY1, i, c) = e0_prev - energy_change;
-          AT(CLV_DENSITY1, i, c) = rho0_prev * volume_change_s;
+        if (own_cell) {
+          stage[r & 1][0][pos] = e0_prev - energy_change;
+          stage[r & 1][1][pos] = rho0_prev * volume_change_s;
         }
         // flux_calc, src/serial/flux_calc.cpp:39: x-face (i, r - 1)
-        if (own_node && c < jc1) AT(CLV_VOL_FLUX_X, i, c) = 0.25 * dt * xa_prev * (u0_prev + u_r + u1_prev + u1);
+        if (own_node) stage[r & 1][2][pos] = 0.25 * dt * xa_prev * (u0_prev + u_r + u1_prev + u1);
       }
       if (r < jn1) {
         if (own_node) {
-          AT(CLV_XVEL1, i, r) = u1;
-          AT(CLV_YVEL1, i, r) = v1;
+          stage[r & 1][3][pos] = u1;
+          stage[r & 1][4][pos] = v1;
         }
         // flux_calc, src/serial/flux_calc.cpp:40: y-face (i, r)
-        if (own_cell) AT(CLV_VOL_FLUX_Y, i, r) = 0.25 * dt * a.ya0 * (v_r + v0_e + v1 + v1_e);
+        if (own_cell) stage[r & 1][5][pos] = 0.25 * dt * a.ya0 * (v_r + v0_e + v1 + v1_e);
       }
       u1_prev = u1;
       v1_prev = v1;
+      // the block's rows (r, j0 uniform over the block)
+      __syncthreads();
+      const int t = threadIdx.x, xs = out.column(t);
+      if (r > j0) {
+        const int c = r - 1;
+        if (out.cell(t)) {
+          AT(CLV_ENERGY1, xs, c) = stage[r & 1][0][t];
+          AT(CLV_DENSITY1, xs, c) = stage[r & 1][1][t];
+        }
+        if (out.node(t)) AT(CLV_VOL_FLUX_X, xs, c) = stage[r & 1][2][t];
+      }
+      if (r < jn1) {
+        if (out.node(t)) {
+          AT(CLV_XVEL1, xs, r) = stage[r & 1][3][t];
+          AT(CLV_YVEL1, xs, r) = stage[r & 1][4][t];
+        }
+        if (out.cell(t)) AT(CLV_VOL_FLUX_Y, xs, r) = stage[r & 1][5][t];
+      }
     }
     pp_prev = pp;
     m_prev = m;
@@ -405,6 +455,7 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
     u_r = u_r1;
     v_r = v_r1;
   }
+  __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
 __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips, int seg) {
@@ -495,12 +546,15 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
   constexpr int MOM_SWEEP = SWEEP == 1 ? 1 : 3;
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
+  const bool live = strip < nstrips;  // past the chunk: march on clamped loads, own nothing
   const int nx = d.nx, ny = d.ny;
-  const int i = 57 * strip - 2 + lane;
+  const int i = kOwnCmX * strip - 2 + lane;
   const int ic = i < 0 ? 0 : (i < nx + 4 ? i : nx + 4);
-  const bool own = lane >= 4 && lane <= 60;
+  const bool own = live && lane >= 4 && lane <= 3 + kOwnCmX;
   const bool own_cell = own && i <= nx + 1, own_face = own && i <= nx + 2;
+  __shared__ double stage[2][5][BlockRows<kOwnCmX>::kSlots];
+  const BlockRows<kOwnCmX> out(bx, nx);
+  const int pos = out.pos(i);
   // halo mirrors of the columns: cells, then x faces (negated)
   const int lane_c = lane + (mirror_cell(i, nx) - i);
   const int fm = i <= 1 ? 4 - i : (i >= nx + 3 ? 2 * nx + 4 - i : i);
@@ -552,10 +606,10 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
     const double mf = rf == rc ? cr.mf : cell_row(rf).mf;
     if (r >= j0 && r <= ny + 1) {
       if (own_cell) {
-        rho_out[(int64_t)r * d.pitch + i] = cr.rho;
-        e_out[(int64_t)r * d.pitch + i] = cr.e;
+        stage[r & 1][0][pos] = cr.rho;
+        stage[r & 1][1][pos] = cr.e;
       }
-      if (own_face) AT(CLV_MASS_FLUX_X, i, r) = cr.mf;
+      if (own_face) stage[r & 1][2][pos] = cr.mf;
     }
     // advec_mom's view of the cell row: ghost columns mirrored
     const double rho_h = __shfl(cr.rho, lane_c), mfx = sgn_f * __shfl(mf, lane_f);
@@ -586,13 +640,29 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
       }
       const double mu_w = up1(mu), mv_w = up1(mv);
       if (own_face) {  // nodes [2, nx + 2]
-        u_out[(int64_t)r * d.pitch + i] = (u * node_mass_pre + mu_w - mu) / node_mass_post;
-        v_out[(int64_t)r * d.pitch + i] = (v * node_mass_pre + mv_w - mv) / node_mass_post;
+        stage[r & 1][3][pos] = (u * node_mass_pre + mu_w - mu) / node_mass_post;
+        stage[r & 1][4][pos] = (v * node_mass_pre + mv_w - mv) / node_mass_post;
+      }
+      // the block's rows (r, j0 uniform over the block)
+      __syncthreads();
+      const int t = threadIdx.x, xs = out.column(t);
+      const int64_t o = (int64_t)r * d.pitch + xs;
+      if (r <= ny + 1) {
+        if (out.cell(t)) {
+          rho_out[o] = stage[r & 1][0][t];
+          e_out[o] = stage[r & 1][1][t];
+        }
+        if (out.node(t)) AT(CLV_MASS_FLUX_X, xs, r) = stage[r & 1][2][t];
+      }
+      if (out.node(t)) {
+        u_out[o] = stage[r & 1][3][t];
+        v_out[o] = stage[r & 1][4][t];
       }
     }
     mp_prev = mp;
     mfx_prev = mfx;
   }
+  __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
 template <int SWEEP>
@@ -649,9 +719,9 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   const int nx = d.nx, ny = d.ny;
-  const int i = 62 * strip + lane;
+  const int i = kOwnCmY * strip + lane;
   const int ic = i < nx + 3 ? i : nx + 3;
-  const bool own = lane >= 2;
+  const bool own = lane >= 2 && lane <= 1 + kOwnCmY;
   const bool own_cell = own && i <= nx + 1, own_node = own && i <= nx + 2;
   const int lane_c = lane + (i == 1 ? 1 : (i == nx + 2 ? -1 : 0));
   const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
@@ -854,13 +924,13 @@ __device__ __forceinline__ ItemRange batch_items(int n) {
   return {lo + k, hi, nbx};
 }
 
-// strips of each tile family (owned columns per wave: 62, 61, 57, 62)
+// strips of each tile family
 __device__ __forceinline__ int strips_of(int kind, int nx) {
   switch (kind) {
-    case BK_TIMESTEP: return (nx + 61) / 62;
-    case BK_LAGRANGE: return (nx + 60) / 61;
-    case BK_CM_X1: case BK_CM_X2: return (nx + 1 + 56) / 57;
-    default: return (nx + 1 + 61) / 62;  // cm y
+    case BK_TIMESTEP: return (nx + kOwnTimestep - 1) / kOwnTimestep;
+    case BK_LAGRANGE: return (nx + kOwnLagrange - 1) / kOwnLagrange;
+    case BK_CM_X1: case BK_CM_X2: return (nx + kOwnCmX) / kOwnCmX;  // nx + 1 face columns
+    default: return (nx + kOwnCmY) / kOwnCmY;                     // cm y: nx + 1 node columns
   }
 }
 
@@ -1028,8 +1098,8 @@ static void choose_segments(clv_chunk *c) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
   const int nx = c->d.nx, ny = c->d.ny;
-  const int strips[8] = {ceil_div(nx, 62),     ceil_div(nx, 61),     ceil_div(nx, 60),     ceil_div(nx, 64),
-                         ceil_div(nx + 1, 60), ceil_div(nx + 1, 64), ceil_div(nx + 1, 57), ceil_div(nx + 1, 62)};
+  const int strips[8] = {ceil_div(nx, kOwnTimestep), ceil_div(nx, kOwnLagrange), ceil_div(nx, 60),          ceil_div(nx, 64),
+                         ceil_div(nx + 1, 60),        ceil_div(nx + 1, 64),        ceil_div(nx + 1, kOwnCmX), ceil_div(nx + 1, kOwnCmY)};
   const int rows[8] = {ny, ny, ny, ny, ny + 1, ny + 1, ny + 1, ny + 1};
   for (int f = 0; f < 8; ++f) {
     int seg = kSegLarge[f];
@@ -1038,10 +1108,10 @@ static void choose_segments(clv_chunk *c) {
   }
 }
 
-// strips of the timestep kernel (62 owned columns each) times row segments (of
+// strips of the timestep kernel (kOwnTimestep owned columns each) times row segments (of
 // at least kSegMin rows), and the second level of the minimum
 constexpr int kMinLevel2 = 256;
-size_t fused_partials_needed(int nx, int ny) { return (size_t)ceil_div(nx, 62) * (size_t)ceil_div(ny, kSegMin) + kMinLevel2; }
+size_t fused_partials_needed(int nx, int ny) { return (size_t)ceil_div(nx, kOwnTimestep) * (size_t)ceil_div(ny, kSegMin) + kMinLevel2; }
 
 static clv_status halo(clv_chunk *c, std::initializer_list<int> ids, int depth) {
   int fields[CLV_NUM_HALO_FIELDS] = {0};
@@ -1069,7 +1139,7 @@ clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *d
   if (st != CLV_OK) return st;
   const Dev &d = c->d;
   const int seg = c->seg[SF_TIMESTEP];
-  const int ns = ceil_div(d.nx, 62), nseg = ceil_div(d.ny, seg);
+  const int ns = ceil_div(d.nx, kOwnTimestep), nseg = ceil_div(d.ny, seg);
   const int n1 = ns * nseg;
   if ((size_t)n1 + kMinLevel2 > c->partials_len) return set_error(CLV_ERR_ARG, "fused timestep: partials buffer too small");
   LAUNCH_F("f_timestep", timestep_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe,
@@ -1097,7 +1167,7 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
   if (st != CLV_OK) return st;
   const Dev &d = c->d;
   const int seg = c->seg[SF_LAGRANGE];
-  const int ns = ceil_div(d.nx, 61), nseg = ceil_div(d.ny, seg);
+  const int ns = ceil_div(d.nx, kOwnLagrange), nseg = ceil_div(d.ny, seg);
   LAUNCH_F("f_lagrange", lagrange_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg);
   return CLV_OK;
 }
@@ -1110,12 +1180,12 @@ static clv_status cm_sweep(clv_chunk *c, int sweep, int dir) {
   const Dev &d = c->d;
   double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2], *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
   if (dir == CLV_XDIR) {
-    const int ns = ceil_div(d.nx + 1, 57), seg = c->seg[SF_CM_X];
+    const int ns = ceil_div(d.nx + 1, kOwnCmX), seg = c->seg[SF_CM_X];
     const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
     if (sweep == 1) LAUNCH_F("f_cm_x1", cm_x_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
     else LAUNCH_F("f_cm_x2", cm_x_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
   } else {
-    const int ns = ceil_div(d.nx + 1, 62), seg = c->seg[SF_CM_Y];
+    const int ns = ceil_div(d.nx + 1, kOwnCmY), seg = c->seg[SF_CM_Y];
     const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
     if (sweep == 1) LAUNCH_F("f_cm_y1", cm_y_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
     else LAUNCH_F("f_cm_y2", cm_y_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);

@@ -1,4 +1,6 @@
 // gfx950 kernels for the self halo, the two reductions and problem set-up.
+#include <vector>
+
 #include "clv_internal.h"
 
 namespace clv {
@@ -431,6 +433,35 @@ __global__ __launch_bounds__(256) void stream_copy_k(const double2 *__restrict__
   if (t < n2) b[t] = a[t];
 }
 
+// The fused kernels' access pattern without their arithmetic: R arrays read and W
+// written per element, one wave64 per 64-column strip marching a segment of rows,
+// strips x-fastest and each XCD's workgroups on one contiguous band of tiles.  Its
+// time is the ceiling of a fused kernel with the same arrays.
+struct MarchProbe {
+  const double *in[9];
+  double *out[7];
+  int64_t pitch;
+  int nx, ny, seg;
+};
+template <int R, int W>
+__global__ __launch_bounds__(256) void march_probe_k(MarchProbe p) {
+  const int n = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+  const int xcd = b % 8, k = b / 8, per = n / 8, rem = n % 8;
+  const int t = xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+  const int bx = t % gridDim.x, by = t / gridDim.x;
+  const int i = 2 + (bx * 4 + (int)(threadIdx.x >> 6)) * 64 + (int)(threadIdx.x & 63);
+  if (i > p.nx + 1) return;
+  const int j0 = 2 + by * p.seg, j1 = min(j0 + p.seg, p.ny + 2);
+  for (int j = j0; j < j1; ++j) {
+    const int64_t o = (int64_t)j * p.pitch + i;
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < R; ++a) s += p.in[a][o];
+#pragma unroll
+    for (int w = 0; w < W; ++w) p.out[w][o] = s + (double)w;
+  }
+}
+
 // ---- set-up -----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void fill_zero_k(double *p, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.0;
@@ -680,6 +711,56 @@ extern "C" clv_status clv_bandwidth_probe(size_t bytes, int iters, double *copy_
   if (e1) (void)hipEventDestroy(e1);
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
+  return st;
+}
+
+extern "C" clv_status clv_march_probe(int x_cells, int y_cells, int reads, int writes, int seg, int iters, double *ms_per_launch) {
+  using namespace clv;
+  if (!ms_per_launch || x_cells < 1 || y_cells < 1 || seg < 1 || iters < 1)
+    return set_error(CLV_ERR_ARG, "clv_march_probe: bad argument");
+  typedef void (*Kern)(MarchProbe);
+  Kern kern = nullptr;
+  if (reads == 7 && writes == 1) kern = march_probe_k<7, 1>;
+  if (reads == 8 && writes == 6) kern = march_probe_k<8, 6>;
+  if (reads == 7 && writes == 5) kern = march_probe_k<7, 5>;
+  if (reads == 6 && writes == 5) kern = march_probe_k<6, 5>;
+  if (!kern) return set_error(CLV_ERR_ARG, "clv_march_probe: (reads, writes) must be (7, 1), (8, 6), (7, 5) or (6, 5)");
+  MarchProbe p{};
+  p.pitch = pitch_for(x_cells);
+  p.nx = x_cells;
+  p.ny = y_cells;
+  p.seg = seg;
+  const size_t n = (size_t)p.pitch * (size_t)(y_cells + 5);
+  std::vector<double *> bufs((size_t)(reads + writes), nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  clv_status st = CLV_OK;
+  // laid out like a chunk's arena: kLeadPad doubles before element (0, 0), so
+  // element (2, j) starts a 128-byte line and every wave's loads are line aligned
+  const size_t alloc = (n + 32) * sizeof(double);
+  for (double *&q : bufs)
+    if (st == CLV_OK && (st = hip_check(hipMalloc(&q, alloc), "march probe alloc")) == CLV_OK)
+      st = hip_check(hipMemset(q, 0, alloc), "march probe memset");
+  for (int a = 0; a < reads; ++a) p.in[a] = bufs[a] + kLeadPad;
+  for (int w = 0; w < writes; ++w) p.out[w] = bufs[reads + w] + kLeadPad;
+  if (st == CLV_OK) st = hip_check(hipEventCreate(&e0), "march probe event");
+  if (st == CLV_OK) st = hip_check(hipEventCreate(&e1), "march probe event");
+  const dim3 grid((unsigned)((x_cells + 255) / 256), (unsigned)((y_cells + seg - 1) / seg)), block(256);
+  if (st == CLV_OK) {
+    hipLaunchKernelGGL(kern, grid, block, 0, nullptr, p);
+    (void)hipEventRecord(e0, nullptr);
+    for (int k = 0; k < iters; ++k) hipLaunchKernelGGL(kern, grid, block, 0, nullptr, p);
+    (void)hipEventRecord(e1, nullptr);
+    st = hip_check(hipEventSynchronize(e1), "march probe sync");
+  }
+  if (st == CLV_OK) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_launch = (double)ms / iters;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (double *q : bufs)
+    if (q) (void)hipFree(q);
   return st;
 }
 

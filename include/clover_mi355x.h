@@ -397,6 +397,14 @@ clv_status clv_profile_reset(clv_chunk *c);
  * launches with HIP events; *copy_gbps = 2 * bytes * iters / time.  Allocates and
  * frees two buffers of `bytes`. */
 clv_status clv_bandwidth_probe(size_t bytes, int iters, double *copy_gbps);
+/* Access-pattern probe on the current device (no counterpart in the reference):
+ * the fused kernels' march (64-column strips, `seg`-row segments, workgroups
+ * x-fastest on one XCD band each) over the interior of an x_cells * y_cells
+ * grid, reading `reads` and writing `writes` FP64 arrays per cell with no
+ * arithmetic; the arrays are padded like a chunk's, element (2, j) on a 128-byte
+ * line.  (reads, writes) is (7, 1), (8, 6), (7, 5) or (6, 5).  Allocates and
+ * frees the arrays; *ms_per_launch = the mean of `iters` timed launches. */
+clv_status clv_march_probe(int x_cells, int y_cells, int reads, int writes, int seg, int iters, double *ms_per_launch);
 
 #ifdef __cplusplus
 }

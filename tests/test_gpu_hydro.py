@@ -88,11 +88,14 @@ def test_small_deck_bitwise_vs_oracle(gpu, deck, plan):
 
 @pytest.mark.parametrize("nx,ny", [(61, 64), (62, 65), (122, 129), (183, 70), (240, 191), (257, 300), (60, 8), (64, 9),
                                    (121, 15), (59, 16), (65, 17), (181, 127), (63, 128), (124, 257), (56, 15), (57, 16),
-                                   (113, 31), (114, 33), (170, 47), (4, 4), (5, 6)])
+                                   (113, 31), (114, 33), (170, 47), (4, 4), (5, 6), (55, 14), (120, 21), (223, 19),
+                                   (225, 22), (239, 20), (241, 18), (447, 9), (448, 10), (480, 17)])
 def test_fused_plan_strip_and_segment_edges(gpu, nx, ny):
-    """Grid sizes around the fused kernels' strip widths (57/60/61/62/64 columns) and
-    row segments (8, 16 and 128 rows), and the smallest grids: every step's dt and the
-    state bit-identical to the oracle."""
+    """Grid sizes around the fused kernels' strip widths (56/60/62 columns), the
+    224- and 240-column blocks whose rows are stored through LDS (blocks with idle
+    strips, the boundary node column on the slot past a block), row segments (8, 16
+    and 128 rows), and the smallest grids: every step's dt and the state bit-identical
+    to the oracle."""
     text = _geom_deck(nx, ny, 12)
     cfg = clv.parse_deck(text)
     o = T.Oracle(T.parse_deck(text))
