@@ -124,8 +124,9 @@ struct BlockRows {
 };
 
 // ---- timestep: ideal_gas + viscosity + calc_dt ----------------------------------------
-// Lanes: column x0 + L, x0 = 1 + 62 * strip; lanes 1..62 own cells (p of the
-// neighbouring columns comes from lanes 0 and 63).
+// Lanes: column x0 + L, x0 = 1 + kOwnTimestep * strip; lanes 1..62 own cells (p of the
+// neighbouring columns comes from lanes 0 and 63).  Its one stored array goes straight
+// from the lanes (staged through LDS it ran no faster).
 __device__ __forceinline__ void ideal_gas_cell(double rho, double e, double &p, double &c) {
   // src/serial/ideal_gas.cpp:40-45
   double v = 1.0 / rho;
@@ -537,10 +538,12 @@ __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor
 // (src/serial/update_halo.cpp: density1 ghost col 1 - k <- 2 + k, nx + 2 + k <- nx + 1 - k,
 // rows alike; mass_flux_x col 1 - k <- -(3 + k), nx + 3 + k <- -(nx + 1 - k), row 1 <- 3,
 // row ny + 2 <- ny), columns by lane shuffles, rows by computing the mirrored row
-// (cell rows are independent in an x sweep).  Lanes: column x0 + L, x0 = 57 * strip - 2;
-// cells, faces and nodes owned on lanes 4..60.  Cell values hold on lanes 2..61,
-// face fluxes on 2..62, and advec_mom's stencil narrows that to 4..60.  March: cell
-// row r (r from j0 - 1), then node row r from cell rows r - 1 and r.
+// (cell rows are independent in an x sweep).  Lanes: column x0 + L, x0 = kOwnCmX * strip
+// - 2; cells, faces and nodes owned on lanes 4..3 + kOwnCmX (4..59).  Cell values hold on
+// lanes 2..61, face fluxes on 2..62, and advec_mom's stencil narrows that to 4..60.
+// March: cell row r (r from j0 - 1), then node row r from cell rows r - 1 and r.  The
+// block's results go out through LDS (BlockRows): its four strips own 224 whole-line
+// columns.
 template <int SWEEP, class V, class O>
 __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 1 : 3;
@@ -674,9 +677,10 @@ __global__ __launch_bounds__(256) void cm_x_k(Dev d, int nstrips, int seg, doubl
 }
 
 // y direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
-// with mom_sweep 2, cell sweep 2 with mom_sweep 4).  Lanes: column 62 * strip + L;
-// nodes, cells and faces are owned on lanes 2..63, and lanes 0 and 1 supply the two
-// columns left of them.  Each lane walks its column: the cell part (face r needs cells
+// with mom_sweep 2, cell sweep 2 with mom_sweep 4).  Lanes: column kOwnCmY * strip + L;
+// nodes, cells and faces are owned on lanes 2..1 + kOwnCmY (2..63), and lanes 0 and 1
+// supply the two columns left of them.  Stores go straight from the lanes: staged
+// through LDS (whole lines, 240-column blocks) this walk ran 5-9 % slower.  Each lane walks its column: the cell part (face r needs cells
 // r-2 .. r+1; cell r - 1 is updated from faces r - 1 and r) runs four faces ahead of
 // the mom part (node row k: node_flux(k) = 0.25 (mfy(i-1,k) + mfy(i,k) + mfy(i-1,k+1) +
 // mfy(i,k+1)), the masses of cells (i-1 .. i, k-1 .. k), the flux of face k - 1 and the
@@ -717,11 +721,11 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   constexpr int MOM_SWEEP = SWEEP == 1 ? 2 : 4;
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
+  const bool live = strip < nstrips;  // past the chunk: walk on clamped loads, own nothing
   const int nx = d.nx, ny = d.ny;
   const int i = kOwnCmY * strip + lane;
   const int ic = i < nx + 3 ? i : nx + 3;
-  const bool own = lane >= 2 && lane <= 1 + kOwnCmY;
+  const bool own = live && lane >= 2 && lane <= 1 + kOwnCmY;
   const bool own_cell = own && i <= nx + 1, own_node = own && i <= nx + 2;
   const int lane_c = lane + (i == 1 ? 1 : (i == nx + 2 ? -1 : 0));
   const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
@@ -892,6 +896,9 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       v_m1 = v_0;
       v_0 = v_p1;
     }
+    // the block's four walks stay on the same rows (3-6 % faster at 15360^2 than
+    // free-running waves; F, j1 are uniform over the block)
+    __syncthreads();
     cur = nxt;
     u_ahead = u_n;
     v_ahead = v_n;
