@@ -107,6 +107,7 @@ SIGNATURES = [
     ("clv_chunk_wrap", _I, [C.POINTER(ChunkView), C.POINTER(_P)]),
     ("clv_chunk_destroy", _I, [_P]),
     ("clv_chunk_get_view", _I, [_P, C.POINTER(ChunkView)]),
+    ("clv_chunk_geometry_from_mesh", _I, [_P]),
     ("clv_chunk_set_stream", _I, [_P, _P]),
     ("clv_chunk_synchronize", _I, [_P]),
     ("clv_chunk_bytes", C.c_size_t, [_P]),
@@ -405,6 +406,11 @@ class Chunk:
         """Every requested field of this tile into (or out of) one face message."""
         fn = self.L.clv_unpack_fields if unpack else self.L.clv_pack_fields
         _check(fn(self.h, int(face), _mask(fields), depth, int(chunk_cells), int(tile_offset), buf.ptr, buf.n), "pack_fields")
+
+    def geometry_from_mesh(self) -> bool:
+        """True while volume, xarea and yarea hold what initialise_chunk wrote (the fused
+        plan then derives them from the mesh instead of reading them)."""
+        return self.L.clv_chunk_geometry_from_mesh(self.h) == 1
 
     def field_ptr(self, name: str) -> int:
         """Device address of element (0, 0) of a 2D field."""

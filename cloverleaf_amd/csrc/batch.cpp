@@ -56,7 +56,11 @@ Dev after_pass(Dev d) {
   swap_in(d, CLV_YVEL1, CLV_WORK4);
   return d;
 }
-bool same_view(const Dev &a, const Dev &b) { return std::memcmp(&a, &b, sizeof(Dev)) == 0; }
+// the same arrays and shape (the geometry flag is checked on its own)
+bool same_view(const Dev &a, const Dev &b) {
+  return std::memcmp(a.f, b.f, sizeof(a.f)) == 0 && std::memcmp(a.m, b.m, sizeof(a.m)) == 0 && a.nx == b.nx && a.ny == b.ny &&
+         a.pitch == b.pitch;
+}
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // host mirror of the device strips_of
@@ -100,6 +104,7 @@ struct clv_batch {
   std::vector<clv_config> cfgs;
   std::vector<clv_hydro_state> st0;  // member states at creation
   std::vector<Dev> d0;               // member views at creation
+  int mesh = 0;                      // every member's geometry from its mesh at creation (Dev::geom_mesh)
   int advect_x0 = 1;
   long long bstep = 0;  // batch steps issued since creation
   int hist_len = 0;
@@ -218,7 +223,7 @@ clv_status apply_schedule(clv_batch *b, const clv_batch_schedule &s) {
 
 clv_status launch_family(clv_batch *b, int kind, int fam, const Dev *tab) {
   const int grid = grid_for(b, b->sched.fam[fam], b->nitems[fam]);
-  return hip_check(batch_launch_tiles(kind, b->stream, grid, tab, b->pc, b->ps, b->items[fam], b->nitems[fam]), "batch tile launch");
+  return hip_check(batch_launch_tiles(kind, b->stream, grid, tab, b->pc, b->ps, b->items[fam], b->nitems[fam], b->mesh), "batch tile launch");
 }
 
 clv_status launch_halo(clv_batch *b, const Dev *tab, std::initializer_list<int> ids, int depth) {
@@ -269,7 +274,11 @@ clv_status capture(clv_batch *b, int q) {
 
 // member views after the steps they ran; the members' own chunks are updated
 void reconcile(clv_batch *b) {
-  for (int p = 0; p < b->n; ++p) b->chunks[p]->d = (b->ps_host[p].ran & 1) ? after_reset(b->d0[p]) : b->d0[p];
+  for (int p = 0; p < b->n; ++p) {
+    const int geom_mesh = b->chunks[p]->d.geom_mesh;  // the chunk's own, possibly cleared since creation
+    b->chunks[p]->d = (b->ps_host[p].ran & 1) ? after_reset(b->d0[p]) : b->d0[p];
+    b->chunks[p]->d.geom_mesh = geom_mesh;
+  }
 }
 
 clv_status pull_states(clv_batch *b) {
@@ -357,6 +366,8 @@ clv_status clv_batch_create(clv_chunk *const *chunks, const clv_config *cfgs, co
     if ((st = hip_check(hipStreamSynchronize(chunks[p]->stream), "clv_batch_create: member sync")) != CLV_OK) return fail(st);
   if ((st = hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "clv_batch_create: stream")) != CLV_OK) return fail(st);
   for (int p = 0; p < n; ++p) b->d0.push_back(chunks[p]->d);
+  b->mesh = 1;
+  for (int p = 0; p < n; ++p) b->mesh = b->mesh && chunks[p]->d.geom_mesh;
   // stage views of both parities
   std::vector<Dev> tabs((size_t)2 * kStages * n);
   for (int p = 0; p < n; ++p) {
@@ -454,11 +465,11 @@ clv_status clv_batch_autotune(clv_batch *b, int reps, clv_batch_schedule *out, d
     if ((s = upload_items(&b->probe_items, &b->probe_cap, v)) != CLV_OK) return s;
     const int grid = grid_for(b, fs, (int)v.size());
     const int kind = probe_kind(fam);
-    if ((s = hip_check(batch_launch_tiles(kind, b->stream, grid, tab, b->pc, b->ps, b->probe_items, (int)v.size()), "autotune warm-up")) != CLV_OK)
+    if ((s = hip_check(batch_launch_tiles(kind, b->stream, grid, tab, b->pc, b->ps, b->probe_items, (int)v.size(), b->mesh), "autotune warm-up")) != CLV_OK)
       return s;
     (void)hipEventRecord(e0, b->stream);
     for (int r = 0; r < reps; ++r)
-      if ((s = hip_check(batch_launch_tiles(kind, b->stream, grid, tab, b->pc, b->ps, b->probe_items, (int)v.size()), "autotune launch")) != CLV_OK)
+      if ((s = hip_check(batch_launch_tiles(kind, b->stream, grid, tab, b->pc, b->ps, b->probe_items, (int)v.size(), b->mesh), "autotune launch")) != CLV_OK)
         return s;
     (void)hipEventRecord(e1, b->stream);
     if ((s = hip_check(hipEventSynchronize(e1), "autotune sync")) != CLV_OK) return s;
@@ -525,6 +536,8 @@ clv_status clv_batch_run(clv_batch *b, int max_steps, int use_graph, int *steps_
   for (int p = 0; p < b->n; ++p) {
     const Dev want = (b->ps_host[p].ran & 1) ? after_reset(b->d0[p]) : b->d0[p];
     if (!same_view(b->chunks[p]->d, want)) return set_error(CLV_ERR_ARG, "clv_batch_run: a member was stepped outside the batch");
+    if (b->mesh && !b->chunks[p]->d.geom_mesh)
+      return set_error(CLV_ERR_ARG, "clv_batch_run: a member's volume or face areas may have changed since the batch was created");
     if ((st = hip_check(hipStreamSynchronize(b->chunks[p]->stream), "clv_batch_run: member sync")) != CLV_OK) return st;
   }
   // no member runs past its end_step: stop there at the latest

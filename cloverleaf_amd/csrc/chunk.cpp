@@ -209,6 +209,8 @@ clv_status clv_chunk_destroy(clv_chunk *c) {
 clv_status clv_chunk_get_view(const clv_chunk *c, clv_chunk_view *out) {
   CHECK_CHUNK(c);
   if (!out) return set_error(CLV_ERR_ARG, "null view");
+  // the caller may write any array through the view: read the geometry arrays from now on
+  const_cast<clv_chunk *>(c)->d.geom_mesh = 0;
   out->x_cells = c->d.nx;
   out->y_cells = c->d.ny;
   out->pitch = c->d.pitch;
@@ -216,6 +218,8 @@ clv_status clv_chunk_get_view(const clv_chunk *c, clv_chunk_view *out) {
   for (int m = 0; m < CLV_NUM_MESH; ++m) out->mesh[m] = c->d.m[m];
   return CLV_OK;
 }
+
+int clv_chunk_geometry_from_mesh(const clv_chunk *c) { return c ? c->d.geom_mesh : -1; }
 
 clv_status clv_chunk_set_stream(clv_chunk *c, void *s) {
   CHECK_CHUNK(c);
@@ -256,6 +260,7 @@ clv_status clv_copy_to_device(void *dst, const void *src, size_t bytes) {
 clv_status clv_field_upload(clv_chunk *c, int field, const double *host, int64_t host_pitch) {
   CHECK_CHUNK(c);
   if (!valid_field(field) || !host || host_pitch < c->d.nx + 5) return set_error(CLV_ERR_ARG, "clv_field_upload: bad argument");
+  if (field == CLV_VOLUME || field == CLV_XAREA || field == CLV_YAREA) c->d.geom_mesh = 0;
   clv_status st = hip_check(hipMemcpy2DAsync(c->d.f[field], c->d.pitch * sizeof(double), host, host_pitch * sizeof(double),
                                              (c->d.nx + 5) * sizeof(double), c->d.ny + 5, hipMemcpyHostToDevice, c->stream),
                             "clv_field_upload");
@@ -276,6 +281,8 @@ clv_status clv_field_download(clv_chunk *c, int field, double *host, int64_t hos
 clv_status clv_mesh_upload(clv_chunk *c, int mesh, const double *host) {
   CHECK_CHUNK(c);
   if (!valid_mesh(mesh) || !host) return set_error(CLV_ERR_ARG, "clv_mesh_upload: bad argument");
+  // the areas were derived from the old spacings
+  if (mesh == CLV_CELLDX || mesh == CLV_CELLDY) c->d.geom_mesh = 0;
   clv_status st = hip_check(hipMemcpyAsync(c->d.m[mesh], host, mesh_len(c, mesh) * sizeof(double), hipMemcpyHostToDevice, c->stream),
                             "clv_mesh_upload");
   if (st != CLV_OK) return st;

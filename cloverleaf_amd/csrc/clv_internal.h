@@ -15,12 +15,18 @@ constexpr double g_small = 1.0e-16;  // driver/definitions.h:32
 constexpr double g_big = 1.0e+21;    // driver/definitions.h:33
 
 // Kernel-side view of a chunk: one base pointer per array, one row pitch.
-// Passed by value (kernarg), 288 bytes.
+// Passed by value (kernarg), 296 bytes.
 struct Dev {
   double *f[CLV_NUM_FIELDS];
   double *m[CLV_NUM_MESH];
   int nx, ny;
   int64_t pitch;
+  // geom_mesh: volume, xarea and yarea still hold what initialise_chunk wrote
+  // (volume = dxdy, xarea(i, j) = celldy(j), yarea(i, j) = celldx(i)), so the fused
+  // kernels take them from dxdy and the 1D mesh.  Set by initialise_chunk, cleared by
+  // every entry point through which they could change.
+  double dxdy;
+  int geom_mesh;
 };
 
 // Row pitch for an x_cells-wide chunk: holds nx + 5 doubles, a multiple of 16
@@ -128,8 +134,9 @@ struct BatchState {
   int failed_step;  // step whose dt fell below dtmin (0: none)
   int pad;
 };
+// mesh: every member's geometry comes from its mesh (Dev::geom_mesh)
 hipError_t batch_launch_tiles(int kind, hipStream_t s, int grid, const Dev *tab, const BatchConst *pc, const BatchState *ps,
-                              const BatchItem *items, int nitems);
+                              const BatchItem *items, int nitems, int mesh);
 hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, BatchState *ps);
 // halo of the listed fields for every running problem (grid: problems x fields)
 hipError_t batch_launch_halo(hipStream_t s, int nprob, const Dev *tab, const BatchState *ps, const int *ids, int nids, int depth);

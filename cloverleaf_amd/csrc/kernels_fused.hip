@@ -139,13 +139,40 @@ __device__ __forceinline__ void ideal_gas_cell(double rho, double e, double &p, 
 
 __device__ __forceinline__ double pressure_of(double rho, double e) { return (1.4 - 1.0) * rho * e; }
 
+// ---- cell volumes and face areas --------------------------------------------------------
+// initialise_chunk (src/serial/initialise_chunk.cpp:78-84) sets volume = dx * dy,
+// xarea(i, j) = celldy(j) and yarea(i, j) = celldx(i) over i < nx + 4, j < ny + 4 (the
+// elements past those, the last column of xarea, the last row of yarea and the row
+// padding, keep build_field's zero), and no
+// kernel of a step writes them.  While a chunk's arrays hold those values
+// (d.geom_mesh) the fused kernels compute them from d.dxdy and the 1D mesh -- the same
+// doubles -- instead of streaming three 2D arrays (3 of timestep's 8 array sweeps, 3 of
+// lagrange's 14, one of each advection pass's); otherwise they read the arrays.  The
+// host picks the instantiation (MESH) per launch.
+template <bool MESH, class V>
+__device__ __forceinline__ double geo_volume(const V &d, int i, int j) {
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? d.dxdy : 0.0;
+  else return AT(CLV_VOLUME, i, j);
+}
+template <bool MESH, class V>
+__device__ __forceinline__ double geo_xarea(const V &d, int i, int j) {
+  if constexpr (MESH) return i < d.nx + 4 ? M(CLV_CELLDY, j) : 0.0;
+  else return AT(CLV_XAREA, i, j);
+}
+// cdx_i: celldx(i) (used for i < nx + 4 only)
+template <bool MESH, class V>
+__device__ __forceinline__ double geo_yarea(const V &d, double cdx_i, int i, int j) {
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? cdx_i : 0.0;
+  else return AT(CLV_YAREA, i, j);
+}
+
 // Every kernel below is a tile function -- strip block bx (WAVES strips of one
 // 256-thread block) over rows [j0, j1) of one chunk -- called either once per
 // block of a plain launch (<name>_k) or in the loop of a persistent batched launch
 // over many chunks (batch_<name>_k).  Both run the same instructions per tile.
 
 // The wave's minimum goes to part[strip].
-template <class V>
+template <bool MESH, class V>
 __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1, double dtc_safe, double dtu_safe, double dtv_safe,
                                               double dtdiv_safe, double *part, int nstrips) {
   const int lane = threadIdx.x & 63;
@@ -166,7 +193,7 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
   ideal_gas_cell(rho_c, e_c, p_c, c_c);
   // node row r velocities (own column and column + 1 via shuffle)
   double u_r = AT(CLV_XVEL0, ic, j0), v_r = AT(CLV_YVEL0, ic, j0);
-  double ya_r = AT(CLV_YAREA, ic, j0);
+  double ya_r = geo_yarea<MESH>(d, cdx, ic, j0);
   double dtmin = g_big;
   for (int r = j0; r < j1; ++r) {
     const int rm = mirror_cell(r + 1, d.ny);
@@ -174,9 +201,9 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
     double p_p, c_p;
     ideal_gas_cell(rho_p, e_p, p_p, c_p);
     const double u_r1 = AT(CLV_XVEL0, ic, r + 1), v_r1 = AT(CLV_YVEL0, ic, r + 1);
-    const double ya_r1 = AT(CLV_YAREA, ic, r + 1);
-    const double xa = AT(CLV_XAREA, ic, r);
-    const double vol = AT(CLV_VOLUME, ic, r);
+    const double ya_r1 = geo_yarea<MESH>(d, cdx, ic, r + 1);
+    const double xa = geo_xarea<MESH>(d, ic, r);
+    const double vol = geo_volume<MESH>(d, ic, r);
     // neighbours along x
     const double p_w = up1(p_c), p_e = dn1(p_c);
     const double u10 = dn1(u_r), u11 = dn1(u_r1), v10 = dn1(v_r), v11 = dn1(v_r1), xa1 = dn1(xa);
@@ -252,12 +279,13 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
 }
 
 // 5 waves per SIMD fit without spilling (96 VGPRs)
+template <bool MESH>
 __global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
                                                   double *partials, int nstrips, int seg) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  timestep_tile(d, bx, j0, min(j0 + seg, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
+  timestep_tile<MESH>(d, bx, j0, min(j0 + seg, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
 }
 
 // Minimum of partials[0, n): block b takes the b-th of gridDim.x contiguous
@@ -306,7 +334,7 @@ __device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt)
 }
 
 // cells [j0, jc1)
-template <class V>
+template <bool MESH, class V>
 __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc1, double dt, int nstrips) {
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
@@ -323,14 +351,15 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   const BlockRows<kOwnLagrange, true> out(bx, nx);
   const int pos = out.pos(i);
   const int jn1 = jc1 == ny + 2 ? ny + 3 : jc1;    // owned node rows [j0, jn1)
+  const double cdx_i = MESH ? M(CLV_CELLDX, ic < nx + 4 ? ic : nx + 3) : 0.0;
   const double halfdt = 0.5 * dt;                  // src/serial/accelerate.cpp:33
 
   // per-row loads of column ic: node velocities (rows r, r+1), areas, cell data
   auto load_pred = [&](int r, double u_r, double v_r, double u_r1, double v_r1, PredIn &a) {
-    a.xa0 = AT(CLV_XAREA, ic, r);
+    a.xa0 = geo_xarea<MESH>(d, ic, r);
     a.xa1 = dn1(a.xa0);
-    a.ya0 = AT(CLV_YAREA, ic, r);
-    a.ya1 = AT(CLV_YAREA, ic, r + 1);
+    a.ya0 = geo_yarea<MESH>(d, cdx_i, ic, r);
+    a.ya1 = geo_yarea<MESH>(d, cdx_i, ic, r + 1);
     a.u00 = u_r;
     a.u01 = u_r1;
     a.u10 = dn1(u_r);
@@ -339,7 +368,7 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
     a.v01 = v_r1;
     a.v10 = dn1(v_r);
     a.v11 = dn1(v_r1);
-    a.vol = AT(CLV_VOLUME, ic, r);
+    a.vol = geo_volume<MESH>(d, ic, r);
     // ghost rho0 through the mirror (the halo of driver/timestep.cpp:53-60)
     a.rho0 = AT(CLV_DENSITY0, mirror_cell(ic, nx), mirror_cell(r, ny));
     a.e0 = AT(CLV_ENERGY0, ic, r);
@@ -459,11 +488,12 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
+template <bool MESH>
 __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips, int seg) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  lagrange_tile(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips);
+  lagrange_tile<MESH>(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips);
 }
 
 // ---- advec_cell -------------------------------------------------------------------------
@@ -544,7 +574,7 @@ __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor
 // March: cell row r (r from j0 - 1), then node row r from cell rows r - 1 and r.  The
 // block's results go out through LDS (BlockRows): its four strips own 224 whole-line
 // columns.
-template <int SWEEP, class V, class O>
+template <int SWEEP, bool MESH, class V, class O>
 __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 1 : 3;
   const int lane = threadIdx.x & 63;
@@ -576,7 +606,7 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
   };
   auto cell_row = [&](int row) {
     const double rho = AT(CLV_DENSITY1, ic, row), e = AT(CLV_ENERGY1, ic, row);
-    const double vol = AT(CLV_VOLUME, ic, row), vfx = AT(CLV_VOL_FLUX_X, ic, row);
+    const double vol = geo_volume<MESH>(d, ic, row), vfx = AT(CLV_VOL_FLUX_X, ic, row);
     const double vfx_e = dn1(vfx);
     double pre, y0 = 0.0, y1 = 0.0;
     if (SWEEP == 1) {
@@ -618,8 +648,8 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
     const double rho_h = __shfl(cr.rho, lane_c), mfx = sgn_f * __shfl(mf, lane_f);
     // density1 * post_vol of cell (i, r) (src/serial/advec_mom.cpp:47 / :65, :98-99)
     double post;
-    if (MOM_SWEEP == 1) post = rc == r ? cr.vol + cr.vfy1 - cr.vfy0 : AT(CLV_VOLUME, ic, r) + AT(CLV_VOL_FLUX_Y, ic, r + 1) - AT(CLV_VOL_FLUX_Y, ic, r);
-    else post = rc == r ? cr.vol : AT(CLV_VOLUME, ic, r);
+    if (MOM_SWEEP == 1) post = rc == r ? cr.vol + cr.vfy1 - cr.vfy0 : geo_volume<MESH>(d, ic, r) + AT(CLV_VOL_FLUX_Y, ic, r + 1) - AT(CLV_VOL_FLUX_Y, ic, r);
+    else post = rc == r ? cr.vol : geo_volume<MESH>(d, ic, r);
     const double mp = rho_h * post;
     if (r >= j0) {
       // node row r (src/serial/advec_mom.cpp:88-149) on the values in registers:
@@ -668,12 +698,12 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
   __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
-template <int SWEEP>
+template <int SWEEP, bool MESH>
 __global__ __launch_bounds__(256) void cm_x_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  cm_x_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+  cm_x_tile<SWEEP, MESH>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
 // y direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
@@ -716,7 +746,7 @@ struct CmHist {
   }
   __device__ double rho(int q) const { return q == 0 ? r0 : q == 2 ? r2 : q == 4 ? r4 : q == 6 ? r6 : r3; }
 };
-template <int SWEEP, class V, class O>
+template <int SWEEP, bool MESH, class V, class O>
 __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 2 : 4;
   const int lane = threadIdx.x & 63;
@@ -738,10 +768,10 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   };
   auto post_from = [&](double vol, double vfx0, double vfx1) { return MOM_SWEEP == 2 ? vol + vfx1 - vfx0 : vol; };
   auto post_of = [&](int c) {
-    return post_from(AT(CLV_VOLUME, ic, c), MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic, c) : 0.0, MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic + 1, c) : 0.0);
+    return post_from(geo_volume<MESH>(d, ic, c), MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic, c) : 0.0, MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic + 1, c) : 0.0);
   };
   auto pre_load = [&](int row, double vfy0, double &vfy1, double &post) {
-    const double vol = AT(CLV_VOLUME, ic, row);
+    const double vol = geo_volume<MESH>(d, ic, row);
     vfy1 = AT(CLV_VOL_FLUX_Y, ic, row + 1);
     const double vfx0 = SWEEP == 1 ? AT(CLV_VOL_FLUX_X, ic, row) : 0.0, vfx1 = SWEEP == 1 ? AT(CLV_VOL_FLUX_X, ic + 1, row) : 0.0;
     post = post_from(vol, vfx0, vfx1);
@@ -756,7 +786,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
     double vol = 0.0, vfy1 = 0.0, vfx0 = 0.0, vfx1 = 0.0, vd = 0.0, rho_p1 = 0.0, e_p1 = 0.0, vd_p1 = 0.0;
     if (F <= Fe) {
       const int rp = F + 1 <= ny + 2 ? F + 1 : ny + 2;  // src/serial/advec_cell.cpp:169
-      vol = AT(CLV_VOLUME, ic, F);
+      vol = geo_volume<MESH>(d, ic, F);
       vfy1 = AT(CLV_VOL_FLUX_Y, ic, F + 1);
       if (SWEEP == 1) {
         vfx0 = AT(CLV_VOL_FLUX_X, ic, F);
@@ -905,12 +935,12 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   }
 }
 
-template <int SWEEP>
+template <int SWEEP, bool MESH>
 __global__ __launch_bounds__(256) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  cm_y_tile<SWEEP>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+  cm_y_tile<SWEEP, MESH>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
 // ---- batched form: persistent launches over the tiles of many chunks ----------------------
@@ -951,6 +981,8 @@ struct GDev {
   gdouble *m[CLV_NUM_MESH];
   int nx, ny;
   int64_t pitch;
+  double dxdy;
+  int geom_mesh;
 };
 __device__ __forceinline__ GDev global_view(const Dev &s) {
   GDev d;
@@ -959,6 +991,8 @@ __device__ __forceinline__ GDev global_view(const Dev &s) {
   d.nx = s.nx;
   d.ny = s.ny;
   d.pitch = s.pitch;
+  d.dxdy = s.dxdy;
+  d.geom_mesh = s.geom_mesh;
   return d;
 }
 
@@ -968,6 +1002,7 @@ __device__ __forceinline__ BatchItem load_item(const BatchItem *__restrict__ ite
   return BatchItem{(uint16_t)v, (uint16_t)(v >> 16), (uint16_t)(v >> 32), (uint16_t)(v >> 48)};
 }
 
+template <bool MESH>
 __global__ __launch_bounds__(256, 5) void batch_timestep_k(const Dev *__restrict__ tab, const BatchConst *__restrict__ pc,
                                                         const BatchState *__restrict__ ps, const BatchItem *__restrict__ items,
                                                         int nitems) {
@@ -979,11 +1014,12 @@ __global__ __launch_bounds__(256, 5) void batch_timestep_k(const Dev *__restrict
     const BatchConst c = pc[it.p];
     const int ns = strips_of(BK_TIMESTEP, d.nx);
     const int j0 = 2 + (int)it.k * it.seg;
-    timestep_tile(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), c.dtc_safe, c.dtu_safe, c.dtv_safe, c.dtdiv_safe,
+    timestep_tile<MESH>(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), c.dtc_safe, c.dtu_safe, c.dtv_safe, c.dtdiv_safe,
                   c.partials + (size_t)it.k * ns, ns);
   }
 }
 
+template <bool MESH>
 __global__ __launch_bounds__(256) void batch_lagrange_k(const Dev *__restrict__ tab, const BatchState *__restrict__ ps,
                                                       const BatchItem *__restrict__ items, int nitems) {
   const ItemRange rg = batch_items(nitems);
@@ -993,13 +1029,13 @@ __global__ __launch_bounds__(256) void batch_lagrange_k(const Dev *__restrict__ 
     const GDev d = global_view(tab[it.p]);
     const double dt = ps[it.p].dt;
     const int j0 = 2 + (int)it.k * it.seg;
-    lagrange_tile(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), dt, strips_of(BK_LAGRANGE, d.nx));
+    lagrange_tile<MESH>(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), dt, strips_of(BK_LAGRANGE, d.nx));
   }
 }
 
 // advection sweeps write the updated pair into work arrays 1/2 (cells) or 3/4
 // (nodes) of the stage's view; the next stage's view has them renamed.
-template <int KIND>
+template <int KIND, bool MESH>
 __global__ __launch_bounds__(256) void batch_advec_k(const Dev *__restrict__ tab, const BatchState *__restrict__ ps,
                                                    const BatchItem *__restrict__ items, int nitems) {
   const ItemRange rg = batch_items(nitems);
@@ -1009,10 +1045,10 @@ __global__ __launch_bounds__(256) void batch_advec_k(const Dev *__restrict__ tab
     const GDev d = global_view(tab[it.p]);
     const int ns = strips_of(KIND, d.nx);
     const int j0 = 2 + (int)it.k * it.seg, jn1 = min(j0 + (int)it.seg, d.ny + 3);
-    if constexpr (KIND == BK_CM_X1) cm_x_tile<1>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_X2) cm_x_tile<2>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_Y1) cm_y_tile<1>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_Y2) cm_y_tile<2>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_X1) cm_x_tile<1, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_X2) cm_x_tile<2, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y1) cm_y_tile<1, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y2) cm_y_tile<2, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
   }
 }
 
@@ -1058,19 +1094,25 @@ __global__ __launch_bounds__(256) void batch_dt_k(const BatchConst *pc, BatchSta
 
 }  // namespace
 
-hipError_t batch_launch_tiles(int kind, hipStream_t s, int grid, const Dev *tab, const BatchConst *pc, const BatchState *ps,
-                              const BatchItem *items, int nitems) {
+template <bool MESH>
+static hipError_t launch_tiles(int kind, hipStream_t s, int grid, const Dev *tab, const BatchConst *pc, const BatchState *ps,
+                               const BatchItem *items, int nitems) {
   const dim3 g(grid, 1, 1), blk(256, 1, 1);
   switch (kind) {
-    case BK_TIMESTEP: hipLaunchKernelGGL(batch_timestep_k, g, blk, 0, s, tab, pc, ps, items, nitems); break;
-    case BK_LAGRANGE: hipLaunchKernelGGL(batch_lagrange_k, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CM_X1: hipLaunchKernelGGL(batch_advec_k<BK_CM_X1>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CM_X2: hipLaunchKernelGGL(batch_advec_k<BK_CM_X2>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CM_Y1: hipLaunchKernelGGL(batch_advec_k<BK_CM_Y1>, g, blk, 0, s, tab, ps, items, nitems); break;
-    case BK_CM_Y2: hipLaunchKernelGGL(batch_advec_k<BK_CM_Y2>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_TIMESTEP: hipLaunchKernelGGL(batch_timestep_k<MESH>, g, blk, 0, s, tab, pc, ps, items, nitems); break;
+    case BK_LAGRANGE: hipLaunchKernelGGL(batch_lagrange_k<MESH>, g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_X1: hipLaunchKernelGGL((batch_advec_k<BK_CM_X1, MESH>), g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_X2: hipLaunchKernelGGL((batch_advec_k<BK_CM_X2, MESH>), g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_Y1: hipLaunchKernelGGL((batch_advec_k<BK_CM_Y1, MESH>), g, blk, 0, s, tab, ps, items, nitems); break;
+    case BK_CM_Y2: hipLaunchKernelGGL((batch_advec_k<BK_CM_Y2, MESH>), g, blk, 0, s, tab, ps, items, nitems); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t batch_launch_tiles(int kind, hipStream_t s, int grid, const Dev *tab, const BatchConst *pc, const BatchState *ps,
+                              const BatchItem *items, int nitems, int mesh) {
+  return mesh ? launch_tiles<true>(kind, s, grid, tab, pc, ps, items, nitems) : launch_tiles<false>(kind, s, grid, tab, pc, ps, items, nitems);
 }
 
 hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, BatchState *ps) {
@@ -1149,7 +1191,8 @@ clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *d
   const int ns = ceil_div(d.nx, kOwnTimestep), nseg = ceil_div(d.ny, seg);
   const int n1 = ns * nseg;
   if ((size_t)n1 + kMinLevel2 > c->partials_len) return set_error(CLV_ERR_ARG, "fused timestep: partials buffer too small");
-  LAUNCH_F("f_timestep", timestep_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe,
+  const auto kern = d.geom_mesh ? timestep_k<true> : timestep_k<false>;
+  LAUNCH_F("f_timestep", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe,
            c->partials, ns, seg);
   before_launch(c, "calc_dt_final");
   if (n1 <= 16384) {
@@ -1175,7 +1218,8 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
   const Dev &d = c->d;
   const int seg = c->seg[SF_LAGRANGE];
   const int ns = ceil_div(d.nx, kOwnLagrange), nseg = ceil_div(d.ny, seg);
-  LAUNCH_F("f_lagrange", lagrange_k, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg);
+  const auto kern = d.geom_mesh ? lagrange_k<true> : lagrange_k<false>;
+  LAUNCH_F("f_lagrange", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg);
   return CLV_OK;
 }
 
@@ -1189,13 +1233,15 @@ static clv_status cm_sweep(clv_chunk *c, int sweep, int dir) {
   if (dir == CLV_XDIR) {
     const int ns = ceil_div(d.nx + 1, kOwnCmX), seg = c->seg[SF_CM_X];
     const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
-    if (sweep == 1) LAUNCH_F("f_cm_x1", cm_x_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
-    else LAUNCH_F("f_cm_x2", cm_x_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+    const auto k1 = d.geom_mesh ? cm_x_k<1, true> : cm_x_k<1, false>, k2 = d.geom_mesh ? cm_x_k<2, true> : cm_x_k<2, false>;
+    if (sweep == 1) LAUNCH_F("f_cm_x1", k1, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+    else LAUNCH_F("f_cm_x2", k2, g, d, ns, seg, rho_out, e_out, u_out, v_out);
   } else {
     const int ns = ceil_div(d.nx + 1, kOwnCmY), seg = c->seg[SF_CM_Y];
     const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
-    if (sweep == 1) LAUNCH_F("f_cm_y1", cm_y_k<1>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
-    else LAUNCH_F("f_cm_y2", cm_y_k<2>, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+    const auto k1 = d.geom_mesh ? cm_y_k<1, true> : cm_y_k<1, false>, k2 = d.geom_mesh ? cm_y_k<2, true> : cm_y_k<2, false>;
+    if (sweep == 1) LAUNCH_F("f_cm_y1", k1, g, d, ns, seg, rho_out, e_out, u_out, v_out);
+    else LAUNCH_F("f_cm_y2", k2, g, d, ns, seg, rho_out, e_out, u_out, v_out);
   }
   swap_fields(c, CLV_DENSITY1, CLV_WORK1);
   swap_fields(c, CLV_ENERGY1, CLV_WORK2);

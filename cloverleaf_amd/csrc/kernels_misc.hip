@@ -720,11 +720,16 @@ extern "C" clv_status clv_march_probe(int x_cells, int y_cells, int reads, int w
     return set_error(CLV_ERR_ARG, "clv_march_probe: bad argument");
   typedef void (*Kern)(MarchProbe);
   Kern kern = nullptr;
+  // the fused kernels' shapes with the geometry arrays read / derived from the mesh
   if (reads == 7 && writes == 1) kern = march_probe_k<7, 1>;
   if (reads == 8 && writes == 6) kern = march_probe_k<8, 6>;
   if (reads == 7 && writes == 5) kern = march_probe_k<7, 5>;
   if (reads == 6 && writes == 5) kern = march_probe_k<6, 5>;
-  if (!kern) return set_error(CLV_ERR_ARG, "clv_march_probe: (reads, writes) must be (7, 1), (8, 6), (7, 5) or (6, 5)");
+  if (reads == 4 && writes == 1) kern = march_probe_k<4, 1>;
+  if (reads == 5 && writes == 6) kern = march_probe_k<5, 6>;
+  if (reads == 5 && writes == 5) kern = march_probe_k<5, 5>;
+  if (!kern)
+    return set_error(CLV_ERR_ARG, "clv_march_probe: (reads, writes) must be (7, 1), (8, 6), (7, 5), (6, 5), (4, 1), (5, 6) or (5, 5)");
   MarchProbe p{};
   p.pitch = pitch_for(x_cells);
   p.nx = x_cells;
@@ -767,6 +772,7 @@ extern "C" clv_status clv_march_probe(int x_cells, int y_cells, int reads, int w
 namespace clv {
 
 clv_status k_fill_zero(clv_chunk *c) {
+  c->d.geom_mesh = 0;  // volume and the areas become zero
   const Dev &d = c->d;
   const size_t n2 = (size_t)(d.ny + 5) * (size_t)d.pitch;
   for (int f = 0; f < CLV_NUM_FIELDS; ++f) {
@@ -791,6 +797,9 @@ clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg, int t_left, i
   LAUNCH_G("initialise_chunk", init_mesh_k, dim3((n + 255) / 256, 1, 1), dim3(256, 1, 1), d, xmin, ymin, dx, dy);
   const dim3 g((unsigned)((d.nx + 4 + 63) / 64), (unsigned)((d.ny + 4 + 3) / 4), 1);
   LAUNCH_G("initialise_chunk", init_geom_k, g, dim3(64, 4, 1), d, dx * dy);
+  // volume = dx * dy, xarea = celldy, yarea = celldx: the fused kernels derive them
+  c->d.dxdy = dx * dy;
+  c->d.geom_mesh = 1;
   return CLV_OK;
 }
 

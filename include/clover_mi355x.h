@@ -128,7 +128,16 @@ clv_status clv_chunk_create(int x_cells, int y_cells, clv_chunk **out);
  * storage); the caller keeps them alive while the chunk exists. */
 clv_status clv_chunk_wrap(const clv_chunk_view *view, clv_chunk **out);
 clv_status clv_chunk_destroy(clv_chunk *c);
+/* The view's pointers allow writes to any array, so taking a view also switches the
+ * fused plan to reading volume, xarea and yarea (see clv_chunk_geometry_from_mesh). */
 clv_status clv_chunk_get_view(const clv_chunk *c, clv_chunk_view *out);
+/* 1 while the chunk's volume, xarea and yarea hold what clv_initialise_chunk wrote
+ * (volume = dx * dy, xarea(i, j) = celldy(j), yarea(i, j) = celldx(i)): the fused plan
+ * then computes them from the mesh instead of reading the three arrays (no counterpart
+ * in the reference, whose kernels always read them).  clv_initialise_chunk sets it;
+ * clv_build_field, clv_field_upload of those fields, clv_mesh_upload of celldx or
+ * celldy and clv_chunk_get_view clear it.  0 otherwise, -1 for a null chunk. */
+int clv_chunk_geometry_from_mesh(const clv_chunk *c);
 /* Orders the chunk's work on a caller stream (a hipStream_t passed as void*;
  * NULL = the null stream). */
 clv_status clv_chunk_set_stream(clv_chunk *c, void *hip_stream);
@@ -309,7 +318,10 @@ clv_status clv_batch_deck(uint64_t seed, int index, char *text, size_t len);
 /* Members: n owned chunks on the current device, already started, with their
  * configs and driver states (all with the same advect_x).  hist_len: dt values
  * kept per member (step s at index s - 1).  While a chunk is in a batch, step it
- * only through the batch; reading it (downloads, summaries) is fine between runs. */
+ * only through the batch; reading it (downloads, summaries) is fine between runs.
+ * If every member's volume and face areas come from its mesh at creation
+ * (clv_chunk_geometry_from_mesh), the batch computes them for all members, and
+ * clv_batch_run refuses to run once a member's flag has been cleared. */
 clv_status clv_batch_create(clv_chunk *const *chunks, const clv_config *cfgs, const clv_hydro_state *states, int n, int hist_len,
                             clv_batch **out);
 clv_status clv_batch_destroy(clv_batch *b);
@@ -402,8 +414,9 @@ clv_status clv_bandwidth_probe(size_t bytes, int iters, double *copy_gbps);
  * x-fastest on one XCD band each) over the interior of an x_cells * y_cells
  * grid, reading `reads` and writing `writes` FP64 arrays per cell with no
  * arithmetic; the arrays are padded like a chunk's, element (2, j) on a 128-byte
- * line.  (reads, writes) is (7, 1), (8, 6), (7, 5) or (6, 5).  Allocates and
- * frees the arrays; *ms_per_launch = the mean of `iters` timed launches. */
+ * line.  (reads, writes) is (7, 1), (8, 6), (7, 5), (6, 5), (4, 1), (5, 6) or (5, 5).
+ * Allocates and frees the arrays; *ms_per_launch = the mean of `iters` timed
+ * launches. */
 clv_status clv_march_probe(int x_cells, int y_cells, int reads, int writes, int seg, int iters, double *ms_per_launch);
 
 #ifdef __cplusplus

@@ -168,6 +168,14 @@ def test_batch_rejects_bad_use(gpu, small_decks):
         chunks[1].step(cfgs[1], st, clv.PLAN_FUSED)
         with pytest.raises(clv.CloverError, match="outside the batch"):
             b.run(1)
+    # the batch computes the members' volumes and areas from their meshes: once a
+    # member's arrays may have changed (a view was taken) it refuses to run
+    cfgs, chunks, states = _start(small_decks[:2])
+    with clv.Batch(chunks, cfgs, states) as b:
+        b.run(1)
+        chunks[0].field_ptr("volume")
+        with pytest.raises(clv.CloverError, match="face areas may have changed"):
+            b.run(1)
 
 
 def test_batch_small_dt_member_stops(gpu, small_decks, small_plain):

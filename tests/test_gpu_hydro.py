@@ -113,6 +113,55 @@ def test_fused_plan_strip_and_segment_edges(gpu, nx, ny):
     assert s_gpu == o.summary()
 
 
+def test_fused_plan_reads_changed_geometry(gpu):
+    """The fused plan computes volume and the face areas from the mesh while they hold
+    what initialise_chunk wrote; once they are changed (here scaled by 1 + U[0, 1e-3) per
+    element and uploaded) it reads the arrays: every step's dt and the state bit-identical
+    to the oracle run on the same changed arrays."""
+    nx, ny = 67, 45
+    text = _geom_deck(nx, ny, 8)
+    cfg = clv.parse_deck(text)
+    o = T.Oracle(T.parse_deck(text))
+    o.start()
+    rng = np.random.default_rng(7)
+    with clv.Chunk(nx, ny) as ch:
+        st, _ = ch.start(cfg)
+        assert ch.geometry_from_mesh()
+        for name in ("volume", "xarea", "yarea"):
+            a = o.f(name)
+            a *= 1.0 + 1e-3 * rng.random(a.shape)
+            ch.upload(name, a.copy())
+        assert not ch.geometry_from_mesh()
+        while not st.complete:
+            ch.step(cfg, st, clv.PLAN_FUSED)
+            assert o.step() == 0
+            assert (st.step, st.dt) == (o.g.step, o.g.dt), st.step
+            bad = _fused_state_equal(ch, o, nx, ny)
+            assert not bad, (st.step, bad)
+
+
+def test_geometry_flag_follows_the_arrays(gpu):
+    """initialise_chunk sets the flag; build_field, uploads of volume / xarea / yarea /
+    celldx / celldy and taking a view clear it; other uploads keep it."""
+    cfg = _cfg("small_geom_mix")
+    with clv.Chunk(cfg.x_cells, cfg.y_cells) as ch:
+        assert not ch.geometry_from_mesh()
+        ch.start(cfg)
+        assert ch.geometry_from_mesh()
+        ch.upload("density0", ch.download("density0"))
+        ch.upload_mesh("cellx", ch.download_mesh("cellx"))
+        assert ch.geometry_from_mesh()
+        for clear in (lambda: ch.upload("volume", ch.download("volume")), lambda: ch.upload("xarea", ch.download("xarea")),
+                      lambda: ch.upload("yarea", ch.download("yarea")),
+                      lambda: ch.upload_mesh("celldx", ch.download_mesh("celldx")),
+                      lambda: ch.upload_mesh("celldy", ch.download_mesh("celldy")), lambda: ch.field_ptr("density0"),
+                      ch.build_field):
+            ch.start(cfg)
+            assert ch.geometry_from_mesh()
+            clear()
+            assert not ch.geometry_from_mesh()
+
+
 def test_plans_can_alternate(gpu):
     """Fused and reference steps interleave on one chunk with the same result."""
     text = _geom_deck(130, 97, 10)
