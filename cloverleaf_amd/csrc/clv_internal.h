@@ -55,7 +55,7 @@ struct clv_chunk {
   double *result_host = nullptr;
   clv::Profiler *prof = nullptr;
   int summary_serial = 0;  // CLV_SUMMARY_SERIAL: reproduce the serial model's summation order
-  int seg[8] = {0};        // rows per segment of the fused plan's tile families (chosen on first use)
+  int seg[4] = {0};        // rows per segment of the fused plan's kernel families (chosen on first use)
 };
 
 namespace clv {

@@ -1131,15 +1131,15 @@ hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, Batch
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-// Rows per wave segment of each plain launch, per tile family (timestep,
-// lagrange, cell x, cell y, mom x, mom y, cell + mom x, cell + mom y).  Longer segments re-read fewer window
-// rows at segment edges; shorter ones give more, shorter workgroups (a smaller
-// tail, a tighter working set per XCD L2).  At 15360^2 (ms per launch over
-// segments of 8 ... 256 rows) the kernels with x shuffles are fastest at 8-16
-// rows and the per-lane y sweeps at 128.  On a smaller grid a family's segment
-// is halved until it has 4 workgroups per CU (or 8 rows).
-enum { SF_TIMESTEP, SF_LAGRANGE, SF_CELL_X, SF_CELL_Y, SF_MOM_X, SF_MOM_Y, SF_CM_X, SF_CM_Y };
-constexpr int kSegLarge[8] = {16, 16, 8, 128, 8, 128, 8, 128};
+// Rows per wave segment of each plain launch, per kernel family (timestep,
+// lagrange, x advection pass, y advection pass).  Longer segments re-read fewer
+// window rows at segment edges; shorter ones give more, shorter workgroups (a
+// smaller tail, a tighter working set per XCD L2).  At 15360^2 (ms per launch over
+// segments of 8 ... 256 rows) the row marches are fastest at 8-16 rows and the
+// per-lane y walk at 128 (64 and 256 within 1 %).  On a smaller grid a family's
+// segment is halved until it has 4 workgroups per CU (or 8 rows).
+enum { SF_TIMESTEP, SF_LAGRANGE, SF_CM_X, SF_CM_Y, SF_COUNT };
+constexpr int kSegLarge[SF_COUNT] = {16, 16, 8, 128};
 constexpr int kSegMin = 8;
 
 static void choose_segments(clv_chunk *c) {
@@ -1147,10 +1147,9 @@ static void choose_segments(clv_chunk *c) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1) cus = 256;
   const int nx = c->d.nx, ny = c->d.ny;
-  const int strips[8] = {ceil_div(nx, kOwnTimestep), ceil_div(nx, kOwnLagrange), ceil_div(nx, 60),          ceil_div(nx, 64),
-                         ceil_div(nx + 1, 60),        ceil_div(nx + 1, 64),        ceil_div(nx + 1, kOwnCmX), ceil_div(nx + 1, kOwnCmY)};
-  const int rows[8] = {ny, ny, ny, ny, ny + 1, ny + 1, ny + 1, ny + 1};
-  for (int f = 0; f < 8; ++f) {
+  const int strips[SF_COUNT] = {ceil_div(nx, kOwnTimestep), ceil_div(nx, kOwnLagrange), ceil_div(nx + 1, kOwnCmX), ceil_div(nx + 1, kOwnCmY)};
+  const int rows[SF_COUNT] = {ny, ny, ny + 1, ny + 1};
+  for (int f = 0; f < SF_COUNT; ++f) {
     int seg = kSegLarge[f];
     while (seg > kSegMin && ceil_div(strips[f], WAVES) * ceil_div(rows[f], seg) < 4 * cus) seg /= 2;
     c->seg[f] = seg;
