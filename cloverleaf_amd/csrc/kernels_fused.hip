@@ -746,12 +746,13 @@ struct CmHist {
   }
   __device__ double rho(int q) const { return q == 0 ? r0 : q == 2 ? r2 : q == 4 ? r4 : q == 6 ? r6 : r3; }
 };
-template <int SWEEP, bool MESH, class V, class O>
+template <int SWEEP, bool MESH, bool LOCKSTEP = true, class V, class O>
 __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 2 : 4;
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
-  const bool live = strip < nstrips;  // past the chunk: walk on clamped loads, own nothing
+  const bool live = strip < nstrips;  // past the chunk: walk on clamped loads, own nothing (or leave)
+  if (!LOCKSTEP && !live) return;
   const int nx = d.nx, ny = d.ny;
   const int i = kOwnCmY * strip + lane;
   const int ic = i < nx + 3 ? i : nx + 3;
@@ -926,9 +927,11 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       v_m1 = v_0;
       v_0 = v_p1;
     }
-    // the block's four walks stay on the same rows (3-6 % faster at 15360^2 than
-    // free-running waves; F, j1 are uniform over the block)
-    __syncthreads();
+    // LOCKSTEP: the block's four walks stay on the same rows (3-6 % faster at 15360^2
+    // than free-running waves; F, j1 are uniform over the block).  The batched form
+    // runs them free, and a walk past its problem's last strip exits at once: over
+    // many small problems that is 7 % faster for the whole batch.
+    if (LOCKSTEP) __syncthreads();
     cur = nxt;
     u_ahead = u_n;
     v_ahead = v_n;
@@ -1047,8 +1050,8 @@ __global__ __launch_bounds__(256) void batch_advec_k(const Dev *__restrict__ tab
     const int j0 = 2 + (int)it.k * it.seg, jn1 = min(j0 + (int)it.seg, d.ny + 3);
     if constexpr (KIND == BK_CM_X1) cm_x_tile<1, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
     if constexpr (KIND == BK_CM_X2) cm_x_tile<2, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_Y1) cm_y_tile<1, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_Y2) cm_y_tile<2, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y1) cm_y_tile<1, MESH, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y2) cm_y_tile<2, MESH, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
   }
 }
 
