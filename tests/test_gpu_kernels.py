@@ -146,6 +146,18 @@ def test_setup_kernels_match_oracle(gpu):
             assert np.array_equal(got, o.m(n)[:len(got)]), n
 
 
+def test_measurement_probes(gpu):
+    """The copy-bandwidth and access-pattern probes run and time something; the
+    pattern probe accepts only the fused kernels' shapes."""
+    assert clv.bandwidth_probe(64 << 20, 2) > 0.0
+    for r, w in [(4, 1), (5, 6), (6, 5), (5, 5), (7, 1), (8, 6), (7, 5)]:
+        assert clv.march_probe(300, 200, r, w, 16, 2) > 0.0
+    with pytest.raises(clv.CloverError, match="reads, writes"):
+        clv.march_probe(300, 200, 3, 3, 16, 1)
+    with pytest.raises(clv.CloverError, match="bad argument"):
+        clv.march_probe(0, 200, 4, 1, 16, 1)
+
+
 def test_bad_arguments_are_rejected(gpu):
     with clv.Chunk(16, 16) as ch:
         with pytest.raises(clv.CloverError):
