@@ -2,6 +2,7 @@
 #include <vector>
 
 #include "clv_internal.h"
+#include "geom.h"
 
 namespace clv {
 namespace {
@@ -269,13 +270,13 @@ __global__ __launch_bounds__(RED_THREADS) void calc_dt_k(Dev d, double dtc_safe,
       cc = fmax(sqrt(cc), g_small);
       double dtct = dtc_safe * fmin(dsx, dsy) / cc;
       double div = 0.0;
-      const double vol = AT(CLV_VOLUME, i, j);
-      double dv1 = (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1)) * AT(CLV_XAREA, i, j);
-      double dv2 = (AT(CLV_XVEL0, i + 1, j) + AT(CLV_XVEL0, i + 1, j + 1)) * AT(CLV_XAREA, i + 1, j);
+      const double vol = geom_volume(d, i, j);
+      double dv1 = (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1)) * geom_xarea(d, i, j);
+      double dv2 = (AT(CLV_XVEL0, i + 1, j) + AT(CLV_XVEL0, i + 1, j + 1)) * geom_xarea(d, i + 1, j);
       div = div + dv2 - dv1;
       double dtut = dtu_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
-      dv1 = (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j)) * AT(CLV_YAREA, i, j);
-      dv2 = (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1)) * AT(CLV_YAREA, i, j + 1);
+      dv1 = (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j)) * geom_yarea(d, i, j);
+      dv2 = (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1)) * geom_yarea(d, i, j + 1);
       div = div + dv2 - dv1;
       double dtvt = dtv_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
       div = div / (2.0 * vol);
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(RED_THREADS) void field_summary_k(Dev d, double *pa
           vsqrd += 0.25 * (u * u + v * v);
         }
       }
-      double cell_vol = AT(CLV_VOLUME, j, k);
+      double cell_vol = geom_volume(d, j, k);
       double cell_mass = cell_vol * AT(CLV_DENSITY0, j, k);
       vol += cell_vol;
       mass += cell_mass;
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(RED_THREADS) void field_summary_serial_k(Dev d, dou
           vsqrd += 0.25 * (u * u + v * v);
         }
       }
-      const double cell_vol = AT(CLV_VOLUME, j, k);
+      const double cell_vol = geom_volume(d, j, k);
       const double cell_mass = cell_vol * AT(CLV_DENSITY0, j, k);
       terms[0][t] = cell_vol;
       terms[1][t] = cell_mass;

@@ -6,6 +6,7 @@
 // of 64 x 4 threads, one wave64 per row segment so that every wave reads and
 // writes whole 128-byte lines of x-contiguous rows; one thread per cell/node.
 #include "clv_internal.h"
+#include "geom.h"
 
 namespace clv {
 namespace {
@@ -81,22 +82,22 @@ __global__ __launch_bounds__(256) void pdv_k(Dev d, double dt) {
   const int V1 = PREDICT ? CLV_YVEL0 : CLV_YVEL1;
   double left_flux, right_flux, bottom_flux, top_flux;
   if (PREDICT) {
-    left_flux = (AT(CLV_XAREA, i, j) * (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1) + AT(U1, i, j) + AT(U1, i, j + 1))) * 0.25 * dt * 0.5;
-    right_flux = (AT(CLV_XAREA, i + 1, j) * (AT(CLV_XVEL0, i + 1, j) + AT(CLV_XVEL0, i + 1, j + 1) + AT(U1, i + 1, j) + AT(U1, i + 1, j + 1))) *
+    left_flux = (geom_xarea(d, i, j) * (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1) + AT(U1, i, j) + AT(U1, i, j + 1))) * 0.25 * dt * 0.5;
+    right_flux = (geom_xarea(d, i + 1, j) * (AT(CLV_XVEL0, i + 1, j) + AT(CLV_XVEL0, i + 1, j + 1) + AT(U1, i + 1, j) + AT(U1, i + 1, j + 1))) *
                  0.25 * dt * 0.5;
-    bottom_flux = (AT(CLV_YAREA, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(V1, i, j) + AT(V1, i + 1, j))) * 0.25 * dt * 0.5;
-    top_flux = (AT(CLV_YAREA, i, j + 1) * (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1) + AT(V1, i, j + 1) + AT(V1, i + 1, j + 1))) *
+    bottom_flux = (geom_yarea(d, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(V1, i, j) + AT(V1, i + 1, j))) * 0.25 * dt * 0.5;
+    top_flux = (geom_yarea(d, i, j + 1) * (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1) + AT(V1, i, j + 1) + AT(V1, i + 1, j + 1))) *
                0.25 * dt * 0.5;
   } else {
-    left_flux = (AT(CLV_XAREA, i, j) * (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1) + AT(U1, i, j) + AT(U1, i, j + 1))) * 0.25 * dt;
-    right_flux = (AT(CLV_XAREA, i + 1, j) * (AT(CLV_XVEL0, i + 1, j) + AT(CLV_XVEL0, i + 1, j + 1) + AT(U1, i + 1, j) + AT(U1, i + 1, j + 1))) *
+    left_flux = (geom_xarea(d, i, j) * (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1) + AT(U1, i, j) + AT(U1, i, j + 1))) * 0.25 * dt;
+    right_flux = (geom_xarea(d, i + 1, j) * (AT(CLV_XVEL0, i + 1, j) + AT(CLV_XVEL0, i + 1, j + 1) + AT(U1, i + 1, j) + AT(U1, i + 1, j + 1))) *
                  0.25 * dt;
-    bottom_flux = (AT(CLV_YAREA, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(V1, i, j) + AT(V1, i + 1, j))) * 0.25 * dt;
-    top_flux = (AT(CLV_YAREA, i, j + 1) * (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1) + AT(V1, i, j + 1) + AT(V1, i + 1, j + 1))) *
+    bottom_flux = (geom_yarea(d, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(V1, i, j) + AT(V1, i + 1, j))) * 0.25 * dt;
+    top_flux = (geom_yarea(d, i, j + 1) * (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1) + AT(V1, i, j + 1) + AT(V1, i + 1, j + 1))) *
                0.25 * dt;
   }
   double total_flux = right_flux - left_flux + top_flux - bottom_flux;
-  const double vol = AT(CLV_VOLUME, i, j);
+  const double vol = geom_volume(d, i, j);
   double volume_change_s = vol / (vol + total_flux);
   double recip_volume = 1.0 / vol;
   const double rho0 = AT(CLV_DENSITY0, i, j);
@@ -115,11 +116,11 @@ __global__ __launch_bounds__(256) void revert_k(Dev d) {
 // ---- accelerate: src/serial/accelerate.cpp:41-55 -----------------------------------------
 __global__ __launch_bounds__(256) void accelerate_k(Dev d, double halfdt) {
   RANGE_GUARD(2, d.nx + 3, 2, d.ny + 3)
-  double stepbymass_s = halfdt / ((AT(CLV_DENSITY0, i - 1, j - 1) * AT(CLV_VOLUME, i - 1, j - 1) + AT(CLV_DENSITY0, i - 1, j) * AT(CLV_VOLUME, i - 1, j) +
-                                   AT(CLV_DENSITY0, i, j) * AT(CLV_VOLUME, i, j) + AT(CLV_DENSITY0, i, j - 1) * AT(CLV_VOLUME, i, j - 1)) *
+  double stepbymass_s = halfdt / ((AT(CLV_DENSITY0, i - 1, j - 1) * geom_volume(d, i - 1, j - 1) + AT(CLV_DENSITY0, i - 1, j) * geom_volume(d, i - 1, j) +
+                                   AT(CLV_DENSITY0, i, j) * geom_volume(d, i, j) + AT(CLV_DENSITY0, i, j - 1) * geom_volume(d, i, j - 1)) *
                                   0.25);
-  const double xa0 = AT(CLV_XAREA, i, j), xa1 = AT(CLV_XAREA, i, j - 1);
-  const double ya0 = AT(CLV_YAREA, i, j), ya1 = AT(CLV_YAREA, i - 1, j);
+  const double xa0 = geom_xarea(d, i, j), xa1 = geom_xarea(d, i, j - 1);
+  const double ya0 = geom_yarea(d, i, j), ya1 = geom_yarea(d, i - 1, j);
   const double p00 = AT(CLV_PRESSURE, i, j), pm0 = AT(CLV_PRESSURE, i - 1, j), p0m = AT(CLV_PRESSURE, i, j - 1), pmm = AT(CLV_PRESSURE, i - 1, j - 1);
   const double q00 = AT(CLV_VISCOSITY, i, j), qm0 = AT(CLV_VISCOSITY, i - 1, j), q0m = AT(CLV_VISCOSITY, i, j - 1), qmm = AT(CLV_VISCOSITY, i - 1, j - 1);
   double u = AT(CLV_XVEL0, i, j) - stepbymass_s * (xa0 * (p00 - pm0) + xa1 * (p0m - pmm));
@@ -134,9 +135,9 @@ __global__ __launch_bounds__(256) void accelerate_k(Dev d, double halfdt) {
 __global__ __launch_bounds__(256) void flux_calc_k(Dev d, double dt) {
   RANGE_GUARD(2, d.nx + 3, 2, d.ny + 3)
   AT(CLV_VOL_FLUX_X, i, j) =
-      0.25 * dt * AT(CLV_XAREA, i, j) * (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1) + AT(CLV_XVEL1, i, j) + AT(CLV_XVEL1, i, j + 1));
+      0.25 * dt * geom_xarea(d, i, j) * (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1) + AT(CLV_XVEL1, i, j) + AT(CLV_XVEL1, i, j + 1));
   AT(CLV_VOL_FLUX_Y, i, j) =
-      0.25 * dt * AT(CLV_YAREA, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(CLV_YVEL1, i, j) + AT(CLV_YVEL1, i + 1, j));
+      0.25 * dt * geom_yarea(d, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(CLV_YVEL1, i, j) + AT(CLV_YVEL1, i + 1, j));
 }
 
 // ---- reset_field: src/serial/reset_field.cpp:35-50 (both loops in one launch) ------------
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(256) void reset_field_k(Dev d) {
 template <int DIR, int SWEEP>
 __global__ __launch_bounds__(256) void advec_cell_vol_k(Dev d) {
   RANGE_GUARD(0, d.nx + 4, 0, d.ny + 4)
-  const double vol = AT(CLV_VOLUME, i, j);
+  const double vol = geom_volume(d, i, j);
   double pre, post;
   if (DIR == 1) {
     const double fx = AT(CLV_VOL_FLUX_X, i + 1, j) - AT(CLV_VOL_FLUX_X, i, j);
@@ -258,16 +259,16 @@ __global__ __launch_bounds__(256) void advec_mom_vol_k(Dev d) {
   RANGE_GUARD(0, d.nx + 4, 0, d.ny + 4)
   double post, pre;
   if (MOM_SWEEP == 1) {
-    post = AT(CLV_VOLUME, i, j) + AT(CLV_VOL_FLUX_Y, i, j + 1) - AT(CLV_VOL_FLUX_Y, i, j);
+    post = geom_volume(d, i, j) + AT(CLV_VOL_FLUX_Y, i, j + 1) - AT(CLV_VOL_FLUX_Y, i, j);
     pre = post + AT(CLV_VOL_FLUX_X, i + 1, j) - AT(CLV_VOL_FLUX_X, i, j);
   } else if (MOM_SWEEP == 2) {
-    post = AT(CLV_VOLUME, i, j) + AT(CLV_VOL_FLUX_X, i + 1, j) - AT(CLV_VOL_FLUX_X, i, j);
+    post = geom_volume(d, i, j) + AT(CLV_VOL_FLUX_X, i + 1, j) - AT(CLV_VOL_FLUX_X, i, j);
     pre = post + AT(CLV_VOL_FLUX_Y, i, j + 1) - AT(CLV_VOL_FLUX_Y, i, j);
   } else if (MOM_SWEEP == 3) {
-    post = AT(CLV_VOLUME, i, j);
+    post = geom_volume(d, i, j);
     pre = post + AT(CLV_VOL_FLUX_Y, i, j + 1) - AT(CLV_VOL_FLUX_Y, i, j);
   } else {
-    post = AT(CLV_VOLUME, i, j);
+    post = geom_volume(d, i, j);
     pre = post + AT(CLV_VOL_FLUX_X, i + 1, j) - AT(CLV_VOL_FLUX_X, i, j);
   }
   AT(CLV_WORK6, i, j) = post;
