@@ -129,12 +129,13 @@ clv_status clv_chunk_create(int x_cells, int y_cells, clv_chunk **out);
 clv_status clv_chunk_wrap(const clv_chunk_view *view, clv_chunk **out);
 clv_status clv_chunk_destroy(clv_chunk *c);
 /* The view's pointers allow writes to any array, so taking a view also switches the
- * fused plan to reading volume, xarea and yarea (see clv_chunk_geometry_from_mesh). */
+ * kernels to reading volume, xarea and yarea (see clv_chunk_geometry_from_mesh). */
 clv_status clv_chunk_get_view(const clv_chunk *c, clv_chunk_view *out);
 /* 1 while the chunk's volume, xarea and yarea hold what clv_initialise_chunk wrote
- * (volume = dx * dy, xarea(i, j) = celldy(j), yarea(i, j) = celldx(i)): the fused plan
- * then computes them from the mesh instead of reading the three arrays (no counterpart
- * in the reference, whose kernels always read them).  clv_initialise_chunk sets it;
+ * (volume = dx * dy, xarea(i, j) = celldy(j), yarea(i, j) = celldx(i)): the kernels
+ * (both plans, the per-kernel entry points, calc_dt and field_summary) then compute
+ * them from the mesh instead of reading the three arrays (no counterpart in the
+ * reference, whose kernels always read them).  clv_initialise_chunk sets it;
  * clv_build_field, clv_field_upload of those fields, clv_mesh_upload of celldx or
  * celldy and clv_chunk_get_view clear it.  0 otherwise, -1 for a null chunk. */
 int clv_chunk_geometry_from_mesh(const clv_chunk *c);
