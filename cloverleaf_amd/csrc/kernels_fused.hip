@@ -29,7 +29,10 @@
 // shuffles); the lanes at a strip's edges only supply those values, so strips
 // overlap by the stencil width and each lane of the interior writes results.
 // Inputs needed at x+1 / x-1 also come from shuffles where the lane ranges
-// allow, or from a second (cache-hitting) load.
+// allow, or from a second (cache-hitting) load.  lagrange_k and cm_x_k store each
+// row through LDS so that every store instruction writes whole 128-byte lines
+// (BlockRows).  Cell volumes and face areas come from the 1D mesh while the
+// chunk's arrays hold what initialise_chunk wrote (geo_*).
 //
 // In-place hazards: a kernel that updates a field its neighbours' stencils read
 // (rho1, e1 in advec_cell; u1, v1 in advec_mom) writes into a work array, and
