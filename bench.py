@@ -2,19 +2,26 @@
 
 One step = one pass of driver/hydro.cpp:57-89 (timestep with the calc_dt
 minimum, PdV predict/correct, accelerate, flux_calc, both advection sweeps,
-reset_field) over the whole chunk, everything resident in HBM.  Default
-workload: InputDecks/clover_bm256_short.in (15360 x 15360 cells, the two-state
-ideal-gas box, FP64).  Metric: cell-steps/s = x_cells * y_cells * steps / wall
-time (driver/hydro.cpp:213-217).
+reset_field) over the whole chunk, everything resident in HBM, with the
+field_summary of driver/hydro.cpp:92-94 every summary_frequency steps inside the
+timed loop (the reference's grind time includes it).  Default workload:
+InputDecks/clover_bm256_short.in (15360 x 15360 cells, the two-state ideal-gas
+box, FP64), the volume and face areas read from their arrays.  Metric:
+cell-steps/s = x_cells * y_cells * steps / wall time (driver/hydro.cpp:213-217).
 
 Prints ONE JSON line.  `roofline` is the dominant kernel (largest share of the
 timed region by HIP events on the chunk's stream): achieved = its algorithmic
-bytes per launch / its mean launch duration, against 8 TB/s.  Unless --no-probe,
-`pattern_ceiling` times each fused kernel's access pattern without arithmetic
-(clv_march_probe: the same march, arrays read and written), the ceiling of that
-kernel on this GPU.  `cpu_baseline`
-times the OpenMP CPU oracle (oracle/liboracle.so) on a bounded sample of the
-same problem.
+bytes per launch / its mean launch duration, against 8 TB/s.
+`geometry_from_mesh` (unless --no-mesh) is the same run with the geometry
+computed from the 1D spacings, a separately labelled figure.  `dropin` (unless
+--no-dropin) runs the reference's own driver with the MI355X plugin model on the
+same deck trimmed to --dropin-steps and reports the driver's own grind time.
+Unless --no-probe, `pattern_ceiling` times each fused kernel's access pattern
+without arithmetic (clv_march_probe: the same march, arrays read and written),
+the ceiling of that kernel on this GPU.  `cpu_baseline` is the reference's own
+omp model (oracle/_ref/omp-cloverleaf, compiled from the reference's sources) on
+a bounded sample, with the OpenMP CPU oracle on the same sample beside it
+(`cpu_baseline.port`).
 
 `batched` (unless --no-batch) runs the seeded 64-problem workload (decks from
 clv_batch_deck, seed 355, 100 steps each) one problem at a time on the plain
@@ -28,9 +35,10 @@ against its forward pass, and checks it against central differences.
 chunk, as 4 tiles and as 4 chunks on the one GPU, 50 steps each.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--plan fused|reference]
-                       [--deck NAME] [--no-cpu] [--no-batch] [--batch-only]
-                       [--no-adjoint] [--adjoint-only] [--adjoint-directions D]
-                       [--no-decomposed] [--decomposed-only] [--profile-out FILE]
+                       [--deck NAME] [--no-cpu] [--no-mesh] [--no-dropin] [--no-batch]
+                       [--batch-only] [--no-adjoint] [--adjoint-only]
+                       [--adjoint-directions D] [--no-decomposed] [--decomposed-only]
+                       [--profile-out FILE]
 """
 import argparse
 import json
@@ -59,7 +67,8 @@ def _extents(nx, ny):
             "NMY": (nx + 1) * (ny + 3), "MFX": (nx + 2) * (ny + 1), "MFY": (nx + 1) * (ny + 2)}
 
 
-# name -> list of (arrays, extent) terms per launch
+# name -> list of (arrays, extent) terms per launch, counted with the volume and face
+# areas read from their arrays (the default geometry mode)
 ALG_TERMS = {
     "ideal_gas": [(4, "C")],                   # rho, e -> p, c
     "viscosity": [(5, "C")],                   # u0, v0, p, rho0 -> q
@@ -84,47 +93,49 @@ ALG_TERMS = {
     "advec_mom_flux_y": [(4, "MFY")],
     "advec_mom_update_x": [(5, "N")],          # vel, pre, mom_flux, post -> vel
     "advec_mom_update_y": [(5, "N")],
-    "field_summary": [(6, "C")],
-    # fused plan: compulsory traffic of each fused kernel (every input array read
-    # once, every output written once over the interior).  On a chunk initialised by
-    # initialise_chunk the volume and face areas come from the mesh (dx * dy, celldy,
-    # celldx) and are not streamed
-    "f_timestep": [(5, "C")],                  # rho0, e0, u0, v0 -> q
-    "f_lagrange": [(11, "C")],                 # rho0, e0, q, u0, v0 -> u1, v1, rho1, e1, vfx, vfy
-    "f_cell_x1": [(8, "C")],                   # vol, vfx, vfy, rho1, e1 -> rho1, e1, mfx
-    "f_cell_y1": [(8, "C")],
-    "f_cell_x2": [(7, "C")],                   # vol, vfx, rho1, e1 -> rho1, e1, mfx
-    "f_cell_y2": [(7, "C")],
-    "f_mom_x1": [(8, "N")],                    # vol, vfy, mfx, rho1, u1, v1 -> u1, v1
-    "f_mom_y1": [(8, "N")],
-    "f_mom_x2": [(7, "N")],                    # vol, mfx, rho1, u1, v1 -> u1, v1
-    "f_mom_y2": [(7, "N")],
-    "f_cm_x1": [(11, "C")],                    # vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
-    "f_cm_y1": [(11, "C")],                    # vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
-    "f_cm_x2": [(10, "C")],                    # vfx, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
-    "f_cm_y2": [(10, "C")],                    # vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
+    "field_summary": [(6, "C")],               # vol, rho0, e0, p, u0, v0
+    # fused plan: compulsory traffic of each fused kernel (every input array read once,
+    # every output written once over the interior)
+    "f_timestep": [(8, "C")],                  # rho0, e0, u0, v0, vol, xarea, yarea -> q
+    "f_lagrange": [(14, "C")],                 # rho0, e0, q, u0, v0, vol, xarea, yarea -> u1, v1, rho1, e1, vfx, vfy
+    "f_cm_x1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
+    "f_cm_y1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
+    "f_cm_x2": [(11, "C")],                    # vol, vfx, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
+    "f_cm_y2": [(11, "C")],                    # vol, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
 }
-# compulsory traffic of one fused step, array sweeps: timestep 5 + lagrange 11 + the
-# first advection pass 11 + the second 10
-B_FUSED_PER_CELL_STEP = 37 * 8.0
+# of those, the volume / face-area arrays a kernel does not stream when the chunk's
+# geometry comes from the mesh (CLV_GEOMETRY_MESH), on the first term's extent
+GEOM_ARRAYS = {"calc_dt": 3, "PdV_predict": 3, "PdV_correct": 3, "accelerate": 3, "flux_calc": 2, "advec_cell_vol": 1,
+               "advec_mom_vol": 1, "field_summary": 1, "f_timestep": 3, "f_lagrange": 3, "f_cm_x1": 1, "f_cm_y1": 1,
+               "f_cm_x2": 1, "f_cm_y2": 1}
+# compulsory traffic of one fused step, array sweeps: timestep 8 + lagrange 14 + the
+# first advection pass 12 + the second 11 (geometry read); 5 + 11 + 11 + 10 from the mesh
+B_FUSED_PER_CELL_STEP = {False: 45 * 8.0, True: 37 * 8.0}
 B_ALG_PER_CELL_STEP = 1440.0  # 180 array sweeps of 8 B per interior cell-step at reference kernel granularity
 
 
 # arrays read and written per interior cell by each fused kernel (the march probe's shape)
-PATTERN = {"f_timestep": (4, 1), "f_lagrange": (5, 6), "f_cm_x1": (6, 5), "f_cm_y1": (6, 5), "f_cm_x2": (5, 5), "f_cm_y2": (5, 5)}
+PATTERN = {False: {"f_timestep": (7, 1), "f_lagrange": (8, 6), "f_cm_x1": (7, 5), "f_cm_y1": (7, 5), "f_cm_x2": (6, 5),
+                   "f_cm_y2": (6, 5)},
+           True: {"f_timestep": (4, 1), "f_lagrange": (5, 6), "f_cm_x1": (6, 5), "f_cm_y1": (6, 5), "f_cm_x2": (5, 5),
+                  "f_cm_y2": (5, 5)}}
 
 
-def alg_bytes(name, nx, ny):
+def alg_bytes(name, nx, ny, geom_mesh=False):
     ext = _extents(nx, ny)
     terms = ALG_TERMS.get(name)
     if terms is None:
         return None
-    return sum(k * ext[e] for k, e in terms) * 8.0
+    total = sum(k * ext[e] for k, e in terms)
+    if geom_mesh and name in GEOM_ARRAYS:
+        total -= GEOM_ARRAYS[name] * ext[terms[0][1]]
+    return total * 8.0
 
 
 def cpu_baseline(deck_text, steps):
-    """The OpenMP oracle on a bounded sample: same problem (deck geometry and states)
-    on a 3840^2 grid, `steps` steps."""
+    """The OpenMP CPU oracle (oracle/liboracle.so, this build's restatement of the
+    reference) on a bounded sample: same problem (deck geometry and states) on a
+    3840^2 grid, `steps` steps."""
     sys.path.insert(0, str(REPO / "tests"))
     import clover_testlib as T
     d = T.parse_deck(deck_text)
@@ -358,6 +369,148 @@ def decomposed_leg(clv, deck="clover_bm16_short", pieces=4, steps=50, warmup=2):
     return out
 
 
+def headline_leg(clv, deck_text, plan, steps, warmup, geometry, profile_out):
+    """The timed loop of driver/hydro.cpp:57-111 on one chunk: `warmup` untimed steps,
+    then `steps` timed steps, each followed by field_summary when the step count is a
+    multiple of the deck's summary_frequency (driver/hydro.cpp:92-94), as the
+    reference's own grind time includes it.  Returns the rate and the per-kernel
+    HIP-event times of the timed region."""
+    cfg = clv.parse_deck(deck_text)
+    nx, ny = cfg.x_cells, cfg.y_cells
+    cfg.end_step = 1 << 30  # the bench decides the step count
+    cfg.end_time = 1e30
+    freq = cfg.summary_frequency
+    gname = "arrays" if geometry == clv.GEOMETRY_ARRAYS else "mesh"
+    print(f"# bench: {nx}x{ny} plan={plan} geometry={gname} warmup={warmup} steps={steps}", file=sys.stderr, flush=True)
+    ch = clv.Chunk(nx, ny, geometry)
+    st, _ = ch.start(cfg)
+    assert ch.geometry_from_mesh() == (geometry == clv.GEOMETRY_MESH)
+    for _ in range(warmup):
+        ch.step(cfg, st, plan)
+    ch.synchronize()
+    ch.profile(True)
+    ch.profile_reset()
+    summaries = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ch.step(cfg, st, plan)
+        if freq and st.step % freq == 0:
+            ch.summary()
+            summaries += 1
+    ch.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ch.profile_read()
+    ch.profile(False)
+    s_end = ch.summary()
+    ch.close()
+    print(f"# bench: {steps} steps in {wall:.3f} s ({nx * ny * steps / wall / 1e9:.3f} Gcell-steps/s), last dt {st.dt:.6g}, "
+          f"step {st.step}", file=sys.stderr, flush=True)
+    kern = sorted(((ms, name, n) for name, (n, ms) in prof.items()), reverse=True)
+    total_kernel_ms = sum(ms for ms, _, _ in kern)
+    mesh = geometry == clv.GEOMETRY_MESH
+    per_kernel = {}
+    for ms, name, n in kern:
+        b = alg_bytes(name, nx, ny, mesh)
+        per_kernel[name] = {"launches": n, "total_ms": round(ms, 3), "share": round(ms / total_kernel_ms, 4),
+                            "avg_us": round(ms / n * 1e3, 2),
+                            "alg_GBps": round(b / (ms / n / 1e3) / 1e9, 1) if b else None}
+    if profile_out:
+        Path(profile_out).write_text(json.dumps({"plan": plan, "geometry": gname, "steps": steps, "wall_s": wall,
+                                                 "kernels": per_kernel}, indent=1))
+    return {"nx": nx, "ny": ny, "value": nx * ny * steps / wall, "wall_s": wall, "per_kernel": per_kernel,
+            "dominant": kern[0][1], "kernel_ms": total_kernel_ms, "summaries": summaries, "summary_frequency": freq,
+            "final_summary": {"step": st.step, "ke": s_end[3], "mass": s_end[1]}}
+
+
+def _omp_env():
+    env = dict(os.environ)
+    threads = int(env.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    env.update({"OMP_NUM_THREADS": str(threads), "OMP_PLACES": "cores", "OMP_PROC_BIND": "true"})
+    return env, threads
+
+
+def _cpu_model():
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        info = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        return {"model": info.get("Model name", "").strip(), "sockets": info.get("Socket(s)", "").strip(),
+                "cores_per_socket": info.get("Core(s) per socket", "").strip(),
+                "threads_per_core": info.get("Thread(s) per core", "").strip()}
+    except Exception:
+        return {}
+
+
+def cpu_baseline_reference(steps, deck="clover_bm16_300"):
+    """The reference's own omp model (driver/ + src/omp/ compiled from the reference's
+    sources by oracle/Makefile into oracle/_ref/omp-cloverleaf) on a bounded sample:
+    InputDecks/clover_bm16_300.in (3840^2) trimmed to `steps` steps, OMP_PLACES=cores,
+    OMP_PROC_BIND=true.  The rate is the reference's own printed `Average time per cell`
+    (driver/hydro.cpp:212-215).  None where the binary was not built."""
+    import re
+    import subprocess
+    import tempfile
+    exe = REPO / "oracle" / "_ref" / "omp-cloverleaf"
+    if not exe.exists():
+        return None
+    text = (REPO / "tests" / "golden" / "decks" / f"{deck}.in").read_text()
+    text = re.sub(r"end_step=\d+", f"end_step={steps}", text)
+    env, threads = _omp_env()
+    with tempfile.TemporaryDirectory() as td:
+        (Path(td) / "deck.in").write_text(text)
+        t0 = time.perf_counter()
+        p = subprocess.run([str(exe), "--in", "deck.in", "--out", "clover.out"], cwd=td, env=env, capture_output=True,
+                           text=True, timeout=600)
+        whole = time.perf_counter() - t0
+    walls = re.findall(r"^ Wall clock\s+([0-9.eE+-]+)", p.stdout, re.M)
+    done = re.findall(r"^ Step (\d+) time", p.stdout, re.M)
+    if p.returncode != 0 or not walls or not done or int(done[-1]) != steps:
+        return {"value": None, "error": (p.stderr or p.stdout)[-500:]}
+    wall = float(walls[-1])
+    grind = wall / (steps * 3840.0 * 3840.0)
+    return {"value": 1.0 / grind, "unit": "cell-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"reference omp model (oracle/_ref/omp-cloverleaf, built from the reference's sources), "
+                      f"{deck}.in = 3840x3840 trimmed to {steps} steps, {threads} threads, OMP_PLACES=cores "
+                      f"OMP_PROC_BIND=true; grind time from its final hydro 'Wall clock' {wall:.2f} s: {grind:.5g} s "
+                      f"per cell-step; process {whole:.1f} s",
+            "cpu": _cpu_model()}
+
+
+def dropin_leg(deck_text, steps):
+    """The drop-in: the reference's unchanged driver (driver/*.cpp) linked with the MI355X
+    plugin model (cloverleaf_amd/model_mi355x/_dropin/mi355x-cloverleaf) on the headline
+    deck trimmed to `steps` steps; the rate is the driver's own `Average time per cell`
+    (driver/hydro.cpp:212-215: hydro wall clock over steps x cells, from the first step,
+    field summaries included)."""
+    import re
+    import subprocess
+    import tempfile
+    exe = REPO / "cloverleaf_amd" / "model_mi355x" / "_dropin" / "mi355x-cloverleaf"
+    if not exe.exists():
+        return {"value": None, "note": "drop-in binary not built (needs the reference tree at build time)"}
+    text = re.sub(r"end_step=\d+", f"end_step={steps}", deck_text)
+    with tempfile.TemporaryDirectory() as td:
+        (Path(td) / "deck.in").write_text(text)
+        p = subprocess.run([str(exe), "--in", "deck.in", "--out", "clover.out"], cwd=td, capture_output=True, text=True,
+                           timeout=900)
+    # the driver prints the grind lines after every step but the last; at the end it
+    # prints the whole hydro wall clock (driver/hydro.cpp:108-118, 205-218)
+    walls = re.findall(r"^ Wall clock\s+([0-9.eE+-]+)", p.stdout, re.M)
+    grinds = re.findall(r"^ Average time per cell\s+([0-9.eE+-]+)", p.stdout, re.M)
+    done = re.findall(r"^ Step (\d+) time", p.stdout, re.M)
+    if p.returncode != 0 or not walls or not done or int(done[-1]) != steps:
+        return {"value": None, "error": (p.stderr or p.stdout)[-800:]}
+    nx, ny = (int(re.search(rf"{k}=(\d+)", text).group(1)) for k in ("x_cells", "y_cells"))
+    wall = float(walls[-1])
+    grind = wall / (steps * float(nx) * ny)
+    return {"value": 1.0 / grind, "unit": "cell-steps/s", "steps": steps, "hydro_wall_s": wall,
+            "last_printed_grind_s": float(grinds[-1]) if grinds else None,
+            "effective_hbm_frac_at_1440B": B_ALG_PER_CELL_STEP / grind / (HBM_PEAK_GBS * 1e9),
+            "workload": f"the headline deck trimmed to end_step={steps}, run by the reference's own driver with the "
+                        f"MI355X model; rate = steps x cells / its final printed hydro 'Wall clock' (the grind time of "
+                        f"driver/hydro.cpp:212-215, {grind:.5g} s per cell-step, field summaries included)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,7 +530,10 @@ def main():
     ap.add_argument("--decomposed-only", action="store_true", help="only the decomposed-domain workload")
     ap.add_argument("--probe-only", action="store_true",
                     help="only run the copy kernel (4 GiB, 20 launches): the counter calibration run")
-    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--cpu-steps", type=int, default=60, help="steps of the CPU baselines on the 3840^2 sample")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the timed run with the geometry from the mesh")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (reference driver) leg")
+    ap.add_argument("--dropin-steps", type=int, default=30)
     ap.add_argument("--profile-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -401,53 +557,20 @@ def main():
         return
     deck_path = REPO / "tests" / "golden" / "decks" / f"{args.deck}.in"
     deck_text = deck_path.read_text()
-    cfg = clv.parse_deck(deck_text)
     plan = clv.PLAN_FUSED if args.plan == "fused" else clv.PLAN_REFERENCE
-    nx, ny = cfg.x_cells, cfg.y_cells
-    cfg.end_step = 1 << 30  # the bench decides the step count
-    cfg.end_time = 1e30
 
-    print(f"# bench: {args.deck} {nx}x{ny} plan={args.plan} warmup={args.warmup} steps={args.steps}", file=sys.stderr, flush=True)
-    ch = clv.Chunk(nx, ny)
-    st, s0 = ch.start(cfg)
-    geom_mesh = ch.geometry_from_mesh()  # volume and face areas computed from the mesh
-    for _ in range(args.warmup):
-        ch.step(cfg, st, plan)
-    ch.synchronize()
-    ch.profile(True)
-    ch.profile_reset()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ch.step(cfg, st, plan)
-    ch.synchronize()
-    wall = time.perf_counter() - t0
-    prof = ch.profile_read()
-    ch.profile(False)
-    s_end = ch.summary()
-    print(f"# bench: {args.steps} steps in {wall:.3f} s, last dt {st.dt:.6g}, step {st.step}", file=sys.stderr, flush=True)
-
-    cell_steps = float(nx) * ny * args.steps
-    value = cell_steps / wall
-    # dominant kernel
-    kern = sorted(((ms, name, n) for name, (n, ms) in prof.items()), reverse=True)
-    total_kernel_ms = sum(ms for ms, _, _ in kern)
-    dom_ms, dom, dom_n = kern[0]
+    # the headline: volume and face areas read from their arrays (driver/clover_leaf.cpp:31-36)
+    head = headline_leg(clv, deck_text, plan, args.steps, args.warmup, clv.GEOMETRY_ARRAYS, args.profile_out)
+    nx, ny = head["nx"], head["ny"]
+    value = head["value"]
+    per_kernel, dom = head["per_kernel"], head["dominant"]
     dom_bytes = alg_bytes(dom, nx, ny)
-    avg_s = dom_ms / dom_n / 1e3
+    avg_s = per_kernel[dom]["avg_us"] / 1e6
     achieved = dom_bytes / avg_s / 1e9 if dom_bytes else None
     traffic = None
     pmc = REPO / "profiles" / f"{dom}_{args.deck}_{args.plan}_traffic.json"
     if pmc.exists():
         traffic = json.loads(pmc.read_text()).get("bytes_per_launch")
-    per_kernel = {}
-    for ms, name, n in kern:
-        b = alg_bytes(name, nx, ny)
-        per_kernel[name] = {"launches": n, "total_ms": round(ms, 3), "share": round(ms / total_kernel_ms, 4),
-                            "avg_us": round(ms / n * 1e3, 2),
-                            "alg_GBps": round(b / (ms / n / 1e3) / 1e9, 1) if b else None}
-    if args.profile_out:
-        Path(args.profile_out).write_text(json.dumps({"deck": args.deck, "plan": args.plan, "steps": args.steps,
-                                                      "wall_s": wall, "kernels": per_kernel}, indent=1))
     out = {
         "metric": "Gcell-steps/s + % HBM peak at 15360^2 FP64 (cell-steps/s, clover_bm256_short)",
         "value": value,
@@ -455,27 +578,44 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["wall_s"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": f"synthetic (deck {args.deck}.in generated on the GPU)",
         "config": {"workload": f"{args.deck}: {nx}x{ny} cells two-state ideal gas, FP64, steps "
-                               f"{args.warmup + 1}..{args.warmup + args.steps}",
-                   "x_cells": nx, "y_cells": ny, "plan": args.plan, "parallelism": "1 GPU, 1 chunk"},
+                               f"{args.warmup + 1}..{args.warmup + args.steps} with field_summary every "
+                               f"{head['summary_frequency']} steps; volume and face areas read from their arrays",
+                   "x_cells": nx, "y_cells": ny, "plan": args.plan, "geometry": "arrays", "parallelism": "1 GPU, 1 chunk"},
         "gcell_steps_per_s": value / 1e9,
         "effective_hbm_frac_at_1440B": value * B_ALG_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9),
-        "fused_compulsory_B_per_cell_step": B_FUSED_PER_CELL_STEP if args.plan == "fused" else None,
-        "compulsory_hbm_frac_fused": value * B_FUSED_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9) if args.plan == "fused" else None,
-        "geometry_from_mesh": geom_mesh,
+        "fused_compulsory_B_per_cell_step": B_FUSED_PER_CELL_STEP[False] if args.plan == "fused" else None,
+        "compulsory_hbm_frac_fused": value * B_FUSED_PER_CELL_STEP[False] / (HBM_PEAK_GBS * 1e9) if args.plan == "fused" else None,
+        "summaries_in_timed_region": head["summaries"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                     "alg_bytes_per_launch": dom_bytes, "avg_launch_us": avg_s * 1e6, "share_of_kernel_time": dom_ms / total_kernel_ms},
-        "kernel_time_fraction_of_wall": total_kernel_ms / 1e3 / wall,
-        "final_summary": {"step": st.step, "ke": s_end[3], "mass": s_end[1]},
+                     "alg_bytes_per_launch": dom_bytes, "avg_launch_us": avg_s * 1e6,
+                     "share_of_kernel_time": per_kernel[dom]["share"]},
+        "kernel_time_fraction_of_wall": head["kernel_ms"] / 1e3 / head["wall_s"],
+        "kernels": per_kernel,
+        "final_summary": head["final_summary"],
     }
-    ch.close()
+    if not args.no_mesh:
+        # the same run with the geometry computed from the 1D spacings (CLV_GEOMETRY_MESH):
+        # a separately labelled figure, not the headline
+        mesh = headline_leg(clv, deck_text, plan, args.steps, args.warmup, clv.GEOMETRY_MESH, None)
+        out["geometry_from_mesh"] = {
+            "note": "volume = celldx(i) * celldy(j), xarea = celldy(j), yarea = celldx(i) computed from the 1D mesh "
+                    "instead of read (the same doubles); not the headline",
+            "cell_steps_per_s": mesh["value"], "ms_per_step": mesh["wall_s"] / args.steps * 1e3,
+            "compulsory_B_per_cell_step": B_FUSED_PER_CELL_STEP[True] if args.plan == "fused" else None,
+            "final_summary_identical": mesh["final_summary"] == head["final_summary"],
+            "kernels": {k: {"avg_us": v["avg_us"], "alg_GBps": alg_bytes(k, nx, ny, True) / (v["avg_us"] / 1e6) / 1e9
+                            if alg_bytes(k, nx, ny, True) else None} for k, v in mesh["per_kernel"].items()}}
+    if not args.no_dropin:
+        print("# bench: drop-in (reference driver + MI355X model) ...", file=sys.stderr, flush=True)
+        out["dropin"] = dropin_leg(deck_text, args.dropin_steps)
     if not args.no_batch:
         print("# bench: batched workload ...", file=sys.stderr, flush=True)
         out["batched"] = batch_leg(clv)
@@ -495,7 +635,7 @@ def main():
             # the ceiling of each fused kernel's access pattern: the same march over the
             # same grid with its arrays read and written and no arithmetic
             ceil = {}
-            for name, (r, w) in PATTERN.items():
+            for name, (r, w) in PATTERN[False].items():
                 if name in per_kernel:
                     probe_ms = clv.march_probe(nx, ny, r, w, 16, 10)
                     ceil[name] = {"arrays_read": r, "arrays_written": w, "probe_ms": round(probe_ms, 3),
@@ -505,8 +645,15 @@ def main():
             if dom in ceil:
                 out["roofline"]["frac_of_pattern_ceiling"] = ceil[dom]["probe_ms"] / ceil[dom]["kernel_ms"]
     if not args.no_cpu:
-        print("# bench: CPU baseline (oracle) ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(deck_text, args.cpu_steps)
+        print("# bench: CPU baseline (reference omp model) ...", file=sys.stderr, flush=True)
+        ref = cpu_baseline_reference(args.cpu_steps)
+        print("# bench: CPU baseline (OpenMP oracle) ...", file=sys.stderr, flush=True)
+        port = cpu_baseline(deck_text, args.cpu_steps)
+        if ref is not None:
+            ref["port"] = port
+            out["cpu_baseline"] = ref
+        else:
+            out["cpu_baseline"] = port
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -32,6 +32,7 @@ MESH = ["cellx", "celly", "vertexx", "vertexy", "celldx", "celldy", "vertexdx", 
 MID = {n: i for i, n in enumerate(MESH)}
 PLAN_REFERENCE, PLAN_FUSED = 0, 1
 SUMMARY_TREE, SUMMARY_SERIAL = 0, 1
+GEOMETRY_ARRAYS, GEOMETRY_MESH = 0, 1
 MAX_STATES = 16
 ERR_SMALL_DT = -4
 
@@ -108,6 +109,7 @@ SIGNATURES = [
     ("clv_chunk_destroy", _I, [_P]),
     ("clv_chunk_get_view", _I, [_P, C.POINTER(ChunkView)]),
     ("clv_chunk_geometry_from_mesh", _I, [_P]),
+    ("clv_chunk_set_geometry_mode", _I, [_P, _I]),
     ("clv_chunk_set_stream", _I, [_P, _P]),
     ("clv_chunk_synchronize", _I, [_P]),
     ("clv_chunk_bytes", C.c_size_t, [_P]),
@@ -282,12 +284,19 @@ def _mask(names) -> "C.Array":
 class Chunk:
     """One chunk resident on the current GPU (one arena allocation)."""
 
-    def __init__(self, x_cells: int, y_cells: int):
+    def __init__(self, x_cells: int, y_cells: int, geometry: int = GEOMETRY_ARRAYS):
+        """geometry: GEOMETRY_ARRAYS (default: the kernels read volume, xarea and yarea)
+        or GEOMETRY_MESH (computed from the 1D spacings while the arrays hold what
+        initialise_chunk wrote)."""
         self.L = lib()
         self.nx, self.ny = int(x_cells), int(y_cells)
         h = C.c_void_p()
         _check(self.L.clv_chunk_create(self.nx, self.ny, C.byref(h)), "clv_chunk_create")
         self.h = h
+        self.set_geometry_mode(geometry)
+
+    def set_geometry_mode(self, mode: int):
+        _check(self.L.clv_chunk_set_geometry_mode(self.h, int(mode)), "set_geometry_mode")
 
     def close(self):
         if getattr(self, "h", None):
@@ -468,10 +477,11 @@ class Chunk:
         return {names[i].decode(): (int(launches[i]), float(ms[i])) for i in range(min(k, n))}
 
 
-def run(cfg: Config, plan: int = PLAN_FUSED, trace: bool = False, summary_order: int = 0):
+def run(cfg: Config, plan: int = PLAN_FUSED, trace: bool = False, summary_order: int = 0,
+        geometry: int = GEOMETRY_ARRAYS):
     """Run a deck to completion (driver/hydro.cpp:53-111); returns the summary records."""
     recs = []
-    with Chunk(cfg.x_cells, cfg.y_cells) as ch:
+    with Chunk(cfg.x_cells, cfg.y_cells, geometry) as ch:
         ch.set_summary_order(summary_order)
         st, s = ch.start(cfg)
         recs.append(dict(step=0, time=st.time, dt=st.dt, vol=s[0], mass=s[1], ie=s[2], ke=s[3], press=s[4]))

@@ -221,6 +221,18 @@ clv_status clv_chunk_get_view(const clv_chunk *c, clv_chunk_view *out) {
 
 int clv_chunk_geometry_from_mesh(const clv_chunk *c) { return c ? c->d.geom_mesh : -1; }
 
+clv_status clv_chunk_set_geometry_mode(clv_chunk *c, int mode) {
+  CHECK_CHUNK(c);
+  if (mode != CLV_GEOMETRY_ARRAYS && mode != CLV_GEOMETRY_MESH)
+    return set_error(CLV_ERR_ARG, "clv_chunk_set_geometry_mode: mode is CLV_GEOMETRY_ARRAYS or CLV_GEOMETRY_MESH");
+  // wrapped chunks: the caller keeps write access to every array, so never assume them
+  if (mode == CLV_GEOMETRY_MESH && !c->owns)
+    return set_error(CLV_ERR_ARG, "clv_chunk_set_geometry_mode: CLV_GEOMETRY_MESH needs a chunk made by clv_chunk_create");
+  c->geom_mode = mode;
+  if (mode == CLV_GEOMETRY_ARRAYS) c->d.geom_mesh = 0;
+  return CLV_OK;
+}
+
 clv_status clv_chunk_set_stream(clv_chunk *c, void *s) {
   CHECK_CHUNK(c);
   c->stream = static_cast<hipStream_t>(s);
@@ -448,6 +460,8 @@ static clv_status pack_array(clv_chunk *c, double *field, int field_type, int fa
   const int y_inc = (field_type == CLV_VERTEX_DATA || field_type == CLV_Y_FACE_DATA) ? 1 : 0;
   if (offset < 0 || offset + message_extent(c, face, x_inc, y_inc, depth) > (int64_t)buffer_len)
     return set_error(CLV_ERR_ARG, std::string(who) + ": the message does not fit in buffer_len");
+  // ghosts written into the geometry arrays: read them from now on
+  if (unpack && (field == c->d.f[CLV_VOLUME] || field == c->d.f[CLV_XAREA] || field == c->d.f[CLV_YAREA])) c->d.geom_mesh = 0;
   return k_pack_array(c, field, field_type, face, depth, buffer, offset, unpack);
 }
 

@@ -15,17 +15,17 @@ constexpr double g_small = 1.0e-16;  // driver/definitions.h:32
 constexpr double g_big = 1.0e+21;    // driver/definitions.h:33
 
 // Kernel-side view of a chunk: one base pointer per array, one row pitch.
-// Passed by value (kernarg), 296 bytes.
+// Passed by value (kernarg), 288 bytes.
 struct Dev {
   double *f[CLV_NUM_FIELDS];
   double *m[CLV_NUM_MESH];
   int nx, ny;
   int64_t pitch;
-  // geom_mesh: volume, xarea and yarea still hold what initialise_chunk wrote
-  // (volume = dxdy, xarea(i, j) = celldy(j), yarea(i, j) = celldx(i)), so the fused
-  // kernels take them from dxdy and the 1D mesh.  Set by initialise_chunk, cleared by
-  // every entry point through which they could change.
-  double dxdy;
+  // geom_mesh: the chunk is in CLV_GEOMETRY_MESH mode and volume, xarea and yarea
+  // still hold what initialise_chunk wrote (volume = dx * dy, xarea(i, j) = celldy(j),
+  // yarea(i, j) = celldx(i)), so the kernels compute them from the 1D mesh.  Set by
+  // initialise_chunk in that mode, cleared by every entry point through which they
+  // could change.
   int geom_mesh;
 };
 
@@ -56,6 +56,7 @@ struct clv_chunk {
   clv::Profiler *prof = nullptr;
   int summary_serial = 0;  // CLV_SUMMARY_SERIAL: reproduce the serial model's summation order
   int seg[4] = {0};        // rows per segment of the fused plan's kernel families (chosen on first use)
+  int geom_mode = CLV_GEOMETRY_ARRAYS;  // CLV_GEOMETRY_MESH: initialise_chunk sets d.geom_mesh
 };
 
 namespace clv {

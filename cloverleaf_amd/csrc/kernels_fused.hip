@@ -146,20 +146,30 @@ __device__ __forceinline__ double pressure_of(double rho, double e) { return (1.
 // initialise_chunk (src/serial/initialise_chunk.cpp:78-84) sets volume = dx * dy,
 // xarea(i, j) = celldy(j) and yarea(i, j) = celldx(i) over i < nx + 4, j < ny + 4 (the
 // elements past those, the last column of xarea, the last row of yarea and the row
-// padding, keep build_field's zero), and no
-// kernel of a step writes them.  While a chunk's arrays hold those values
-// (d.geom_mesh) the fused kernels compute them from d.dxdy and the 1D mesh -- the same
-// doubles -- instead of streaming three 2D arrays (3 of timestep's 8 array sweeps, 3 of
-// lagrange's 14, one of each advection pass's); otherwise they read the arrays.  The
-// host picks the instantiation (MESH) per launch.
+// padding, keep build_field's zero), and no kernel of a step writes them.  By default
+// the kernels read the three arrays, as the reference asks of any optimisation
+// (driver/clover_leaf.cpp:31-36: the volume stays an array).  A chunk in
+// CLV_GEOMETRY_MESH mode whose arrays still hold what initialise_chunk wrote
+// (d.geom_mesh) has them computed instead, per cell from the 1D spacings:
+// volume = celldx(i) * celldy(j), which is dx * dy bit for bit on the reference's
+// uniform mesh and the tensor-product cell area on any other.  That streams three
+// fewer 2D arrays (3 of timestep's 8 array sweeps, 3 of lagrange's 14, one of each
+// advection pass's).  The host picks the instantiation (MESH) per launch.
 template <bool MESH, class V>
 __device__ __forceinline__ double geo_volume(const V &d, int i, int j) {
-  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? d.dxdy : 0.0;
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? M(CLV_CELLDX, i) * M(CLV_CELLDY, j) : 0.0;
+  else return AT(CLV_VOLUME, i, j);
+}
+// the same with celldx(i) already in registers (cdx_i; its value does not matter for
+// i >= nx + 4): one load of celldy(j) instead of two loads
+template <bool MESH, class V>
+__device__ __forceinline__ double geo_volume_x(const V &d, double cdx_i, int i, int j) {
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? cdx_i * M(CLV_CELLDY, j) : 0.0;
   else return AT(CLV_VOLUME, i, j);
 }
 template <bool MESH, class V>
 __device__ __forceinline__ double geo_xarea(const V &d, int i, int j) {
-  if constexpr (MESH) return i < d.nx + 4 ? M(CLV_CELLDY, j) : 0.0;
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? M(CLV_CELLDY, j) : 0.0;
   else return AT(CLV_XAREA, i, j);
 }
 // cdx_i: celldx(i) (used for i < nx + 4 only)
@@ -206,46 +216,51 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
     const double u_r1 = AT(CLV_XVEL0, ic, r + 1), v_r1 = AT(CLV_YVEL0, ic, r + 1);
     const double ya_r1 = geo_yarea<MESH>(d, cdx, ic, r + 1);
     const double xa = geo_xarea<MESH>(d, ic, r);
-    const double vol = geo_volume<MESH>(d, ic, r);
+    const double vol = geo_volume_x<MESH>(d, cdx, ic, r);
     // neighbours along x
     const double p_w = up1(p_c), p_e = dn1(p_c);
     const double u10 = dn1(u_r), u11 = dn1(u_r1), v10 = dn1(v_r), v11 = dn1(v_r1), xa1 = dn1(xa);
     const double u00 = u_r, u01 = u_r1, v00 = v_r, v01 = v_r1;
     const double cdy = M(CLV_CELLDY, r), cdy2 = M(CLV_CELLDY, r + 2);
-    // viscosity, src/serial/viscosity.cpp:39-65
+    // viscosity, src/serial/viscosity.cpp:39-65.  The reference sets q = 0 whenever
+    // div >= 0, whatever the limiter (:50), so the limiter and its seven divisions are
+    // formed only where !(div >= 0): in the at-rest part of the box (div exactly 0)
+    // whole waves skip them.  The same doubles come out: where the branch runs, every
+    // expression is the reference's.
     double ugrad = (u10 + u11) - (u00 + u01);
     double vgrad = (v01 + v11) - (v00 + v10);
     double div = (cdx * (ugrad) + cdy * (vgrad));
-    double strain2 = 0.5 * (u01 + u11 - u00 - u10) / cdy + 0.5 * (v10 + v11 - v00 - v01) / cdx;
-    double pgradx = (p_e - p_w) / (cdx + cdx1);
-    double pgrady = (p_p - p_m) / (cdy + cdy2);
-    double pgradx2 = pgradx * pgradx;
-    double pgrady2 = pgrady * pgrady;
-    double limiter = ((0.5 * (ugrad) / cdx) * pgradx2 + (0.5 * (vgrad) / cdy) * pgrady2 + strain2 * pgradx * pgrady) /
-                     fmax(pgradx2 + pgrady2, g_small);
-    double q;
-    if ((limiter > 0.0) || (div >= 0.0)) {
-      q = 0.0;
-    } else {
-      double dirx = 1.0;
-      if (pgradx < 0.0) dirx = -1.0;
-      pgradx = dirx * fmax(g_small, fabs(pgradx));
-      double diry = 1.0;
-      if (pgradx < 0.0) diry = -1.0;
-      pgrady = diry * fmax(g_small, fabs(pgrady));
-      double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
-      double xgrad = fabs(cdx * pgrad / pgradx);
-      double ygrad = fabs(cdy * pgrad / pgrady);
-      double grad = fmin(xgrad, ygrad);
-      double grad2 = grad * grad;
-      q = 2.0 * rho_c * grad2 * limiter * limiter;
+    double q = 0.0;
+    if (!(div >= 0.0)) {
+      double strain2 = 0.5 * (u01 + u11 - u00 - u10) / cdy + 0.5 * (v10 + v11 - v00 - v01) / cdx;
+      double pgradx = (p_e - p_w) / (cdx + cdx1);
+      double pgrady = (p_p - p_m) / (cdy + cdy2);
+      double pgradx2 = pgradx * pgradx;
+      double pgrady2 = pgrady * pgrady;
+      double limiter = ((0.5 * (ugrad) / cdx) * pgradx2 + (0.5 * (vgrad) / cdy) * pgrady2 + strain2 * pgradx * pgrady) /
+                       fmax(pgradx2 + pgrady2, g_small);
+      if (!(limiter > 0.0)) {
+        double dirx = 1.0;
+        if (pgradx < 0.0) dirx = -1.0;
+        pgradx = dirx * fmax(g_small, fabs(pgradx));
+        double diry = 1.0;
+        if (pgradx < 0.0) diry = -1.0;
+        pgrady = diry * fmax(g_small, fabs(pgrady));
+        double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
+        double xgrad = fabs(cdx * pgrad / pgradx);
+        double ygrad = fabs(cdy * pgrad / pgrady);
+        double grad = fmin(xgrad, ygrad);
+        double grad2 = grad * grad;
+        q = 2.0 * rho_c * grad2 * limiter * limiter;
+      }
     }
     // calc_dt, src/serial/calc_dt.cpp:52-75
     {
       double dsx = cdx;
       double dsy = cdy;
       double cc = c_c * c_c;
-      cc = cc + 2.0 * q / rho_c;
+      // 2 q / rho is +-0 when q is +-0 and rho > 0, and cc + (+-0) is cc: skip the division
+      if (!(q == 0.0 && rho_c > 0.0)) cc = cc + 2.0 * q / rho_c;
       cc = fmax(sqrt(cc), g_small);
       double dtct = dtc_safe * fmin(dsx, dsy) / cc;
       double dv = 0.0;
@@ -257,7 +272,8 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
       dv2 = (v01 + v11) * ya_r1;
       dv = dv + dv2 - dv1;
       double dtvt = dtv_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
-      dv = dv / (2.0 * vol);
+      // dv only meets `dv < -g_small`: a +-0 dv compares the same divided or not
+      if (dv != 0.0) dv = dv / (2.0 * vol);
       double dtdivt;
       if (dv < -g_small) {
         dtdivt = dtdiv_safe * (-1.0 / dv);
@@ -371,7 +387,7 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
     a.v01 = v_r1;
     a.v10 = dn1(v_r);
     a.v11 = dn1(v_r1);
-    a.vol = geo_volume<MESH>(d, ic, r);
+    a.vol = geo_volume_x<MESH>(d, cdx_i, ic, r);
     // ghost rho0 through the mirror (the halo of driver/timestep.cpp:53-60)
     a.rho0 = AT(CLV_DENSITY0, mirror_cell(ic, nx), mirror_cell(r, ny));
     a.e0 = AT(CLV_ENERGY0, ic, r);
@@ -609,7 +625,7 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
   };
   auto cell_row = [&](int row) {
     const double rho = AT(CLV_DENSITY1, ic, row), e = AT(CLV_ENERGY1, ic, row);
-    const double vol = geo_volume<MESH>(d, ic, row), vfx = AT(CLV_VOL_FLUX_X, ic, row);
+    const double vol = geo_volume_x<MESH>(d, width, ic, row), vfx = AT(CLV_VOL_FLUX_X, ic, row);
     const double vfx_e = dn1(vfx);
     double pre, y0 = 0.0, y1 = 0.0;
     if (SWEEP == 1) {
@@ -651,8 +667,8 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
     const double rho_h = __shfl(cr.rho, lane_c), mfx = sgn_f * __shfl(mf, lane_f);
     // density1 * post_vol of cell (i, r) (src/serial/advec_mom.cpp:47 / :65, :98-99)
     double post;
-    if (MOM_SWEEP == 1) post = rc == r ? cr.vol + cr.vfy1 - cr.vfy0 : geo_volume<MESH>(d, ic, r) + AT(CLV_VOL_FLUX_Y, ic, r + 1) - AT(CLV_VOL_FLUX_Y, ic, r);
-    else post = rc == r ? cr.vol : geo_volume<MESH>(d, ic, r);
+    if (MOM_SWEEP == 1) post = rc == r ? cr.vol + cr.vfy1 - cr.vfy0 : geo_volume_x<MESH>(d, width, ic, r) + AT(CLV_VOL_FLUX_Y, ic, r + 1) - AT(CLV_VOL_FLUX_Y, ic, r);
+    else post = rc == r ? cr.vol : geo_volume_x<MESH>(d, width, ic, r);
     const double mp = rho_h * post;
     if (r >= j0) {
       // node row r (src/serial/advec_mom.cpp:88-149) on the values in registers:
@@ -764,6 +780,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   const int lane_c = lane + (i == 1 ? 1 : (i == nx + 2 ? -1 : 0));
   const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
   const int Fs = j0 - 2 > 2 ? j0 - 2 : 2, Fe = min(j1 + 1, ny + 2);  // faces the cell part walks
+  const double cdx_ic = MESH ? M(CLV_CELLDX, ic) : 0.0;                 // ic <= nx + 3
   // pre_vol of cell row `row` (src/serial/advec_cell.cpp:139 / :149) from its bottom and
   // top vol_flux_y, and advec_mom's post_vol (src/serial/advec_mom.cpp:56 / :74)
   auto pre_of = [&](int row, double vfy0, double vfy1, double vol, double vfx0, double vfx1) {
@@ -772,10 +789,10 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   };
   auto post_from = [&](double vol, double vfx0, double vfx1) { return MOM_SWEEP == 2 ? vol + vfx1 - vfx0 : vol; };
   auto post_of = [&](int c) {
-    return post_from(geo_volume<MESH>(d, ic, c), MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic, c) : 0.0, MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic + 1, c) : 0.0);
+    return post_from(geo_volume_x<MESH>(d, cdx_ic, ic, c), MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic, c) : 0.0, MOM_SWEEP == 2 ? AT(CLV_VOL_FLUX_X, ic + 1, c) : 0.0);
   };
   auto pre_load = [&](int row, double vfy0, double &vfy1, double &post) {
-    const double vol = geo_volume<MESH>(d, ic, row);
+    const double vol = geo_volume_x<MESH>(d, cdx_ic, ic, row);
     vfy1 = AT(CLV_VOL_FLUX_Y, ic, row + 1);
     const double vfx0 = SWEEP == 1 ? AT(CLV_VOL_FLUX_X, ic, row) : 0.0, vfx1 = SWEEP == 1 ? AT(CLV_VOL_FLUX_X, ic + 1, row) : 0.0;
     post = post_from(vol, vfx0, vfx1);
@@ -790,7 +807,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
     double vol = 0.0, vfy1 = 0.0, vfx0 = 0.0, vfx1 = 0.0, vd = 0.0, rho_p1 = 0.0, e_p1 = 0.0, vd_p1 = 0.0;
     if (F <= Fe) {
       const int rp = F + 1 <= ny + 2 ? F + 1 : ny + 2;  // src/serial/advec_cell.cpp:169
-      vol = geo_volume<MESH>(d, ic, F);
+      vol = geo_volume_x<MESH>(d, cdx_ic, ic, F);
       vfy1 = AT(CLV_VOL_FLUX_Y, ic, F + 1);
       if (SWEEP == 1) {
         vfx0 = AT(CLV_VOL_FLUX_X, ic, F);
@@ -987,7 +1004,6 @@ struct GDev {
   gdouble *m[CLV_NUM_MESH];
   int nx, ny;
   int64_t pitch;
-  double dxdy;
   int geom_mesh;
 };
 __device__ __forceinline__ GDev global_view(const Dev &s) {
@@ -997,7 +1013,6 @@ __device__ __forceinline__ GDev global_view(const Dev &s) {
   d.nx = s.nx;
   d.ny = s.ny;
   d.pitch = s.pitch;
-  d.dxdy = s.dxdy;
   d.geom_mesh = s.geom_mesh;
   return d;
 }

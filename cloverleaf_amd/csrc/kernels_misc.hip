@@ -266,7 +266,14 @@ __global__ __launch_bounds__(RED_THREADS) void calc_dt_k(Dev d, double dtc_safe,
       double dsx = M(CLV_CELLDX, i);
       const double ss = AT(CLV_SOUNDSPEED, i, j);
       double cc = ss * ss;
-      cc = cc + 2.0 * AT(CLV_VISCOSITY, i, j) / AT(CLV_DENSITY0, i, j);
+      // 2 q / rho is +-0 when q is +-0 and rho > 0, and cc + (+-0) is cc
+      const double q = AT(CLV_VISCOSITY, i, j);
+      if (q != 0.0) {
+        cc = cc + 2.0 * q / AT(CLV_DENSITY0, i, j);
+      } else {
+        const double rho = AT(CLV_DENSITY0, i, j);
+        if (!(rho > 0.0)) cc = cc + 2.0 * q / rho;
+      }
       cc = fmax(sqrt(cc), g_small);
       double dtct = dtc_safe * fmin(dsx, dsy) / cc;
       double div = 0.0;
@@ -279,7 +286,8 @@ __global__ __launch_bounds__(RED_THREADS) void calc_dt_k(Dev d, double dtc_safe,
       dv2 = (AT(CLV_YVEL0, i, j + 1) + AT(CLV_YVEL0, i + 1, j + 1)) * geom_yarea(d, i, j + 1);
       div = div + dv2 - dv1;
       double dtvt = dtv_safe * 2.0 * vol / fmax(fmax(fabs(dv1), fabs(dv2)), g_small * vol);
-      div = div / (2.0 * vol);
+      // div only meets `div < -g_small`: a +-0 div compares the same divided or not
+      if (div != 0.0) div = div / (2.0 * vol);
       double dtdivt;
       if (div < -g_small) {
         dtdivt = dtdiv_safe * (-1.0 / div);
@@ -798,9 +806,8 @@ clv_status k_initialise_chunk(clv_chunk *c, const clv_config *cfg, int t_left, i
   LAUNCH_G("initialise_chunk", init_mesh_k, dim3((n + 255) / 256, 1, 1), dim3(256, 1, 1), d, xmin, ymin, dx, dy);
   const dim3 g((unsigned)((d.nx + 4 + 63) / 64), (unsigned)((d.ny + 4 + 3) / 4), 1);
   LAUNCH_G("initialise_chunk", init_geom_k, g, dim3(64, 4, 1), d, dx * dy);
-  // volume = dx * dy, xarea = celldy, yarea = celldx: the fused kernels derive them
-  c->d.dxdy = dx * dy;
-  c->d.geom_mesh = 1;
+  // volume = dx * dy, xarea = celldy, yarea = celldx: in mesh mode the kernels derive them
+  c->d.geom_mesh = c->geom_mode == CLV_GEOMETRY_MESH;
   return CLV_OK;
 }
 
@@ -818,3 +825,4 @@ clv_status k_generate_chunk(clv_chunk *c, const clv_config *cfg) {
 }
 
 }  // namespace clv
+

@@ -47,29 +47,32 @@ __global__ __launch_bounds__(256) void viscosity_k(Dev d) {
   double ugrad = (u10 + u11) - (u00 + u01);
   double vgrad = (v01 + v11) - (v00 + v10);
   double div = (cdx * (ugrad) + cdy * (vgrad));
-  double strain2 = 0.5 * (u01 + u11 - u00 - u10) / cdy + 0.5 * (v10 + v11 - v00 - v01) / cdx;
-  double pgradx = (AT(CLV_PRESSURE, i + 1, j) - AT(CLV_PRESSURE, i - 1, j)) / (cdx + M(CLV_CELLDX, i + 1));
-  double pgrady = (AT(CLV_PRESSURE, i, j + 1) - AT(CLV_PRESSURE, i, j - 1)) / (cdy + M(CLV_CELLDY, j + 2));
-  double pgradx2 = pgradx * pgradx;
-  double pgrady2 = pgrady * pgrady;
-  double limiter = ((0.5 * (ugrad) / cdx) * pgradx2 + (0.5 * (vgrad) / cdy) * pgrady2 + strain2 * pgradx * pgrady) /
-                   fmax(pgradx2 + pgrady2, g_small);
-  double q;
-  if ((limiter > 0.0) || (div >= 0.0)) {
-    q = 0.0;
-  } else {
-    double dirx = 1.0;
-    if (pgradx < 0.0) dirx = -1.0;
-    pgradx = dirx * fmax(g_small, fabs(pgradx));
-    double diry = 1.0;
-    if (pgradx < 0.0) diry = -1.0;
-    pgrady = diry * fmax(g_small, fabs(pgrady));
-    double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
-    double xgrad = fabs(cdx * pgrad / pgradx);
-    double ygrad = fabs(cdy * pgrad / pgrady);
-    double grad = fmin(xgrad, ygrad);
-    double grad2 = grad * grad;
-    q = 2.0 * AT(CLV_DENSITY0, i, j) * grad2 * limiter * limiter;
+  // q = 0 whenever div >= 0, whatever the limiter (src/serial/viscosity.cpp:50): the
+  // limiter, its seven divisions and the pressure and density loads are needed only
+  // where !(div >= 0), and there every expression is the reference's
+  double q = 0.0;
+  if (!(div >= 0.0)) {
+    double strain2 = 0.5 * (u01 + u11 - u00 - u10) / cdy + 0.5 * (v10 + v11 - v00 - v01) / cdx;
+    double pgradx = (AT(CLV_PRESSURE, i + 1, j) - AT(CLV_PRESSURE, i - 1, j)) / (cdx + M(CLV_CELLDX, i + 1));
+    double pgrady = (AT(CLV_PRESSURE, i, j + 1) - AT(CLV_PRESSURE, i, j - 1)) / (cdy + M(CLV_CELLDY, j + 2));
+    double pgradx2 = pgradx * pgradx;
+    double pgrady2 = pgrady * pgrady;
+    double limiter = ((0.5 * (ugrad) / cdx) * pgradx2 + (0.5 * (vgrad) / cdy) * pgrady2 + strain2 * pgradx * pgrady) /
+                     fmax(pgradx2 + pgrady2, g_small);
+    if (!(limiter > 0.0)) {
+      double dirx = 1.0;
+      if (pgradx < 0.0) dirx = -1.0;
+      pgradx = dirx * fmax(g_small, fabs(pgradx));
+      double diry = 1.0;
+      if (pgradx < 0.0) diry = -1.0;
+      pgrady = diry * fmax(g_small, fabs(pgrady));
+      double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
+      double xgrad = fabs(cdx * pgrad / pgradx);
+      double ygrad = fabs(cdy * pgrad / pgrady);
+      double grad = fmin(xgrad, ygrad);
+      double grad2 = grad * grad;
+      q = 2.0 * AT(CLV_DENSITY0, i, j) * grad2 * limiter * limiter;
+    }
   }
   AT(CLV_VISCOSITY, i, j) = q;
 }

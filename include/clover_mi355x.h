@@ -125,19 +125,33 @@ int clv_device_count(void);
  * Replaces the per-field allocations of field_type (driver/definitions.cpp:4-41). */
 clv_status clv_chunk_create(int x_cells, int y_cells, clv_chunk **out);
 /* Wraps device arrays the caller owns (e.g. a reference model's Buffer2D
- * storage); the caller keeps them alive while the chunk exists. */
+ * storage); the caller keeps them alive while the chunk exists.  A wrapped
+ * chunk always reads volume, xarea and yarea from their arrays (the caller
+ * may write them at any time), so CLV_GEOMETRY_MESH is refused for it. */
 clv_status clv_chunk_wrap(const clv_chunk_view *view, clv_chunk **out);
 clv_status clv_chunk_destroy(clv_chunk *c);
 /* The view's pointers allow writes to any array, so taking a view also switches the
  * kernels to reading volume, xarea and yarea (see clv_chunk_geometry_from_mesh). */
 clv_status clv_chunk_get_view(const clv_chunk *c, clv_chunk_view *out);
-/* 1 while the chunk's volume, xarea and yarea hold what clv_initialise_chunk wrote
- * (volume = dx * dy, xarea(i, j) = celldy(j), yarea(i, j) = celldx(i)): the kernels
- * (both plans, the per-kernel entry points, calc_dt and field_summary) then compute
- * them from the mesh instead of reading the three arrays (no counterpart in the
- * reference, whose kernels always read them).  clv_initialise_chunk sets it;
- * clv_build_field, clv_field_upload of those fields, clv_mesh_upload of celldx or
- * celldy and clv_chunk_get_view clear it.  0 otherwise, -1 for a null chunk. */
+/* Where the kernels take the cell volume and face areas from (no counterpart in
+ * the reference, whose kernels always read the arrays).
+ * CLV_GEOMETRY_ARRAYS (the default): from the volume, xarea and yarea arrays, as
+ *   the reference asks of every optimisation (driver/clover_leaf.cpp:31-36: "the
+ *   volume, though constant for all cells, should remain array").
+ * CLV_GEOMETRY_MESH: while the three arrays hold what clv_initialise_chunk wrote,
+ *   from the 1D spacings: volume(i, j) = celldx(i) * celldy(j), xarea(i, j) =
+ *   celldy(j), yarea(i, j) = celldx(i) -- the same doubles (dx * dy on the
+ *   reference's uniform mesh), three fewer 2D arrays streamed per kernel.  Takes
+ *   effect at the next clv_initialise_chunk; refused for wrapped chunks.
+ * Setting CLV_GEOMETRY_ARRAYS takes effect at once. */
+enum { CLV_GEOMETRY_ARRAYS = 0, CLV_GEOMETRY_MESH = 1 };
+clv_status clv_chunk_set_geometry_mode(clv_chunk *c, int mode);
+/* 1 while the kernels (both plans, the per-kernel entry points, calc_dt and
+ * field_summary) compute the geometry from the mesh: the chunk is in
+ * CLV_GEOMETRY_MESH mode and clv_initialise_chunk has run since.  clv_build_field,
+ * clv_field_upload of volume / xarea / yarea, clv_unpack_message into them,
+ * clv_mesh_upload of celldx or celldy and clv_chunk_get_view clear it.  0 otherwise,
+ * -1 for a null chunk. */
 int clv_chunk_geometry_from_mesh(const clv_chunk *c);
 /* Orders the chunk's work on a caller stream (a hipStream_t passed as void*;
  * NULL = the null stream). */

@@ -49,11 +49,11 @@ def _state_regions(nx, ny):
             "mass_flux_y": (slice(None), slice(None))}
 
 
-def _start(decks):
+def _start(decks, geometry=clv.GEOMETRY_ARRAYS):
     cfgs = [clv.parse_deck(t) for t in decks]
     chunks, states = [], []
     for c in cfgs:
-        ch = clv.Chunk(c.x_cells, c.y_cells)
+        ch = clv.Chunk(c.x_cells, c.y_cells, geometry)
         st, _ = ch.start(c)
         chunks.append(ch)
         states.append(st)
@@ -101,9 +101,12 @@ def small_plain(small_decks):
     return _plain(small_decks)
 
 
+@pytest.mark.parametrize("geometry", [clv.GEOMETRY_ARRAYS, clv.GEOMETRY_MESH], ids=["arrays", "mesh"])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_bitwise_vs_plain(gpu, small_decks, small_plain, use_graph):
-    cfgs, chunks, states = _start(small_decks)
+def test_batch_bitwise_vs_plain(gpu, small_decks, small_plain, use_graph, geometry):
+    """Both geometry modes: the volume and face areas read from their arrays, or
+    computed from the mesh (the plain runs read the arrays: the same doubles)."""
+    cfgs, chunks, states = _start(small_decks, geometry)
     with clv.Batch(chunks, cfgs, states, hist_len=64) as b:
         done = b.run(100, use_graph=use_graph)
         assert done == 15  # the largest end_step: no member runs past its own
@@ -168,9 +171,9 @@ def test_batch_rejects_bad_use(gpu, small_decks):
         chunks[1].step(cfgs[1], st, clv.PLAN_FUSED)
         with pytest.raises(clv.CloverError, match="outside the batch"):
             b.run(1)
-    # the batch computes the members' volumes and areas from their meshes: once a
-    # member's arrays may have changed (a view was taken) it refuses to run
-    cfgs, chunks, states = _start(small_decks[:2])
+    # in mesh mode the batch computes the members' volumes and areas from their meshes:
+    # once a member's arrays may have changed (a view was taken) it refuses to run
+    cfgs, chunks, states = _start(small_decks[:2], clv.GEOMETRY_MESH)
     with clv.Batch(chunks, cfgs, states) as b:
         b.run(1)
         chunks[0].field_ptr("volume")

@@ -64,15 +64,19 @@ def _geom_deck(nx, ny, steps):
         "y_cells=56", f"y_cells={ny}").replace("end_step=80", f"end_step={steps}")
 
 
+GEOMETRIES = pytest.mark.parametrize("geometry", [clv.GEOMETRY_ARRAYS, clv.GEOMETRY_MESH], ids=["arrays", "mesh"])
+
+
 @pytest.mark.parametrize("deck", SMALL)
 @pytest.mark.parametrize("plan", [clv.PLAN_REFERENCE, clv.PLAN_FUSED])
-def test_small_deck_bitwise_vs_oracle(gpu, deck, plan):
+@GEOMETRIES
+def test_small_deck_bitwise_vs_oracle(gpu, deck, plan, geometry):
     cfg = _cfg(deck)
     if plan == clv.PLAN_FUSED and min(cfg.x_cells, cfg.y_cells) < 4:
         pytest.skip("the fused plan needs x_cells, y_cells >= 4")
     o = T.Oracle(T.load_deck(deck))
     o.start()
-    with clv.Chunk(cfg.x_cells, cfg.y_cells) as ch:
+    with clv.Chunk(cfg.x_cells, cfg.y_cells, geometry) as ch:
         st, _ = ch.start(cfg)
         assert not _fields_equal(ch, o, PERSISTENT), "after start"
         while not st.complete:
@@ -90,18 +94,21 @@ def test_small_deck_bitwise_vs_oracle(gpu, deck, plan):
                                    (121, 15), (59, 16), (65, 17), (181, 127), (63, 128), (124, 257), (56, 15), (57, 16),
                                    (113, 31), (114, 33), (170, 47), (4, 4), (5, 6), (55, 14), (120, 21), (223, 19),
                                    (225, 22), (239, 20), (241, 18), (447, 9), (448, 10), (480, 17)])
-def test_fused_plan_strip_and_segment_edges(gpu, nx, ny):
+@GEOMETRIES
+def test_fused_plan_strip_and_segment_edges(gpu, nx, ny, geometry):
     """Grid sizes around the fused kernels' strip widths (56/60/62 columns), the
     224- and 240-column blocks whose rows are stored through LDS (blocks with idle
     strips, the boundary node column on the slot past a block), row segments (8, 16
-    and 128 rows), and the smallest grids: every step's dt and the state bit-identical
-    to the oracle."""
+    and 128 rows), and the smallest grids, with the geometry read from its arrays
+    and computed from the mesh: every step's dt and the state bit-identical to the
+    oracle."""
     text = _geom_deck(nx, ny, 12)
     cfg = clv.parse_deck(text)
     o = T.Oracle(T.parse_deck(text))
     o.start()
-    with clv.Chunk(nx, ny) as ch:
+    with clv.Chunk(nx, ny, geometry) as ch:
         st, _ = ch.start(cfg)
+        assert ch.geometry_from_mesh() == (geometry == clv.GEOMETRY_MESH)
         while not st.complete:
             ch.step(cfg, st, clv.PLAN_FUSED)
             assert o.step() == 0
@@ -114,17 +121,17 @@ def test_fused_plan_strip_and_segment_edges(gpu, nx, ny):
 
 
 def test_fused_plan_reads_changed_geometry(gpu):
-    """The fused plan computes volume and the face areas from the mesh while they hold
-    what initialise_chunk wrote; once they are changed (here scaled by 1 + U[0, 1e-3) per
-    element and uploaded) it reads the arrays: every step's dt and the state bit-identical
-    to the oracle run on the same changed arrays."""
+    """In mesh mode the fused plan computes volume and the face areas from the mesh while
+    they hold what initialise_chunk wrote; once they are changed (here scaled by
+    1 + U[0, 1e-3) per element and uploaded) it reads the arrays: every step's dt and the
+    state bit-identical to the oracle run on the same changed arrays."""
     nx, ny = 67, 45
     text = _geom_deck(nx, ny, 8)
     cfg = clv.parse_deck(text)
     o = T.Oracle(T.parse_deck(text))
     o.start()
     rng = np.random.default_rng(7)
-    with clv.Chunk(nx, ny) as ch:
+    with clv.Chunk(nx, ny, clv.GEOMETRY_MESH) as ch:
         st, _ = ch.start(cfg)
         assert ch.geometry_from_mesh()
         for name in ("volume", "xarea", "yarea"):
@@ -141,10 +148,21 @@ def test_fused_plan_reads_changed_geometry(gpu):
 
 
 def test_geometry_flag_follows_the_arrays(gpu):
-    """initialise_chunk sets the flag; build_field, uploads of volume / xarea / yarea /
-    celldx / celldy and taking a view clear it; other uploads keep it."""
+    """The default mode reads the arrays and never sets the flag.  In mesh mode
+    initialise_chunk sets it; build_field, uploads of volume / xarea / yarea / celldx /
+    celldy, taking a view and switching back to the arrays clear it; other uploads keep
+    it."""
     cfg = _cfg("small_geom_mix")
     with clv.Chunk(cfg.x_cells, cfg.y_cells) as ch:
+        ch.start(cfg)
+        assert not ch.geometry_from_mesh()
+        ch.set_geometry_mode(clv.GEOMETRY_MESH)
+        assert not ch.geometry_from_mesh()  # from the next initialise_chunk
+        ch.start(cfg)
+        assert ch.geometry_from_mesh()
+        ch.set_geometry_mode(clv.GEOMETRY_ARRAYS)
+        assert not ch.geometry_from_mesh()
+    with clv.Chunk(cfg.x_cells, cfg.y_cells, clv.GEOMETRY_MESH) as ch:
         assert not ch.geometry_from_mesh()
         ch.start(cfg)
         assert ch.geometry_from_mesh()
@@ -160,6 +178,16 @@ def test_geometry_flag_follows_the_arrays(gpu):
             assert ch.geometry_from_mesh()
             clear()
             assert not ch.geometry_from_mesh()
+        # a message unpacked into the volume array (its pointer taken before the start)
+        vol = ch.field_ptr("volume")
+        ch.start(cfg)
+        assert ch.geometry_from_mesh()
+        buf = clv.DeviceBuffer(4 * (cfg.y_cells + 8))
+        ch.pack_message(vol, clv.CELL_DATA, clv.FACE_LEFT, 2, buf, 0)
+        assert ch.geometry_from_mesh()
+        ch.pack_message(vol, clv.CELL_DATA, clv.FACE_LEFT, 2, buf, 0, unpack=True)
+        assert not ch.geometry_from_mesh()
+        buf.close()
 
 
 def test_plans_can_alternate(gpu):
@@ -212,11 +240,19 @@ def test_small_golden_trace(gpu, deck):
     _run_vs_golden(deck, clv.PLAN_REFERENCE, clv.SUMMARY_TREE, 1e-11)
 
 
-def test_problem_2_clover_bm_short(gpu):
+@pytest.mark.parametrize("deck", ["small_bm_64x48", "small_geom_mix"])
+def test_small_golden_trace_fused(gpu, deck):
+    """The fused plan against the serial reference's full-precision trace: dt, time and
+    every sum bit-identical (serial summation order)."""
+    _run_vs_golden(deck, clv.PLAN_FUSED, clv.SUMMARY_SERIAL, 0.0)
+
+
+@pytest.mark.parametrize("plan", [clv.PLAN_REFERENCE, clv.PLAN_FUSED], ids=["reference", "fused"])
+def test_problem_2_clover_bm_short(gpu, plan):
     """960^2, 87 steps (test problem 2): with the serial summation order every dt and
     every printed summary is bit-identical to the serial reference; the reference's
     QA check passes."""
-    recs = _run_vs_golden("clover_bm_short", clv.PLAN_REFERENCE, clv.SUMMARY_SERIAL, 0.0)
+    recs = _run_vs_golden("clover_bm_short", plan, clv.SUMMARY_SERIAL, 0.0)
     ke = recs[-1]["ke"]
     assert abs(100.0 * (ke / KNOWN[2]) - 100.0) < 0.001
 
@@ -253,12 +289,66 @@ def test_problem_2_tree_summary(gpu):
 
 
 @pytest.mark.parametrize("deck", ["clover_bm16_very_short", "clover_bm2_short", "clover_bm4_short"])
-def test_known_answer_decks(gpu, deck):
+@pytest.mark.parametrize("plan", [clv.PLAN_REFERENCE, clv.PLAN_FUSED], ids=["reference", "fused"])
+def test_known_answer_decks(gpu, deck, plan):
     """Against the reference omp model's trace: dt identical at every step; sums within
     the gap two summation orders may show at this size (the omp sum is itself an order)."""
     d = T.load_deck(deck)
-    recs = _run_vs_golden(deck, clv.PLAN_REFERENCE, clv.SUMMARY_TREE, summation_tolerance(d.x_cells * d.y_cells))
+    recs = _run_vs_golden(deck, plan, clv.SUMMARY_TREE, summation_tolerance(d.x_cells * d.y_cells))
     assert abs(100.0 * (recs[-1]["ke"] / KNOWN[d.test_problem]) - 100.0) < 0.001
+
+
+SUMS = ("vol", "mass", "ie", "ke", "press")
+# Bar of the large decks: the GPU's tree sums against the exact sums of the reference's
+# own state (x_vol .. x_press of the golden, order-free).  A tree sum of n non-negative
+# terms is within ~(log2 n + a few hundred sequential terms) * 2^-53 of the exact sum,
+# so with the state bit-identical the gap is ~1e-14; 1e-12 is ten times tighter than
+# the 1e-11 bar against the reference's field_summary.
+EXACT_TOL = 1e-12
+
+
+def _run_vs_large_golden(deck, plan, geometry):
+    """dt and time bit-identical to the reference omp model at every step; the five
+    sums within EXACT_TOL of the exact sums of the reference's state and within the
+    summation-order bound of the omp model's own printed sums.  Returns the worst
+    relative gaps (to the exact sums, to the omp sums)."""
+    cfg = _cfg(deck)
+    gold = T.golden(deck)["summaries"]
+    assert all("x_vol" in g for g in gold), "golden without exact sums: regenerate with tests/golden/make_golden.py"
+    print(f"# {deck}: {cfg.x_cells}x{cfg.y_cells}, {len(gold) - 1} steps, plan {plan}, geometry {geometry}", flush=True)
+    recs = clv.run(cfg, plan=plan, trace=True, summary_order=clv.SUMMARY_TREE, geometry=geometry)
+    assert len(recs) == len(gold)
+    w_exact = w_omp = 0.0
+    tol_omp = summation_tolerance(cfg.x_cells * cfg.y_cells)
+    for a, b in zip(recs, gold):
+        assert a["step"] == b["step"]
+        assert a["dt"] == b["dt"], (a["step"], a["dt"], b["dt"])
+        assert a["time"] == b["time"], a["step"]
+        for k in SUMS:
+            x = b["x_" + k]
+            if x == 0.0:
+                assert a[k] == 0.0 and b[k] == 0.0, (a["step"], k)
+                continue
+            w_exact = max(w_exact, abs(a[k] - x) / abs(x))
+            w_omp = max(w_omp, abs(a[k] - b[k]) / abs(b[k]))
+    assert w_exact <= EXACT_TOL, (w_exact, deck, plan)
+    assert w_omp <= tol_omp, (w_omp, deck, plan)
+    print(f"# {deck}: worst gap to the exact sums {w_exact:.2e}, to the omp model's sums {w_omp:.2e}", flush=True)
+    return w_exact, w_omp
+
+
+@pytest.mark.parametrize("plan", [clv.PLAN_REFERENCE, clv.PLAN_FUSED], ids=["reference", "fused"])
+def test_bm16_300_vs_reference(gpu, plan):
+    """InputDecks/clover_bm16_300.in (3840^2, 300 steps) against the reference omp model."""
+    _run_vs_large_golden("clover_bm16_300", plan, clv.GEOMETRY_ARRAYS)
+
+
+@pytest.mark.parametrize("plan,geometry", [(clv.PLAN_FUSED, clv.GEOMETRY_ARRAYS), (clv.PLAN_REFERENCE, clv.GEOMETRY_ARRAYS),
+                                           (clv.PLAN_FUSED, clv.GEOMETRY_MESH)], ids=["fused", "reference", "fused-mesh"])
+def test_bm256_short_vs_reference(gpu, plan, geometry):
+    """The headline workload, InputDecks/clover_bm256_short.in (15360^2, 87 steps, 47 GB
+    resident), against the reference omp model run in full on the same deck."""
+    _run_vs_large_golden("clover_bm256_short", plan, geometry)
 
 
 @pytest.mark.parametrize("plan", ["reference", "fused"])
