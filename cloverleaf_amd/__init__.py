@@ -142,6 +142,7 @@ SIGNATURES = [
     ("clv_unpack_fields", _I, [_P, _I, _PI, _I, _I, C.c_int64, _P, C.c_size_t]),
     ("clv_field_summary", _I, [_P, _PD]),
     ("clv_chunk_set_summary_order", _I, [_P, _I]),
+    ("clv_field_summary_noalloc", _I, [C.POINTER(ChunkView), _P, _PD]),
     ("clv_deck_parse", _I, [C.c_char_p, C.POINTER(Config)]),
     ("clv_hydro_start", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _PD]),
     ("clv_timestep", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _I]),
@@ -430,6 +431,15 @@ class Chunk:
     def field_summary(self):
         out = (C.c_double * 5)()
         _check(self.L.clv_field_summary(self.h, out), "field_summary")
+        return list(out)
+
+    def field_summary_noalloc(self, stream: int = 0):
+        """clv_field_summary over this chunk's arrays through the allocation-free entry
+        point (a view of the arrays, the library's fixed scratch array)."""
+        out = (C.c_double * 5)()
+        v = ChunkView()
+        _check(self.L.clv_chunk_get_view(self.h, C.byref(v)), "clv_chunk_get_view")
+        _check(self.L.clv_field_summary_noalloc(C.byref(v), C.c_void_p(stream or None), out), "clv_field_summary_noalloc")
         return list(out)
 
     def set_summary_order(self, order: int):

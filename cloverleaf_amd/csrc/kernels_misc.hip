@@ -1,4 +1,5 @@
 // gfx950 kernels for the self halo, the two reductions and problem set-up.
+#include <mutex>
 #include <vector>
 
 #include "clv_internal.h"
@@ -826,3 +827,76 @@ clv_status k_generate_chunk(clv_chunk *c, const clv_config *cfg) {
 
 }  // namespace clv
 
+// ---- field_summary with no allocation ------------------------------------------------------
+// clv_field_summary_noalloc: the sums of clv_field_summary over a caller's arrays, for
+// callers that cannot allocate device memory.  Its reduction partials and results live
+// in `scratch`, one fixed array per device for the life of the process: 5 sums x
+// RED_MAX_BLOCKS block partials, then the 5 results.  The reduction never runs more
+// than RED_MAX_BLOCKS blocks (they stride over the rows), so every grid fits.  In host
+// code `scratch` names no device address: the per-device address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+namespace clv {
+namespace {
+constexpr int kScratchResults = 5 * RED_MAX_BLOCKS;                 // offset of the 5 results
+constexpr size_t kScratchDoubles = (size_t)kScratchResults + 8;     // 81,984 bytes
+}  // namespace
+}  // namespace clv
+
+static __device__ __align__(256) double scratch[clv::kScratchDoubles];
+
+namespace clv {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// the address of this device's `scratch` (looked up on the first call per device)
+clv_status scratch_on(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    clv_status st = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "clv_field_summary_noalloc: hipGetSymbolAddress(scratch)");
+    if (st != CLV_OK) return st;
+    if (!p) return set_error(CLV_ERR_HIP, "clv_field_summary_noalloc: no address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return CLV_OK;
+}
+}  // namespace
+}  // namespace clv
+
+extern "C" clv_status clv_field_summary_noalloc(const clv_chunk_view *v, void *hip_stream, double out[5]) {
+  using namespace clv;
+  if (!v || !out) return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: null view or out");
+  if (v->x_cells < 2 || v->y_cells < 2 || v->pitch < (int64_t)v->x_cells + 5)
+    return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: need x_cells, y_cells >= 2 and pitch >= x_cells + 5");
+  const int read[] = {CLV_DENSITY0, CLV_ENERGY0, CLV_PRESSURE, CLV_XVEL0, CLV_YVEL0, CLV_VOLUME};
+  for (int f : read)
+    if (!v->field[f]) return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: density0, energy0, pressure, xvel0, yvel0 and volume are required");
+  const int nb = red_blocks(v->y_cells);
+  // the partials of nb blocks and the results must fit the fixed scratch array
+  if ((size_t)nb * 5 > (size_t)kScratchResults || nb > RED_MAX_BLOCKS)
+    return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: the reduction needs more partials than the scratch array holds");
+  int device = 0;
+  clv_status st = hip_check(hipGetDevice(&device), "clv_field_summary_noalloc: hipGetDevice");
+  if (st != CLV_OK) return st;
+  double *ptr = nullptr;
+  if ((st = scratch_on(device, &ptr)) != CLV_OK) return st;
+  Dev d{};
+  for (int f = 0; f < CLV_NUM_FIELDS; ++f) d.f[f] = v->field[f];
+  for (int m = 0; m < CLV_NUM_MESH; ++m) d.m[m] = v->mesh[m];
+  d.nx = v->x_cells;
+  d.ny = v->y_cells;
+  d.pitch = v->pitch;
+  d.geom_mesh = 0;  // the caller's volume array
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  hipLaunchKernelGGL(field_summary_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), 0, s, d, ptr);
+  hipLaunchKernelGGL(reduce_sum5_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), 0, s, (const double *)ptr, nb, ptr + kScratchResults);
+  if ((st = hip_check(hipGetLastError(), "clv_field_summary_noalloc: launch")) != CLV_OK) return st;
+  if ((st = hip_check(hipMemcpyAsync(out, ptr + kScratchResults, 5 * sizeof(double), hipMemcpyDeviceToHost, s),
+                      "clv_field_summary_noalloc: copy")) != CLV_OK)
+    return st;
+  return hip_check(hipStreamSynchronize(s), "clv_field_summary_noalloc: sync");
+}

@@ -278,6 +278,19 @@ clv_status clv_field_summary(clv_chunk *c, double out[5]);
  * (a few hundred M cells/s), meant for validation runs. */
 enum { CLV_SUMMARY_TREE = 0, CLV_SUMMARY_SERIAL = 1 };
 clv_status clv_chunk_set_summary_order(clv_chunk *c, int order);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the sums of clv_field_summary (tree order, the same kernels, so the
+ * same doubles) over the arrays of a caller's view -- density0, energy0, pressure,
+ * xvel0, yvel0 and volume are read, the geometry always from the volume array --
+ * with no chunk and no allocation.  Its block partials and results live in one fixed
+ * device array per GPU (81,984 bytes: 5 x 2048 partials + 5 results), kept until the
+ * process ends; the reduction never runs more than 2048 blocks, so no grid is too
+ * large (views with x_cells or y_cells < 2 or pitch < x_cells + 5 are refused).
+ * Ordered on hip_stream (a hipStream_t as void*, NULL = the null stream); blocks
+ * for out.  Calls that overlap on one GPU (other threads, other streams) share that
+ * array and race: serialise them. */
+clv_status clv_field_summary_noalloc(const clv_chunk_view *view, void *hip_stream, double out[5]);
 
 /* ---- whole-step forms (the host loop of driver/hydro.cpp) -------------------------------- */
 /* Parses deck text with driver/read_input.cpp semantics (defaults :39-69). */
