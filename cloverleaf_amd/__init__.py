@@ -33,6 +33,7 @@ MID = {n: i for i, n in enumerate(MESH)}
 PLAN_REFERENCE, PLAN_FUSED = 0, 1
 SUMMARY_TREE, SUMMARY_SERIAL = 0, 1
 GEOMETRY_ARRAYS, GEOMETRY_MESH = 0, 1
+ENTRY_REFERENCE, ENTRY_FUSED = 0, 1
 MAX_STATES = 16
 ERR_SMALL_DT = -4
 
@@ -110,6 +111,7 @@ SIGNATURES = [
     ("clv_chunk_get_view", _I, [_P, C.POINTER(ChunkView)]),
     ("clv_chunk_geometry_from_mesh", _I, [_P]),
     ("clv_chunk_set_geometry_mode", _I, [_P, _I]),
+    ("clv_chunk_set_entry_plan", _I, [_P, _I]),
     ("clv_chunk_set_stream", _I, [_P, _P]),
     ("clv_chunk_synchronize", _I, [_P]),
     ("clv_chunk_bytes", C.c_size_t, [_P]),
@@ -148,6 +150,7 @@ SIGNATURES = [
     ("clv_timestep", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _I]),
     ("clv_hydro_step", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _I]),
     ("clv_hydro_summary", _I, [_P, _PD]),
+    ("clv_hydro_phase", _I, [_P, C.POINTER(Config), C.POINTER(HydroState), _I]),
     ("clv_profile_enable", _I, [_P, _I]),
     ("clv_profile_read", _I, [_P, C.POINTER(C.c_char_p), C.POINTER(C.c_int64), _PD, _I]),
     ("clv_profile_reset", _I, [_P]),
@@ -298,6 +301,11 @@ class Chunk:
 
     def set_geometry_mode(self, mode: int):
         _check(self.L.clv_chunk_set_geometry_mode(self.h, int(mode)), "set_geometry_mode")
+
+    def set_entry_plan(self, plan: int):
+        """ENTRY_REFERENCE (one launch per reference loop) or ENTRY_FUSED (advec_cell,
+        advec_mom and the PdV predictor chain one launch each)."""
+        _check(self.L.clv_chunk_set_entry_plan(self.h, int(plan)), "set_entry_plan")
 
     def close(self):
         if getattr(self, "h", None):

@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include <utility>
+
 #include "clv_internal.h"
 
 namespace clv {
@@ -362,6 +364,7 @@ clv_status clv_PdV(clv_chunk *c, int predict, double dt) {
 
 clv_status clv_PdV_driver(clv_chunk *c, int predict, double dt) {
   CHECK_CHUNK(c);
+  if (predict && c->entry_fused) return e_PdV_predict_chain(c, dt);
   clv_status st = k_PdV(c, predict, dt);
   if (st != CLV_OK || !predict) return st;
   if ((st = k_ideal_gas(c, 1)) != CLV_OK) return st;
@@ -390,14 +393,33 @@ clv_status clv_advec_cell(clv_chunk *c, int sweep_number, int direction) {
   CHECK_CHUNK(c);
   if ((sweep_number != 1 && sweep_number != 2) || (direction != CLV_XDIR && direction != CLV_YDIR))
     return set_error(CLV_ERR_ARG, "clv_advec_cell: sweep_number must be 1|2 and direction 1|2");
-  return k_advec_cell(c, sweep_number, direction);
+  if (!c->entry_fused) return k_advec_cell(c, sweep_number, direction);
+  // the new density1 / energy1 go to work arrays 6 / 7, which then take their place
+  clv_status st = e_advec_cell(c, sweep_number, direction, c->d.f[CLV_WORK6], c->d.f[CLV_WORK7]);
+  if (st != CLV_OK) return st;
+  std::swap(c->d.f[CLV_DENSITY1], c->d.f[CLV_WORK6]);
+  std::swap(c->d.f[CLV_ENERGY1], c->d.f[CLV_WORK7]);
+  return CLV_OK;
 }
 
 clv_status clv_advec_mom(clv_chunk *c, int which_vel, int direction, int sweep_number) {
   CHECK_CHUNK(c);
   if ((which_vel != 1 && which_vel != 2) || (sweep_number != 1 && sweep_number != 2) || (direction != CLV_XDIR && direction != CLV_YDIR))
     return set_error(CLV_ERR_ARG, "clv_advec_mom: which_vel, direction and sweep_number must each be 1|2");
-  return k_advec_mom(c, which_vel, direction, sweep_number);
+  if (!c->entry_fused) return k_advec_mom(c, which_vel, direction, sweep_number);
+  // the new velocity goes to work array 5, which then takes its place
+  clv_status st = e_advec_mom(c, which_vel, direction, sweep_number, c->d.f[CLV_WORK5]);
+  if (st != CLV_OK) return st;
+  std::swap(c->d.f[which_vel == 1 ? CLV_XVEL1 : CLV_YVEL1], c->d.f[CLV_WORK5]);
+  return CLV_OK;
+}
+
+clv_status clv_chunk_set_entry_plan(clv_chunk *c, int plan) {
+  CHECK_CHUNK(c);
+  if (plan != CLV_ENTRY_REFERENCE && plan != CLV_ENTRY_FUSED)
+    return set_error(CLV_ERR_ARG, "clv_chunk_set_entry_plan: plan is CLV_ENTRY_REFERENCE or CLV_ENTRY_FUSED");
+  c->entry_fused = plan == CLV_ENTRY_FUSED;
+  return CLV_OK;
 }
 
 clv_status clv_reset_field(clv_chunk *c) {

@@ -57,6 +57,7 @@ struct clv_chunk {
   int summary_serial = 0;  // CLV_SUMMARY_SERIAL: reproduce the serial model's summation order
   int seg[4] = {0};        // rows per segment of the fused plan's kernel families (chosen on first use)
   int geom_mode = CLV_GEOMETRY_ARRAYS;  // CLV_GEOMETRY_MESH: initialise_chunk sets d.geom_mesh
+  int entry_fused = 0;  // CLV_ENTRY_FUSED: the per-kernel entry points run one launch each (kernels_entry.hip)
 };
 
 namespace clv {
@@ -88,6 +89,12 @@ clv_status k_pack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_F
                          double *buf, int unpack);
 clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, double *dt_min_val);
 clv_status k_field_summary(clv_chunk *c, double out[5]);
+
+// one launch per entry point (kernels_entry.hip); the advection kernels write the
+// updated fields to the given arrays
+clv_status e_PdV_predict_chain(clv_chunk *c, double dt);
+clv_status e_advec_cell(clv_chunk *c, int sweep_number, int dir, double *rho_out, double *e_out);
+clv_status e_advec_mom(clv_chunk *c, int which_vel, int direction, int sweep_number, double *vel_out);
 
 // fused plan (kernels_fused.hip)
 size_t fused_partials_needed(int nx, int ny);

@@ -236,23 +236,42 @@ clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st
   return CLV_OK;
 }
 
+// One phase of driver/hydro.cpp:57-89 on the reference plan; CLV_PHASE_END_STEP is the
+// bookkeeping after reset_field: flip advect_x, advance time, set complete (:86-89, :108).
+clv_status clv_hydro_phase(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int phase) {
+  if (!c || !cfg || !st) return set_error(CLV_ERR_ARG, "clv_hydro_phase: null argument");
+  const double dt = st->dt;
+  switch (phase) {
+    case CLV_PHASE_TIMESTEP:
+      st->step += 1;
+      return clv_timestep(c, cfg, st, CLV_PLAN_REFERENCE);
+    case CLV_PHASE_PDV_PREDICT: return clv_PdV_driver(c, 1, dt);
+    case CLV_PHASE_ACCELERATE: return k_accelerate(c, dt);
+    case CLV_PHASE_PDV_CORRECT: return clv_PdV_driver(c, 0, dt);
+    case CLV_PHASE_FLUX_CALC: return k_flux_calc(c, dt);
+    case CLV_PHASE_ADVECTION: return advection_reference(c, st->advect_x);
+    case CLV_PHASE_RESET_FIELD: return k_reset_field(c);
+    case CLV_PHASE_END_STEP:
+      st->advect_x = !st->advect_x;
+      st->time += dt;
+      st->complete = (st->time + g_small > cfg->end_time || st->step >= cfg->end_step) ? 1 : 0;
+      return CLV_OK;
+  }
+  return set_error(CLV_ERR_ARG, "clv_hydro_phase: phase is one of CLV_PHASE_*");
+}
+
 // driver/hydro.cpp:57-89 and :108
 clv_status clv_hydro_step(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan) {
   if (!c || !cfg || !st) return set_error(CLV_ERR_ARG, "clv_hydro_step: null argument");
+  if (plan != CLV_PLAN_FUSED) {
+    for (int phase = CLV_PHASE_TIMESTEP; phase <= CLV_PHASE_END_STEP; ++phase) TRY(clv_hydro_phase(c, cfg, st, phase));
+    return CLV_OK;
+  }
   st->step += 1;
   TRY(clv_timestep(c, cfg, st, plan));
   const double dt = st->dt;
-  if (plan == CLV_PLAN_FUSED) {
-    TRY(fused_lagrange(c, dt));
-    TRY(fused_advection(c, st->advect_x));
-  } else {
-    TRY(clv_PdV_driver(c, 1, dt));
-    TRY(k_accelerate(c, dt));
-    TRY(clv_PdV_driver(c, 0, dt));
-    TRY(k_flux_calc(c, dt));
-    TRY(advection_reference(c, st->advect_x));
-    TRY(k_reset_field(c));
-  }
+  TRY(fused_lagrange(c, dt));
+  TRY(fused_advection(c, st->advect_x));
   st->advect_x = !st->advect_x;
   st->time += dt;
   st->complete = (st->time + g_small > cfg->end_time || st->step >= cfg->end_step) ? 1 : 0;

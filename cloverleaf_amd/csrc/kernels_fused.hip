@@ -445,7 +445,6 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
       const double xa_e_prev = dn1(xa_prev);
       if (r > j0) {
         // corrector for cell (i, r - 1), src/serial/PdV.cpp:72-83
-        const int c = r - 1;
         double left_flux = (xa_prev * (u0_prev + u_r + u1_prev + u1)) * 0.25 * dt;
         double right_flux = (xa_e_prev * (u0_e_prev + u0_e + u1_e_prev + u1_e)) * 0.25 * dt;
         double bottom_flux = (ya_prev * (v0_prev + v0_e_prev + v1_prev + v1_e_prev)) * 0.25 * dt;

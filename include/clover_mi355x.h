@@ -184,6 +184,23 @@ clv_status clv_initialise_chunk(clv_chunk *c, const clv_config *cfg);
 clv_status clv_generate_chunk(clv_chunk *c, const clv_config *cfg);
 
 /* ---- per-step kernels (one reference entry point each) ---------------------------------- */
+/* How the per-kernel entry points below run (no counterpart in the reference, whose
+ * models all run one loop nest per loop).
+ * CLV_ENTRY_REFERENCE (the default): one launch per reference loop, every work array
+ *   written as the reference writes it.
+ * CLV_ENTRY_FUSED: clv_advec_cell, clv_advec_mom and clv_PdV_driver(predict = 1) run
+ *   one launch each, the values the reference passes between its loops through work
+ *   arrays kept in registers (the same doubles).  clv_advec_cell writes the new
+ *   density1 / energy1 into work arrays 6 / 7 and exchanges the two pairs of pointers
+ *   in the chunk; clv_advec_mom does the same with the velocity and work array 5.  A
+ *   caller that holds the arrays' addresses (a wrapped chunk's owner) reads them again
+ *   with clv_chunk_get_view after those calls.  Every array the reference reads later
+ *   holds the reference's values; the work arrays and the ghost layers of an
+ *   exchanged array hold other values until the next halo update of that field.
+ *   clv_PdV_driver's fused chain writes the pressure halo of every face (a tile whose
+ *   four faces are external). */
+enum { CLV_ENTRY_REFERENCE = 0, CLV_ENTRY_FUSED = 1 };
+clv_status clv_chunk_set_entry_plan(clv_chunk *c, int plan);
 /* void ideal_gas(global_variables&, const int tile, bool predict)   driver/ideal_gas.h:24 */
 clv_status clv_ideal_gas(clv_chunk *c, int predict);
 /* void viscosity(global_variables&)                                 driver/viscosity.h:24 */
@@ -310,6 +327,23 @@ clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st
 clv_status clv_hydro_step(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan);
 /* field_summary with its ideal_gas (src/serial/field_summary.cpp:51-99). */
 clv_status clv_hydro_summary(clv_chunk *c, double out[5]);
+/* The phases of one step of driver/hydro.cpp:59-85 on the reference plan, for a host
+ * loop that acts between them as the reference's does (its --dump writes the fields
+ * after each, driver/hydro.cpp:60-85).  Called in order they make one
+ * clv_hydro_step(..., CLV_PLAN_REFERENCE): CLV_PHASE_TIMESTEP advances st->step and
+ * chooses dt (clv_timestep; CLV_ERR_SMALL_DT as there); CLV_PHASE_END_STEP runs no
+ * kernel: it flips advect_x, advances time and sets complete (:86-89, :108). */
+enum {
+  CLV_PHASE_TIMESTEP = 0,    /* timestep()            driver/timestep.cpp:37-108 */
+  CLV_PHASE_PDV_PREDICT = 1, /* PdV(globals, true)    src/serial/PdV.cpp:92-133 */
+  CLV_PHASE_ACCELERATE = 2,  /* accelerate()          src/serial/accelerate.cpp */
+  CLV_PHASE_PDV_CORRECT = 3, /* PdV(globals, false) */
+  CLV_PHASE_FLUX_CALC = 4,   /* flux_calc() */
+  CLV_PHASE_ADVECTION = 5,   /* advection()           driver/advection.cpp:29-105 */
+  CLV_PHASE_RESET_FIELD = 6, /* reset_field() */
+  CLV_PHASE_END_STEP = 7     /* advect_x, time, complete */
+};
+clv_status clv_hydro_phase(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int phase);
 
 /* ---- batched form: many independent problems advanced together ------------------------- */
 /* No counterpart in the reference, whose driver advances one problem
