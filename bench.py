@@ -477,6 +477,14 @@ def cpu_baseline_reference(steps, deck="clover_bm16_300"):
 
 
 def dropin_leg(deck_text, steps):
+    """The drop-in with the model's one-launch entry points (the default) and with one
+    launch per reference loop (CLOVER_MI355X_ENTRY=reference)."""
+    out = dropin_run(deck_text, steps, {})
+    out["one_launch_per_reference_loop"] = dropin_run(deck_text, steps, {"CLOVER_MI355X_ENTRY": "reference"})
+    return out
+
+
+def dropin_run(deck_text, steps, env_extra):
     """The drop-in: the reference's unchanged driver (driver/*.cpp) linked with the MI355X
     plugin model (cloverleaf_amd/model_mi355x/_dropin/mi355x-cloverleaf) on the headline
     deck trimmed to `steps` steps; the rate is the driver's own `Average time per cell`
@@ -491,8 +499,10 @@ def dropin_leg(deck_text, steps):
     text = re.sub(r"end_step=\d+", f"end_step={steps}", deck_text)
     with tempfile.TemporaryDirectory() as td:
         (Path(td) / "deck.in").write_text(text)
+        env = dict(os.environ)
+        env.update(env_extra)
         p = subprocess.run([str(exe), "--in", "deck.in", "--out", "clover.out"], cwd=td, capture_output=True, text=True,
-                           timeout=900)
+                           timeout=900, env=env)
     # the driver prints the grind lines after every step but the last; at the end it
     # prints the whole hydro wall clock (driver/hydro.cpp:108-118, 205-218)
     walls = re.findall(r"^ Wall clock\s+([0-9.eE+-]+)", p.stdout, re.M)

@@ -38,6 +38,7 @@
 // (rho1, e1 in advec_cell; u1, v1 in advec_mom) writes into a work array, and
 // the host swaps the two pointers afterwards.
 #include "clv_internal.h"
+#include "lanes.h"
 
 namespace clv {
 namespace {
@@ -46,26 +47,6 @@ namespace {
 #define M(mid, i) d.m[mid][i]
 
 constexpr int WAVES = 4;       // waves (strips) per 256-thread block
-
-// Lane shifts across the whole wave64 with DPP (GFX9 wave_shr:1 / wave_shl:1):
-// one VALU move per 32-bit half instead of an LDS-crossbar bpermute.  The lane
-// shifted in at the wave's edge keeps its own value; edge lanes never produce
-// owned results, so that value is never used.
-__device__ __forceinline__ double dpp_shift(double v, int ctrl_is_shr) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  if (ctrl_is_shr) {
-    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);  // wave_shr:1
-    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
-  } else {
-    lo = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);  // wave_shl:1
-    hi = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
-  }
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double up1(double v) { return dpp_shift(v, 1); }          // value of lane - 1
-__device__ __forceinline__ double up2(double v) { return dpp_shift(dpp_shift(v, 1), 1); }  // value of lane - 2
-__device__ __forceinline__ double dn1(double v) { return dpp_shift(v, 0); }          // value of lane + 1
-__device__ __forceinline__ double dn2(double v) { return dpp_shift(dpp_shift(v, 0), 0); }  // value of lane + 2
 
 __device__ __forceinline__ double wave_min(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));

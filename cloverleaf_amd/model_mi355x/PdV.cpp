@@ -1,6 +1,7 @@
 // void PdV(global_variables&, bool predict)   driver/PdV.h:24
 // The sequence of src/serial/PdV.cpp:92-133: the kernel, the error check, and for
-// the predictor ideal_gas(predict), the pressure halo and revert.
+// the predictor ideal_gas(predict), the pressure halo and revert -- as one launch
+// (clv_PdV_driver under CLV_ENTRY_FUSED) for a single tile with every face external.
 #include "PdV.h"
 
 #include "comms.h"
@@ -15,6 +16,20 @@ void PdV(global_variables &globals, bool predict) {
   double kernel_time = 0;
   if (globals.profiler_on) kernel_time = timer();
   globals.error_condition = 0;
+  // one tile with every face external: the whole predictor chain in one launch (its
+  // pressure halo is the reflective one of every face; the profiler counts it as PdV)
+  if (predict && mi355x::fused_entries(globals) && globals.config.tiles_per_chunk == 1) {
+    int external[4];
+    mi355x::external_faces(globals, 0, external);
+    if (external[0] && external[1] && external[2] && external[3]) {
+      mi355x::check(clv_PdV_driver(mi355x::chunk(globals, 0), 1, globals.dt), "PdV");
+      mi355x::sync_if_profiling(globals, 0);
+      clover_check_error(globals.error_condition);
+      if (globals.profiler_on) globals.profiler.PdV += timer() - kernel_time;
+      if (globals.error_condition == 1) report_error((char *)"PdV", (char *)"error in PdV");
+      return;
+    }
+  }
   for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {
     mi355x::check(clv_PdV(mi355x::chunk(globals, tile), predict ? 1 : 0, globals.dt), "PdV");
     mi355x::sync_if_profiling(globals, tile);

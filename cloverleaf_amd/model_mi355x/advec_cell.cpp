@@ -5,5 +5,6 @@
 
 void advec_cell_driver(global_variables &globals, int tile, int sweep_number, int direction) {
   mi355x::check(clv_advec_cell(mi355x::chunk(globals, tile), sweep_number, direction), "advec_cell");
+  mi355x::refresh_buffers(globals, tile);  // density1 / energy1 may now be other arrays
   mi355x::sync_if_profiling(globals, tile);
 }

@@ -14,7 +14,12 @@
 // covers (y_cells + 5) rows of pitch >= x_cells + 5).  The pitch is fixed by the
 // first 2D buffer a context allocates (density0, a cell array, xrange = nx + 4,
 // driver/definitions.cpp:6) as the next multiple of 16 doubles above it, so the
-// one-larger node and face arrays fit the same pitch.
+// one-larger node and face arrays fit the same pitch.  Element (0, 0) sits 14 elements
+// past the allocation's 256-byte boundary, so element (2, j), the first interior cell
+// of every row, starts a 128-byte line: a wave's 64 lanes then store whole lines (a line
+// written partly by two waves costs HBM time).  With CLV_ENTRY_FUSED the model exchanges
+// buffers between handles (advection), so every 2D buffer has the same offset and
+// release() frees data - kLead2D.
 #pragma once
 
 #include <cstddef>
@@ -66,17 +71,18 @@ template <typename T> struct Buffer1D {
 };
 
 template <typename T> struct Buffer2D {
+  static constexpr size_t kLead2D = 128 / sizeof(T) - 2;  // element (2, j) on a 128-byte line
   size_t sizeX, sizeY, pitch;
   T *data;
   Buffer2D(context &ctx, size_t sizeX, size_t sizeY) : sizeX(sizeX), sizeY(sizeY), pitch(0), data(nullptr) {
     if (ctx.pitch == 0 || ctx.pitch < sizeX) ctx.pitch = (sizeX + 1 + 15) / 16 * 16;
     pitch = ctx.pitch;
     void *p = nullptr;
-    check_clv(clv_device_alloc((sizeY + 1) * pitch * sizeof(T), &p), "Buffer2D allocation");
-    data = static_cast<T *>(p);
+    check_clv(clv_device_alloc(((sizeY + 1) * pitch + kLead2D) * sizeof(T), &p), "Buffer2D allocation");
+    data = static_cast<T *>(p) + kLead2D;
   }
   void release() {
-    check_clv(clv_device_free(data), "Buffer2D release");
+    check_clv(clv_device_free(data - kLead2D), "Buffer2D release");
     data = nullptr;
   }
   T *actual() { return data; }

@@ -95,8 +95,22 @@ clv_chunk *chunk(global_variables &globals, int tile) {
   check(clv_chunk_wrap(&v, &c), "clv_chunk_wrap");
   if (const char *order = std::getenv("CLOVER_MI355X_SUMMARY"); order && std::string(order) == "serial")
     check(clv_chunk_set_summary_order(c, CLV_SUMMARY_SERIAL), "clv_chunk_set_summary_order");
+  if (fused_entries(globals)) check(clv_chunk_set_entry_plan(c, CLV_ENTRY_FUSED), "clv_chunk_set_entry_plan");
   g_chunks[f.density0.data] = c;
   return c;
+}
+
+bool fused_entries(const global_variables &globals) {
+  const char *entry = std::getenv("CLOVER_MI355X_ENTRY");
+  return globals.config.dumpDir.empty() && !(entry && std::string(entry) == "reference");
+}
+
+void refresh_buffers(global_variables &globals, int tile) {
+  clv_chunk_view v{};
+  check(clv_chunk_get_view(chunk(globals, tile), &v), "clv_chunk_get_view");
+  clover::Buffer2D<double> *b2[CLV_NUM_FIELDS];
+  fields_of(globals.chunk.tiles[tile].field, b2);
+  for (int k = 0; k < CLV_NUM_FIELDS; ++k) b2[k]->data = v.field[k];
 }
 
 void sync_if_profiling(global_variables &globals, int tile) {

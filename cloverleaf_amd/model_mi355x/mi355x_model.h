@@ -22,6 +22,14 @@ void external_faces(const global_variables &globals, int tile, int external[4]);
 void sync_if_profiling(global_variables &globals, int tile);
 // Destroys the chunk bindings (finalise).
 void release_chunks();
+// Whether the tiles run advec_cell, advec_mom and the PdV predictor chain as one launch
+// each (CLV_ENTRY_FUSED): yes unless the driver dumps fields (--dump writes the work
+// arrays after every phase, which the fused kernels leave as scratch) or
+// CLOVER_MI355X_ENTRY=reference.
+bool fused_entries(const global_variables &globals);
+// After a call that may exchange a tile's buffers (CLV_ENTRY_FUSED advection): point the
+// driver's Buffer2D handles at the arrays the chunk now uses.
+void refresh_buffers(global_variables &globals, int tile);
 
 inline void check(clv_status st, const char *what) { clover::check_clv(st, what); }
 [[noreturn]] void unsupported(const char *what);

@@ -166,3 +166,140 @@ def test_bad_arguments_are_rejected(gpu):
             ch.update_halo([0], 3)
         with pytest.raises(clv.CloverError):
             ch.advec_mom(1, 0, 1)
+
+
+# ---- CLV_ENTRY_FUSED: advec_cell, advec_mom and the PdV predictor chain in one launch ----
+def _interior_rows_cols(n, ext):
+    return slice(2, n + 2 + ext)
+
+
+def _same_except(ch, o, what, regions):
+    """Every array equal to the oracle's over its whole extent, except those in
+    `regions` (compared over the given (rows, cols) region only) and the work arrays."""
+    bad = []
+    for n in T.FIELDS:
+        if n.startswith("work"):
+            continue
+        a, b = ch.download(n), o.f(n)
+        if n in regions:
+            a, b = a[regions[n]], b[regions[n]]
+        same = (a == b) | (np.isnan(a) & np.isnan(b))
+        if not same.all():
+            bad.append((n, np.argwhere(~same)[:3].tolist()))
+    assert not bad, f"{what}: {bad}"
+
+
+ENTRY_SIZES = SIZES + [(4, 4), (5, 9), (63, 66), (65, 3)]
+
+
+@pytest.mark.parametrize("nx,ny", ENTRY_SIZES)
+@pytest.mark.parametrize("sweep,d", [(1, 1), (1, 2), (2, 1), (2, 2)])
+def test_entry_fused_advec_cell_matches_oracle(gpu, nx, ny, sweep, d):
+    """One launch: density1 and energy1 on the interior cells, the mass flux over its
+    whole array (the reference's face range written, the rest kept) and every other
+    array bit-identical to the oracle's three loops; density1/energy1 now live in
+    other arrays (their ghosts wait for the driver's next halo update)."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=zlib.crc32(repr(("entry_cell", sweep, d, nx, ny)).encode()))
+    o = _oracle(nx, ny, f, m)
+    with clv.Chunk(nx, ny) as ch:
+        _load(ch, f, m)
+        ch.set_entry_plan(clv.ENTRY_FUSED)
+        before = ch.field_ptr("density1")
+        ch.advec_cell(sweep, d)
+        o.L.orc_advec_cell(C.byref(o.c), sweep, d)
+        assert ch.field_ptr("density1") != before
+        inner = (slice(2, ny + 2), slice(2, nx + 2))
+        _same_except(ch, o, f"entry advec_cell({sweep}, {d}) {nx}x{ny}", {"density1": inner, "energy1": inner})
+
+
+@pytest.mark.parametrize("nx,ny", ENTRY_SIZES)
+@pytest.mark.parametrize("d,sweep", [(1, 1), (2, 1), (1, 2), (2, 2)])
+def test_entry_fused_advec_mom_matches_oracle(gpu, nx, ny, d, sweep):
+    """xvel then yvel (the reference's yvel call reuses the xvel call's node arrays; the
+    fused yvel call recomputes them): both velocities on the interior and boundary
+    nodes and every other array bit-identical to the oracle."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=zlib.crc32(repr(("entry_mom", sweep, d, nx, ny)).encode()))
+    o = _oracle(nx, ny, f, m)
+    with clv.Chunk(nx, ny) as ch:
+        _load(ch, f, m)
+        ch.set_entry_plan(clv.ENTRY_FUSED)
+        nodes = (slice(2, ny + 3), slice(2, nx + 3))
+        ch.advec_mom(1, d, sweep)
+        o.L.orc_advec_mom(C.byref(o.c), 1, sweep, d)
+        _same_except(ch, o, f"entry advec_mom(xvel, {d}, {sweep}) {nx}x{ny}", {"xvel1": nodes})
+        ch.advec_mom(2, d, sweep)
+        o.L.orc_advec_mom(C.byref(o.c), 2, sweep, d)
+        _same_except(ch, o, f"entry advec_mom(yvel, {d}, {sweep}) {nx}x{ny}", {"xvel1": nodes, "yvel1": nodes})
+
+
+@pytest.mark.parametrize("nx,ny", ENTRY_SIZES)
+def test_entry_fused_pdv_predict_chain_matches_oracle(gpu, nx, ny):
+    """PdV predictor + ideal_gas(predict) + pressure halo + revert in one launch: every
+    array bit-identical to the oracle's chain over its whole extent."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=zlib.crc32(repr(("entry_pdv", nx, ny)).encode()))
+    o = _oracle(nx, ny, f, m)
+    with clv.Chunk(nx, ny) as ch:
+        _load(ch, f, m)
+        ch.set_entry_plan(clv.ENTRY_FUSED)
+        ch.PdV_driver(1, 0.0123)
+        o.L.orc_PdV_driver(C.byref(o.c), 1, 0.0123)
+        _same(ch, o, f"entry PdV predict chain {nx}x{ny}")
+        # the corrector is not fused: the same as the reference plan
+        ch.PdV_driver(0, 0.0123)
+        o.L.orc_PdV_driver(C.byref(o.c), 0, 0.0123)
+        _same(ch, o, f"entry PdV corrector {nx}x{ny}")
+
+
+@pytest.mark.parametrize("deck", ["small_bm_64x48", "small_geom_mix", "clover_sodxy"])
+def test_entry_fused_whole_runs_match_oracle(gpu, deck):
+    """A run through the per-kernel entry points under CLV_ENTRY_FUSED, sequenced like
+    driver/hydro.cpp (timestep, PdV chain, accelerate, PdV, flux_calc, advection with
+    its halo updates, reset_field): dt at every step and every persistent array equal
+    to the oracle's."""
+    d = T.load_deck(deck)
+    cfg = clv.parse_deck((T.DECKS / f"{deck}.in").read_text())
+    o = T.Oracle(d)
+    o.start()
+    nx, ny = d.x_cells, d.y_cells
+    with clv.Chunk(nx, ny) as ch:
+        st, _ = ch.start(cfg)
+        ch.set_entry_plan(clv.ENTRY_FUSED)
+        advect_x = True
+        for step in range(1, 21):
+            # driver/timestep.cpp:47-60
+            ch.ideal_gas(False)
+            ch.update_halo(["pressure", "energy0", "density0", "xvel0", "yvel0"], 1)
+            ch.viscosity()
+            ch.update_halo(["viscosity"], 1)
+            dtc = ch.calc_dt(d.dtmin, d.dtc_safe, d.dtu_safe, d.dtv_safe, d.dtdiv_safe)[0]
+            assert o.step() == 0
+            dt = min(dtc, st.dtold * d.dtrise, d.dtmax)
+            assert dt == o.g.dt, step
+            st.dtold = dt
+            ch.PdV_driver(1, dt)
+            ch.accelerate(dt)
+            ch.PdV_driver(0, dt)
+            ch.flux_calc(dt)
+            # driver/advection.cpp:29-105
+            dirs = (1, 2) if advect_x else (2, 1)
+            ch.update_halo(["energy1", "density1", "vol_flux_x", "vol_flux_y"], 2)
+            ch.advec_cell(1, dirs[0])
+            mom = ["density1", "energy1", "xvel1", "yvel1", "mass_flux_x", "mass_flux_y"]
+            ch.update_halo(mom, 2)
+            ch.advec_mom(1, dirs[0], 1)
+            ch.advec_mom(2, dirs[0], 1)
+            ch.advec_cell(2, dirs[1])
+            ch.update_halo(mom, 2)
+            ch.advec_mom(1, dirs[1], 2)
+            ch.advec_mom(2, dirs[1], 2)
+            ch.reset_field()
+            advect_x = not advect_x
+        cells = (slice(2, ny + 2), slice(2, nx + 2))
+        nodes = (slice(2, ny + 3), slice(2, nx + 3))
+        for n, reg in (("density0", cells), ("energy0", cells), ("xvel0", nodes), ("yvel0", nodes), ("density1", cells),
+                       ("energy1", cells), ("xvel1", nodes), ("yvel1", nodes), ("viscosity", None), ("pressure", None),
+                       ("vol_flux_x", None), ("vol_flux_y", None), ("mass_flux_x", None), ("mass_flux_y", None)):
+            a, b = ch.download(n), o.f(n)
+            if reg is not None:
+                a, b = a[reg], b[reg]
+            assert np.array_equal(a, b), n
