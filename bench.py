@@ -543,7 +543,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60, help="steps of the CPU baselines on the 3840^2 sample")
     ap.add_argument("--no-mesh", action="store_true", help="skip the timed run with the geometry from the mesh")
     ap.add_argument("--no-dropin", action="store_true", help="skip the drop-in (reference driver) leg")
-    ap.add_argument("--dropin-steps", type=int, default=30)
+    ap.add_argument("--dropin-steps", type=int, default=87, help="steps of the drop-in run (87: the deck as it is)")
     ap.add_argument("--profile-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -48,13 +48,27 @@ inline Dual operator/(Dual a, Dual b) {
 inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
 inline Dual &operator+=(Dual &a, Dual b) { return a = a + b; }
 inline Dual &operator-=(Dual &a, Dual b) { return a = a - b; }
-inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
-inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
-inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
-inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
-inline Dual fabs(Dual a) { return a.v < 0.0 ? -a : a; }
-inline Dual fmin(Dual a, Dual b) { return b.v < a.v ? b : a; }
-inline Dual fmax(Dual a, Dual b) { return b.v > a.v ? b : a; }
+// Kink census (orc_tl_set_kink_eps / orc_tl_kinks).  Every piecewise choice of the
+// tangent-linear run goes through one of the comparisons below, fabs, fmin or fmax,
+// and turns on the sign of a difference x = a - b.  Perturbing the inputs by
+// +-eps along the run's direction moves x by +-eps * dx to first order, so the choice
+// differs between the two runs of a central difference when |x| < eps |dx|.  With
+// eps > 0 each such choice is counted (a tie that does not move, dx = 0, is not).
+double g_kink_eps = 0.0;
+long long g_kinks = 0;
+inline void kink(double x, double dx) {
+  if (g_kink_eps > 0.0 && ::fabs(x) < g_kink_eps * ::fabs(dx)) {
+#pragma omp atomic
+    ++g_kinks;
+  }
+}
+inline bool operator<(Dual a, Dual b) { kink(a.v - b.v, a.d - b.d); return a.v < b.v; }
+inline bool operator>(Dual a, Dual b) { kink(a.v - b.v, a.d - b.d); return a.v > b.v; }
+inline bool operator<=(Dual a, Dual b) { kink(a.v - b.v, a.d - b.d); return a.v <= b.v; }
+inline bool operator>=(Dual a, Dual b) { kink(a.v - b.v, a.d - b.d); return a.v >= b.v; }
+inline Dual fabs(Dual a) { kink(a.v, a.d); return a.v < 0.0 ? -a : a; }
+inline Dual fmin(Dual a, Dual b) { kink(a.v - b.v, a.d - b.d); return b.v < a.v ? b : a; }
+inline Dual fmax(Dual a, Dual b) { kink(a.v - b.v, a.d - b.d); return b.v > a.v ? b : a; }
 inline Dual sqrt(Dual a) {
   const double s = ::sqrt(a.v);
   return Dual(s, s > 0.0 ? a.d / (2.0 * s) : 0.0);
@@ -1118,3 +1132,8 @@ void orc_tl_field_summary(orc_tl_chunk *t, orc_dual out[5]) {
 void orc_tl_PdV_driver(orc_tl_chunk *t, int predict, double dt) { k_PdV_driver(V(t), predict, dt); }
 void orc_tl_advection(orc_tl_chunk *t, int advect_x) { k_advection(V(t), advect_x); }
 void orc_tl_step_fixed(orc_tl_chunk *t, double dt, int advect_x) { k_step_fixed(V(t), dt, advect_x); }
+void orc_tl_set_kink_eps(double eps) {
+  g_kink_eps = eps;
+  g_kinks = 0;
+}
+long long orc_tl_kinks(void) { return g_kinks; }

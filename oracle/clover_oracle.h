@@ -167,6 +167,12 @@ void orc_tl_field_summary(orc_tl_chunk *t, orc_dual out[5]);
 void orc_tl_PdV_driver(orc_tl_chunk *t, int predict, double dt);
 void orc_tl_advection(orc_tl_chunk *t, int advect_x);
 void orc_tl_step_fixed(orc_tl_chunk *t, double dt, int advect_x);
+/* Kink census of tangent-linear runs: with eps > 0, every piecewise choice (comparison,
+ * fabs, fmin, fmax) whose deciding difference x satisfies |x| < eps |dx| is counted --
+ * a choice that differs between the +eps and -eps runs of a central difference along
+ * the run's direction, to first order.  Setting eps resets the count. */
+void orc_tl_set_kink_eps(double eps);
+long long orc_tl_kinks(void);
 
 #ifdef __cplusplus
 }

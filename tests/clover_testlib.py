@@ -426,6 +426,8 @@ def _tl_lib():
         L.orc_tl_field_summary.argtypes = [P(_OrcTLChunk), P(_OrcDual)]
         L.orc_tl_field_summary_kernel.argtypes = [P(_OrcTLChunk), P(_OrcDual)]
         L.orc_step_fixed.argtypes = [P(_OrcChunk), C.c_double, C.c_int]
+        L.orc_tl_set_kink_eps.argtypes = [C.c_double]
+        L.orc_tl_kinks.restype = C.c_longlong
         for k, args in _TL_SIGS.items():
             getattr(L, f"orc_tl_{k}").argtypes = [P(_OrcTLChunk)] + args
         L._tl_ready = True
