@@ -257,22 +257,22 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Rows of a reduction block: block b takes the b-th of gridDim.x contiguous row ranges,
-// so the node rows j and j + 1 a cell row reads stay in one block (one XCD's L2)
-// instead of being fetched twice by blocks on different XCDs.
-__device__ __forceinline__ void block_rows(int ny, int &j0, int &j1) {
-  const int per = (ny + gridDim.x - 1) / gridDim.x;
-  j0 = 2 + blockIdx.x * per;
-  j1 = min(j0 + per, ny + 2);
+// First row of a reduction block; the block then takes every gridDim.x-th row, so the
+// grid streams one band of gridDim.x rows at a time.  Blocks are dealt to the 8 XCDs
+// round-robin (block b on XCD b % 8); XCD x takes the x-th eighth of each band, so the
+// node rows j and j + 1 that cell row j reads are read by blocks of one XCD (one L2)
+// instead of being fetched twice.
+__device__ __forceinline__ int band_row(int G) {
+  const int b = blockIdx.x;
+  if (G % 8) return b;
+  return (b % 8) * (G / 8) + b / 8;
 }
 
 // calc_dt: src/serial/calc_dt.cpp:50-77
 __global__ __launch_bounds__(RED_THREADS) void calc_dt_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
                                                           double *partials) {
   double dt_min_val0 = g_big;
-  int jb, je;
-  block_rows(d.ny, jb, je);
-  for (int j = jb; j < je; ++j) {
+  for (int j = 2 + band_row(gridDim.x); j < d.ny + 2; j += gridDim.x) {
     const double dsy = M(CLV_CELLDY, j);
     for (int i = 2 + threadIdx.x; i < d.nx + 2; i += RED_THREADS) {
       double dsx = M(CLV_CELLDX, i);
@@ -337,14 +337,11 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_min_k(const double *partia
 
 // field_summary: src/serial/field_summary.cpp:76-92.  The per-cell terms keep the
 // reference's expressions; the sum order is a fixed tree (thread stride within a
-// row, a contiguous range of rows per block, wave xor-shuffle, waves, blocks in index
-// order), so results are reproducible run to run and differ from the serial order
-// only by rounding.
+// row, rows by block, wave xor-shuffle, waves, blocks in index order), so results
+// are reproducible run to run and differ from the serial order only by rounding.
 __global__ __launch_bounds__(RED_THREADS) void field_summary_k(Dev d, double *partials) {
   double vol = 0.0, mass = 0.0, ie = 0.0, ke = 0.0, press = 0.0;
-  int kb, ke_;
-  block_rows(d.ny, kb, ke_);
-  for (int k = kb; k < ke_; ++k) {
+  for (int k = 2 + band_row(gridDim.x); k < d.ny + 2; k += gridDim.x) {
     for (int j = 2 + threadIdx.x; j < d.nx + 2; j += RED_THREADS) {
       double vsqrd = 0.0;
       for (int kv = k; kv <= k + 1; ++kv) {
