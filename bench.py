@@ -291,7 +291,15 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
     # on 1920^2 a step of 1e-7 or more moves some van Leer limiter / upwind switch
     # across its kink, and the difference quotient then carries an O(1e-5) error
     # that does not shrink with eps (at 10-40 steps, 1e-6 and 1e-7 agree to 1e-8)
-    rel = []
+    rel, per_dir = [], []
+    # the oracle's tangent-linear derivative and kink census of the same directions
+    # (tests/golden/adjoint_fd_kinks.json, tests/golden/make_adjoint_kinks.py)
+    tl = {}
+    kinks_file = REPO / "tests" / "golden" / "adjoint_fd_kinks.json"
+    if kinks_file.exists():
+        g = json.loads(kinks_file.read_text())
+        if g.get("fd_eps") == eps and g.get("workload") == {k: w[k] for k in g.get("workload", {})}:
+            tl = {r["k"]: r for r in g["directions"]}
     for k in range(directions):
         dr, de = clv.smooth_field(nx, ny, 300 + k), clv.smooth_field(nx, ny, 400 + k)
         an = float(np.sum(gr * dr) + np.sum(ge * de))
@@ -301,6 +309,12 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
         km = adj.forward()[3]
         fd = (kp - km) / (2 * eps)
         rel.append(abs(an - fd) / abs(fd))
+        rec = {"k": k, "adjoint": an, "fd": fd, "fd_rel_diff": rel[-1]}
+        if k in tl:
+            t = tl[k]["dke"]
+            rec.update({"tangent_linear": t, "adjoint_rel_diff_vs_tl": abs(an - t) / abs(t), "fd_rel_diff_vs_tl": abs(fd - t) / abs(t),
+                        "kinks": tl[k]["kinks"]})
+        per_dir.append(rec)
     adj.close()
     rel = np.array(rel)
     return {
@@ -319,6 +333,18 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
         "fd_max_rel_diff": float(rel.max()) if directions else None,
         "fd_median_rel_diff": float(np.median(rel)) if directions else None,
         "fd_within_1e-6": int(np.sum(rel <= 1e-6)),
+        "tl_max_rel_diff": max((r["adjoint_rel_diff_vs_tl"] for r in per_dir if "tangent_linear" in r), default=None),
+        "fd_kink_crossings": {
+            "note": "kinks: piecewise choices of the oracle's tangent-linear run whose deciding difference x has |x| < "
+                    "fd_eps |dx| -- taken differently by the +eps and -eps runs of the central difference (to first "
+                    "order); tangent_linear: the oracle's exact directional derivative on the branches taken "
+                    "(tests/golden/adjoint_fd_kinks.json)",
+            "directions_above_1e-6": [{"k": r["k"], "fd_rel_diff": r["fd_rel_diff"], "kinks": r.get("kinks"),
+                                       "adjoint_rel_diff_vs_tl": r.get("adjoint_rel_diff_vs_tl")}
+                                      for r in per_dir if r["fd_rel_diff"] > 1e-6],
+            "min_kinks_per_direction": min((r["kinks"] for r in per_dir if "kinks" in r), default=None),
+        },
+        "directions": per_dir,
     }
 
 
