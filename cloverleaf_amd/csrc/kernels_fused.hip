@@ -45,6 +45,9 @@ namespace {
 
 #define AT(fid, i, j) d.f[fid][(int64_t)(j) * d.pitch + (i)]
 #define M(mid, i) d.m[mid][i]
+// a 1D mesh value at a wave-uniform index, read through the scalar cache (the mesh
+// arrays do not change during a launch)
+#define MU(mid, i) ((const __attribute__((address_space(4))) double *)(d.m[mid]))[i]
 
 constexpr int WAVES = 4;       // waves (strips) per 256-thread block
 
@@ -138,19 +141,20 @@ __device__ __forceinline__ double pressure_of(double rho, double e) { return (1.
 // advection pass's).  The host picks the instantiation (MESH) per launch.
 template <bool MESH, class V>
 __device__ __forceinline__ double geo_volume(const V &d, int i, int j) {
-  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? M(CLV_CELLDX, i) * M(CLV_CELLDY, j) : 0.0;
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? M(CLV_CELLDX, i) * MU(CLV_CELLDY, j) : 0.0;
   else return AT(CLV_VOLUME, i, j);
 }
 // the same with celldx(i) already in registers (cdx_i; its value does not matter for
-// i >= nx + 4): one load of celldy(j) instead of two loads
+// i >= nx + 4): one load of celldy(j) instead of two loads.  Every caller's row j is
+// wave-uniform (lanes run along x), so celldy(j) is a scalar load.
 template <bool MESH, class V>
 __device__ __forceinline__ double geo_volume_x(const V &d, double cdx_i, int i, int j) {
-  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? cdx_i * M(CLV_CELLDY, j) : 0.0;
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? cdx_i * MU(CLV_CELLDY, j) : 0.0;
   else return AT(CLV_VOLUME, i, j);
 }
 template <bool MESH, class V>
 __device__ __forceinline__ double geo_xarea(const V &d, int i, int j) {
-  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? M(CLV_CELLDY, j) : 0.0;
+  if constexpr (MESH) return i < d.nx + 4 && j < d.ny + 4 ? MU(CLV_CELLDY, j) : 0.0;
   else return AT(CLV_XAREA, i, j);
 }
 // cdx_i: celldx(i) (used for i < nx + 4 only)
@@ -721,29 +725,30 @@ __global__ __launch_bounds__(256) void cm_x_k(Dev d, int nstrips, int seg, doubl
 // 1 <- 2, nx + 2 <- nx + 1; mass_flux_y columns 1 <- 3, nx + 2 <- nx.  Here the ghost
 // rows are read from the rows they mirror (kept in the registers) and the ghost
 // columns from lane shuffles.
-// History slots at mom row k: slot q holds row k - 3 + q (mass flux rows k-3..k+4,
-// density rows k-3..k+3).
-__device__ __forceinline__ int cm_y_rho_slot(int k, int ny) { return mirror_cell(k, ny) - k + 3; }
-__device__ __forceinline__ int cm_y_mf_slot(int x, int k, int ny, double &sgn) {
-  // mass flux row x (k or k + 1) through the mirror
+// The rows the mom part reads back are kept per lane in LDS rings (registers would
+// hold them as shift registers; the walk is issue-bound at 2-3 waves per SIMD, and the
+// rings bring it to 4): mass flux rows k-3 .. k+4 at slot row & 7, advected density
+// rows k-3 .. k+3 at kRingMf + row % 7, advec_mom's post_vol rows k .. k+4 at
+// kRingMf + kRingRho + row % 5.
+constexpr int kRingMf = 8, kRingRho = 7, kRingPost = 5, kRing = kRingMf + kRingRho + kRingPost;
+// mass flux row x (k or k + 1) through the mirror: the row it is read from
+__device__ __forceinline__ int cm_y_mf_src(int x, int ny, double &sgn) {
   int src = x;
   if (x < 2) src = 4 - x;
   else if (x > ny + 2) src = 2 * ny + 4 - x;
   sgn = src == x ? 1.0 : -1.0;
-  return src - k + 3;
+  return src;
 }
-// the history is held in named registers (an array indexed by a run-time slot would
-// live in scratch memory): density rows in r0..r6, mass flux rows in m0..m7
-struct CmHist {
-  double m0, m1, m2, m3, m4, m5, m6, m7, r0, r1, r2, r3, r4, r5, r6;
-  __device__ void push(double mf, double rho) {
-    m0 = m1; m1 = m2; m2 = m3; m3 = m4; m4 = m5; m5 = m6; m6 = m7; m7 = mf;
-    r0 = r1; r1 = r2; r2 = r3; r3 = r4; r4 = r5; r5 = r6; r6 = rho;
-  }
-  __device__ double mf(int q) const {
-    return q == 0 ? m0 : q == 1 ? m1 : q == 2 ? m2 : q == 3 ? m3 : q == 4 ? m4 : q == 5 ? m5 : q == 6 ? m6 : m7;
-  }
-  __device__ double rho(int q) const { return q == 0 ? r0 : q == 2 ? r2 : q == 4 ? r4 : q == 6 ? r6 : r3; }
+struct CmRing {
+  double *lane_base;  // this lane's slot 0; slot q at lane_base[64 q]
+  __device__ void put(int q, double v) const { lane_base[64 * q] = v; }
+  __device__ double get(int q) const { return lane_base[64 * q]; }
+  __device__ void put_mf(int row, double v) const { put(row & 7, v); }
+  __device__ double mf(int row) const { return get(row & 7); }
+  __device__ void put_rho(int row, double v) const { put(kRingMf + row % kRingRho, v); }
+  __device__ double rho(int row) const { return get(kRingMf + row % kRingRho); }
+  __device__ void put_post(int row, double v) const { put(kRingMf + kRingRho + row % kRingPost, v); }
+  __device__ double post(int row) const { return get(kRingMf + kRingRho + row % kRingPost); }
 };
 template <int SWEEP, bool MESH, bool LOCKSTEP = true, class V, class O>
 __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
@@ -761,6 +766,9 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
   const int Fs = j0 - 2 > 2 ? j0 - 2 : 2, Fe = min(j1 + 1, ny + 2);  // faces the cell part walks
   const double cdx_ic = MESH ? M(CLV_CELLDX, ic) : 0.0;                 // ic <= nx + 3
+  __shared__ double ring_lds[WAVES * kRing * 64];
+  const CmRing ring{ring_lds + (threadIdx.x >> 6) * (kRing * 64) + lane};
+  for (int q = 0; q < kRing; ++q) ring.put(q, 0.0);
   // pre_vol of cell row `row` (src/serial/advec_cell.cpp:139 / :149) from its bottom and
   // top vol_flux_y, and advec_mom's post_vol (src/serial/advec_mom.cpp:56 / :74)
   auto pre_of = [&](int row, double vfy0, double vfy1, double vol, double vfx0, double vfx1) {
@@ -793,10 +801,10 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
         vfx0 = AT(CLV_VOL_FLUX_X, ic, F);
         vfx1 = AT(CLV_VOL_FLUX_X, ic + 1, F);
       }
-      vd = M(CLV_VERTEXDY, F);
+      vd = MU(CLV_VERTEXDY, F);
       rho_p1 = AT(CLV_DENSITY1, ic, rp);
       e_p1 = AT(CLV_ENERGY1, ic, rp);
-      vd_p1 = M(CLV_VERTEXDY, rp);
+      vd_p1 = MU(CLV_VERTEXDY, rp);
     }
     return CellLd{vol, vfy1, vfx0, vfx1, vd, rho_p1, e_p1, vd_p1};
   };
@@ -805,11 +813,8 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   double e_m2 = AT(CLV_ENERGY1, ic, Fs - 2), e_m1 = AT(CLV_ENERGY1, ic, Fs - 1), e_0 = AT(CLV_ENERGY1, ic, Fs);
   double vfy_c = AT(CLV_VOL_FLUX_Y, ic, Fs - 1), post_unused;  // vol_flux_y of the face below the cell in hand
   double pre_m1 = pre_load(Fs - 1, vfy_c, vfy_c, post_unused);
-  // post_vol of the last five cell rows: row F - 4 + q in p[q]
-  double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0;
-  double vd_m1 = M(CLV_VERTEXDY, Fs - 1);
+  double vd_m1 = MU(CLV_VERTEXDY, Fs - 1);
   double mf_prev = 0.0, ef_prev = 0.0, vf_prev = 0.0;
-  CmHist h{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   // mom part state: node flux and masses of the previous row, the previous face's
   // momentum fluxes, velocities of rows k-2 .. k
   double nf_prev = 0.0, mo_prev = 0.0, mw_prev = 0.0, pre_prev = 0.0, post_prev = 0.0, mu_prev = 0.0, mv_prev = 0.0;
@@ -818,11 +823,11 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   // post_vol of row k from the history where the cell part passed that row, else loaded
   auto mom_row = [&](int k, double &nf, double &mo, double &mw) {
     double s0, s1;
-    const int q0 = cm_y_mf_slot(k, k, ny, s0), q1 = cm_y_mf_slot(k + 1, k, ny, s1);
-    const double m0 = s0 * h.mf(q0), m1 = s1 * h.mf(q1);
+    const int x0 = cm_y_mf_src(k, ny, s0), x1 = cm_y_mf_src(k + 1, ny, s1);
+    const double m0 = s0 * ring.mf(x0), m1 = s1 * ring.mf(x1);
     nf = 0.25 * (up1(m0) + m0 + up1(m1) + m1);
-    const double post = k >= Fs && k <= Fe ? p0 : post_of(k);
-    mo = h.rho(cm_y_rho_slot(k, ny)) * post;
+    const double post = k >= Fs && k <= Fe ? ring.post(k) : post_of(k);
+    mo = ring.rho(mirror_cell(k, ny)) * post;
     mw = up1(mo);
   };
   CellLd cur = cell_ld(Fs);
@@ -873,12 +878,9 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       vd_m1 = vd_face;
     }
     // history: mass flux row F and density row F - 1, ghost columns mirrored
-    h.push(__shfl(mf_h, lane_f), __shfl(rho_h, lane_c));
-    p0 = p1;
-    p1 = p2;
-    p2 = p3;
-    p3 = p4;
-    p4 = post_h;
+    ring.put_mf(F, __shfl(mf_h, lane_f));
+    ring.put_rho(F - 1, __shfl(rho_h, lane_c));
+    ring.put_post(F, post_h);
     // mom part at node row k = F - 4
     const int k = F - 4;
     if (k == j0 - 2) {
@@ -896,14 +898,14 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       if (k >= j0) {
         // momentum flux of face f = k - 1
         const int f = k - 1;
-        const double wd = M(CLV_CELLDY, f);
+        const double wd = MU(CLV_CELLDY, f);
         double mu, mv;
         if (nf_prev < 0.0) {  // upwind f+2, donor f+1, downwind f, dif = donor (:186-191)
-          const double wdif = M(CLV_CELLDY, f + 1);
+          const double wdif = MU(CLV_CELLDY, f + 1);
           mu = mom_flux_of(nf_prev, nm_pre, wd, wdif, u_p1, u_0, u_m1);
           mv = mom_flux_of(nf_prev, nm_pre, wd, wdif, v_p1, v_0, v_m1);
         } else {  // upwind f-1, donor f, downwind f+1, dif = upwind (:192-196)
-          const double wdif = M(CLV_CELLDY, f - 1);
+          const double wdif = MU(CLV_CELLDY, f - 1);
           mu = mom_flux_of(nf_prev, pre_prev, wd, wdif, u_m2, u_m1, u_0);
           mv = mom_flux_of(nf_prev, pre_prev, wd, wdif, v_m2, v_m1, v_0);
         }
@@ -939,7 +941,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
 }
 
 template <int SWEEP, bool MESH>
-__global__ __launch_bounds__(256) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
+__global__ __launch_bounds__(256, 4) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
