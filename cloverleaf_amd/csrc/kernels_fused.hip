@@ -206,7 +206,7 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
     const double p_w = up1(p_c), p_e = dn1(p_c);
     const double u10 = dn1(u_r), u11 = dn1(u_r1), v10 = dn1(v_r), v11 = dn1(v_r1), xa1 = dn1(xa);
     const double u00 = u_r, u01 = u_r1, v00 = v_r, v01 = v_r1;
-    const double cdy = M(CLV_CELLDY, r), cdy2 = M(CLV_CELLDY, r + 2);
+    const double cdy = MU(CLV_CELLDY, r), cdy2 = MU(CLV_CELLDY, r + 2);
     // viscosity, src/serial/viscosity.cpp:39-65.  The reference sets q = 0 whenever
     // div >= 0, whatever the limiter (:50), so the limiter and its seven divisions are
     // formed only where !(div >= 0): in the at-rest part of the box (div exactly 0)
@@ -834,12 +834,10 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   double u_ahead = 0.0, v_ahead = 0.0;  // velocities of row k + 1 for the next mom step
   for (int F = Fs; F <= j1 + 4; ++F) {
     const CellLd nxt = cell_ld(F + 1);
-    // the next mom step (k + 1 = F - 3) reads velocity row F - 2
-    double u_n = 0.0, v_n = 0.0;
-    if (F - 3 >= j0 - 1 && F - 3 <= j1) {
-      u_n = AT(CLV_XVEL1, ic, F - 2);
-      v_n = AT(CLV_YVEL1, ic, F - 2);
-    }
+    // the next mom step (k + 1 = F - 3) reads velocity row F - 2; it runs for
+    // F - 2 in [j0, j1 + 1], and outside that a row in it is loaded and not used
+    const int rv = min(max(F - 2, j0), j1 + 1);
+    const double u_n = AT(CLV_XVEL1, ic, rv), v_n = AT(CLV_YVEL1, ic, rv);
     double mf_h = 0.0, rho_h = 0.0, post_h = 0.0;
     if (F <= Fe) {
       const double vf = vfy_c;
