@@ -124,8 +124,10 @@ __global__ __launch_bounds__(256) void update_halo_k(Dev d, HaloList list, int d
   if (t < halo_items(d, r, depth)) halo_item(d, r, fid, depth, t);
 }
 
-// Batched: blockIdx.x = problem, blockIdx.y = field slot; the block strides over
-// the problem's ghost cells.  Problems not running this step are skipped.
+// Batched: blockIdx.x = problem, blockIdx.y = field slot, blockIdx.z = one of
+// kBatchHaloSplit blocks that share the problem's ghost columns and rows of that
+// field (4100 on a 2048^2 problem).  Problems not running this step are skipped.
+constexpr int kBatchHaloSplit = 8;
 __global__ __launch_bounds__(256) void batch_halo_k(const Dev *tab, const BatchState *ps, HaloList list, int depth) {
   const int p = blockIdx.x;
   if (!ps[p].run) return;
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void batch_halo_k(const Dev *tab, const BatchS
   const int fid = list.ids[blockIdx.y];
   const HaloRule r = kRules[fid];
   const int n = halo_items(d, r, depth);
-  for (int t = threadIdx.x; t < n; t += 256) halo_item(d, r, fid, depth, t);
+  for (int t = blockIdx.z * 256 + threadIdx.x; t < n; t += 256 * kBatchHaloSplit) halo_item(d, r, fid, depth, t);
 }
 
 // Small grids (nx or ny < 4): a mirror source can itself be a ghost written by an
@@ -667,7 +669,7 @@ hipError_t batch_launch_halo(hipStream_t s, int nprob, const Dev *tab, const Bat
   HaloList list{};
   for (int k = 0; k < nids && k < CLV_NUM_HALO_FIELDS; ++k) list.ids[list.n++] = ids[k];
   if (list.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_halo_k, dim3(nprob, list.n, 1), dim3(256, 1, 1), 0, s, tab, ps, list, depth);
+  hipLaunchKernelGGL(batch_halo_k, dim3(nprob, list.n, kBatchHaloSplit), dim3(256, 1, 1), 0, s, tab, ps, list, depth);
   return hipGetLastError();
 }
 
