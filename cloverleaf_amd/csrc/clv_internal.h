@@ -89,6 +89,8 @@ clv_status k_pack_fields(clv_chunk *c, int face, const int fields[CLV_NUM_HALO_F
                          double *buf, int unpack);
 clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, double *dt_min_val);
 clv_status k_field_summary(clv_chunk *c, double out[5]);
+// ideal_gas(predict = false) then field_summary, one pass over the cells
+clv_status k_field_summary_gas(clv_chunk *c, double out[5]);
 
 // one launch per entry point (kernels_entry.hip); the advection kernels write the
 // updated fields to the given arrays

@@ -176,8 +176,7 @@ clv_status clv_deck_parse(const char *text, clv_config *g) {
 
 clv_status clv_hydro_summary(clv_chunk *c, double out[5]) {
   if (!c || !out) return set_error(CLV_ERR_ARG, "clv_hydro_summary: null argument");
-  TRY(k_ideal_gas(c, 0));
-  return k_field_summary(c, out);
+  return k_field_summary_gas(c, out);
 }
 
 // driver/start.cpp:69-127 and driver/definitions.cpp:53-55

@@ -38,6 +38,7 @@
 // (rho1, e1 in advec_cell; u1, v1 in advec_mom) writes into a work array, and
 // the host swaps the two pointers afterwards.
 #include "clv_internal.h"
+#include "eos.h"
 #include "lanes.h"
 
 namespace clv {
@@ -114,17 +115,6 @@ struct BlockRows {
 // Lanes: column x0 + L, x0 = 1 + kOwnTimestep * strip; lanes 1..62 own cells (p of the
 // neighbouring columns comes from lanes 0 and 63).  Its one stored array goes straight
 // from the lanes (staged through LDS it ran no faster).
-__device__ __forceinline__ void ideal_gas_cell(double rho, double e, double &p, double &c) {
-  // src/serial/ideal_gas.cpp:40-45
-  double v = 1.0 / rho;
-  p = (1.4 - 1.0) * rho * e;
-  double pressurebyenergy = (1.4 - 1.0) * rho;
-  double pressurebyvolume = -rho * p;
-  double sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
-  c = sqrt(sound_speed_squared);
-}
-
-__device__ __forceinline__ double pressure_of(double rho, double e) { return (1.4 - 1.0) * rho * e; }
 
 // ---- cell volumes and face areas --------------------------------------------------------
 // initialise_chunk (src/serial/initialise_chunk.cpp:78-84) sets volume = dx * dy,

@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "clv_internal.h"
+#include "eos.h"
 #include "geom.h"
 
 namespace clv {
@@ -375,6 +376,51 @@ __global__ __launch_bounds__(RED_THREADS) void field_summary_k(Dev d, double *pa
   }
 }
 
+// ideal_gas (the predict = false call of src/serial/field_summary.cpp:52) and the sums
+// of field_summary_k in one pass: each cell's pressure and sound speed are formed
+// with ideal_gas's expressions and written, and the pressure term of the sum uses
+// the value just written, so arrays and sums equal ideal_gas_k followed by
+// field_summary_k bit for bit; density and energy are read once instead of twice and
+// pressure is not read back.
+__global__ __launch_bounds__(RED_THREADS) void field_summary_gas_k(Dev d, double *partials) {
+  double vol = 0.0, mass = 0.0, ie = 0.0, ke = 0.0, press = 0.0;
+  for (int k = 2 + band_row(gridDim.x); k < d.ny + 2; k += gridDim.x) {
+    for (int j = 2 + threadIdx.x; j < d.nx + 2; j += RED_THREADS) {
+      double vsqrd = 0.0;
+      for (int kv = k; kv <= k + 1; ++kv) {
+        for (int jv = j; jv <= j + 1; ++jv) {
+          const double u = AT(CLV_XVEL0, jv, kv), v = AT(CLV_YVEL0, jv, kv);
+          vsqrd += 0.25 * (u * u + v * v);
+        }
+      }
+      const double rho = AT(CLV_DENSITY0, j, k), e = AT(CLV_ENERGY0, j, k);
+      double p, c;
+      ideal_gas_cell(rho, e, p, c);
+      AT(CLV_PRESSURE, j, k) = p;
+      AT(CLV_SOUNDSPEED, j, k) = c;
+      double cell_vol = geom_volume(d, j, k);
+      double cell_mass = cell_vol * rho;
+      vol += cell_vol;
+      mass += cell_mass;
+      ie += cell_mass * e;
+      ke += cell_mass * 0.5 * vsqrd;
+      press += cell_vol * p;
+    }
+  }
+  __shared__ double lds[5][RED_THREADS / 64];
+  double s[5] = {vol, mass, ie, ke, press};
+  for (int q = 0; q < 5; ++q) {
+    double v = wave_sum(s[q]);
+    if ((threadIdx.x & 63) == 0) lds[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double r = lds[threadIdx.x][0];
+    for (int w = 1; w < RED_THREADS / 64; ++w) r += lds[threadIdx.x][w];
+    partials[(size_t)threadIdx.x * RED_MAX_BLOCKS + blockIdx.x] = r;
+  }
+}
+
 __global__ __launch_bounds__(RED_THREADS) void reduce_sum5_k(const double *partials, int n, double *out) {
   __shared__ double lds[5][RED_THREADS / 64];
   for (int q = 0; q < 5; ++q) {
@@ -687,12 +733,19 @@ clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_
   return CLV_OK;
 }
 
-clv_status k_field_summary(clv_chunk *c, double out[5]) {
+// with_gas: ideal_gas first (src/serial/field_summary.cpp:52), in the same pass as the
+// sums unless the serial summation order is asked for
+static clv_status summary(clv_chunk *c, double out[5], bool with_gas) {
   const int nb = red_blocks(c->d.ny);
   if (c->summary_serial) {
+    if (with_gas) {
+      clv_status st = k_ideal_gas(c, 0);
+      if (st != CLV_OK) return st;
+    }
     LAUNCH_G("field_summary_serial", field_summary_serial_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->result_dev);
   } else {
-    LAUNCH_G("field_summary", field_summary_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->partials);
+    if (with_gas) LAUNCH_G("field_summary_gas", field_summary_gas_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->partials);
+    else LAUNCH_G("field_summary", field_summary_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, c->partials);
     LAUNCH_G("field_summary_final", reduce_sum5_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), (const double *)c->partials, nb,
              c->result_dev);
   }
@@ -703,6 +756,9 @@ clv_status k_field_summary(clv_chunk *c, double out[5]) {
   for (int q = 0; q < 5; ++q) out[q] = c->result_host[q];
   return CLV_OK;
 }
+
+clv_status k_field_summary(clv_chunk *c, double out[5]) { return summary(c, out, false); }
+clv_status k_field_summary_gas(clv_chunk *c, double out[5]) { return summary(c, out, true); }
 
 }  // namespace clv
 

@@ -325,7 +325,10 @@ clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st
  * st->complete when driver/hydro.cpp:108 says so.  Returns CLV_ERR_SMALL_DT
  * when dt < dtmin. */
 clv_status clv_hydro_step(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan);
-/* field_summary with its ideal_gas (src/serial/field_summary.cpp:51-99). */
+/* field_summary with its ideal_gas (src/serial/field_summary.cpp:51-99).  With the
+ * tree summation order the two run as one pass over the cells: pressure and
+ * soundspeed are written as ideal_gas writes them, and the sums use the pressure
+ * just written (the same doubles as clv_ideal_gas then clv_field_summary). */
 clv_status clv_hydro_summary(clv_chunk *c, double out[5]);
 /* The phases of one step of driver/hydro.cpp:59-85 on the reference plan, for a host
  * loop that acts between them as the reference's does (its --dump writes the fields

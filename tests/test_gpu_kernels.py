@@ -127,6 +127,28 @@ def test_field_summary_close_to_oracle(gpu, nx, ny):
         assert abs(a - b) <= 1e-13 * abs(b), (a, b)
 
 
+@pytest.mark.parametrize("nx,ny", SIZES)
+def test_summary_one_pass_equals_ideal_gas_then_sums(gpu, nx, ny):
+    """clv_hydro_summary (ideal_gas and the sums in one pass) against clv_ideal_gas
+    followed by clv_field_summary: the same sums, pressure and soundspeed bit for
+    bit, and the oracle's ideal_gas arrays."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=5 * nx + ny)
+    f["pressure"][:] = -1.0
+    f["soundspeed"][:] = -1.0
+    o = _oracle(nx, ny, f, m)
+    o.L.orc_ideal_gas(C.byref(o.c), 0)
+    with clv.Chunk(nx, ny) as a, clv.Chunk(nx, ny) as b:
+        _load(a, f, m)
+        _load(b, f, m)
+        one = a.summary()
+        b.ideal_gas(False)
+        two = b.field_summary()
+        assert one == two
+        for name in ("pressure", "soundspeed"):
+            assert np.array_equal(a.download(name), b.download(name), equal_nan=True), name
+            assert np.array_equal(a.download(name), o.f(name), equal_nan=True), name
+
+
 def test_setup_kernels_match_oracle(gpu):
     """build_field + initialise_chunk + generate_chunk (rectangle, circle, point states)."""
     d = T.load_deck("small_geom_mix")
