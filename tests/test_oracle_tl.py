@@ -10,10 +10,14 @@ Pinned by:
   final kinetic energy against central finite differences of oracle runs;
 - linearity: the derivative of a sum of directions is the sum of derivatives.
 """
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 import clover_testlib as T
+import cloverleaf_amd as clv
 
 SIZES = [(9, 7), (24, 19)]
 DT = 0.004
@@ -183,3 +187,16 @@ def test_linearity_in_the_direction():
     a, b = dke(*dirs[0]), dke(*dirs[1])
     ab = dke(dirs[0][0] + dirs[1][0], dirs[0][1] + dirs[1][1])
     assert abs(ab - (a + b)) <= 1e-12 * (abs(a) + abs(b))
+
+
+def test_fd_kink_fixture_matches_the_adjoint_workload():
+    """tests/golden/adjoint_fd_kinks.json (make_adjoint_kinks.py) is the oracle's
+    tangent-linear derivative and kink census for the 32 directions bench.py's
+    adjoint leg checks: same workload, same seeds, the leg's central-difference step."""
+    g = json.loads((Path(__file__).parent / "golden" / "adjoint_fd_kinks.json").read_text())
+    assert g["workload"] == clv.ADJOINT_WORKLOAD
+    assert g["fd_eps"] == 1e-8
+    assert [r["k"] for r in g["directions"]] == list(range(32))
+    for r in g["directions"]:
+        assert (r["seed_density"], r["seed_energy"]) == (300 + r["k"], 400 + r["k"])
+        assert np.isfinite(r["dke"]) and r["kinks"] > 0
