@@ -48,6 +48,12 @@ namespace {
 #define M(mid, i) d.m[mid][i]
 // a 1D mesh value at a wave-uniform index, read through the scalar cache (the mesh
 // arrays do not change during a launch)
+// a store that bypasses the caches' normal retention (the value is read next by
+// another kernel, from HBM): on the block-staged whole-line row stores of lagrange_k
+// and cm_x_k it takes 1-2 % off those kernels at 15360^2; on the y walk's direct
+// stores it costs 1 %, and non-temporal loads cost lagrange_k 5 % (its strips
+// re-read each other's edge columns from L2)
+#define ST_STREAM(lhs, v) __builtin_nontemporal_store((v), &(lhs))
 #define MU(mid, i) ((const __attribute__((address_space(4))) double *)(d.m[mid]))[i]
 
 constexpr int WAVES = 4;       // waves (strips) per 256-thread block
@@ -451,17 +457,17 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
       if (r > j0) {
         const int c = r - 1;
         if (out.cell(t)) {
-          AT(CLV_ENERGY1, xs, c) = stage[r & 1][0][t];
-          AT(CLV_DENSITY1, xs, c) = stage[r & 1][1][t];
+          ST_STREAM(AT(CLV_ENERGY1, xs, c), stage[r & 1][0][t]);
+          ST_STREAM(AT(CLV_DENSITY1, xs, c), stage[r & 1][1][t]);
         }
-        if (out.node(t)) AT(CLV_VOL_FLUX_X, xs, c) = stage[r & 1][2][t];
+        if (out.node(t)) ST_STREAM(AT(CLV_VOL_FLUX_X, xs, c), stage[r & 1][2][t]);
       }
       if (r < jn1) {
         if (out.node(t)) {
-          AT(CLV_XVEL1, xs, r) = stage[r & 1][3][t];
-          AT(CLV_YVEL1, xs, r) = stage[r & 1][4][t];
+          ST_STREAM(AT(CLV_XVEL1, xs, r), stage[r & 1][3][t]);
+          ST_STREAM(AT(CLV_YVEL1, xs, r), stage[r & 1][4][t]);
         }
-        if (out.cell(t)) AT(CLV_VOL_FLUX_Y, xs, r) = stage[r & 1][5][t];
+        if (out.cell(t)) ST_STREAM(AT(CLV_VOL_FLUX_Y, xs, r), stage[r & 1][5][t]);
       }
     }
     pp_prev = pp;
@@ -675,14 +681,14 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
       const int64_t o = (int64_t)r * d.pitch + xs;
       if (r <= ny + 1) {
         if (out.cell(t)) {
-          rho_out[o] = stage[r & 1][0][t];
-          e_out[o] = stage[r & 1][1][t];
+          ST_STREAM(rho_out[o], stage[r & 1][0][t]);
+          ST_STREAM(e_out[o], stage[r & 1][1][t]);
         }
-        if (out.node(t)) AT(CLV_MASS_FLUX_X, xs, r) = stage[r & 1][2][t];
+        if (out.node(t)) ST_STREAM(AT(CLV_MASS_FLUX_X, xs, r), stage[r & 1][2][t]);
       }
       if (out.node(t)) {
-        u_out[o] = stage[r & 1][3][t];
-        v_out[o] = stage[r & 1][4][t];
+        ST_STREAM(u_out[o], stage[r & 1][3][t]);
+        ST_STREAM(v_out[o], stage[r & 1][4][t]);
       }
     }
     mp_prev = mp;
