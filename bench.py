@@ -318,6 +318,13 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
         per_dir.append(rec)
     adj.close()
     rel = np.array(rel)
+    # the oracle's central differences of the same directions over a range of eps, with
+    # ke summed exactly (tests/golden/adjoint_fd_scan.json, make_adjoint_fd_scan.py)
+    scan = {}
+    scan_file = REPO / "tests" / "golden" / "adjoint_fd_scan.json"
+    if scan_file.exists():
+        for r in json.loads(scan_file.read_text())["records"]:
+            scan.setdefault(r["k"], {})[f"{r['eps']:g}"] = r["rel_diff_exact_sum"]
     return {
         "workload": f"two-state problem (InputDecks/clover_bm_short.in) on {nx}x{ny}, initial density x (1 + "
                     f"{w['amplitude']} s), s = seeded smooth modes; {steps} steps at fixed dt {dt} (0.004 diverges at "
@@ -341,8 +348,13 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
                     "order); tangent_linear: the oracle's exact directional derivative on the branches taken "
                     "(tests/golden/adjoint_fd_kinks.json)",
             "directions_above_1e-6": [{"k": r["k"], "fd_rel_diff": r["fd_rel_diff"], "kinks": r.get("kinks"),
-                                       "adjoint_rel_diff_vs_tl": r.get("adjoint_rel_diff_vs_tl")}
+                                       "adjoint_rel_diff_vs_tl": r.get("adjoint_rel_diff_vs_tl"),
+                                       "oracle_fd_rel_diff_by_eps": scan.get(r["k"])}
                                       for r in per_dir if r["fd_rel_diff"] > 1e-6],
+            "oracle_fd_scan_note": "oracle_fd_rel_diff_by_eps: the oracle's central difference of the same direction "
+                                   "at each eps, ke summed exactly, against the tangent-linear derivative "
+                                   "(tests/golden/adjoint_fd_scan.json): an error that does not shrink with eps and "
+                                   "changes from one eps to the next comes from branch flips, not rounding",
             "min_kinks_per_direction": min((r["kinks"] for r in per_dir if "kinks" in r), default=None),
         },
         "directions": per_dir,

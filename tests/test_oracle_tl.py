@@ -200,3 +200,23 @@ def test_fd_kink_fixture_matches_the_adjoint_workload():
     for r in g["directions"]:
         assert (r["seed_density"], r["seed_energy"]) == (300 + r["k"], 400 + r["k"])
         assert np.isfinite(r["dke"]) and r["kinks"] > 0
+
+
+def test_fd_scan_fixture_accounts_for_the_outliers():
+    """tests/golden/adjoint_fd_scan.json (make_adjoint_fd_scan.py): the oracle's central
+    differences of the directions whose eps = 1e-8 difference misses 1e-6, with ke
+    summed exactly, over eps 1e-9 .. 1e-6.  Their error does not shrink with eps from
+    1e-6 to 1e-8 (switch crossings, not rounding) and is below 1e-6 at eps = 1e-9."""
+    g = json.loads((Path(__file__).parent / "golden" / "adjoint_fd_scan.json").read_text())
+    assert g["workload"] == clv.ADJOINT_WORKLOAD
+    kinks = json.loads((Path(__file__).parent / "golden" / "adjoint_fd_kinks.json").read_text())
+    tl = {r["k"]: r["dke"] for r in kinks["directions"]}
+    by = {}
+    for r in g["records"]:
+        assert r["tangent_linear"] == tl[r["k"]]
+        assert r["rel_diff_exact_sum"] == pytest.approx(abs(r["fd_exact_sum"] - tl[r["k"]]) / abs(tl[r["k"]]))
+        by.setdefault(r["k"], {})[r["eps"]] = r["rel_diff_exact_sum"]
+    for k in (25, 28):
+        e = by[k]
+        assert e[1e-9] < 1e-6
+        assert e[1e-8] > 1e-6 and max(e[1e-7], e[1e-6]) > 1e-6
