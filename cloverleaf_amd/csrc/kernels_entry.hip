@@ -32,6 +32,9 @@ namespace {
 
 #define AT(fid, i, j) d.f[fid][(int64_t)(j) * d.pitch + (i)]
 #define M(mid, i) d.m[mid][i]
+// every output element is stored once and read next by another launch, from HBM:
+// streaming (non-temporal) stores, 2.7 % off the drop-in's step at 15360^2
+#define ST_STREAM(lhs, v) __builtin_nontemporal_store((v), &(lhs))
 
 constexpr int BX = 64, BY = 4;
 
@@ -71,17 +74,17 @@ __global__ __launch_bounds__(256) void pdv_predict_chain_k(Dev d, double dt) {
   double pressurebyenergy = (1.4 - 1.0) * rho1;
   double pressurebyvolume = -rho1 * p;
   double sound_speed_squared = v * v * (p * pressurebyenergy - pressurebyvolume);
-  AT(CLV_PRESSURE, i, j) = p;
-  AT(CLV_SOUNDSPEED, i, j) = sqrt(sound_speed_squared);
+  ST_STREAM(AT(CLV_PRESSURE, i, j), p);
+  ST_STREAM(AT(CLV_SOUNDSPEED, i, j), sqrt(sound_speed_squared));
   // update_halo(pressure, 1)
   const int gi = i == 2 ? 1 : (i == d.nx + 1 ? d.nx + 2 : 0);
   const int gj = j == 2 ? 1 : (j == d.ny + 1 ? d.ny + 2 : 0);
-  if (gi) AT(CLV_PRESSURE, gi, j) = p;
-  if (gj) AT(CLV_PRESSURE, i, gj) = p;
-  if (gi && gj) AT(CLV_PRESSURE, gi, gj) = p;
+  if (gi) ST_STREAM(AT(CLV_PRESSURE, gi, j), p);
+  if (gj) ST_STREAM(AT(CLV_PRESSURE, i, gj), p);
+  if (gi && gj) ST_STREAM(AT(CLV_PRESSURE, gi, gj), p);
   // revert
-  AT(CLV_DENSITY1, i, j) = rho0;
-  AT(CLV_ENERGY1, i, j) = e0;
+  ST_STREAM(AT(CLV_DENSITY1, i, j), rho0);
+  ST_STREAM(AT(CLV_ENERGY1, i, j), e0);
 }
 
 // ---- advec_cell: one launch per call (src/serial/advec_cell.cpp:37-219) ------------------------
@@ -165,15 +168,15 @@ __global__ __launch_bounds__(256) void advec_cell_x_entry_k(Dev d, double *__res
   const double mf_e = dn1(mf), ef_e = dn1(ef);
   const bool own = lane >= 2 && lane <= 1 + kOwnEX;
   if (!own) return;
-  if (i <= nx + 3) AT(CLV_MASS_FLUX_X, i, j) = mf;
+  if (i <= nx + 3) ST_STREAM(AT(CLV_MASS_FLUX_X, i, j), mf);
   if (i > nx + 1) return;
   double pre_mass_s = rho * pre;
   double post_mass_s = pre_mass_s + mf - mf_e;
   double post_ener_s = (e * pre_mass_s + ef - ef_e) / post_mass_s;
   double advec_vol_s = pre + vfx - vfx_e;
   const int64_t o = (int64_t)j * d.pitch + i;
-  rho_out[o] = post_mass_s / advec_vol_s;
-  e_out[o] = post_ener_s;
+  ST_STREAM(rho_out[o], post_mass_s / advec_vol_s);
+  ST_STREAM(e_out[o], post_ener_s);
 }
 
 // y sweep.  One lane per cell column i in [2, nx + 1], marching up the faces
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(256) void advec_cell_y_entry_k(Dev d, int seg, doub
   double vf0 = AT(CLV_VOL_FLUX_Y, i, c0), mf0, ef0;
   flux(c0, vf0, mf0, ef0);
   for (int c = c0; c < c1; ++c) {
-    AT(CLV_MASS_FLUX_Y, i, c) = mf0;
+    ST_STREAM(AT(CLV_MASS_FLUX_Y, i, c), mf0);
     if (c + 1 > ny + 3 || (c > ny + 1 && c + 1 >= c1)) break;
     // the window at face c + 1
     const double rho_c = r_0, e_c = e_0, pre_c = p_0;
@@ -220,8 +223,8 @@ __global__ __launch_bounds__(256) void advec_cell_y_entry_k(Dev d, int seg, doub
       double post_ener_s = (e_c * pre_mass_s + ef0 - ef1) / post_mass_s;
       double advec_vol_s = pre_c + vf0 - vf1;
       const int64_t o = (int64_t)c * d.pitch + i;
-      rho_out[o] = post_mass_s / advec_vol_s;
-      e_out[o] = post_ener_s;
+      ST_STREAM(rho_out[o], post_mass_s / advec_vol_s);
+      ST_STREAM(e_out[o], post_ener_s);
     }
     mf0 = mf1;
     ef0 = ef1;
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(256) void advec_mom_x_entry_k(Dev d, int vel_id, do
   }
   const double mom_flux_w = up1(mom_flux);
   if (lane >= 2 && lane <= 1 + kOwnEX && i <= nx + 2)
-    vel_out[(int64_t)j * d.pitch + i] = (vel * node_mass_pre + mom_flux_w - mom_flux) / node_mass_post;
+    ST_STREAM(vel_out[(int64_t)j * d.pitch + i], (vel * node_mass_pre + mom_flux_w - mom_flux) / node_mass_post);
 }
 
 // y sweep (mom_sweep 2 or 4).  Lane L holds node column i = 1 + kOwnEY * s + L (nodes
