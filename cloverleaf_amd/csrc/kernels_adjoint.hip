@@ -165,6 +165,9 @@ __device__ __forceinline__ Dn<N> sqrt(Dn<N> a) {
 #define PA(fid, i, j) p.f[fid][(int64_t)(j) * p.pitch + (i)]
 #define AA(fid, i, j) a.f[fid][(int64_t)(j) * a.pitch + (i)]
 #define PM(mid, i) p.m[mid][i]
+// adjoint and primal work values are stored once and read next by another launch:
+// streaming (non-temporal) stores, 1.6 % off the backward pass at 1920^2
+#define ST_STREAM(lhs, v) __builtin_nontemporal_store((v), &(lhs))
 
 __device__ __forceinline__ bool in(int i, int lo, int hi) { return i >= lo && i <= hi; }
 
@@ -434,7 +437,7 @@ __global__ void adj_ideal_gas_k(Dev p, Dev a, int rho_f, int e_f) {
   const double pb = AA(CLV_PRESSURE, i, j);
   AA(rho_f, i, j) += (1.4 - 1.0) * PA(e_f, i, j) * pb;
   AA(e_f, i, j) += (1.4 - 1.0) * PA(rho_f, i, j) * pb;
-  AA(CLV_PRESSURE, i, j) = 0.0;
+  ST_STREAM(AA(CLV_PRESSURE, i, j), 0.0);
 }
 
 // ---- update_halo: ghost = sign * source, transposed ----------------------------------------
@@ -503,7 +506,7 @@ __global__ void adj_visc_cell_k(Dev p, Dev a) {
   const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x, j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > p.nx + 1 || j > p.ny + 1) return;
   const double qb = AA(CLV_VISCOSITY, i, j);
-  AA(CLV_VISCOSITY, i, j) = 0.0;
+  ST_STREAM(AA(CLV_VISCOSITY, i, j), 0.0);
   double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (qb != 0.0) {
     const double u00 = PA(CLV_XVEL0, i, j), u10 = PA(CLV_XVEL0, i + 1, j), u01 = PA(CLV_XVEL0, i, j + 1), u11 = PA(CLV_XVEL0, i + 1, j + 1);
@@ -519,12 +522,12 @@ __global__ void adj_visc_cell_k(Dev p, Dev a) {
     for (int k = 0; k < 6; ++k) g[k] = q.d[k] * qb;
     AA(CLV_DENSITY0, i, j) += q.d[6] * qb;
   }
-  AA(CLV_WORK1, i, j) = g[0];
-  AA(CLV_WORK2, i, j) = g[1];
-  AA(CLV_WORK3, i, j) = g[2];
-  AA(CLV_WORK4, i, j) = g[3];
-  AA(CLV_WORK5, i, j) = g[4];
-  AA(CLV_WORK6, i, j) = g[5];
+  ST_STREAM(AA(CLV_WORK1, i, j), g[0]);
+  ST_STREAM(AA(CLV_WORK2, i, j), g[1]);
+  ST_STREAM(AA(CLV_WORK3, i, j), g[2]);
+  ST_STREAM(AA(CLV_WORK4, i, j), g[3]);
+  ST_STREAM(AA(CLV_WORK5, i, j), g[4]);
+  ST_STREAM(AA(CLV_WORK6, i, j), g[5]);
 }
 
 // V2, sites (i, j) in [1, nx+2]^2: node (i, j) velocities from its four cells,
@@ -562,9 +565,9 @@ __global__ void adj_pdv_cell_k(Dev p, Dev a, int predict, double dt) {
   AA(CLV_ENERGY0, i, j) += r1.d[1] * rb + e1.d[1] * eb;
   AA(CLV_PRESSURE, i, j) += r1.d[2] * rb + e1.d[2] * eb;
   AA(CLV_VISCOSITY, i, j) += r1.d[3] * rb + e1.d[3] * eb;
-  AA(CLV_WORK1, i, j) = r1.d[4] * rb + e1.d[4] * eb;
-  AA(CLV_DENSITY1, i, j) = 0.0;
-  AA(CLV_ENERGY1, i, j) = 0.0;
+  ST_STREAM(AA(CLV_WORK1, i, j), r1.d[4] * rb + e1.d[4] * eb);
+  ST_STREAM(AA(CLV_DENSITY1, i, j), 0.0);
+  ST_STREAM(AA(CLV_ENERGY1, i, j), 0.0);
 }
 
 // P2, nodes [2, nx+2]^2.  d(face flux)/d(u of a face node) = area * 0.25 dt in
@@ -600,8 +603,8 @@ __global__ void adj_accel_node_k(Dev p, Dev a, double dt) {
   const double ub = AA(CLV_XVEL1, i, j), vb = AA(CLV_YVEL1, i, j);
   AA(CLV_XVEL0, i, j) += ub;
   AA(CLV_YVEL0, i, j) += vb;
-  AA(CLV_XVEL1, i, j) = 0.0;
-  AA(CLV_YVEL1, i, j) = 0.0;
+  ST_STREAM(AA(CLV_XVEL1, i, j), 0.0);
+  ST_STREAM(AA(CLV_YVEL1, i, j), 0.0);
   const double m = (PA(CLV_DENSITY0, i - 1, j - 1) * PA(CLV_VOLUME, i - 1, j - 1) + PA(CLV_DENSITY0, i - 1, j) * PA(CLV_VOLUME, i - 1, j) +
                     PA(CLV_DENSITY0, i, j) * PA(CLV_VOLUME, i, j) + PA(CLV_DENSITY0, i, j - 1) * PA(CLV_VOLUME, i, j - 1)) *
                    0.25;
@@ -613,9 +616,9 @@ __global__ void adj_accel_node_k(Dev p, Dev a, double dt) {
   };
   const double2 dp = D(CLV_PRESSURE), dq = D(CLV_VISCOSITY);
   const double sb = -(ub * (dp.x + dq.x) + vb * (dp.y + dq.y));
-  AA(CLV_WORK1, i, j) = -s * ub;
-  AA(CLV_WORK2, i, j) = -s * vb;
-  AA(CLV_WORK3, i, j) = sb * (-s / m) * 0.25;  // ds/d(rho_c) = -s / m * 0.25 vol_c
+  ST_STREAM(AA(CLV_WORK1, i, j), -s * ub);
+  ST_STREAM(AA(CLV_WORK2, i, j), -s * vb);
+  ST_STREAM(AA(CLV_WORK3, i, j), sb * (-s / m) * 0.25);  // ds/d(rho_c) = -s / m * 0.25 vol_c
 }
 
 // A2, cells [1, nx+2]^2: cell c is 00 of node c, m0 of node c+x, 0m of node c+y,
@@ -728,7 +731,7 @@ __global__ void adj_cell_values_k(Dev p, int sweep, int dir) {
   int s, t;
   sweep_pos(dir, 0, 2, s, t);
   if (s > smax + 3 || t > tmax + 1) return;
-  PS(CLV_WORK1, s, t) = pre_at(p, sweep, sw, s, t);
+  ST_STREAM(PS(CLV_WORK1, s, t), pre_at(p, sweep, sw, s, t));
   if (s < 2) return;
   const int vf_id = dir == 1 ? CLV_VOL_FLUX_X : CLV_VOL_FLUX_Y, vd = dir == 1 ? CLV_VERTEXDX : CLV_VERTEXDY;
   const double vfv = PS(vf_id, s, t);
@@ -737,8 +740,8 @@ __global__ void adj_cell_values_k(Dev p, int sweep, int dir) {
   el_face<0>(cst<0>(vfv), cst<0>(pre_at(p, sweep, sw, r.dn, t)), cst<0>(PS(CLV_DENSITY1, r.uw, t)), cst<0>(PS(CLV_DENSITY1, r.dn, t)),
              cst<0>(PS(CLV_DENSITY1, r.dw, t)), cst<0>(PS(CLV_ENERGY1, r.uw, t)), cst<0>(PS(CLV_ENERGY1, r.dn, t)),
              cst<0>(PS(CLV_ENERGY1, r.dw, t)), PM(vd, s), PM(vd, r.dif), mf, ef);
-  PS(CLV_WORK2, s, t) = mf.v;
-  PS(CLV_WORK3, s, t) = ef.v;
+  ST_STREAM(PS(CLV_WORK2, s, t), mf.v);
+  ST_STREAM(PS(CLV_WORK3, s, t), ef.v);
 }
 
 // One cell's update, src/serial/advec_cell.cpp:120-125 / :211-216:
@@ -880,7 +883,7 @@ __global__ __launch_bounds__(256) void adj_cell_x_k(Dev p, Dev a, double *rn, do
   if (fval) {
     const double vfs = face_adj(vfv, AS(CLV_MASS_FLUX_X, s, t), s <= smax + 1, u, s - 1 >= 2 && s - 1 <= smax + 1, Ml, El, Vl, pre_dn, r_uw, r_dn,
                                 r_dw, e_uw, e_dn, e_dw, PM(CLV_VERTEXDX, s), PM(CLV_VERTEXDX, r.dif), g);
-    if (owned) AS(CLV_WORK3, s, t) = vfs;
+    if (owned) ST_STREAM(AS(CLV_WORK3, s, t), vfs);
   }
   // role cells as offsets + 2 from the face (uw in [-2, 1], dn and dw in [-1, 0]); bit 6: a face
   const int roles = fval ? (64 | (r.uw - s + 2) | ((r.dn - s + 2) << 2) | ((r.dw - s + 2) << 4)) : 0;
@@ -897,7 +900,7 @@ __global__ __launch_bounds__(256) void adj_cell_x_k(Dev p, Dev a, double *rn, do
   if (owned && s <= smax + 3) {
     rn[o] = w.rb;
     en[o] = w.eb;
-    PS(CLV_WORK5, s, t) = w.pb;
+    ST_STREAM(PS(CLV_WORK5, s, t), w.pb);
   }
 }
 
@@ -927,7 +930,7 @@ __global__ __launch_bounds__(256) void adj_cell_y_k(Dev p, Dev a, double *rn, do
   auto put = [&](const CellAdj &w, int c) {
     rn[sw.at(a, c, t)] = w.rb;
     en[sw.at(a, c, t)] = w.eb;
-    PS(CLV_WORK5, c, t) = w.pb;
+    ST_STREAM(PS(CLV_WORK5, c, t), w.pb);
   };
   const int fs = max(2, c0 - 1), fe = min(smax + 3, cl + 2);
   CellAdj w0 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs - 2, t), w1 = cell_adj(p, a, sw, smax, CLV_VOL_FLUX_Y, fs - 1, t);
@@ -952,7 +955,7 @@ __global__ __launch_bounds__(256) void adj_cell_y_k(Dev p, Dev a, double *rn, do
     const double vfs = face_adj(vfv, mfb0, f <= smax + 1, w2, f - 1 >= 2 && f - 1 <= smax + 1, w1.M, w1.E, w1.V, qd == 1 ? w1.pre : w2.pre,
                                 r_uw, qd == 1 ? w1.rho : w2.rho, qw == 1 ? w1.rho : w2.rho, e_uw, qd == 1 ? w1.e : w2.e,
                                 qw == 1 ? w1.e : w2.e, vd0, vd_dif, g);
-    if (f >= c0 && f <= c1) AS(CLV_WORK3, f, t) = vfs;
+    if (f >= c0 && f <= c1) ST_STREAM(AS(CLV_WORK3, f, t), vfs);
     take_shares(w0, qu == 0, false, false, g);
     take_shares(w1, qu == 1, qd == 1, qw == 1, g);
     take_shares(w2, qu == 2, qd == 2, qw == 2, g);
@@ -991,7 +994,7 @@ __global__ void adj_cell_vol_k(Dev p, Dev a, int sweep, int dir) {
     double g = PB(s - 1, t) - PB(s, t);
     if (s >= 2 && s <= smax + 3) {
       g += AS(CLV_WORK3, s, t);
-      AS(mf_id, s, t) = 0.0;  // consumed by C1
+      ST_STREAM(AS(mf_id, s, t), 0.0);  // consumed by C1
     }
     AS(vf_id, s, t) += g;
   }
@@ -1033,12 +1036,12 @@ __global__ void adj_mom_values_k(Dev p, int mom_sweep, int dir) {
   sweep_pos(dir, 0, 2, s, t);
   if (s > smax + 3 || t > tmax + 2) return;
   const double nf = node_flux_val(p, sw, s, t);
-  PS(CLV_WORK1, s, t) = nf;
+  ST_STREAM(PS(CLV_WORK1, s, t), nf);
   if (s < 1) return;
   const int i = dir == 1 ? s : t, j = dir == 1 ? t : s;
   const double post = node_mass_post_val(p, mom_sweep, i, j);
-  PS(CLV_WORK2, s, t) = post;
-  PS(CLV_WORK3, s, t) = post - node_flux_val(p, sw, s - 1, t) + nf;
+  ST_STREAM(PS(CLV_WORK2, s, t), post);
+  ST_STREAM(PS(CLV_WORK3, s, t), post - node_flux_val(p, sw, s - 1, t) + nf);
 }
 
 // the nodes node face f (between nodes f and f+1) reads, by the sign of its node flux
@@ -1102,8 +1105,8 @@ __global__ __launch_bounds__(256) void adj_mom_x_k(Dev p, Dev a, double *vn) {
       if (LAST) {
         AS(CLV_WORK5, s, t) += g[0];
       } else {
-        AS(CLV_WORK5, s, t) = g[0];
-        PS(CLV_WORK5, s, t) = g[1];
+        ST_STREAM(AS(CLV_WORK5, s, t), g[0]);
+        ST_STREAM(PS(CLV_WORK5, s, t), g[1]);
       }
     }
   }
@@ -1113,8 +1116,8 @@ __global__ __launch_bounds__(256) void adj_mom_x_k(Dev p, Dev a, double *vn) {
     const double vnew = (vel * mpre + flux_l - flux) / mpost;
     gv = vb * mpre / mpost;
     mpreb = (LAST ? AS(CLV_WORK3, s, t) : 0.0) + vb * vel / mpost;
-    AS(CLV_WORK4, s, t) = (LAST ? AS(CLV_WORK4, s, t) : 0.0) - vb * vnew / mpost;
-    if (!LAST) AS(CLV_WORK3, s, t) = mpreb;
+    ST_STREAM(AS(CLV_WORK4, s, t), (LAST ? AS(CLV_WORK4, s, t) : 0.0) - vb * vnew / mpost);
+    if (!LAST) ST_STREAM(AS(CLV_WORK3, s, t), mpreb);
   }
   // role nodes as offsets from the face: uw + 1 in {0, 3}, dn and dw in {0, 1}; bit 6: a face
   const int roles = fval ? (64 | (r.uw - s + 1) | ((r.dn - s) << 2) | ((r.dw - s) << 4)) : 0;
@@ -1135,7 +1138,7 @@ __global__ __launch_bounds__(256) void adj_mom_x_k(Dev p, Dev a, double *vn) {
   }
   if (owned) {
     vn[o] = gv;
-    if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
+    if (LAST) ST_STREAM(AS(CLV_WORK3, s, t), mpreb + gm);
   }
 }
 
@@ -1194,10 +1197,10 @@ __device__ __forceinline__ MomFace mom_face_y(const Dev &p, const Dev &a, const 
     if (LAST) ds = pw5 + g[1];
     if (own) {
       if (LAST) {
-        AS(CLV_WORK5, f, t) = aw5 + g[0];
+        ST_STREAM(AS(CLV_WORK5, f, t), aw5 + g[0]);
       } else {
-        AS(CLV_WORK5, f, t) = g[0];
-        PS(CLV_WORK5, f, t) = g[1];
+        ST_STREAM(AS(CLV_WORK5, f, t), g[0]);
+        ST_STREAM(PS(CLV_WORK5, f, t), g[1]);
       }
     }
     roles = 64 | (r.uw - f + 1) | ((r.dn - f) << 2) | ((r.dw - f) << 4);
@@ -1253,8 +1256,8 @@ __global__ __launch_bounds__(256) void adj_mom_y_k(Dev p, Dev a, double *vn) {
       const double vnew = (n0.vel * n0.mpre + F1.flux - F2.flux) / n0.mpost;
       gv = n0.vb * n0.mpre / n0.mpost;
       mpreb = (LAST ? n0.w3 : 0.0) + n0.vb * n0.vel / n0.mpost;
-      AS(CLV_WORK4, s, t) = (LAST ? n0.w4 : 0.0) - n0.vb * vnew / n0.mpost;
-      if (!LAST) AS(CLV_WORK3, s, t) = mpreb;
+      ST_STREAM(AS(CLV_WORK4, s, t), (LAST ? n0.w4 : 0.0) - n0.vb * vnew / n0.mpost);
+      if (!LAST) ST_STREAM(AS(CLV_WORK3, s, t), mpreb);
     }
     double gm = 0.0;
     mom_take<LAST>(gv, gm, F0, -2);
@@ -1262,7 +1265,7 @@ __global__ __launch_bounds__(256) void adj_mom_y_k(Dev p, Dev a, double *vn) {
     mom_take<LAST>(gv, gm, F2, 0);
     mom_take<LAST>(gv, gm, F3, 1);
     vn[sw.at(a, s, t)] = gv;
-    if (LAST) AS(CLV_WORK3, s, t) = mpreb + gm;
+    if (LAST) ST_STREAM(AS(CLV_WORK3, s, t), mpreb + gm);
     n0 = n1;
     n1 = n2;
     n2 = n3;
@@ -1300,7 +1303,7 @@ __global__ void adj_mom_mass_k(Dev p, Dev a, int mom_sweep, int dir) {
     for (int ni = i; ni <= i + 1; ++ni) g += mpost_bar(a, sw, smax, tmax, ni, nj);
   g *= 0.25;
   AA(CLV_DENSITY1, i, j) += mom_post_vol(p, mom_sweep, i, j) * g;
-  AA(CLV_WORK6, i, j) = PA(CLV_DENSITY1, i, j) * g;  // post_vol adjoint of cell (i, j)
+  ST_STREAM(AA(CLV_WORK6, i, j), PA(CLV_DENSITY1, i, j) * g);  // post_vol adjoint of cell (i, j)
 }
 
 // M5b: post_vol = vol + vfy(j+1) - vfy(j) (mom_sweep 1) or vol + vfx(i+1) - vfx(i)
