@@ -396,8 +396,8 @@ __global__ __launch_bounds__(RED_THREADS) void field_summary_gas_k(Dev d, double
       const double rho = AT(CLV_DENSITY0, j, k), e = AT(CLV_ENERGY0, j, k);
       double p, c;
       ideal_gas_cell(rho, e, p, c);
-      AT(CLV_PRESSURE, j, k) = p;
-      AT(CLV_SOUNDSPEED, j, k) = c;
+      __builtin_nontemporal_store(p, &AT(CLV_PRESSURE, j, k));  // streaming: read next by another launch
+      __builtin_nontemporal_store(c, &AT(CLV_SOUNDSPEED, j, k));
       double cell_vol = geom_volume(d, j, k);
       double cell_mass = cell_vol * rho;
       vol += cell_vol;
