@@ -720,10 +720,37 @@ hipError_t batch_launch_halo(hipStream_t s, int nprob, const Dev *tab, const Bat
   return hipGetLastError();
 }
 
-static int red_blocks(int ny) { return ny < RED_MAX_BLOCKS ? ny : RED_MAX_BLOCKS; }
+// Blocks of a reduction launch (calc_dt_k, field_summary_k, field_summary_gas_k): one
+// per row, up to the number of blocks the device holds at once (CUs x resident blocks
+// per CU, the least of the three kernels, so that the two summaries tree-sum in the
+// same order).  More blocks than fit make a second, partly filled round: 2048 blocks
+// at 6 per CU on 256 CUs kept calc_dt_k 25 % above its array traffic's time.
+static int red_capacity(int device) {
+  static std::mutex mu;
+  static int cap[64] = {0};
+  if (device < 0 || device >= 64) return RED_MAX_BLOCKS;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!cap[device]) {
+    int cus = 0, least = 1 << 30;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) return RED_MAX_BLOCKS;
+    const void *kerns[] = {(const void *)calc_dt_k, (const void *)field_summary_k, (const void *)field_summary_gas_k};
+    for (const void *k : kerns) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, RED_THREADS, 0) != hipSuccess || n < 1) return RED_MAX_BLOCKS;
+      least = n < least ? n : least;
+    }
+    const int c = cus * least;
+    cap[device] = c < RED_MAX_BLOCKS ? c : RED_MAX_BLOCKS;
+  }
+  return cap[device];
+}
+static int red_blocks(int ny, int device) {
+  const int cap = red_capacity(device);
+  return ny < cap ? ny : cap;
+}
 
 clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, double *dt_min_val) {
-  const int nb = red_blocks(c->d.ny);
+  const int nb = red_blocks(c->d.ny, c->device);
   LAUNCH_G("calc_dt", calc_dt_k, dim3(nb, 1, 1), dim3(RED_THREADS, 1, 1), c->d, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, c->partials);
   LAUNCH_G("calc_dt_final", reduce_min_k, dim3(1, 1, 1), dim3(RED_THREADS, 1, 1), (const double *)c->partials, nb, c->result_dev);
   clv_status st = hip_check(hipMemcpyAsync(c->result_host, c->result_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream), "calc_dt copy");
@@ -737,7 +764,7 @@ clv_status k_calc_dt(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_
 // with_gas: ideal_gas first (src/serial/field_summary.cpp:52), in the same pass as the
 // sums unless the serial summation order is asked for
 static clv_status summary(clv_chunk *c, double out[5], bool with_gas) {
-  const int nb = red_blocks(c->d.ny);
+  const int nb = red_blocks(c->d.ny, c->device);
   if (c->summary_serial) {
     if (with_gas) {
       clv_status st = k_ideal_gas(c, 0);
@@ -945,13 +972,13 @@ extern "C" clv_status clv_field_summary_noalloc(const clv_chunk_view *v, void *h
   const int read[] = {CLV_DENSITY0, CLV_ENERGY0, CLV_PRESSURE, CLV_XVEL0, CLV_YVEL0, CLV_VOLUME};
   for (int f : read)
     if (!v->field[f]) return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: density0, energy0, pressure, xvel0, yvel0 and volume are required");
-  const int nb = red_blocks(v->y_cells);
-  // the partials of nb blocks and the results must fit the fixed scratch array
-  if ((size_t)nb * 5 > (size_t)kScratchResults || nb > RED_MAX_BLOCKS)
-    return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: the reduction needs more partials than the scratch array holds");
   int device = 0;
   clv_status st = hip_check(hipGetDevice(&device), "clv_field_summary_noalloc: hipGetDevice");
   if (st != CLV_OK) return st;
+  const int nb = red_blocks(v->y_cells, device);
+  // the partials of nb blocks and the results must fit the fixed scratch array
+  if ((size_t)nb * 5 > (size_t)kScratchResults || nb > RED_MAX_BLOCKS)
+    return set_error(CLV_ERR_ARG, "clv_field_summary_noalloc: the reduction needs more partials than the scratch array holds");
   double *ptr = nullptr;
   if ((st = scratch_on(device, &ptr)) != CLV_OK) return st;
   Dev d{};
