@@ -125,6 +125,7 @@ SIGNATURES = [
     ("clv_ideal_gas", _I, [_P, _I]),
     ("clv_viscosity", _I, [_P]),
     ("clv_calc_dt", _I, [_P, _D, _D, _D, _D, _D, _PD, _PI, _PI, _PI, _PI]),
+    ("clv_timestep_kernels", _I, [_P, _D, _D, _D, _D, _D, _I, _PD, _PI, _PI, _PI, _PI]),
     ("clv_PdV", _I, [_P, _I, _D]),
     ("clv_PdV_driver", _I, [_P, _I, _D]),
     ("clv_revert", _I, [_P]),
@@ -372,6 +373,16 @@ class Chunk:
         ctl, jl, kl, small = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _check(self.L.clv_calc_dt(self.h, dtmin, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, C.byref(dt), C.byref(ctl),
                                   C.byref(jl), C.byref(kl), C.byref(small)), "calc_dt")
+        return dt.value, ctl.value, jl.value, kl.value, small.value
+
+    def timestep_kernels(self, pressure_halo_depth=1, dtmin=1e-7, dtc_safe=0.7, dtu_safe=0.5, dtv_safe=0.5, dtdiv_safe=0.7):
+        """ideal_gas, viscosity and calc_dt in one pass, with the pressure and viscosity
+        halos (clv_timestep_kernels)."""
+        dt = C.c_double()
+        ctl, jl, kl, small = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(self.L.clv_timestep_kernels(self.h, dtmin, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, int(pressure_halo_depth),
+                                           C.byref(dt), C.byref(ctl), C.byref(jl), C.byref(kl), C.byref(small)),
+               "timestep_kernels")
         return dt.value, ctl.value, jl.value, kl.value, small.value
 
     def PdV(self, predict, dt):

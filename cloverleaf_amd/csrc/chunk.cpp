@@ -357,6 +357,25 @@ clv_status clv_calc_dt(clv_chunk *c, double dtmin, double dtc_safe, double dtu_s
   return CLV_OK;
 }
 
+clv_status clv_timestep_kernels(clv_chunk *c, double dtmin, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                                int pressure_halo_depth, double *dt_min_val, int *dt_control, int *jldt, int *kldt, int *small) {
+  CHECK_CHUNK(c);
+  if (!dt_min_val) return set_error(CLV_ERR_ARG, "clv_timestep_kernels: null dt_min_val");
+  if (pressure_halo_depth != 1 && pressure_halo_depth != 2) return set_error(CLV_ERR_ARG, "clv_timestep_kernels: pressure_halo_depth must be 1 or 2");
+  clv_status st = fused_timestep_entry(c, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, pressure_halo_depth, dt_min_val);
+  if (st != CLV_OK) return st;
+  // the bookkeeping of clv_calc_dt (src/serial/calc_dt.cpp:40,80-83)
+  double jk_control = 1.1;
+  const int x_max = c->d.nx;
+  const int l_control = (int)(10.01 * (jk_control - (int)(jk_control)));
+  jk_control = jk_control - (jk_control - (int)(jk_control));
+  if (dt_control) *dt_control = l_control;
+  if (jldt) *jldt = ((int)jk_control) % x_max;
+  if (kldt) *kldt = (int)(1.f + (jk_control / x_max));
+  if (small) *small = *dt_min_val < dtmin ? 1 : 0;
+  return CLV_OK;
+}
+
 clv_status clv_PdV(clv_chunk *c, int predict, double dt) {
   CHECK_CHUNK(c);
   return k_PdV(c, predict, dt);

@@ -101,6 +101,8 @@ clv_status e_advec_mom(clv_chunk *c, int which_vel, int direction, int sweep_num
 // fused plan (kernels_fused.hip)
 size_t fused_partials_needed(int nx, int ny);
 clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val);
+clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, int p_depth,
+                                double *dt_min_val);
 clv_status fused_lagrange(clv_chunk *c, double dt);
 clv_status fused_advection(clv_chunk *c, int advect_x);
 

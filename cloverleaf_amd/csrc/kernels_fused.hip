@@ -165,8 +165,10 @@ __device__ __forceinline__ double geo_yarea(const V &d, double cdx_i, int i, int
 // block of a plain launch (<name>_k) or in the loop of a persistent batched launch
 // over many chunks (batch_<name>_k).  Both run the same instructions per tile.
 
-// The wave's minimum goes to part[strip].
-template <bool MESH, class V>
+// The wave's minimum goes to part[strip].  WRITE_PC: also store each owned cell's
+// pressure and sound speed, as ideal_gas writes them (the per-entry-point path, whose
+// later kernels read the arrays; the fused step recomputes them instead).
+template <bool MESH, bool WRITE_PC = false, class V>
 __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1, double dtc_safe, double dtu_safe, double dtv_safe,
                                               double dtdiv_safe, double *part, int nstrips) {
   const int lane = threadIdx.x & 63;
@@ -264,8 +266,12 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
       double mins = fmin(dtct, fmin(dtut, fmin(dtvt, fmin(dtdivt, g_big))));
       if (own) dtmin = fmin(mins, dtmin);
     }
-    // p is not stored: lagrange_tile recomputes it from the same rho0, e0
+    // p is not stored in the fused step: lagrange_tile recomputes it from the same rho0, e0
     if (own) AT(CLV_VISCOSITY, i, r) = q;
+    if (WRITE_PC && own) {
+      AT(CLV_PRESSURE, i, r) = p_c;
+      AT(CLV_SOUNDSPEED, i, r) = c_c;
+    }
     p_m = p_c;
     p_c = p_p;
     c_c = c_p;
@@ -279,13 +285,14 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
 }
 
 // 5 waves per SIMD fit without spilling (96 VGPRs)
-template <bool MESH>
+template <bool MESH, bool WRITE_PC = false>
 __global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
                                                   double *partials, int nstrips, int seg) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  timestep_tile<MESH>(d, bx, j0, min(j0 + seg, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, partials + (size_t)by * nstrips, nstrips);
+  timestep_tile<MESH, WRITE_PC>(d, bx, j0, min(j0 + seg, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe,
+                                partials + (size_t)by * nstrips, nstrips);
 }
 
 // Minimum of partials[0, n): block b takes the b-th of gridDim.x contiguous
@@ -1179,17 +1186,22 @@ static clv_status require_owning(clv_chunk *c) {
   return CLV_OK;
 }
 
-clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val) {
-  clv_status st = require_owning(c);
-  if (st != CLV_OK) return st;
+// timestep_k over the chunk and the two-level minimum of its partials into
+// c->result_dev, then copied to c->result_host (asynchronously)
+static clv_status timestep_launch(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, bool write_pc) {
   const Dev &d = c->d;
   const int seg = c->seg[SF_TIMESTEP];
   const int ns = ceil_div(d.nx, kOwnTimestep), nseg = ceil_div(d.ny, seg);
   const int n1 = ns * nseg;
   if ((size_t)n1 + kMinLevel2 > c->partials_len) return set_error(CLV_ERR_ARG, "fused timestep: partials buffer too small");
-  const auto kern = d.geom_mesh ? timestep_k<true> : timestep_k<false>;
-  LAUNCH_F("f_timestep", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe,
-           c->partials, ns, seg);
+  const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+  if (write_pc) {
+    const auto kern = d.geom_mesh ? timestep_k<true, true> : timestep_k<false, true>;
+    LAUNCH_F("f_timestep_pc", kern, g, d, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, c->partials, ns, seg);
+  } else {
+    const auto kern = d.geom_mesh ? timestep_k<true> : timestep_k<false>;
+    LAUNCH_F("f_timestep", kern, g, d, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, c->partials, ns, seg);
+  }
   before_launch(c, "calc_dt_final");
   if (n1 <= 16384) {
     hipLaunchKernelGGL(reduce_min_big_k, dim3(1, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)c->partials, n1, c->result_dev);
@@ -1198,10 +1210,34 @@ clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *d
     hipLaunchKernelGGL(reduce_min_big_k, dim3(kMinLevel2, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)c->partials, n1, level2);
     hipLaunchKernelGGL(reduce_min_big_k, dim3(1, 1, 1), dim3(1024, 1, 1), 0, c->stream, (const double *)level2, kMinLevel2, c->result_dev);
   }
-  if ((st = after_launch(c, "calc_dt_final")) != CLV_OK) return st;
-  st = hip_check(hipMemcpyAsync(c->result_host, c->result_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream), "fused dt copy");
+  clv_status st = after_launch(c, "calc_dt_final");
   if (st != CLV_OK) return st;
+  return hip_check(hipMemcpyAsync(c->result_host, c->result_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream), "fused dt copy");
+}
+
+clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val) {
+  clv_status st = require_owning(c);
+  if (st != CLV_OK) return st;
+  if ((st = timestep_launch(c, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe, false)) != CLV_OK) return st;
   // q's reflective ghosts for the accelerate stencil (driver/timestep.cpp:66-69)
+  if ((st = halo(c, {CLV_VISCOSITY}, 1)) != CLV_OK) return st;
+  if ((st = hip_check(hipStreamSynchronize(c->stream), "fused dt sync")) != CLV_OK) return st;
+  *dt_min_val = c->result_host[0];
+  return CLV_OK;
+}
+
+// driver/timestep.cpp:47-69 for the per-entry-point path, the volume and face areas
+// and density0, energy0, xvel0, yvel0 halos already in place: pressure, soundspeed
+// and viscosity written as ideal_gas and viscosity write them, then the pressure
+// halo of depth p_depth and viscosity's of depth 1, then the dt minimum.  Any chunk
+// (a wrapped one too): nothing is renamed.
+clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, int p_depth,
+                                double *dt_min_val) {
+  if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "clv_timestep_kernels needs x_cells, y_cells >= 4");
+  choose_segments(c);
+  clv_status st = timestep_launch(c, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, true);
+  if (st != CLV_OK) return st;
+  if ((st = halo(c, {CLV_PRESSURE}, p_depth)) != CLV_OK) return st;
   if ((st = halo(c, {CLV_VISCOSITY}, 1)) != CLV_OK) return st;
   if ((st = hip_check(hipStreamSynchronize(c->stream), "fused dt sync")) != CLV_OK) return st;
   *dt_min_val = c->result_host[0];

@@ -30,6 +30,10 @@
 #include "clover_mi355x.h"
 #include "shared.h"
 
+// runs the model's recorded timestep chain, if any (initialise.cpp), so that a host
+// copy sees the arrays the reference's sequence leaves
+void mi355x_flush_deferred();
+
 namespace clover {
 
 struct context {
@@ -64,6 +68,7 @@ template <typename T> struct Buffer1D {
     return size;
   }
   std::vector<T> mirrored() const {
+    mi355x_flush_deferred();
     std::vector<T> buffer(size);
     check_clv(clv_copy_to_host(buffer.data(), data, size * sizeof(T)), "Buffer1D mirrored");
     return buffer;
@@ -93,6 +98,7 @@ template <typename T> struct Buffer2D {
   // host copy in the order BufferMirror2D indexes it (driver/shared.h:30-37:
   // element (i, j) at [j + i * sizeY])
   std::vector<T> mirrored() const {
+    mi355x_flush_deferred();
     std::vector<T> rows((sizeY + 1) * pitch);
     check_clv(clv_copy_to_host(rows.data(), data, rows.size() * sizeof(T)), "Buffer2D mirrored");
     std::vector<T> out(sizeX * sizeY);

@@ -70,7 +70,97 @@ void external_faces(const global_variables &globals, int tile, int external[4]) 
   for (int face = 0; face < 4; ++face) external[face] = at[face] && c.chunk_neighbours[face] == external_face;
 }
 
+namespace {
+// the recorded timestep chain (one tile, see mi355x_model.h)
+struct Deferred {
+  clv_chunk *c = nullptr;
+  bool gas = false, visc = false, q_halo = false;
+  int p_depth = 0;
+};
+Deferred g_def;
+
+bool chain_allowed(const global_variables &globals) {
+  const char *e = std::getenv("CLOVER_MI355X_TIMESTEP");
+  if (e && std::string(e) == "reference") return false;
+  if (!fused_entries(globals) || globals.profiler_on || globals.config.tiles_per_chunk != 1) return false;
+  for (int face = 0; face < 4; ++face)
+    if (globals.chunk.chunk_neighbours[face] != external_face) return false;
+  const tile_info &t = globals.chunk.tiles[0].info;
+  return t.t_xmax - t.t_xmin + 1 >= 4 && t.t_ymax - t.t_ymin + 1 >= 4;
+}
+}  // namespace
+
+void flush_deferred() {
+  if (!g_def.gas) return;
+  Deferred d = g_def;
+  g_def = Deferred{};
+  int f[CLV_NUM_HALO_FIELDS] = {0};
+  check(clv_ideal_gas(d.c, 0), "ideal_gas");
+  if (d.p_depth) {
+    f[CLV_PRESSURE] = 1;
+    check(clv_update_halo(d.c, f, d.p_depth), "update_halo");
+    f[CLV_PRESSURE] = 0;
+  }
+  if (d.visc) check(clv_viscosity(d.c), "viscosity");
+  if (d.q_halo) {
+    f[CLV_VISCOSITY] = 1;
+    check(clv_update_halo(d.c, f, 1), "update_halo");
+  }
+}
+
+bool defer_ideal_gas(global_variables &globals, int tile, bool predict) {
+  if (predict || tile != 0 || !chain_allowed(globals)) return false;
+  clv_chunk *c = chunk(globals, tile);  // runs an earlier recorded chain first
+  g_def = Deferred{};
+  g_def.c = c;
+  g_def.gas = true;
+  return true;
+}
+
+bool defer_viscosity(global_variables &globals) {
+  (void)globals;
+  if (!g_def.gas || g_def.visc) return false;
+  g_def.visc = true;
+  return true;
+}
+
+void defer_halo(global_variables &globals, int fields[CLV_NUM_HALO_FIELDS], int depth) {
+  (void)globals;
+  if (!g_def.gas) return;
+  if (!g_def.visc && g_def.p_depth == 0 && fields[CLV_PRESSURE] && !fields[CLV_VISCOSITY] && !fields[CLV_SOUNDSPEED]) {
+    fields[CLV_PRESSURE] = 0;
+    g_def.p_depth = depth;
+    return;
+  }
+  if (g_def.visc && g_def.p_depth && !g_def.q_halo && depth == 1 && fields[CLV_VISCOSITY] && !fields[CLV_PRESSURE] &&
+      !fields[CLV_SOUNDSPEED]) {
+    fields[CLV_VISCOSITY] = 0;
+    g_def.q_halo = true;
+    return;
+  }
+  flush_deferred();  // a request the chain does not expect: the arrays first
+}
+
+bool deferred_calc_dt(global_variables &globals, int tile, double &dt, int &control, int &jldt, int &kldt, int &small) {
+  if (!(g_def.gas && g_def.visc && g_def.q_halo && g_def.p_depth) || tile != 0) {
+    flush_deferred();
+    return false;
+  }
+  const global_config &cfg = globals.config;
+  Deferred d = g_def;
+  g_def = Deferred{};
+  check(clv_timestep_kernels(d.c, cfg.dtmin, cfg.dtc_safe, cfg.dtu_safe, cfg.dtv_safe, cfg.dtdiv_safe, d.p_depth, &dt, &control, &jldt,
+                             &kldt, &small),
+        "timestep_kernels");
+  return true;
+}
+
 clv_chunk *chunk(global_variables &globals, int tile) {
+  if (g_def.gas) flush_deferred();
+  return chunk_keep(globals, tile);
+}
+
+clv_chunk *chunk_keep(global_variables &globals, int tile) {
   tile_type &t = globals.chunk.tiles[tile];
   field_type &f = t.field;
   auto it = g_chunks.find(f.density0.data);
@@ -118,8 +208,11 @@ void sync_if_profiling(global_variables &globals, int tile) {
 }
 
 void release_chunks() {
+  flush_deferred();
   for (auto &kv : g_chunks) clv_chunk_destroy(kv.second);
   g_chunks.clear();
 }
 
 }  // namespace mi355x
+
+void mi355x_flush_deferred() { mi355x::flush_deferred(); }

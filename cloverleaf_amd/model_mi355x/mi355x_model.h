@@ -7,8 +7,12 @@
 
 namespace mi355x {
 
-// The chunk over tile `tile`'s buffers, wrapped on first use (clv_chunk_wrap).
+// The chunk over tile `tile`'s buffers, wrapped on first use (clv_chunk_wrap), after
+// running a recorded timestep chain (below).
 clv_chunk *chunk(global_variables &globals, int tile);
+// The same without running the recorded chain: for work that touches none of the
+// chain's arrays (pressure, soundspeed, viscosity).
+clv_chunk *chunk_keep(global_variables &globals, int tile);
 // The chunk of the tile one of whose 2D buffers is `field` (aborts if none is).
 clv_chunk *chunk_of(global_variables &globals, const double *field);
 // external[face] (face: left, right, bottom, top) = the tile's face lies on the
@@ -30,6 +34,22 @@ bool fused_entries(const global_variables &globals);
 // After a call that may exchange a tile's buffers (CLV_ENTRY_FUSED advection): point the
 // driver's Buffer2D handles at the arrays the chunk now uses.
 void refresh_buffers(global_variables &globals, int tile);
+
+// The timestep chain of driver/timestep.cpp:47-69 -- ideal_gas(predict = false), the
+// halo update with pressure, viscosity, the viscosity halo update, calc_dt -- runs as
+// one launch (clv_timestep_kernels) on a single tile of a single chunk, without the
+// profiler and with the one-launch entry points: ideal_gas and viscosity are recorded,
+// the halo updates run at once for the other fields and are recorded for pressure and
+// viscosity, and calc_dt runs the lot.  Anything else that touches a tile first runs
+// what is recorded one entry point at a time (flush_deferred), so the arrays are the
+// reference's whenever they can be seen.
+bool defer_ideal_gas(global_variables &globals, int tile, bool predict);
+bool defer_viscosity(global_variables &globals);
+// Takes pressure / viscosity out of a halo request when their halo can be recorded.
+void defer_halo(global_variables &globals, int fields[CLV_NUM_HALO_FIELDS], int depth);
+// calc_dt of the recorded chain: true when it ran (outputs as clv_calc_dt).
+bool deferred_calc_dt(global_variables &globals, int tile, double &dt, int &control, int &jldt, int &kldt, int &small);
+void flush_deferred();
 
 inline void check(clv_status st, const char *what) { clover::check_clv(st, what); }
 [[noreturn]] void unsupported(const char *what);

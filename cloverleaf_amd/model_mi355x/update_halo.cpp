@@ -9,7 +9,11 @@
 #include "timer.h"
 #include "update_tile_halo.h"
 
-void update_halo(global_variables &globals, int fields[NUM_FIELDS], const int depth) {
+void update_halo(global_variables &globals, int fields_in[NUM_FIELDS], const int depth) {
+  // pressure / viscosity of a recorded timestep chain get their halo in its launch
+  int fields[NUM_FIELDS];
+  for (int k = 0; k < NUM_FIELDS; ++k) fields[k] = fields_in[k];
+  mi355x::defer_halo(globals, fields, depth);
   double kernel_time = 0;
   if (globals.profiler_on) kernel_time = timer();
   update_tile_halo(globals, fields, depth);
@@ -28,7 +32,8 @@ void update_halo(global_variables &globals, int fields[NUM_FIELDS], const int de
     for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {
       int external[4];
       mi355x::external_faces(globals, tile, external);
-      mi355x::check(clv_update_halo_faces(mi355x::chunk(globals, tile), fields, depth, external), "update_halo");
+      // defer_halo has taken the recorded chain's arrays out of `fields` (or run the chain)
+      mi355x::check(clv_update_halo_faces(mi355x::chunk_keep(globals, tile), fields, depth, external), "update_halo");
       mi355x::sync_if_profiling(globals, tile);
     }
   }

@@ -11,8 +11,8 @@
  * Every entry point returns CLV_OK (0) or a negative clv_status; the message of
  * the last failure on the calling thread is clv_last_error().  Calls on one
  * chunk are ordered on that chunk's stream and do not block the host, except
- * those that return host scalars (clv_calc_dt, clv_field_summary,
- * clv_timestep, clv_hydro_step) and the explicit copies.  A chunk is not
+ * those that return host scalars (clv_calc_dt, clv_timestep_kernels,
+ * clv_field_summary, clv_timestep, clv_hydro_step) and the explicit copies.  A chunk is not
  * thread-safe.  No HIP or C++ type crosses this boundary.
  *
  * Each per-kernel entry point replaces one reference model entry point for a
@@ -211,6 +211,17 @@ clv_status clv_viscosity(clv_chunk *c);
  * bookkeeping (src/serial/calc_dt.cpp:80-83); small = dt_min_val < dtmin. */
 clv_status clv_calc_dt(clv_chunk *c, double dtmin, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
                        double *dt_min_val, int *dt_control, int *jldt, int *kldt, int *small);
+/* driver/timestep.cpp:47-69 in one pass over the cells, for a driver that has already
+ * refreshed the density0, energy0, xvel0 and yvel0 halos: ideal_gas(predict = 0),
+ * viscosity and calc_dt (src/serial/ideal_gas.cpp:28-48, viscosity.cpp:30-68,
+ * calc_dt.cpp:30-111) with pressure, soundspeed and viscosity written as those
+ * entry points write them, the pressure halo of depth `pressure_halo_depth` (1 or 2)
+ * and the viscosity halo of depth 1 (the reflective halo of every face, as
+ * clv_update_halo); outputs as clv_calc_dt.  It replaces the sequence clv_ideal_gas,
+ * clv_update_halo(pressure...), clv_viscosity, clv_update_halo(viscosity, 1),
+ * clv_calc_dt with the same arrays and dt.  x_cells, y_cells >= 4. */
+clv_status clv_timestep_kernels(clv_chunk *c, double dtmin, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
+                                int pressure_halo_depth, double *dt_min_val, int *dt_control, int *jldt, int *kldt, int *small);
 /* The kernel of void PdV(global_variables&, bool predict)           driver/PdV.h:24
  * (src/serial/PdV.cpp:36-87); clv_PdV_driver adds the predictor chain
  * ideal_gas(predict) + update_halo(pressure, 1) + revert (src/serial/PdV.cpp:113-132). */

@@ -37,11 +37,15 @@ def _run(tmp_path, deck, env_extra=None, args=()):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("order", ["tree", "serial"])
-def test_dropin_prints_the_reference_output(gpu, tmp_path, order):
+@pytest.mark.parametrize("order,timestep", [("tree", "chain"), ("serial", "chain"), ("tree", "reference")])
+def test_dropin_prints_the_reference_output(gpu, tmp_path, order, timestep):
     """clover_bm_short (960^2, 87 steps, test problem 2): every dt line and every
-    field-summary line equal to the reference's, and the QA check passes."""
-    p = _run(tmp_path, "clover_bm_short", {"CLOVER_MI355X_SUMMARY": order})
+    field-summary line equal to the reference's, and the QA check passes; with the
+    timestep chain as one launch (the default) and as its five entry points."""
+    env = {"CLOVER_MI355X_SUMMARY": order}
+    if timestep == "reference":
+        env["CLOVER_MI355X_TIMESTEP"] = "reference"
+    p = _run(tmp_path, "clover_bm_short", env)
     assert p.returncode == 0, p.stderr[-2000:]
     got = _printed_lines(p.stdout, (tmp_path / "clover.out").read_text())
     want = (T.GOLDEN / "clover_bm_short.reference_lines.txt").read_text().splitlines()

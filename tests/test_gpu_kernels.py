@@ -127,6 +127,32 @@ def test_field_summary_close_to_oracle(gpu, nx, ny):
         assert abs(a - b) <= 1e-13 * abs(b), (a, b)
 
 
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("nx,ny", [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130)])
+def test_timestep_kernels_equal_the_entry_point_sequence(gpu, nx, ny, depth):
+    """clv_timestep_kernels (ideal_gas, viscosity and calc_dt in one pass, then the
+    pressure and viscosity halos) against driver/timestep.cpp:47-69's sequence of
+    entry points on the same arrays: every array bit for bit, and the same dt and
+    bookkeeping.  Both start with the density0, energy0, xvel0, yvel0 halos done, as
+    the drop-in does before it runs the one-pass form."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=7 * nx + ny + depth)
+    first = ["energy0", "density0", "xvel0", "yvel0"]
+    with clv.Chunk(nx, ny) as a, clv.Chunk(nx, ny) as b:
+        _load(a, f, m)
+        _load(b, f, m)
+        a.update_halo(first, 1)
+        b.update_halo(first, 1)
+        got = a.timestep_kernels(depth)
+        b.ideal_gas(False)
+        b.update_halo(["pressure"], depth)
+        b.viscosity()
+        b.update_halo(["viscosity"], 1)
+        want = b.calc_dt()
+        assert got == want
+        for n in T.FIELDS:
+            assert np.array_equal(a.download(n), b.download(n), equal_nan=True), n
+
+
 @pytest.mark.parametrize("nx,ny", SIZES)
 def test_summary_one_pass_equals_ideal_gas_then_sums(gpu, nx, ny):
     """clv_hydro_summary (ideal_gas and the sums in one pass) against clv_ideal_gas
