@@ -99,6 +99,7 @@ ALG_TERMS = {
     # every output written once over the interior)
     "f_timestep": [(8, "C")],                  # rho0, e0, u0, v0, vol, xarea, yarea -> q
     "f_timestep_pc": [(10, "C")],              # the same -> q, p, c (the per-entry-point path)
+    "f_lagrange_pc": [(16, "C")],              # f_lagrange's arrays -> + p', c' (the per-entry-point path)
     "f_lagrange": [(14, "C")],                 # rho0, e0, q, u0, v0, vol, xarea, yarea -> u1, v1, rho1, e1, vfx, vfy
     "f_cm_x1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
     "f_cm_y1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
@@ -108,7 +109,7 @@ ALG_TERMS = {
 # of those, the volume / face-area arrays a kernel does not stream when the chunk's
 # geometry comes from the mesh (CLV_GEOMETRY_MESH), on the first term's extent
 GEOM_ARRAYS = {"calc_dt": 3, "PdV_predict": 3, "PdV_correct": 3, "accelerate": 3, "flux_calc": 2, "advec_cell_vol": 1,
-               "advec_mom_vol": 1, "field_summary": 1, "field_summary_gas": 1, "f_timestep": 3, "f_timestep_pc": 3, "f_lagrange": 3, "f_cm_x1": 1, "f_cm_y1": 1,
+               "advec_mom_vol": 1, "field_summary": 1, "field_summary_gas": 1, "f_timestep": 3, "f_timestep_pc": 3, "f_lagrange": 3, "f_lagrange_pc": 3, "f_cm_x1": 1, "f_cm_y1": 1,
                "f_cm_x2": 1, "f_cm_y2": 1}
 # compulsory traffic of one fused step, array sweeps: timestep 8 + lagrange 14 + the
 # first advection pass 12 + the second 11 (geometry read); 5 + 11 + 11 + 10 from the mesh

@@ -126,6 +126,7 @@ SIGNATURES = [
     ("clv_viscosity", _I, [_P]),
     ("clv_calc_dt", _I, [_P, _D, _D, _D, _D, _D, _PD, _PI, _PI, _PI, _PI]),
     ("clv_timestep_kernels", _I, [_P, _D, _D, _D, _D, _D, _I, _PD, _PI, _PI, _PI, _PI]),
+    ("clv_lagrange_kernels", _I, [_P, _D]),
     ("clv_PdV", _I, [_P, _I, _D]),
     ("clv_PdV_driver", _I, [_P, _I, _D]),
     ("clv_revert", _I, [_P]),
@@ -384,6 +385,11 @@ class Chunk:
                                            C.byref(dt), C.byref(ctl), C.byref(jl), C.byref(kl), C.byref(small)),
                "timestep_kernels")
         return dt.value, ctl.value, jl.value, kl.value, small.value
+
+    def lagrange_kernels(self, dt):
+        """PdV(predict) chain, accelerate, PdV(correct) and flux_calc in one pass
+        (clv_lagrange_kernels)."""
+        _check(self.L.clv_lagrange_kernels(self.h, float(dt)), "lagrange_kernels")
 
     def PdV(self, predict, dt):
         _check(self.L.clv_PdV(self.h, int(bool(predict)), float(dt)), "PdV")

@@ -376,6 +376,11 @@ clv_status clv_timestep_kernels(clv_chunk *c, double dtmin, double dtc_safe, dou
   return CLV_OK;
 }
 
+clv_status clv_lagrange_kernels(clv_chunk *c, double dt) {
+  CHECK_CHUNK(c);
+  return fused_lagrange_entry(c, dt);
+}
+
 clv_status clv_PdV(clv_chunk *c, int predict, double dt) {
   CHECK_CHUNK(c);
   return k_PdV(c, predict, dt);

@@ -78,6 +78,8 @@ clv_status k_PdV(clv_chunk *c, int predict, double dt);
 clv_status k_revert(clv_chunk *c);
 clv_status k_accelerate(clv_chunk *c, double dt);
 clv_status k_flux_calc(clv_chunk *c, double dt);
+// flux_calc's extra ghost row of vol_flux_x and ghost column of vol_flux_y only
+clv_status k_flux_calc_edges(clv_chunk *c, double dt);
 clv_status k_advec_cell(clv_chunk *c, int sweep_number, int dir);
 clv_status k_advec_mom(clv_chunk *c, int which_vel, int direction, int sweep_number);
 clv_status k_reset_field(clv_chunk *c);
@@ -103,6 +105,7 @@ size_t fused_partials_needed(int nx, int ny);
 clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val);
 clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, int p_depth,
                                 double *dt_min_val);
+clv_status fused_lagrange_entry(clv_chunk *c, double dt);
 clv_status fused_lagrange(clv_chunk *c, double dt);
 clv_status fused_advection(clv_chunk *c, int advect_x);
 

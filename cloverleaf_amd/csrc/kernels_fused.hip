@@ -325,8 +325,9 @@ struct PredIn {
 };
 
 // src/serial/PdV.cpp:49-63 (predict) then src/serial/ideal_gas.cpp:41 (pressure only: the
-// predictor's soundspeed is overwritten before use).
-__device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt) {
+// predictor's soundspeed is overwritten before use; WITH_C: and the sound speed, :42-45).
+template <bool WITH_C = false>
+__device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt, double &c) {
   double left_flux = (a.xa0 * (a.u00 + a.u01 + a.u00 + a.u01)) * 0.25 * dt * 0.5;
   double right_flux = (a.xa1 * (a.u10 + a.u11 + a.u10 + a.u11)) * 0.25 * dt * 0.5;
   double bottom_flux = (a.ya0 * (a.v00 + a.v10 + a.v00 + a.v10)) * 0.25 * dt * 0.5;
@@ -337,11 +338,20 @@ __device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt)
   double energy_change = (a.p / a.rho0 + a.q / a.rho0) * total_flux * recip_volume;
   double e1 = a.e0 - energy_change;
   double rho1 = a.rho0 * volume_change_s;
-  return (1.4 - 1.0) * rho1 * e1;
+  if constexpr (WITH_C) {
+    double p;
+    ideal_gas_cell(rho1, e1, p, c);
+    return p;
+  } else {
+    (void)c;
+    return (1.4 - 1.0) * rho1 * e1;
+  }
 }
 
-// cells [j0, jc1)
-template <bool MESH, class V>
+// cells [j0, jc1).  WRITE_PC: also store the predictor's pressure and sound speed of
+// the owned cells (what ideal_gas(predict) leaves in those arrays; the per-entry-point
+// path, whose caller may read them -- the fused step never does)
+template <bool MESH, bool WRITE_PC = false, class V>
 __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc1, double dt, int nstrips) {
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
@@ -354,7 +364,7 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own_cell = live && lane >= 1 && lane <= kOwnLagrange && i <= nx + 1;
   const bool own_node = live && ((lane >= 1 && lane <= kOwnLagrange && i <= nx + 2) || (lane == kOwnLagrange + 1 && i == nx + 2));
-  __shared__ double stage[2][6][BlockRows<kOwnLagrange, true>::kSlots];
+  __shared__ double stage[2][WRITE_PC ? 8 : 6][BlockRows<kOwnLagrange, true>::kSlots];
   const BlockRows<kOwnLagrange, true> out(bx, nx);
   const int pos = out.pos(i);
   const int jn1 = jc1 == ny + 2 ? ny + 3 : jc1;    // owned node rows [j0, jn1)
@@ -407,13 +417,13 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
     // cell row r inputs
     PredIn a;
     load_pred(r, u_r, v_r, u_r1, v_r1, a);
-    double pp;
+    double pp, cp = 0.0;
     if (r == 1) {
       pp = 0.0;  // set from row 2 below, before any use
     } else if (r == ny + 2) {
       pp = pp_prev;  // ghost row: p'(ny+2) = p'(ny+1)
     } else {
-      pp = mirror_cols(predicted_pressure(a, dt));
+      pp = mirror_cols(predicted_pressure<WRITE_PC>(a, dt, cp));
     }
     if (r == 2) pp_prev = pp;  // ghost row 1 mirrors row 2
     const double m = a.rho0 * a.vol;
@@ -456,6 +466,10 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
         // flux_calc, src/serial/flux_calc.cpp:40: y-face (i, r)
         if (own_cell) stage[r & 1][5][pos] = 0.25 * dt * a.ya0 * (v_r + v0_e + v1 + v1_e);
       }
+      if (WRITE_PC && own_cell && r < jc1) {
+        stage[r & 1][WRITE_PC ? 6 : 0][pos] = pp;
+        stage[r & 1][WRITE_PC ? 7 : 0][pos] = cp;
+      }
       u1_prev = u1;
       v1_prev = v1;
       // the block's rows (r, j0 uniform over the block)
@@ -476,6 +490,10 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
         }
         if (out.cell(t)) ST_STREAM(AT(CLV_VOL_FLUX_Y, xs, r), stage[r & 1][5][t]);
       }
+      if (WRITE_PC && r < jc1 && out.cell(t)) {
+        ST_STREAM(AT(CLV_PRESSURE, xs, r), stage[r & 1][WRITE_PC ? 6 : 0][t]);
+        ST_STREAM(AT(CLV_SOUNDSPEED, xs, r), stage[r & 1][WRITE_PC ? 7 : 0][t]);
+      }
     }
     pp_prev = pp;
     m_prev = m;
@@ -494,12 +512,12 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
-template <bool MESH>
+template <bool MESH, bool WRITE_PC = false>
 __global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips, int seg) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  lagrange_tile<MESH>(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips);
+  lagrange_tile<MESH, WRITE_PC>(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips);
 }
 
 // ---- advec_cell -------------------------------------------------------------------------
@@ -1242,6 +1260,24 @@ clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, 
   if ((st = hip_check(hipStreamSynchronize(c->stream), "fused dt sync")) != CLV_OK) return st;
   *dt_min_val = c->result_host[0];
   return CLV_OK;
+}
+
+// src/serial/PdV.cpp:92-133 (predict), accelerate, PdV (correct) and flux_calc for the
+// per-entry-point path, the timestep's halos in place: every array as those entry
+// points leave it -- the predictor's pressure (with its depth-1 reflective halo) and
+// sound speed included -- in one pass.  Any chunk (a wrapped one too): nothing is
+// renamed.
+clv_status fused_lagrange_entry(clv_chunk *c, double dt) {
+  if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "clv_lagrange_kernels needs x_cells, y_cells >= 4");
+  choose_segments(c);
+  const Dev &d = c->d;
+  const int seg = c->seg[SF_LAGRANGE];
+  const int ns = ceil_div(d.nx, kOwnLagrange), nseg = ceil_div(d.ny, seg);
+  const auto kern = d.geom_mesh ? lagrange_k<true, true> : lagrange_k<false, true>;
+  LAUNCH_F("f_lagrange_pc", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg);
+  clv_status st = k_flux_calc_edges(c, dt);
+  if (st != CLV_OK) return st;
+  return halo(c, {CLV_PRESSURE}, 1);
 }
 
 clv_status fused_lagrange(clv_chunk *c, double dt) {

@@ -146,6 +146,24 @@ __global__ __launch_bounds__(256) void flux_calc_k(Dev d, double dt) {
       0.25 * dt * geom_yarea(d, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(CLV_YVEL1, i, j) + AT(CLV_YVEL1, i + 1, j));
 }
 
+// The one extra flux of each direction that flux_calc's loops compute beyond what the
+// advection reads (src/serial/flux_calc.cpp:33-35): vol_flux_x of the ghost row ny + 2
+// and vol_flux_y of the ghost column nx + 2, from the ghost velocities as they stand.
+// The fused Lagrangian pass leaves them out; the per-entry-point form adds them here.
+__global__ __launch_bounds__(256) void flux_calc_edges_k(Dev d, double dt) {
+  const int t = blockIdx.x * (BX * BY) + threadIdx.y * BX + threadIdx.x;  // BX x BY blocks (LAUNCH)
+  if (t < d.nx + 1) {
+    const int i = 2 + t, j = d.ny + 2;
+    ST_STREAM(AT(CLV_VOL_FLUX_X, i, j),
+              0.25 * dt * geom_xarea(d, i, j) * (AT(CLV_XVEL0, i, j) + AT(CLV_XVEL0, i, j + 1) + AT(CLV_XVEL1, i, j) + AT(CLV_XVEL1, i, j + 1)));
+  }
+  if (t < d.ny + 1) {
+    const int i = d.nx + 2, j = 2 + t;
+    ST_STREAM(AT(CLV_VOL_FLUX_Y, i, j),
+              0.25 * dt * geom_yarea(d, i, j) * (AT(CLV_YVEL0, i, j) + AT(CLV_YVEL0, i + 1, j) + AT(CLV_YVEL1, i, j) + AT(CLV_YVEL1, i + 1, j)));
+  }
+}
+
 // ---- reset_field: src/serial/reset_field.cpp:35-50 (both loops in one launch) ------------
 __global__ __launch_bounds__(256) void reset_field_k(Dev d) {
   RANGE_GUARD(2, d.nx + 3, 2, d.ny + 3)
@@ -409,6 +427,13 @@ clv_status k_accelerate(clv_chunk *c, double dt) {
 clv_status k_flux_calc(clv_chunk *c, double dt) {
   const Dev &d = c->d;
   LAUNCH("flux_calc", flux_calc_k, grid2(2, d.nx + 3, 2, d.ny + 3), d, dt);
+  return CLV_OK;
+}
+
+clv_status k_flux_calc_edges(clv_chunk *c, double dt) {
+  const Dev &d = c->d;
+  const int n = (d.nx > d.ny ? d.nx : d.ny) + 1;
+  LAUNCH("flux_calc_edges", flux_calc_edges_k, dim3((unsigned)((n + BX * BY - 1) / (BX * BY)), 1, 1), d, dt);
   return CLV_OK;
 }
 

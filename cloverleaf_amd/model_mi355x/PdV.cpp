@@ -13,6 +13,7 @@
 #include "update_halo.h"
 
 void PdV(global_variables &globals, bool predict) {
+  if (mi355x::defer_pdv(globals, predict)) return;  // runs in flux_calc's launch
   double kernel_time = 0;
   if (globals.profiler_on) kernel_time = timer();
   globals.error_condition = 0;

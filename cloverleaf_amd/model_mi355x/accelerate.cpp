@@ -5,6 +5,7 @@
 #include "timer.h"
 
 void accelerate(global_variables &globals) {
+  if (mi355x::defer_accelerate(globals)) return;  // runs in flux_calc's launch
   double kernel_time = 0;
   if (globals.profiler_on) kernel_time = timer();
   for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {

@@ -5,6 +5,7 @@
 #include "timer.h"
 
 void flux_calc(global_variables &globals) {
+  if (mi355x::deferred_flux_calc(globals)) return;  // the recorded Lagrangian chain with it
   double kernel_time = 0;
   if (globals.profiler_on) kernel_time = timer();
   for (int tile = 0; tile < globals.config.tiles_per_chunk; ++tile) {

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "mi355x_model.h"
 
@@ -71,92 +72,139 @@ void external_faces(const global_variables &globals, int tile, int external[4]) 
 }
 
 namespace {
-// the recorded timestep chain (one tile, see mi355x_model.h)
-struct Deferred {
-  clv_chunk *c = nullptr;
-  bool gas = false, visc = false, q_halo = false;
-  int p_depth = 0;
+// The calls recorded for a one-launch chain (one tile, see mi355x_model.h), in order.
+enum RecOp { R_GAS, R_P_HALO, R_VISC, R_Q_HALO, R_PDV_PREDICT, R_ACCEL, R_PDV_CORRECT };
+struct Rec {
+  RecOp op;
+  double arg;  // halo depth or dt
 };
-Deferred g_def;
+clv_chunk *g_rec_chunk = nullptr;
+std::vector<Rec> g_rec;
 
 bool chain_allowed(const global_variables &globals) {
-  const char *e = std::getenv("CLOVER_MI355X_TIMESTEP");
-  if (e && std::string(e) == "reference") return false;
+  const char *e = std::getenv("CLOVER_MI355X_CHAINS");
+  if (e && std::string(e) == "off") return false;
   if (!fused_entries(globals) || globals.profiler_on || globals.config.tiles_per_chunk != 1) return false;
   for (int face = 0; face < 4; ++face)
     if (globals.chunk.chunk_neighbours[face] != external_face) return false;
   const tile_info &t = globals.chunk.tiles[0].info;
   return t.t_xmax - t.t_xmin + 1 >= 4 && t.t_ymax - t.t_ymin + 1 >= 4;
 }
+
+bool recorded(std::initializer_list<RecOp> ops) {
+  if (g_rec.size() != ops.size()) return false;
+  size_t k = 0;
+  for (RecOp op : ops)
+    if (g_rec[k++].op != op) return false;
+  return true;
+}
+
+// start recording on tile 0 (after running what an earlier record left)
+bool start_record(global_variables &globals, RecOp op, double arg) {
+  clv_chunk *c = chunk(globals, 0);
+  g_rec_chunk = c;
+  g_rec.push_back({op, arg});
+  return true;
+}
 }  // namespace
 
 void flush_deferred() {
-  if (!g_def.gas) return;
-  Deferred d = g_def;
-  g_def = Deferred{};
-  int f[CLV_NUM_HALO_FIELDS] = {0};
-  check(clv_ideal_gas(d.c, 0), "ideal_gas");
-  if (d.p_depth) {
-    f[CLV_PRESSURE] = 1;
-    check(clv_update_halo(d.c, f, d.p_depth), "update_halo");
-    f[CLV_PRESSURE] = 0;
-  }
-  if (d.visc) check(clv_viscosity(d.c), "viscosity");
-  if (d.q_halo) {
-    f[CLV_VISCOSITY] = 1;
-    check(clv_update_halo(d.c, f, 1), "update_halo");
+  if (g_rec.empty()) return;
+  const std::vector<Rec> rec = g_rec;
+  clv_chunk *c = g_rec_chunk;
+  g_rec.clear();
+  for (const Rec &r : rec) {
+    int f[CLV_NUM_HALO_FIELDS] = {0};
+    switch (r.op) {
+      case R_GAS: check(clv_ideal_gas(c, 0), "ideal_gas"); break;
+      case R_P_HALO:
+        f[CLV_PRESSURE] = 1;
+        check(clv_update_halo(c, f, (int)r.arg), "update_halo");
+        break;
+      case R_VISC: check(clv_viscosity(c), "viscosity"); break;
+      case R_Q_HALO:
+        f[CLV_VISCOSITY] = 1;
+        check(clv_update_halo(c, f, 1), "update_halo");
+        break;
+      case R_PDV_PREDICT: check(clv_PdV_driver(c, 1, r.arg), "PdV"); break;
+      case R_ACCEL: check(clv_accelerate(c, r.arg), "accelerate"); break;
+      case R_PDV_CORRECT: check(clv_PdV_driver(c, 0, r.arg), "PdV"); break;
+    }
   }
 }
 
+// ---- the timestep chain: ideal_gas, update_halo(pressure ...), viscosity, update_halo(viscosity), calc_dt
 bool defer_ideal_gas(global_variables &globals, int tile, bool predict) {
   if (predict || tile != 0 || !chain_allowed(globals)) return false;
-  clv_chunk *c = chunk(globals, tile);  // runs an earlier recorded chain first
-  g_def = Deferred{};
-  g_def.c = c;
-  g_def.gas = true;
-  return true;
+  return start_record(globals, R_GAS, 0);
 }
 
 bool defer_viscosity(global_variables &globals) {
   (void)globals;
-  if (!g_def.gas || g_def.visc) return false;
-  g_def.visc = true;
+  if (!recorded({R_GAS, R_P_HALO})) return false;
+  g_rec.push_back({R_VISC, 0});
   return true;
 }
 
 void defer_halo(global_variables &globals, int fields[CLV_NUM_HALO_FIELDS], int depth) {
   (void)globals;
-  if (!g_def.gas) return;
-  if (!g_def.visc && g_def.p_depth == 0 && fields[CLV_PRESSURE] && !fields[CLV_VISCOSITY] && !fields[CLV_SOUNDSPEED]) {
+  if (g_rec.empty()) return;
+  if (recorded({R_GAS}) && fields[CLV_PRESSURE] && !fields[CLV_VISCOSITY] && !fields[CLV_SOUNDSPEED]) {
     fields[CLV_PRESSURE] = 0;
-    g_def.p_depth = depth;
+    g_rec.push_back({R_P_HALO, (double)depth});
     return;
   }
-  if (g_def.visc && g_def.p_depth && !g_def.q_halo && depth == 1 && fields[CLV_VISCOSITY] && !fields[CLV_PRESSURE] &&
-      !fields[CLV_SOUNDSPEED]) {
+  if (recorded({R_GAS, R_P_HALO, R_VISC}) && depth == 1 && fields[CLV_VISCOSITY] && !fields[CLV_PRESSURE] && !fields[CLV_SOUNDSPEED]) {
     fields[CLV_VISCOSITY] = 0;
-    g_def.q_halo = true;
+    g_rec.push_back({R_Q_HALO, 1});
     return;
   }
   flush_deferred();  // a request the chain does not expect: the arrays first
 }
 
 bool deferred_calc_dt(global_variables &globals, int tile, double &dt, int &control, int &jldt, int &kldt, int &small) {
-  if (!(g_def.gas && g_def.visc && g_def.q_halo && g_def.p_depth) || tile != 0) {
+  if (tile != 0 || !recorded({R_GAS, R_P_HALO, R_VISC, R_Q_HALO})) {
     flush_deferred();
     return false;
   }
   const global_config &cfg = globals.config;
-  Deferred d = g_def;
-  g_def = Deferred{};
-  check(clv_timestep_kernels(d.c, cfg.dtmin, cfg.dtc_safe, cfg.dtu_safe, cfg.dtv_safe, cfg.dtdiv_safe, d.p_depth, &dt, &control, &jldt,
-                             &kldt, &small),
+  const int p_depth = (int)g_rec[1].arg;
+  g_rec.clear();
+  check(clv_timestep_kernels(g_rec_chunk, cfg.dtmin, cfg.dtc_safe, cfg.dtu_safe, cfg.dtv_safe, cfg.dtdiv_safe, p_depth, &dt, &control,
+                             &jldt, &kldt, &small),
         "timestep_kernels");
   return true;
 }
 
+// ---- the Lagrangian chain: PdV(predict), accelerate, PdV(correct), flux_calc
+bool defer_pdv(global_variables &globals, bool predict) {
+  if (predict) {
+    if (!chain_allowed(globals)) return false;
+    return start_record(globals, R_PDV_PREDICT, globals.dt);
+  }
+  if (!recorded({R_PDV_PREDICT, R_ACCEL}) || g_rec[0].arg != globals.dt) return false;
+  g_rec.push_back({R_PDV_CORRECT, globals.dt});
+  return true;
+}
+
+bool defer_accelerate(global_variables &globals) {
+  if (!recorded({R_PDV_PREDICT}) || g_rec[0].arg != globals.dt) return false;
+  g_rec.push_back({R_ACCEL, globals.dt});
+  return true;
+}
+
+bool deferred_flux_calc(global_variables &globals) {
+  if (!recorded({R_PDV_PREDICT, R_ACCEL, R_PDV_CORRECT}) || g_rec[0].arg != globals.dt) {
+    flush_deferred();
+    return false;
+  }
+  g_rec.clear();
+  check(clv_lagrange_kernels(g_rec_chunk, globals.dt), "lagrange_kernels");
+  return true;
+}
+
 clv_chunk *chunk(global_variables &globals, int tile) {
-  if (g_def.gas) flush_deferred();
+  if (!g_rec.empty()) flush_deferred();
   return chunk_keep(globals, tile);
 }
 

@@ -35,20 +35,27 @@ bool fused_entries(const global_variables &globals);
 // driver's Buffer2D handles at the arrays the chunk now uses.
 void refresh_buffers(global_variables &globals, int tile);
 
-// The timestep chain of driver/timestep.cpp:47-69 -- ideal_gas(predict = false), the
-// halo update with pressure, viscosity, the viscosity halo update, calc_dt -- runs as
-// one launch (clv_timestep_kernels) on a single tile of a single chunk, without the
-// profiler and with the one-launch entry points: ideal_gas and viscosity are recorded,
-// the halo updates run at once for the other fields and are recorded for pressure and
-// viscosity, and calc_dt runs the lot.  Anything else that touches a tile first runs
-// what is recorded one entry point at a time (flush_deferred), so the arrays are the
-// reference's whenever they can be seen.
+// Two chains of entry points run as one launch each on a single tile of a single chunk
+// (without the profiler, with the one-launch entry points; CLOVER_MI355X_CHAINS=off
+// turns them off):
+// - the timestep, driver/timestep.cpp:47-69: ideal_gas(predict = false), the halo
+//   update with pressure, viscosity, the viscosity halo update, then calc_dt runs
+//   them (clv_timestep_kernels); the halo updates run at once for the other fields;
+// - the Lagrangian phase, driver/hydro.cpp:63-70: PdV(predict), accelerate,
+//   PdV(correct), then flux_calc runs them (clv_lagrange_kernels).
+// The calls before the last are recorded.  Anything else that touches a tile first
+// runs what is recorded one entry point at a time (flush_deferred), so the arrays are
+// the reference's whenever they can be seen.
 bool defer_ideal_gas(global_variables &globals, int tile, bool predict);
 bool defer_viscosity(global_variables &globals);
 // Takes pressure / viscosity out of a halo request when their halo can be recorded.
 void defer_halo(global_variables &globals, int fields[CLV_NUM_HALO_FIELDS], int depth);
 // calc_dt of the recorded chain: true when it ran (outputs as clv_calc_dt).
 bool deferred_calc_dt(global_variables &globals, int tile, double &dt, int &control, int &jldt, int &kldt, int &small);
+bool defer_pdv(global_variables &globals, bool predict);
+bool defer_accelerate(global_variables &globals);
+// flux_calc of the recorded Lagrangian chain: true when it ran.
+bool deferred_flux_calc(global_variables &globals);
 void flush_deferred();
 
 inline void check(clv_status st, const char *what) { clover::check_clv(st, what); }

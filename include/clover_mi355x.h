@@ -227,6 +227,16 @@ clv_status clv_timestep_kernels(clv_chunk *c, double dtmin, double dtc_safe, dou
  * ideal_gas(predict) + update_halo(pressure, 1) + revert (src/serial/PdV.cpp:113-132). */
 clv_status clv_PdV(clv_chunk *c, int predict, double dt);
 clv_status clv_PdV_driver(clv_chunk *c, int predict, double dt);
+/* The Lagrangian phase of driver/hydro.cpp:63-70 in one pass over the cells, for a
+ * driver whose timestep left pressure = ideal_gas(density0, energy0), the
+ * density0/energy0/xvel0/yvel0/pressure halos of depth 1 and the viscosity halo of
+ * depth 1 (driver/timestep.cpp:47-69): PdV(predict) with its ideal_gas(predict),
+ * pressure halo and revert (src/serial/PdV.cpp:92-133), accelerate, PdV(correct) and
+ * flux_calc, every array as those entry points leave it (the predictor's pressure,
+ * its depth-1 reflective halo on every face, and its soundspeed included).  It
+ * replaces clv_PdV_driver(c, 1, dt), clv_accelerate, clv_PdV_driver(c, 0, dt),
+ * clv_flux_calc.  x_cells, y_cells >= 4. */
+clv_status clv_lagrange_kernels(clv_chunk *c, double dt);
 /* void revert(global_variables&)                                    driver/revert.h:24 */
 clv_status clv_revert(clv_chunk *c);
 /* void accelerate(global_variables&)                                driver/accelerate.h:24 */

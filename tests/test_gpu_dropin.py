@@ -37,14 +37,14 @@ def _run(tmp_path, deck, env_extra=None, args=()):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("order,timestep", [("tree", "chain"), ("serial", "chain"), ("tree", "reference")])
-def test_dropin_prints_the_reference_output(gpu, tmp_path, order, timestep):
+@pytest.mark.parametrize("order,chains", [("tree", "chains"), ("serial", "chains"), ("tree", "off")])
+def test_dropin_prints_the_reference_output(gpu, tmp_path, order, chains):
     """clover_bm_short (960^2, 87 steps, test problem 2): every dt line and every
     field-summary line equal to the reference's, and the QA check passes; with the
-    timestep chain as one launch (the default) and as its five entry points."""
+    chains of entry points as one launch each (the default) and one entry point at a time."""
     env = {"CLOVER_MI355X_SUMMARY": order}
-    if timestep == "reference":
-        env["CLOVER_MI355X_TIMESTEP"] = "reference"
+    if chains == "off":
+        env["CLOVER_MI355X_CHAINS"] = "off"
     p = _run(tmp_path, "clover_bm_short", env)
     assert p.returncode == 0, p.stderr[-2000:]
     got = _printed_lines(p.stdout, (tmp_path / "clover.out").read_text())
