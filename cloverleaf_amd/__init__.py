@@ -127,6 +127,7 @@ SIGNATURES = [
     ("clv_calc_dt", _I, [_P, _D, _D, _D, _D, _D, _PD, _PI, _PI, _PI, _PI]),
     ("clv_timestep_kernels", _I, [_P, _D, _D, _D, _D, _D, _I, _PD, _PI, _PI, _PI, _PI]),
     ("clv_lagrange_kernels", _I, [_P, _D]),
+    ("clv_advection_pass", _I, [_P, _I, _I]),
     ("clv_PdV", _I, [_P, _I, _D]),
     ("clv_PdV_driver", _I, [_P, _I, _D]),
     ("clv_revert", _I, [_P]),
@@ -390,6 +391,11 @@ class Chunk:
         """PdV(predict) chain, accelerate, PdV(correct) and flux_calc in one pass
         (clv_lagrange_kernels)."""
         _check(self.L.clv_lagrange_kernels(self.h, float(dt)), "lagrange_kernels")
+
+    def advection_pass(self, sweep_number, direction):
+        """advec_cell, the halo between, advec_mom for both velocities in one pass
+        (clv_advection_pass)."""
+        _check(self.L.clv_advection_pass(self.h, sweep_number, direction), "advection_pass")
 
     def PdV(self, predict, dt):
         _check(self.L.clv_PdV(self.h, int(bool(predict)), float(dt)), "PdV")

@@ -376,6 +376,13 @@ clv_status clv_timestep_kernels(clv_chunk *c, double dtmin, double dtc_safe, dou
   return CLV_OK;
 }
 
+clv_status clv_advection_pass(clv_chunk *c, int sweep_number, int direction) {
+  CHECK_CHUNK(c);
+  if ((sweep_number != 1 && sweep_number != 2) || (direction != CLV_XDIR && direction != CLV_YDIR))
+    return set_error(CLV_ERR_ARG, "clv_advection_pass: sweep_number must be 1 or 2, direction CLV_XDIR or CLV_YDIR");
+  return fused_advection_entry(c, sweep_number, direction);
+}
+
 clv_status clv_lagrange_kernels(clv_chunk *c, double dt) {
   CHECK_CHUNK(c);
   return fused_lagrange_entry(c, dt);

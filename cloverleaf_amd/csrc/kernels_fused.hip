@@ -1318,6 +1318,22 @@ static clv_status cm_sweep(clv_chunk *c, int sweep, int dir) {
   return CLV_OK;
 }
 
+// One sweep of driver/advection.cpp (:47-73 or :75-103) for the per-entry-point path,
+// the velocities' depth-2 halo (and for sweep 1 energy1, density1 and the volume
+// fluxes') in place: advec_cell, the halo of density1, energy1 and both mass fluxes,
+// advec_mom for xvel1 and yvel1, in one pass.  The new fields are exchanged with work
+// arrays (the caller re-points its handles); the velocities' ghost frames are carried
+// over first, so that they hold the halo the reference's advec_mom leaves untouched.
+clv_status fused_advection_entry(clv_chunk *c, int sweep, int dir) {
+  if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "clv_advection_pass needs x_cells, y_cells >= 4");
+  choose_segments(c);
+  clv_status st = k_copy_frame(c, CLV_XVEL1, CLV_WORK3);
+  if (st == CLV_OK) st = k_copy_frame(c, CLV_YVEL1, CLV_WORK4);
+  if (st == CLV_OK) st = cm_sweep(c, sweep, dir);
+  if (st != CLV_OK) return st;
+  return halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2);
+}
+
 // driver/advection.cpp:29-105 followed by reset_field (src/serial/reset_field.cpp)
 clv_status fused_advection(clv_chunk *c, int advect_x) {
   clv_status st = require_owning(c);

@@ -532,6 +532,27 @@ __global__ __launch_bounds__(256) void march_probe_k(MarchProbe p) {
   }
 }
 
+// ---- ghost frame copy ---------------------------------------------------------------------
+// dst's ghost frame (depth 2 of the node extent: rows 0, 1, ny + 3, ny + 4 and columns
+// 0, 1, nx + 3, nx + 4) <- src's; the interior is untouched.
+__global__ __launch_bounds__(256) void copy_frame_k(Dev d, int src, int dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = d.nx + 5, nrow = 4 * w, ncol = 4 * (d.ny + 1);
+  int i, j;
+  if (t < nrow) {
+    const int r = t / w;
+    i = t % w;
+    j = r < 2 ? r : d.ny + 1 + r;  // 0, 1, ny + 3, ny + 4
+  } else if (t < nrow + ncol) {
+    const int u = t - nrow, q = u / (d.ny + 1);
+    j = 2 + u % (d.ny + 1);
+    i = q < 2 ? q : d.nx + 1 + q;  // 0, 1, nx + 3, nx + 4
+  } else {
+    return;
+  }
+  AT(dst, i, j) = AT(src, i, j);
+}
+
 // ---- set-up -----------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void fill_zero_k(double *p, size_t n) {
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0.0;
@@ -632,6 +653,14 @@ __global__ __launch_bounds__(256) void generate_nodes_k(Dev d, StateTable tab) {
 }
 
 }  // namespace
+
+clv_status k_copy_frame(clv_chunk *c, int src, int dst) {
+  const int n = 4 * (c->d.nx + 5) + 4 * (c->d.ny + 1);
+  before_launch(c, "copy_frame");
+  hipLaunchKernelGGL(copy_frame_k, dim3((unsigned)((n + 255) / 256), 1, 1), dim3(256, 1, 1), 0, c->stream, c->d, src, dst);
+  return after_launch(c, "copy_frame");
+}
+
 
 // ---- host wrappers ------------------------------------------------------------------------
 #define LAUNCH_G(name, kern, grid, block, ...)                                 \

@@ -73,12 +73,14 @@ void external_faces(const global_variables &globals, int tile, int external[4]) 
 
 namespace {
 // The calls recorded for a one-launch chain (one tile, see mi355x_model.h), in order.
-enum RecOp { R_GAS, R_P_HALO, R_VISC, R_Q_HALO, R_PDV_PREDICT, R_ACCEL, R_PDV_CORRECT };
+enum RecOp { R_GAS, R_P_HALO, R_VISC, R_Q_HALO, R_PDV_PREDICT, R_ACCEL, R_PDV_CORRECT, R_CELL, R_ADV_HALO, R_MOM_X };
 struct Rec {
   RecOp op;
-  double arg;  // halo depth or dt
+  double arg;         // halo depth or dt
+  int sweep = 0, dir = 0;  // advection
 };
 clv_chunk *g_rec_chunk = nullptr;
+global_variables *g_rec_globals = nullptr;
 std::vector<Rec> g_rec;
 
 bool chain_allowed(const global_variables &globals) {
@@ -100,10 +102,22 @@ bool recorded(std::initializer_list<RecOp> ops) {
 }
 
 // start recording on tile 0 (after running what an earlier record left)
-bool start_record(global_variables &globals, RecOp op, double arg) {
+bool start_record(global_variables &globals, RecOp op, double arg, int sweep = 0, int dir = 0) {
   clv_chunk *c = chunk(globals, 0);
   g_rec_chunk = c;
-  g_rec.push_back({op, arg});
+  g_rec_globals = &globals;
+  g_rec.push_back({op, arg, sweep, dir});
+  return true;
+}
+
+// the halo request between advec_cell and advec_mom (driver/advection.cpp:57-63, 87-93)
+bool is_advection_halo(const int fields[CLV_NUM_HALO_FIELDS], int depth) {
+  if (depth != 2) return false;
+  for (int f = 0; f < CLV_NUM_HALO_FIELDS; ++f) {
+    const bool want = f == CLV_DENSITY1 || f == CLV_ENERGY1 || f == CLV_XVEL1 || f == CLV_YVEL1 || f == CLV_MASS_FLUX_X ||
+                      f == CLV_MASS_FLUX_Y;
+    if ((fields[f] == 1) != want) return false;
+  }
   return true;
 }
 }  // namespace
@@ -129,6 +143,18 @@ void flush_deferred() {
       case R_PDV_PREDICT: check(clv_PdV_driver(c, 1, r.arg), "PdV"); break;
       case R_ACCEL: check(clv_accelerate(c, r.arg), "accelerate"); break;
       case R_PDV_CORRECT: check(clv_PdV_driver(c, 0, r.arg), "PdV"); break;
+      case R_CELL:
+        check(clv_advec_cell(c, r.sweep, r.dir), "advec_cell");
+        refresh_buffers(*g_rec_globals, 0);
+        break;
+      case R_ADV_HALO:  // the velocities' part ran when it was recorded
+        f[CLV_DENSITY1] = f[CLV_ENERGY1] = f[CLV_MASS_FLUX_X] = f[CLV_MASS_FLUX_Y] = 1;
+        check(clv_update_halo(c, f, 2), "update_halo");
+        break;
+      case R_MOM_X:
+        check(clv_advec_mom(c, 1, r.dir, r.sweep), "advec_mom");
+        refresh_buffers(*g_rec_globals, 0);
+        break;
     }
   }
 }
@@ -157,6 +183,15 @@ void defer_halo(global_variables &globals, int fields[CLV_NUM_HALO_FIELDS], int 
   if (recorded({R_GAS, R_P_HALO, R_VISC}) && depth == 1 && fields[CLV_VISCOSITY] && !fields[CLV_PRESSURE] && !fields[CLV_SOUNDSPEED]) {
     fields[CLV_VISCOSITY] = 0;
     g_rec.push_back({R_Q_HALO, 1});
+    return;
+  }
+  if (recorded({R_CELL}) && is_advection_halo(fields, depth)) {
+    // the velocities' halo now (advec_cell does not change them), the rest recorded
+    int v[CLV_NUM_HALO_FIELDS] = {0};
+    v[CLV_XVEL1] = v[CLV_YVEL1] = 1;
+    check(clv_update_halo(g_rec_chunk, v, 2), "update_halo");
+    for (int f = 0; f < CLV_NUM_HALO_FIELDS; ++f) fields[f] = 0;
+    g_rec.push_back({R_ADV_HALO, 2});
     return;
   }
   flush_deferred();  // a request the chain does not expect: the arrays first
@@ -201,6 +236,28 @@ bool deferred_flux_calc(global_variables &globals) {
   g_rec.clear();
   check(clv_lagrange_kernels(g_rec_chunk, globals.dt), "lagrange_kernels");
   return true;
+}
+
+// ---- one advection sweep: advec_cell, update_halo(density1, energy1, xvel1, yvel1, mass fluxes, 2),
+// advec_mom(xvel), advec_mom(yvel)
+bool defer_advec_cell(global_variables &globals, int tile, int sweep, int dir) {
+  if (tile != 0 || !chain_allowed(globals)) return false;
+  return start_record(globals, R_CELL, 0, sweep, dir);
+}
+
+bool defer_advec_mom(global_variables &globals, int tile, int which_vel, int dir, int sweep) {
+  if (tile != 0 || g_rec.empty() || g_rec[0].op != R_CELL || g_rec[0].sweep != sweep || g_rec[0].dir != dir) return false;
+  if (which_vel == 1 && recorded({R_CELL, R_ADV_HALO})) {
+    g_rec.push_back({R_MOM_X, 0, sweep, dir});
+    return true;
+  }
+  if (which_vel == 2 && recorded({R_CELL, R_ADV_HALO, R_MOM_X})) {
+    g_rec.clear();
+    check(clv_advection_pass(g_rec_chunk, sweep, dir), "advection_pass");
+    refresh_buffers(globals, 0);  // density1, energy1 and the velocities are other arrays now
+    return true;
+  }
+  return false;
 }
 
 clv_chunk *chunk(global_variables &globals, int tile) {

@@ -42,7 +42,11 @@ void refresh_buffers(global_variables &globals, int tile);
 //   update with pressure, viscosity, the viscosity halo update, then calc_dt runs
 //   them (clv_timestep_kernels); the halo updates run at once for the other fields;
 // - the Lagrangian phase, driver/hydro.cpp:63-70: PdV(predict), accelerate,
-//   PdV(correct), then flux_calc runs them (clv_lagrange_kernels).
+//   PdV(correct), then flux_calc runs them (clv_lagrange_kernels);
+// - each advection sweep, driver/advection.cpp:47-73 and :75-103: advec_cell, the
+//   halo update of density1, energy1, the velocities and the mass fluxes, advec_mom
+//   for xvel1, then advec_mom for yvel1 runs them (clv_advection_pass); the
+//   velocities' halo runs at once (advec_cell does not change them).
 // The calls before the last are recorded.  Anything else that touches a tile first
 // runs what is recorded one entry point at a time (flush_deferred), so the arrays are
 // the reference's whenever they can be seen.
@@ -56,6 +60,9 @@ bool defer_pdv(global_variables &globals, bool predict);
 bool defer_accelerate(global_variables &globals);
 // flux_calc of the recorded Lagrangian chain: true when it ran.
 bool deferred_flux_calc(global_variables &globals);
+bool defer_advec_cell(global_variables &globals, int tile, int sweep, int dir);
+// advec_mom: recorded (xvel) or the recorded sweep run (yvel): true when handled.
+bool defer_advec_mom(global_variables &globals, int tile, int which_vel, int dir, int sweep);
 void flush_deferred();
 
 inline void check(clv_status st, const char *what) { clover::check_clv(st, what); }

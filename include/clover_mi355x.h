@@ -237,6 +237,16 @@ clv_status clv_PdV_driver(clv_chunk *c, int predict, double dt);
  * replaces clv_PdV_driver(c, 1, dt), clv_accelerate, clv_PdV_driver(c, 0, dt),
  * clv_flux_calc.  x_cells, y_cells >= 4. */
 clv_status clv_lagrange_kernels(clv_chunk *c, double dt);
+/* One sweep of driver/advection.cpp -- advec_cell_driver(sweep_number, direction),
+ * update_halo(density1, energy1, xvel1, yvel1, mass_flux_x, mass_flux_y, 2),
+ * advec_mom_driver(xvel), advec_mom_driver(yvel) (:47-73 for sweep 1, :75-103 for
+ * sweep 2) -- in one pass, for a driver that has already refreshed the xvel1/yvel1
+ * halo of depth 2 (and, before sweep 1, the energy1, density1, vol_flux_x/y halo of
+ * depth 2, :39-45).  density1, energy1, xvel1 and yvel1 come back in other buffers
+ * (re-read them with clv_chunk_get_view), the mass fluxes in place, every value as
+ * those calls leave it, ghosts included; the work arrays hold scratch.  The halo is
+ * the reflective one of every face.  x_cells, y_cells >= 4. */
+clv_status clv_advection_pass(clv_chunk *c, int sweep_number, int direction);
 /* void revert(global_variables&)                                    driver/revert.h:24 */
 clv_status clv_revert(clv_chunk *c);
 /* void accelerate(global_variables&)                                driver/accelerate.h:24 */

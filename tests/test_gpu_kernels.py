@@ -177,6 +177,37 @@ def test_lagrange_kernels_equal_the_entry_point_sequence(gpu, nx, ny):
             assert np.array_equal(a.download(n), b.download(n), equal_nan=True), n
 
 
+@pytest.mark.parametrize("sweep,direction", [(1, 1), (1, 2), (2, 1), (2, 2)])
+@pytest.mark.parametrize("nx,ny", [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130)])
+def test_advection_pass_equals_the_entry_point_sequence(gpu, nx, ny, sweep, direction):
+    """clv_advection_pass against advec_cell, update_halo(density1, energy1, xvel1,
+    yvel1, mass fluxes, 2), advec_mom(xvel), advec_mom(yvel) on the same arrays after
+    the same halos: every array but the work arrays bit for bit over its extent
+    (driver/definitions.cpp:4-41), ghosts included."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=13 * nx + ny + 7 * sweep + direction)
+    with clv.Chunk(nx, ny) as a, clv.Chunk(nx, ny) as b:
+        for ch in (a, b):
+            _load(ch, f, m)
+            ch.update_halo(["energy1", "density1", "vol_flux_x", "vol_flux_y", "xvel1", "yvel1"], 2)
+        a.advection_pass(sweep, direction)
+        b.advec_cell(sweep, direction)
+        b.update_halo(["density1", "energy1", "xvel1", "yvel1", "mass_flux_x", "mass_flux_y"], 2)
+        b.advec_mom(1, direction, sweep)
+        b.advec_mom(2, direction, sweep)
+        bad = []
+        for n in T.FIELDS:
+            if n.startswith("work"):
+                continue  # scratch of the reference's loops
+            # the reference's extent of the field (the exchanged buffers hold other
+            # values in the padding past it)
+            ex, ey = T.EXTENT[n]
+            x, y = a.download(n)[:ny + ey, :nx + ex], b.download(n)[:ny + ey, :nx + ex]
+            same = (x == y) | (np.isnan(x) & np.isnan(y))
+            if not same.all():
+                bad.append((n, [(tuple(int(v) for v in ij), float(x[tuple(ij)]), float(y[tuple(ij)])) for ij in np.argwhere(~same)[:4]]))
+        assert not bad, bad
+
+
 @pytest.mark.parametrize("nx,ny", SIZES)
 def test_summary_one_pass_equals_ideal_gas_then_sums(gpu, nx, ny):
     """clv_hydro_summary (ideal_gas and the sums in one pass) against clv_ideal_gas
