@@ -518,9 +518,11 @@ def cpu_baseline_reference(steps, deck="clover_bm16_300"):
 
 
 def dropin_leg(deck_text, steps):
-    """The drop-in with the model's one-launch entry points (the default) and with one
-    launch per reference loop (CLOVER_MI355X_ENTRY=reference)."""
+    """The drop-in as it runs by default (the model's chains of entry points as one
+    launch each), with one launch per entry point (CLOVER_MI355X_CHAINS=off), and with
+    one launch per reference loop (CLOVER_MI355X_ENTRY=reference)."""
     out = dropin_run(deck_text, steps, {})
+    out["one_launch_per_entry_point"] = dropin_run(deck_text, steps, {"CLOVER_MI355X_CHAINS": "off"})
     out["one_launch_per_reference_loop"] = dropin_run(deck_text, steps, {"CLOVER_MI355X_ENTRY": "reference"})
     return out
 
