@@ -128,6 +128,7 @@ SIGNATURES = [
     ("clv_timestep_kernels", _I, [_P, _D, _D, _D, _D, _D, _I, _PD, _PI, _PI, _PI, _PI]),
     ("clv_lagrange_kernels", _I, [_P, _D]),
     ("clv_advection_pass", _I, [_P, _I, _I]),
+    ("clv_advection_pass_reset", _I, [_P, _I, _I]),
     ("clv_PdV", _I, [_P, _I, _D]),
     ("clv_PdV_driver", _I, [_P, _I, _D]),
     ("clv_revert", _I, [_P]),
@@ -396,6 +397,11 @@ class Chunk:
         """advec_cell, the halo between, advec_mom for both velocities in one pass
         (clv_advection_pass)."""
         _check(self.L.clv_advection_pass(self.h, sweep_number, direction), "advection_pass")
+
+    def advection_pass_reset(self, sweep_number, direction):
+        """advection_pass followed by reset_field, the copy done by the pass's own stores
+        (clv_advection_pass_reset)."""
+        _check(self.L.clv_advection_pass_reset(self.h, sweep_number, direction), "advection_pass_reset")
 
     def PdV(self, predict, dt):
         _check(self.L.clv_PdV(self.h, int(bool(predict)), float(dt)), "PdV")

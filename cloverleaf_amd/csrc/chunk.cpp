@@ -361,7 +361,7 @@ clv_status clv_timestep_kernels(clv_chunk *c, double dtmin, double dtc_safe, dou
                                 int pressure_halo_depth, double *dt_min_val, int *dt_control, int *jldt, int *kldt, int *small) {
   CHECK_CHUNK(c);
   if (!dt_min_val) return set_error(CLV_ERR_ARG, "clv_timestep_kernels: null dt_min_val");
-  if (pressure_halo_depth != 1 && pressure_halo_depth != 2) return set_error(CLV_ERR_ARG, "clv_timestep_kernels: pressure_halo_depth must be 1 or 2");
+  if (pressure_halo_depth < 0 || pressure_halo_depth > 2) return set_error(CLV_ERR_ARG, "clv_timestep_kernels: pressure_halo_depth must be 0, 1 or 2");
   clv_status st = fused_timestep_entry(c, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, pressure_halo_depth, dt_min_val);
   if (st != CLV_OK) return st;
   // the bookkeeping of clv_calc_dt (src/serial/calc_dt.cpp:40,80-83)
@@ -380,7 +380,14 @@ clv_status clv_advection_pass(clv_chunk *c, int sweep_number, int direction) {
   CHECK_CHUNK(c);
   if ((sweep_number != 1 && sweep_number != 2) || (direction != CLV_XDIR && direction != CLV_YDIR))
     return set_error(CLV_ERR_ARG, "clv_advection_pass: sweep_number must be 1 or 2, direction CLV_XDIR or CLV_YDIR");
-  return fused_advection_entry(c, sweep_number, direction);
+  return fused_advection_entry(c, sweep_number, direction, false);
+}
+
+clv_status clv_advection_pass_reset(clv_chunk *c, int sweep_number, int direction) {
+  CHECK_CHUNK(c);
+  if ((sweep_number != 1 && sweep_number != 2) || (direction != CLV_XDIR && direction != CLV_YDIR))
+    return set_error(CLV_ERR_ARG, "clv_advection_pass_reset: sweep_number must be 1 or 2, direction CLV_XDIR or CLV_YDIR");
+  return fused_advection_entry(c, sweep_number, direction, true);
 }
 
 clv_status clv_lagrange_kernels(clv_chunk *c, double dt) {

@@ -597,8 +597,11 @@ __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor
 // lanes 2..61, face fluxes on 2..62, and advec_mom's stencil narrows that to 4..60.
 // March: cell row r (r from j0 - 1), then node row r from cell rows r - 1 and r.  The
 // block's results go out through LDS (BlockRows): its four strips own 224 whole-line
-// columns.
-template <int SWEEP, bool MESH, class V, class O>
+// columns.  RESET: the pass also stores its new density, energy and velocities into
+// density0, energy0, xvel0 and yvel0 -- reset_field (src/serial/reset_field.cpp:29-42:
+// cells [2, nx + 1] x [2, ny + 1], nodes [2, nx + 2] x [2, ny + 2]) -- over the
+// same cells and nodes as its own stores.
+template <int SWEEP, bool MESH, bool RESET = false, class V, class O>
 __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 1 : 3;
   const int lane = threadIdx.x & 63;
@@ -708,12 +711,20 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
         if (out.cell(t)) {
           ST_STREAM(rho_out[o], stage[r & 1][0][t]);
           ST_STREAM(e_out[o], stage[r & 1][1][t]);
+          if (RESET) {
+            ST_STREAM(AT(CLV_DENSITY0, xs, r), stage[r & 1][0][t]);
+            ST_STREAM(AT(CLV_ENERGY0, xs, r), stage[r & 1][1][t]);
+          }
         }
         if (out.node(t)) ST_STREAM(AT(CLV_MASS_FLUX_X, xs, r), stage[r & 1][2][t]);
       }
       if (out.node(t)) {
         ST_STREAM(u_out[o], stage[r & 1][3][t]);
         ST_STREAM(v_out[o], stage[r & 1][4][t]);
+        if (RESET) {
+          ST_STREAM(AT(CLV_XVEL0, xs, r), stage[r & 1][3][t]);
+          ST_STREAM(AT(CLV_YVEL0, xs, r), stage[r & 1][4][t]);
+        }
       }
     }
     mp_prev = mp;
@@ -722,12 +733,12 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
   __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
-template <int SWEEP, bool MESH>
+template <int SWEEP, bool MESH, bool RESET = false>
 __global__ __launch_bounds__(256) void cm_x_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  cm_x_tile<SWEEP, MESH>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+  cm_x_tile<SWEEP, MESH, RESET>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
 // y direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
@@ -771,7 +782,8 @@ struct CmRing {
   __device__ void put_post(int row, double v) const { put(kRingMf + kRingRho + row % kRingPost, v); }
   __device__ double post(int row) const { return get(kRingMf + kRingRho + row % kRingPost); }
 };
-template <int SWEEP, bool MESH, bool LOCKSTEP = true, class V, class O>
+// RESET as for cm_x_tile.
+template <int SWEEP, bool MESH, bool LOCKSTEP = true, bool RESET = false, class V, class O>
 __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 2 : 4;
   const int lane = threadIdx.x & 63;
@@ -882,6 +894,10 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
         if (own_cell && c >= j0 && c < j1 && c <= ny + 1) {
           rho_out[(int64_t)c * d.pitch + i] = rho_h;
           e_out[(int64_t)c * d.pitch + i] = post_ener_s;
+          if (RESET) {
+            AT(CLV_DENSITY0, i, c) = rho_h;
+            AT(CLV_ENERGY0, i, c) = post_ener_s;
+          }
         }
       }
       mf_prev = mf;
@@ -930,8 +946,13 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
         }
         if (k >= j0 + 1 && own_node) {
           // update node row f from faces f - 1 and f (:221)
-          u_out[(int64_t)f * d.pitch + i] = (u_m1 * pre_prev + mu_prev - mu) / post_prev;
-          v_out[(int64_t)f * d.pitch + i] = (v_m1 * pre_prev + mv_prev - mv) / post_prev;
+          const double u_new = (u_m1 * pre_prev + mu_prev - mu) / post_prev, v_new = (v_m1 * pre_prev + mv_prev - mv) / post_prev;
+          u_out[(int64_t)f * d.pitch + i] = u_new;
+          v_out[(int64_t)f * d.pitch + i] = v_new;
+          if (RESET) {
+            AT(CLV_XVEL0, i, f) = u_new;
+            AT(CLV_YVEL0, i, f) = v_new;
+          }
         }
         mu_prev = mu;
         mv_prev = mv;
@@ -959,12 +980,12 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   }
 }
 
-template <int SWEEP, bool MESH>
+template <int SWEEP, bool MESH, bool RESET = false>
 __global__ __launch_bounds__(256, 4) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  cm_y_tile<SWEEP, MESH>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+  cm_y_tile<SWEEP, MESH, true, RESET>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
 // ---- batched form: persistent launches over the tiles of many chunks ----------------------
@@ -1247,15 +1268,16 @@ clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *d
 // driver/timestep.cpp:47-69 for the per-entry-point path, the volume and face areas
 // and density0, energy0, xvel0, yvel0 halos already in place: pressure, soundspeed
 // and viscosity written as ideal_gas and viscosity write them, then the pressure
-// halo of depth p_depth and viscosity's of depth 1, then the dt minimum.  Any chunk
-// (a wrapped one too): nothing is renamed.
+// halo of depth p_depth and viscosity's of depth 1, then the dt minimum.  p_depth 0:
+// pressure and soundspeed are not stored (nor the pressure halo run), for a caller
+// whose next pass rewrites them.  Any chunk (a wrapped one too): nothing is renamed.
 clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, int p_depth,
                                 double *dt_min_val) {
   if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "clv_timestep_kernels needs x_cells, y_cells >= 4");
   choose_segments(c);
-  clv_status st = timestep_launch(c, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, true);
+  clv_status st = timestep_launch(c, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, p_depth > 0);
   if (st != CLV_OK) return st;
-  if ((st = halo(c, {CLV_PRESSURE}, p_depth)) != CLV_OK) return st;
+  if (p_depth > 0 && (st = halo(c, {CLV_PRESSURE}, p_depth)) != CLV_OK) return st;
   if ((st = halo(c, {CLV_VISCOSITY}, 1)) != CLV_OK) return st;
   if ((st = hip_check(hipStreamSynchronize(c->stream), "fused dt sync")) != CLV_OK) return st;
   *dt_min_val = c->result_host[0];
@@ -1294,23 +1316,26 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
 // advec_cell and advec_mom of one sweep in one launch (cm_x_k, cm_y_k); the halo
 // update the driver runs between the two (driver/advection.cpp:42-46) is split: the
 // velocities' before (the cell sweep does not change them), the rest after (the mom
-// sweep does not change those), and the kernels read the ghosts they need by mirror
-static clv_status cm_sweep(clv_chunk *c, int sweep, int dir) {
+// sweep does not change those), and the kernels read the ghosts they need by mirror.
+// reset: the pass also does reset_field's stores (the kernels' RESET form).
+using CmKernel = void (*)(Dev, int, int, double *, double *, double *, double *);
+template <bool MESH, bool RESET>
+static CmKernel cm_kernel(int sweep, int dir) {
+  if (dir == CLV_XDIR) return sweep == 1 ? cm_x_k<1, MESH, RESET> : cm_x_k<2, MESH, RESET>;
+  return sweep == 1 ? cm_y_k<1, MESH, RESET> : cm_y_k<2, MESH, RESET>;
+}
+
+static clv_status cm_sweep(clv_chunk *c, int sweep, int dir, bool reset = false) {
   const Dev &d = c->d;
   double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2], *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
-  if (dir == CLV_XDIR) {
-    const int ns = ceil_div(d.nx + 1, kOwnCmX), seg = c->seg[SF_CM_X];
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
-    const auto k1 = d.geom_mesh ? cm_x_k<1, true> : cm_x_k<1, false>, k2 = d.geom_mesh ? cm_x_k<2, true> : cm_x_k<2, false>;
-    if (sweep == 1) LAUNCH_F("f_cm_x1", k1, g, d, ns, seg, rho_out, e_out, u_out, v_out);
-    else LAUNCH_F("f_cm_x2", k2, g, d, ns, seg, rho_out, e_out, u_out, v_out);
-  } else {
-    const int ns = ceil_div(d.nx + 1, kOwnCmY), seg = c->seg[SF_CM_Y];
-    const dim3 g(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1);
-    const auto k1 = d.geom_mesh ? cm_y_k<1, true> : cm_y_k<1, false>, k2 = d.geom_mesh ? cm_y_k<2, true> : cm_y_k<2, false>;
-    if (sweep == 1) LAUNCH_F("f_cm_y1", k1, g, d, ns, seg, rho_out, e_out, u_out, v_out);
-    else LAUNCH_F("f_cm_y2", k2, g, d, ns, seg, rho_out, e_out, u_out, v_out);
-  }
+  const CmKernel kern = d.geom_mesh ? (reset ? cm_kernel<true, true>(sweep, dir) : cm_kernel<true, false>(sweep, dir))
+                                    : (reset ? cm_kernel<false, true>(sweep, dir) : cm_kernel<false, false>(sweep, dir));
+  static const char *const names[2][2][2] = {{{"f_cm_x1", "f_cm_x1_reset"}, {"f_cm_x2", "f_cm_x2_reset"}},
+                                             {{"f_cm_y1", "f_cm_y1_reset"}, {"f_cm_y2", "f_cm_y2_reset"}}};
+  const char *name = names[dir == CLV_XDIR ? 0 : 1][sweep == 1 ? 0 : 1][reset ? 1 : 0];
+  const int own = dir == CLV_XDIR ? kOwnCmX : kOwnCmY, seg = c->seg[dir == CLV_XDIR ? SF_CM_X : SF_CM_Y];
+  const int ns = ceil_div(d.nx + 1, own);
+  LAUNCH_F(name, kern, dim3(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1), d, ns, seg, rho_out, e_out, u_out, v_out);
   swap_fields(c, CLV_DENSITY1, CLV_WORK1);
   swap_fields(c, CLV_ENERGY1, CLV_WORK2);
   swap_fields(c, CLV_XVEL1, CLV_WORK3);
@@ -1324,12 +1349,13 @@ static clv_status cm_sweep(clv_chunk *c, int sweep, int dir) {
 // advec_mom for xvel1 and yvel1, in one pass.  The new fields are exchanged with work
 // arrays (the caller re-points its handles); the velocities' ghost frames are carried
 // over first, so that they hold the halo the reference's advec_mom leaves untouched.
-clv_status fused_advection_entry(clv_chunk *c, int sweep, int dir) {
+// reset: reset_field (src/serial/reset_field.cpp) follows, its stores done by the pass.
+clv_status fused_advection_entry(clv_chunk *c, int sweep, int dir, bool reset) {
   if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "clv_advection_pass needs x_cells, y_cells >= 4");
   choose_segments(c);
   clv_status st = k_copy_frame(c, CLV_XVEL1, CLV_WORK3);
   if (st == CLV_OK) st = k_copy_frame(c, CLV_YVEL1, CLV_WORK4);
-  if (st == CLV_OK) st = cm_sweep(c, sweep, dir);
+  if (st == CLV_OK) st = cm_sweep(c, sweep, dir, reset);
   if (st != CLV_OK) return st;
   return halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2);
 }

@@ -73,7 +73,7 @@ void external_faces(const global_variables &globals, int tile, int external[4]) 
 
 namespace {
 // The calls recorded for a one-launch chain (one tile, see mi355x_model.h), in order.
-enum RecOp { R_GAS, R_P_HALO, R_VISC, R_Q_HALO, R_PDV_PREDICT, R_ACCEL, R_PDV_CORRECT, R_CELL, R_ADV_HALO, R_MOM_X };
+enum RecOp { R_GAS, R_P_HALO, R_VISC, R_Q_HALO, R_PDV_PREDICT, R_ACCEL, R_PDV_CORRECT, R_CELL, R_ADV_HALO, R_MOM_X, R_MOM_Y };
 struct Rec {
   RecOp op;
   double arg;         // halo depth or dt
@@ -82,6 +82,20 @@ struct Rec {
 clv_chunk *g_rec_chunk = nullptr;
 global_variables *g_rec_globals = nullptr;
 std::vector<Rec> g_rec;
+// The chunk whose pressure (with its depth-1 halo) and soundspeed the timestep chain
+// left unwritten, since the Lagrangian chain rewrites them; anything else that
+// touches the tile first has them written as ideal_gas and the halo would have.
+clv_chunk *g_pc_owed = nullptr;
+
+void settle_pressure() {
+  if (!g_pc_owed) return;
+  clv_chunk *c = g_pc_owed;
+  g_pc_owed = nullptr;
+  int f[CLV_NUM_HALO_FIELDS] = {0};
+  f[CLV_PRESSURE] = 1;
+  check(clv_ideal_gas(c, 0), "ideal_gas");
+  check(clv_update_halo(c, f, 1), "update_halo");
+}
 
 bool chain_allowed(const global_variables &globals) {
   const char *e = std::getenv("CLOVER_MI355X_CHAINS");
@@ -101,9 +115,11 @@ bool recorded(std::initializer_list<RecOp> ops) {
   return true;
 }
 
-// start recording on tile 0 (after running what an earlier record left)
+// start recording on tile 0 (after running what an earlier record left; the
+// Lagrangian chain keeps an owed pressure, which it rewrites)
 bool start_record(global_variables &globals, RecOp op, double arg, int sweep = 0, int dir = 0) {
-  clv_chunk *c = chunk(globals, 0);
+  if (!g_rec.empty() || op != R_PDV_PREDICT) flush_deferred();
+  clv_chunk *c = chunk_keep(globals, 0);
   g_rec_chunk = c;
   g_rec_globals = &globals;
   g_rec.push_back({op, arg, sweep, dir});
@@ -123,10 +139,16 @@ bool is_advection_halo(const int fields[CLV_NUM_HALO_FIELDS], int depth) {
 }  // namespace
 
 void flush_deferred() {
+  settle_pressure();
   if (g_rec.empty()) return;
   const std::vector<Rec> rec = g_rec;
   clv_chunk *c = g_rec_chunk;
   g_rec.clear();
+  if (rec.back().op == R_MOM_Y) {  // a whole recorded sweep: the pass as advec_mom(yvel) would have run it
+    check(clv_advection_pass(c, rec[0].sweep, rec[0].dir), "advection_pass");
+    refresh_buffers(*g_rec_globals, 0);
+    return;
+  }
   for (const Rec &r : rec) {
     int f[CLV_NUM_HALO_FIELDS] = {0};
     switch (r.op) {
@@ -155,6 +177,7 @@ void flush_deferred() {
         check(clv_advec_mom(c, 1, r.dir, r.sweep), "advec_mom");
         refresh_buffers(*g_rec_globals, 0);
         break;
+      case R_MOM_Y: break;  // only ever last in a record, handled above
     }
   }
 }
@@ -174,7 +197,10 @@ bool defer_viscosity(global_variables &globals) {
 
 void defer_halo(global_variables &globals, int fields[CLV_NUM_HALO_FIELDS], int depth) {
   (void)globals;
-  if (g_rec.empty()) return;
+  if (g_rec.empty()) {
+    settle_pressure();  // an owed pressure gets its halo before any other halo call
+    return;
+  }
   if (recorded({R_GAS}) && fields[CLV_PRESSURE] && !fields[CLV_VISCOSITY] && !fields[CLV_SOUNDSPEED]) {
     fields[CLV_PRESSURE] = 0;
     g_rec.push_back({R_P_HALO, (double)depth});
@@ -205,9 +231,13 @@ bool deferred_calc_dt(global_variables &globals, int tile, double &dt, int &cont
   const global_config &cfg = globals.config;
   const int p_depth = (int)g_rec[1].arg;
   g_rec.clear();
-  check(clv_timestep_kernels(g_rec_chunk, cfg.dtmin, cfg.dtc_safe, cfg.dtu_safe, cfg.dtv_safe, cfg.dtdiv_safe, p_depth, &dt, &control,
-                             &jldt, &kldt, &small),
+  // with the Lagrangian chain next (driver/hydro.cpp:59-63), pressure and soundspeed
+  // are owed rather than stored: that chain rewrites both over the same cells
+  const bool owe = p_depth == 1;
+  check(clv_timestep_kernels(g_rec_chunk, cfg.dtmin, cfg.dtc_safe, cfg.dtu_safe, cfg.dtv_safe, cfg.dtdiv_safe, owe ? 0 : p_depth, &dt,
+                             &control, &jldt, &kldt, &small),
         "timestep_kernels");
+  if (owe) g_pc_owed = g_rec_chunk;
   return true;
 }
 
@@ -234,6 +264,8 @@ bool deferred_flux_calc(global_variables &globals) {
     return false;
   }
   g_rec.clear();
+  if (g_pc_owed && g_pc_owed != g_rec_chunk) settle_pressure();
+  g_pc_owed = nullptr;  // the chain stores the predictor's pressure, its halo and soundspeed
   check(clv_lagrange_kernels(g_rec_chunk, globals.dt), "lagrange_kernels");
   return true;
 }
@@ -252,6 +284,10 @@ bool defer_advec_mom(global_variables &globals, int tile, int which_vel, int dir
     return true;
   }
   if (which_vel == 2 && recorded({R_CELL, R_ADV_HALO, R_MOM_X})) {
+    if (sweep == 2) {  // reset_field comes next in driver/hydro.cpp:79-83
+      g_rec.push_back({R_MOM_Y, 0, sweep, dir});
+      return true;
+    }
     g_rec.clear();
     check(clv_advection_pass(g_rec_chunk, sweep, dir), "advection_pass");
     refresh_buffers(globals, 0);  // density1, energy1 and the velocities are other arrays now
@@ -260,8 +296,17 @@ bool defer_advec_mom(global_variables &globals, int tile, int which_vel, int dir
   return false;
 }
 
+bool deferred_reset_field(global_variables &globals, int tile) {
+  if (tile != 0 || !recorded({R_CELL, R_ADV_HALO, R_MOM_X, R_MOM_Y})) return false;
+  const Rec r = g_rec[0];
+  g_rec.clear();
+  check(clv_advection_pass_reset(g_rec_chunk, r.sweep, r.dir), "advection_pass_reset");
+  refresh_buffers(globals, 0);
+  return true;
+}
+
 clv_chunk *chunk(global_variables &globals, int tile) {
-  if (!g_rec.empty()) flush_deferred();
+  flush_deferred();
   return chunk_keep(globals, tile);
 }
 

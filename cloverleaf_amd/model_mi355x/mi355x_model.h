@@ -35,7 +35,7 @@ bool fused_entries(const global_variables &globals);
 // driver's Buffer2D handles at the arrays the chunk now uses.
 void refresh_buffers(global_variables &globals, int tile);
 
-// Two chains of entry points run as one launch each on a single tile of a single chunk
+// Three chains of entry points run as one launch each on a single tile of a single chunk
 // (without the profiler, with the one-launch entry points; CLOVER_MI355X_CHAINS=off
 // turns them off):
 // - the timestep, driver/timestep.cpp:47-69: ideal_gas(predict = false), the halo
@@ -46,10 +46,15 @@ void refresh_buffers(global_variables &globals, int tile);
 // - each advection sweep, driver/advection.cpp:47-73 and :75-103: advec_cell, the
 //   halo update of density1, energy1, the velocities and the mass fluxes, advec_mom
 //   for xvel1, then advec_mom for yvel1 runs them (clv_advection_pass); the
-//   velocities' halo runs at once (advec_cell does not change them).
+//   velocities' halo runs at once (advec_cell does not change them).  The second
+//   sweep stays recorded until the next call: reset_field runs it with reset_field's
+//   copy done by the pass's own stores (clv_advection_pass_reset), anything else
+//   runs it as clv_advection_pass first.
 // The calls before the last are recorded.  Anything else that touches a tile first
 // runs what is recorded one entry point at a time (flush_deferred), so the arrays are
-// the reference's whenever they can be seen.
+// the reference's whenever they can be seen.  The timestep chain leaves pressure and
+// soundspeed unwritten when the Lagrangian chain, which rewrites them, is next; the
+// first other call writes them (ideal_gas and the depth-1 pressure halo) before it runs.
 bool defer_ideal_gas(global_variables &globals, int tile, bool predict);
 bool defer_viscosity(global_variables &globals);
 // Takes pressure / viscosity out of a halo request when their halo can be recorded.
@@ -61,8 +66,11 @@ bool defer_accelerate(global_variables &globals);
 // flux_calc of the recorded Lagrangian chain: true when it ran.
 bool deferred_flux_calc(global_variables &globals);
 bool defer_advec_cell(global_variables &globals, int tile, int sweep, int dir);
-// advec_mom: recorded (xvel) or the recorded sweep run (yvel): true when handled.
+// advec_mom: recorded (xvel, and yvel of the second sweep) or the recorded sweep run
+// (yvel of the first): true when handled.
 bool defer_advec_mom(global_variables &globals, int tile, int which_vel, int dir, int sweep);
+// reset_field of a recorded second sweep: true when it ran (clv_advection_pass_reset).
+bool deferred_reset_field(global_variables &globals, int tile);
 void flush_deferred();
 
 inline void check(clv_status st, const char *what) { clover::check_clv(st, what); }

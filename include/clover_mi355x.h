@@ -219,7 +219,11 @@ clv_status clv_calc_dt(clv_chunk *c, double dtmin, double dtc_safe, double dtu_s
  * and the viscosity halo of depth 1 (the reflective halo of every face, as
  * clv_update_halo); outputs as clv_calc_dt.  It replaces the sequence clv_ideal_gas,
  * clv_update_halo(pressure...), clv_viscosity, clv_update_halo(viscosity, 1),
- * clv_calc_dt with the same arrays and dt.  x_cells, y_cells >= 4. */
+ * clv_calc_dt with the same arrays and dt.  pressure_halo_depth 0: pressure and
+ * soundspeed are left as they were and no pressure halo runs (viscosity and dt as
+ * above), for a caller whose next call rewrites both before anything reads them --
+ * clv_lagrange_kernels stores the predictor's pressure with its depth-1 halo and
+ * soundspeed over the same cells.  x_cells, y_cells >= 4. */
 clv_status clv_timestep_kernels(clv_chunk *c, double dtmin, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
                                 int pressure_halo_depth, double *dt_min_val, int *dt_control, int *jldt, int *kldt, int *small);
 /* The kernel of void PdV(global_variables&, bool predict)           driver/PdV.h:24
@@ -247,6 +251,12 @@ clv_status clv_lagrange_kernels(clv_chunk *c, double dt);
  * those calls leave it, ghosts included; the work arrays hold scratch.  The halo is
  * the reflective one of every face.  x_cells, y_cells >= 4. */
 clv_status clv_advection_pass(clv_chunk *c, int sweep_number, int direction);
+/* clv_advection_pass followed by void reset_field(global_variables&)   driver/reset_field.h:24
+ * (src/serial/reset_field.cpp:29-42): the pass also stores its new density, energy and
+ * velocities into density0, energy0, xvel0 and yvel0 (interior cells and nodes, as
+ * reset_field copies them), so that no separate copy pass runs.  Every array as
+ * clv_advection_pass then clv_reset_field leave it. */
+clv_status clv_advection_pass_reset(clv_chunk *c, int sweep_number, int direction);
 /* void revert(global_variables&)                                    driver/revert.h:24 */
 clv_status clv_revert(clv_chunk *c);
 /* void accelerate(global_variables&)                                driver/accelerate.h:24 */

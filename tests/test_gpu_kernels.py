@@ -154,6 +154,34 @@ def test_timestep_kernels_equal_the_entry_point_sequence(gpu, nx, ny, depth):
 
 
 @pytest.mark.parametrize("nx,ny", [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130)])
+def test_timestep_kernels_without_pressure_then_lagrange(gpu, nx, ny):
+    """clv_timestep_kernels with pressure_halo_depth 0 stores neither pressure nor
+    soundspeed (viscosity and dt as with depth 1); followed by clv_lagrange_kernels,
+    which rewrites both, every array equals the depth-1 form followed by the same
+    call, bit for bit."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=3 * nx + ny + 1)
+    dt = 1.1e-3
+    with clv.Chunk(nx, ny) as a, clv.Chunk(nx, ny) as b:
+        for ch in (a, b):
+            _load(ch, f, m)
+            ch.update_halo(["energy0", "density0", "xvel0", "yvel0"], 1)
+        got = a.timestep_kernels(0)
+        want = b.timestep_kernels(1)
+        assert got == want
+        for n in T.FIELDS:
+            if n in ("pressure", "soundspeed"):
+                assert np.array_equal(a.download(n), f[n], equal_nan=True), n  # left as they were
+            else:
+                assert np.array_equal(a.download(n), b.download(n), equal_nan=True), n
+        a.lagrange_kernels(dt)
+        b.lagrange_kernels(dt)
+        for n in T.FIELDS:
+            if n.startswith("work"):
+                continue
+            assert np.array_equal(a.download(n), b.download(n), equal_nan=True), n
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130)])
 def test_lagrange_kernels_equal_the_entry_point_sequence(gpu, nx, ny):
     """clv_lagrange_kernels against PdV(predict) chain, accelerate, PdV(correct),
     flux_calc on the same arrays, after the same timestep (its halos and pressure):
@@ -206,6 +234,36 @@ def test_advection_pass_equals_the_entry_point_sequence(gpu, nx, ny, sweep, dire
             if not same.all():
                 bad.append((n, [(tuple(int(v) for v in ij), float(x[tuple(ij)]), float(y[tuple(ij)])) for ij in np.argwhere(~same)[:4]]))
         assert not bad, bad
+
+
+@pytest.mark.parametrize("sweep,direction", [(1, 1), (1, 2), (2, 1), (2, 2)])
+@pytest.mark.parametrize("nx,ny", [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130)])
+def test_advection_pass_reset_equals_pass_then_reset_field(gpu, nx, ny, sweep, direction):
+    """clv_advection_pass_reset against clv_advection_pass followed by clv_reset_field
+    (src/serial/reset_field.cpp:29-42): every array but the work arrays bit for bit
+    over its extent, the ghost frames of density0, energy0, xvel0 and yvel0 (which
+    reset_field leaves alone) included."""
+    f, m = T.random_chunk_arrays(nx, ny, seed=11 * nx + ny + 5 * sweep + direction)
+    with clv.Chunk(nx, ny) as a, clv.Chunk(nx, ny) as b:
+        for ch in (a, b):
+            _load(ch, f, m)
+            ch.update_halo(["energy1", "density1", "vol_flux_x", "vol_flux_y", "xvel1", "yvel1"], 2)
+        a.advection_pass_reset(sweep, direction)
+        b.advection_pass(sweep, direction)
+        b.reset_field()
+        bad = []
+        for n in T.FIELDS:
+            if n.startswith("work"):
+                continue
+            ex, ey = T.EXTENT[n]
+            x, y = a.download(n)[:ny + ey, :nx + ex], b.download(n)[:ny + ey, :nx + ex]
+            same = (x == y) | (np.isnan(x) & np.isnan(y))
+            if not same.all():
+                bad.append((n, [(tuple(int(v) for v in ij), float(x[tuple(ij)]), float(y[tuple(ij)])) for ij in np.argwhere(~same)[:4]]))
+        assert not bad, bad
+        # the copy really happened: density0 now holds the advected density
+        x0, x1 = a.download("density0"), a.download("density1")
+        assert np.array_equal(x0[2:ny + 2, 2:nx + 2], x1[2:ny + 2, 2:nx + 2])
 
 
 @pytest.mark.parametrize("nx,ny", SIZES)
