@@ -4,13 +4,18 @@
 //
 // Forward.  The fixed-dt step is driver/timestep.cpp:47-69 without calc_dt, then
 // driver/hydro.cpp:63-89, on the reference-plan kernels (one launch per
-// reference loop), which leave every intermediate the adjoint needs in its array.
+// reference loop), which leave every intermediate the adjoint needs in its array;
+// each advection sweep runs as one fused pass (advec_cell, the halo between,
+// advec_mom for both velocities: fused_advection_entry), whose outputs are those of
+// the loops bit for bit.  The adjoint kernels recompute the loops' intermediates
+// from their inputs, so only the inputs are logged.
 //
 // Checkpointing (two levels).  The forward pass stores the state (density0,
 // energy0, xvel0, yvel0, whole arrays) every `every` steps.  The backward pass
 // walks the segments last to first: it restores the segment's checkpoint,
 // re-runs the segment recording an undo log -- before each kernel that
-// overwrites arrays, a copy of those arrays -- and then walks the log backwards:
+// overwrites arrays, a copy of those arrays, or the old buffers themselves where a
+// pass writes new ones -- and then walks the log backwards:
 // restore the kernel's input arrays, run its adjoint.  Halo updates need no
 // copy (their adjoint is the transposed reflection and no adjoint reads a value
 // a later halo overwrote).  Memory: (steps / every + 1) x 4 state arrays plus
@@ -92,7 +97,8 @@ struct clv_adjoint {
 namespace {
 
 const int kState[4] = {CLV_DENSITY0, CLV_ENERGY0, CLV_XVEL0, CLV_YVEL0};
-// arrays copied into the undo log per step (forward_step's logged() lists)
+// undo-log slots per step (forward_step's logged() lists: 15 copies, and 8 buffers
+// the advection passes hand over)
 constexpr int kSnapsPerStep = 23;
 
 clv_status copy_array(clv_adjoint *g, double *dst, const double *src) {
@@ -116,8 +122,9 @@ clv_status load_state(clv_adjoint *g, int k) {
   return CLV_OK;
 }
 
-// record (when logging) then run one forward kernel
-clv_status logged(clv_adjoint *g, bool rec, LogEntry e, std::initializer_list<int> snap) {
+// record (when logging) then run one forward kernel: `snap` copied into the log now;
+// the slots of `later` reserved after them, for the caller to fill by buffer exchange
+clv_status logged(clv_adjoint *g, bool rec, LogEntry e, std::initializer_list<int> snap, std::initializer_list<int> later = {}) {
   if (!rec) return CLV_OK;
   e.nsnap = 0;
   e.slot = g->log_used;
@@ -125,6 +132,11 @@ clv_status logged(clv_adjoint *g, bool rec, LogEntry e, std::initializer_list<in
     if (g->log_used >= g->log_slots) return set_error(CLV_ERR_ARG, "adjoint: undo log full");
     clv_status st = copy_array(g, g->slot[g->log_used], base(g->p, f));
     if (st != CLV_OK) return st;
+    e.snap[e.nsnap++] = f;
+    ++g->log_used;
+  }
+  for (int f : later) {
+    if (g->log_used >= g->log_slots) return set_error(CLV_ERR_ARG, "adjoint: undo log full");
     e.snap[e.nsnap++] = f;
     ++g->log_used;
   }
@@ -140,19 +152,47 @@ LogEntry entry(OpKind k, int i0 = 0, int i1 = 0) {
   return e;
 }
 
-clv_status halo_op(clv_adjoint *g, bool rec, std::initializer_list<int> ids, int depth) {
-  LogEntry e = entry(OP_HALO, depth);
-  for (int id : ids) e.fields[id] = 1;
-  clv_status st = logged(g, rec, e, {});
-  if (st != CLV_OK) return st;
-  return k_update_halo(g->p, e.fields, depth);
-}
-
 #define TRY_A(x)                       \
   do {                                 \
     clv_status s_ = (x);               \
     if (s_ != CLV_OK) return s_;       \
   } while (0)
+
+clv_status halo_op(clv_adjoint *g, bool rec, std::initializer_list<int> ids, int depth, bool run = true) {
+  LogEntry e = entry(OP_HALO, depth);
+  for (int id : ids) e.fields[id] = 1;
+  clv_status st = logged(g, rec, e, {});
+  if (st != CLV_OK || !run) return st;
+  return k_update_halo(g->p, e.fields, depth);
+}
+
+// One advection sweep, driver/advection.cpp:47-73 (sweep 1) or :75-103 (sweep 2)
+// after its leading halo: advec_cell, update_halo(density1, energy1, xvel1, yvel1,
+// mass fluxes, 2), advec_mom(xvel1), advec_mom(yvel1), logged in that order and run
+// as one pass.  The velocities' halo runs first (advec_cell does not change them),
+// so both snapshots -- advec_cell's inputs, then advec_mom's velocities -- are the
+// arrays before the pass.  The pass writes the new density1, energy1, xvel1, yvel1
+// into the work arrays and exchanges the buffers, leaving the old ones, unchanged,
+// as work1..work4: those become the snapshots (no copy); the mass flux is copied.
+clv_status advection_sweep(clv_adjoint *g, bool rec, int sweep, int dir) {
+  clv_chunk *p = g->p;
+  int vel[CLV_NUM_HALO_FIELDS] = {0};
+  vel[CLV_XVEL1] = vel[CLV_YVEL1] = 1;
+  TRY_A(k_update_halo(p, vel, 2));
+  const size_t cell_slot = g->log_used;
+  TRY_A(logged(g, rec, entry(OP_CELL, sweep, dir), {dir == CLV_XDIR ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y}, {CLV_DENSITY1, CLV_ENERGY1}));
+  TRY_A(halo_op(g, rec, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2, false));
+  const size_t mom_slot = g->log_used;
+  TRY_A(logged(g, rec, entry(OP_MOM, sweep, dir), {}, {CLV_XVEL1, CLV_YVEL1}));
+  TRY_A(fused_advection_entry(p, sweep, dir, false));
+  if (rec) {
+    std::swap(g->slot[cell_slot + 1], p->d.f[CLV_WORK1]);  // density1 before the pass
+    std::swap(g->slot[cell_slot + 2], p->d.f[CLV_WORK2]);  // energy1
+    std::swap(g->slot[mom_slot], p->d.f[CLV_WORK3]);       // xvel1 after its halo
+    std::swap(g->slot[mom_slot + 1], p->d.f[CLV_WORK4]);   // yvel1
+  }
+  return CLV_OK;
+}
 
 // one fixed-dt step on the reference-plan kernels; advect_x alternates from 1
 clv_status forward_step(clv_adjoint *g, int n, bool rec) {
@@ -186,18 +226,8 @@ clv_status forward_step(clv_adjoint *g, int n, bool rec) {
   // driver/advection.cpp:29-105
   const int d1 = advect_x ? CLV_XDIR : CLV_YDIR, d2 = advect_x ? CLV_YDIR : CLV_XDIR;
   TRY_A(halo_op(g, rec, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}, 2));
-  TRY_A(logged(g, rec, entry(OP_CELL, 1, d1), {CLV_DENSITY1, CLV_ENERGY1, d1 == CLV_XDIR ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y}));
-  TRY_A(k_advec_cell(p, 1, d1));
-  TRY_A(halo_op(g, rec, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2));
-  TRY_A(logged(g, rec, entry(OP_MOM, 1, d1), {CLV_XVEL1, CLV_YVEL1}));
-  TRY_A(k_advec_mom(p, 1, d1, 1));
-  TRY_A(k_advec_mom(p, 2, d1, 1));
-  TRY_A(logged(g, rec, entry(OP_CELL, 2, d2), {CLV_DENSITY1, CLV_ENERGY1, d2 == CLV_XDIR ? CLV_MASS_FLUX_X : CLV_MASS_FLUX_Y}));
-  TRY_A(k_advec_cell(p, 2, d2));
-  TRY_A(halo_op(g, rec, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2));
-  TRY_A(logged(g, rec, entry(OP_MOM, 2, d2), {CLV_XVEL1, CLV_YVEL1}));
-  TRY_A(k_advec_mom(p, 1, d2, 2));
-  TRY_A(k_advec_mom(p, 2, d2, 2));
+  TRY_A(advection_sweep(g, rec, 1, d1));
+  TRY_A(advection_sweep(g, rec, 2, d2));
   TRY_A(logged(g, rec, entry(OP_RESET), {CLV_DENSITY0, CLV_ENERGY0, CLV_XVEL0, CLV_YVEL0}));
   TRY_A(k_reset_field(p));
   return CLV_OK;
