@@ -3,12 +3,13 @@
 // density and energy fields.
 //
 // Forward.  The fixed-dt step is driver/timestep.cpp:47-69 without calc_dt, then
-// driver/hydro.cpp:63-89, on the reference-plan kernels (one launch per
-// reference loop), which leave every intermediate the adjoint needs in its array;
-// each advection sweep runs as one fused pass (advec_cell, the halo between,
-// advec_mom for both velocities: fused_advection_entry), whose outputs are those of
-// the loops bit for bit.  The adjoint kernels recompute the loops' intermediates
-// from their inputs, so only the inputs are logged.
+// driver/hydro.cpp:63-89.  The timestep runs on the reference-plan kernels (one
+// launch per reference loop); the Lagrangian phase (PdV predictor chain,
+// accelerate, PdV corrector, flux_calc) as one fused pass (fused_lagrange_entry),
+// and each advection sweep as one (advec_cell, the halo between, advec_mom for both
+// velocities: fused_advection_entry): their arrays are those of the loops bit for
+// bit.  The adjoint kernels recompute the loops' intermediates from their inputs,
+// so only the inputs are logged.
 //
 // Checkpointing (two levels).  The forward pass stores the state (density0,
 // energy0, xvel0, yvel0, whole arrays) every `every` steps.  The backward pass
@@ -206,23 +207,25 @@ clv_status forward_step(clv_adjoint *g, int n, bool rec) {
   TRY_A(logged(g, rec, entry(OP_VISC), {CLV_VISCOSITY}));
   TRY_A(k_viscosity(p));
   TRY_A(halo_op(g, rec, {CLV_VISCOSITY}, 1));
-  // PdV predictor chain, src/serial/PdV.cpp:113-132
+  // PdV predictor chain (src/serial/PdV.cpp:113-132: PdV, ideal_gas, the pressure
+  // halo, revert), accelerate, the PdV corrector and flux_calc, logged in that order
+  // and run as one pass (fused_lagrange_entry), whose arrays are those of the
+  // kernels bit for bit.  Every snapshot but revert's is an array before the pass;
+  // revert's is density1 / energy1 as PdV(predict) leaves them: the arrays before
+  // the pass (the ghosts, which PdV leaves alone), their interior written by the
+  // pass.  Accelerate and the corrector overwrite u1, v1 and rho1, e1, but no
+  // kernel adjoint reads those values before revert (this step) and advec_mom (the
+  // previous step) restore their own inputs: nothing to log.
+  const size_t revert_slot = g->log_used + 3;
   TRY_A(logged(g, rec, entry(OP_PDV, 1), {CLV_DENSITY1, CLV_ENERGY1}));
-  TRY_A(k_PdV(p, 1, dt));
   TRY_A(logged(g, rec, entry(OP_IDEAL, 1), {CLV_PRESSURE}));
-  TRY_A(k_ideal_gas(p, 1));
-  TRY_A(halo_op(g, rec, {CLV_PRESSURE}, 1));
+  TRY_A(halo_op(g, rec, {CLV_PRESSURE}, 1, false));
   TRY_A(logged(g, rec, entry(OP_REVERT), {CLV_DENSITY1, CLV_ENERGY1}));
-  TRY_A(k_revert(p));
-  // accelerate and the PdV corrector overwrite u1, v1 and rho1, e1, but no kernel
-  // adjoint reads those values before revert (this step) and advec_mom (the
-  // previous step) restore their own inputs: nothing to log
   TRY_A(logged(g, rec, entry(OP_ACCEL), {}));
-  TRY_A(k_accelerate(p, dt));
   TRY_A(logged(g, rec, entry(OP_PDV, 0), {}));
-  TRY_A(k_PdV(p, 0, dt));
   TRY_A(logged(g, rec, entry(OP_FLUX), {CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}));
-  TRY_A(k_flux_calc(p, dt));
+  if (rec) TRY_A(fused_lagrange_entry(p, dt, g->slot[revert_slot], g->slot[revert_slot + 1]));
+  else TRY_A(fused_lagrange_entry(p, dt));
   // driver/advection.cpp:29-105
   const int d1 = advect_x ? CLV_XDIR : CLV_YDIR, d2 = advect_x ? CLV_YDIR : CLV_XDIR;
   TRY_A(halo_op(g, rec, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y}, 2));

@@ -105,7 +105,7 @@ size_t fused_partials_needed(int nx, int ny);
 clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val);
 clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, int p_depth,
                                 double *dt_min_val);
-clv_status fused_lagrange_entry(clv_chunk *c, double dt);
+clv_status fused_lagrange_entry(clv_chunk *c, double dt, double *rho_pred = nullptr, double *e_pred = nullptr);
 clv_status fused_advection_entry(clv_chunk *c, int sweep, int dir, bool reset);
 // dst's depth-2 ghost frame (node extent) <- src's
 clv_status k_copy_frame(clv_chunk *c, int src, int dst);

@@ -326,8 +326,9 @@ struct PredIn {
 
 // src/serial/PdV.cpp:49-63 (predict) then src/serial/ideal_gas.cpp:41 (pressure only: the
 // predictor's soundspeed is overwritten before use; WITH_C: and the sound speed, :42-45).
+// rho1_out / e1_out: the predictor's density1 and energy1.
 template <bool WITH_C = false>
-__device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt, double &c) {
+__device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt, double &c, double &rho1_out, double &e1_out) {
   double left_flux = (a.xa0 * (a.u00 + a.u01 + a.u00 + a.u01)) * 0.25 * dt * 0.5;
   double right_flux = (a.xa1 * (a.u10 + a.u11 + a.u10 + a.u11)) * 0.25 * dt * 0.5;
   double bottom_flux = (a.ya0 * (a.v00 + a.v10 + a.v00 + a.v10)) * 0.25 * dt * 0.5;
@@ -338,6 +339,8 @@ __device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt,
   double energy_change = (a.p / a.rho0 + a.q / a.rho0) * total_flux * recip_volume;
   double e1 = a.e0 - energy_change;
   double rho1 = a.rho0 * volume_change_s;
+  rho1_out = rho1;
+  e1_out = e1;
   if constexpr (WITH_C) {
     double p;
     ideal_gas_cell(rho1, e1, p, c);
@@ -350,9 +353,14 @@ __device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt,
 
 // cells [j0, jc1).  WRITE_PC: also store the predictor's pressure and sound speed of
 // the owned cells (what ideal_gas(predict) leaves in those arrays; the per-entry-point
-// path, whose caller may read them -- the fused step never does)
-template <bool MESH, bool WRITE_PC = false, class V>
-__device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc1, double dt, int nstrips) {
+// path, whose caller may read them -- the fused step never does).  PRED (with
+// WRITE_PC): also the predictor's density1 and energy1 of the owned cells into
+// rho_pred / e_pred (what PdV(predict) leaves in density1 / energy1 before revert;
+// the adjoint's undo log).
+template <bool MESH, bool WRITE_PC = false, bool PRED = false, class V>
+__device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc1, double dt, int nstrips, double *rho_pred = nullptr,
+                                              double *e_pred = nullptr) {
+  static_assert(!PRED || WRITE_PC, "the predictor's density and energy are stored with its pressure");
   const int lane = threadIdx.x & 63;
   const int strip = bx * WAVES + (threadIdx.x >> 6);
   // a strip past the chunk runs the march (every wave meets the block's barriers)
@@ -364,7 +372,7 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   const int ic = i < nx + 4 ? i : nx + 4;
   const bool own_cell = live && lane >= 1 && lane <= kOwnLagrange && i <= nx + 1;
   const bool own_node = live && ((lane >= 1 && lane <= kOwnLagrange && i <= nx + 2) || (lane == kOwnLagrange + 1 && i == nx + 2));
-  __shared__ double stage[2][WRITE_PC ? 8 : 6][BlockRows<kOwnLagrange, true>::kSlots];
+  __shared__ double stage[2][PRED ? 10 : (WRITE_PC ? 8 : 6)][BlockRows<kOwnLagrange, true>::kSlots];
   const BlockRows<kOwnLagrange, true> out(bx, nx);
   const int pos = out.pos(i);
   const int jn1 = jc1 == ny + 2 ? ny + 3 : jc1;    // owned node rows [j0, jn1)
@@ -417,13 +425,13 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
     // cell row r inputs
     PredIn a;
     load_pred(r, u_r, v_r, u_r1, v_r1, a);
-    double pp, cp = 0.0;
+    double pp, cp = 0.0, rp = 0.0, ep = 0.0;
     if (r == 1) {
       pp = 0.0;  // set from row 2 below, before any use
     } else if (r == ny + 2) {
       pp = pp_prev;  // ghost row: p'(ny+2) = p'(ny+1)
     } else {
-      pp = mirror_cols(predicted_pressure<WRITE_PC>(a, dt, cp));
+      pp = mirror_cols(predicted_pressure<WRITE_PC>(a, dt, cp, rp, ep));
     }
     if (r == 2) pp_prev = pp;  // ghost row 1 mirrors row 2
     const double m = a.rho0 * a.vol;
@@ -469,6 +477,10 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
       if (WRITE_PC && own_cell && r < jc1) {
         stage[r & 1][WRITE_PC ? 6 : 0][pos] = pp;
         stage[r & 1][WRITE_PC ? 7 : 0][pos] = cp;
+        if (PRED) {
+          stage[r & 1][PRED ? 8 : 0][pos] = rp;
+          stage[r & 1][PRED ? 9 : 0][pos] = ep;
+        }
       }
       u1_prev = u1;
       v1_prev = v1;
@@ -493,6 +505,10 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
       if (WRITE_PC && r < jc1 && out.cell(t)) {
         ST_STREAM(AT(CLV_PRESSURE, xs, r), stage[r & 1][WRITE_PC ? 6 : 0][t]);
         ST_STREAM(AT(CLV_SOUNDSPEED, xs, r), stage[r & 1][WRITE_PC ? 7 : 0][t]);
+        if (PRED) {
+          ST_STREAM(rho_pred[(int64_t)r * d.pitch + xs], stage[r & 1][PRED ? 8 : 0][t]);
+          ST_STREAM(e_pred[(int64_t)r * d.pitch + xs], stage[r & 1][PRED ? 9 : 0][t]);
+        }
       }
     }
     pp_prev = pp;
@@ -512,12 +528,12 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
-template <bool MESH, bool WRITE_PC = false>
-__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips, int seg) {
+template <bool MESH, bool WRITE_PC = false, bool PRED = false>
+__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips, int seg, double *rho_pred, double *e_pred) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  lagrange_tile<MESH, WRITE_PC>(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips);
+  lagrange_tile<MESH, WRITE_PC, PRED>(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips, rho_pred, e_pred);
 }
 
 // ---- advec_cell -------------------------------------------------------------------------
@@ -1289,14 +1305,21 @@ clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, 
 // points leave it -- the predictor's pressure (with its depth-1 reflective halo) and
 // sound speed included -- in one pass.  Any chunk (a wrapped one too): nothing is
 // renamed.
-clv_status fused_lagrange_entry(clv_chunk *c, double dt) {
+clv_status fused_lagrange_entry(clv_chunk *c, double dt, double *rho_pred, double *e_pred) {
   if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "clv_lagrange_kernels needs x_cells, y_cells >= 4");
+  if ((rho_pred == nullptr) != (e_pred == nullptr)) return set_error(CLV_ERR_ARG, "fused lagrange: both predictor outputs or neither");
   choose_segments(c);
   const Dev &d = c->d;
   const int seg = c->seg[SF_LAGRANGE];
   const int ns = ceil_div(d.nx, kOwnLagrange), nseg = ceil_div(d.ny, seg);
-  const auto kern = d.geom_mesh ? lagrange_k<true, true> : lagrange_k<false, true>;
-  LAUNCH_F("f_lagrange_pc", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg);
+  const dim3 g(ceil_div(ns, WAVES), nseg, 1);
+  if (rho_pred) {
+    const auto kern = d.geom_mesh ? lagrange_k<true, true, true> : lagrange_k<false, true, true>;
+    LAUNCH_F("f_lagrange_pred", kern, g, d, dt, ns, seg, rho_pred, e_pred);
+  } else {
+    const auto kern = d.geom_mesh ? lagrange_k<true, true> : lagrange_k<false, true>;
+    LAUNCH_F("f_lagrange_pc", kern, g, d, dt, ns, seg, rho_pred, e_pred);
+  }
   clv_status st = k_flux_calc_edges(c, dt);
   if (st != CLV_OK) return st;
   return halo(c, {CLV_PRESSURE}, 1);
@@ -1309,7 +1332,7 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
   const int seg = c->seg[SF_LAGRANGE];
   const int ns = ceil_div(d.nx, kOwnLagrange), nseg = ceil_div(d.ny, seg);
   const auto kern = d.geom_mesh ? lagrange_k<true> : lagrange_k<false>;
-  LAUNCH_F("f_lagrange", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg);
+  LAUNCH_F("f_lagrange", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg, nullptr, nullptr);
   return CLV_OK;
 }
 
