@@ -1196,12 +1196,14 @@ static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // smaller tail, a tighter working set per XCD L2).  At 15360^2 (ms per launch over
 // segments of 8 ... 256 rows) the row marches are fastest at 8-16 rows and the
 // per-lane y walk at 128 (64 and 256 within 1 %).  On a smaller grid a family's
-// segment is halved until it has 4 workgroups per CU (or 8 rows); the y walk, which
-// spends 6 extra steps at each segment's start, stops at 3.5 (1920^2: 16 rows, not 8,
-// 10 % faster; 3840^2: 64, not 32, 5 % faster).
+// segment is halved until it has 4 workgroups per CU (or 8 rows).  The timestep and
+// lagrange marches halve until 16 per CU (3840^2: 8 rows, not 16, a 1.5 % faster
+// step; 15360^2 keeps 16, 2-3 % faster there than 8); the y walk, which spends 6
+// extra steps at each segment's start, stops at 3.5 (1920^2: 16 rows, not 8, 10 %
+// faster; 3840^2: 64, not 32, 5 % faster).
 enum { SF_TIMESTEP, SF_LAGRANGE, SF_CM_X, SF_CM_Y, SF_COUNT };
 constexpr int kSegLarge[SF_COUNT] = {16, 16, 8, 128};
-constexpr int kBlocksPer2Cu[SF_COUNT] = {8, 8, 8, 7};
+constexpr int kBlocksPer2Cu[SF_COUNT] = {32, 32, 8, 7};
 constexpr int kSegMin = 8;
 
 static void choose_segments(clv_chunk *c) {
