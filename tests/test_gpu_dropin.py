@@ -73,3 +73,26 @@ def test_dropin_without_gpu_fails_cleanly(tmp_path):
     p = _run(tmp_path, "clover_default")
     assert p.returncode != 0
     assert "no HIP device" in p.stderr
+
+
+@pytest.mark.gpu
+def test_dropin_dumps_equal_reference(gpu, tmp_path):
+    """--dump (driver/shared.cpp:96-189): the field dumps the reference's driver writes
+    after every start-up and step phase, with this model, byte-identical to the
+    reference serial model's own (tests/golden/driver_output.json.gz, case tiny: 6 x 5
+    cells, 2 steps).  Every array of the chunk is dumped, work arrays included."""
+    import gzip
+    import json
+    g = json.load(gzip.open(T.GOLDEN / "driver_output.json.gz", "rt"))["tiny"]
+    (tmp_path / "deck.in").write_text((T.DECKS / g["deck"]).read_text())
+    p = subprocess.run([str(DROPIN), *g["args"]], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    files = sorted(f.name for f in (tmp_path / "d").iterdir())
+    assert files == sorted(g["dumps"])
+    for name in files:
+        got = (tmp_path / "d" / name).read_text()
+        if got != g["dumps"][name]:
+            a, b = got.splitlines(), g["dumps"][name].splitlines()
+            bad = next((k for k in range(min(len(a), len(b))) if a[k] != b[k]), min(len(a), len(b)))
+            raise AssertionError(f"{name}: line {bad}: {a[bad] if bad < len(a) else None!r} != {b[bad] if bad < len(b) else None!r}")
+
