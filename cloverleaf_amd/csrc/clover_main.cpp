@@ -353,6 +353,66 @@ struct Dumper {
   }
 };
 
+// ---- visit output (driver/visit.cpp:38-196) ------------------------------------------------
+// Every visit_frequency steps, after start-up and at the end: pressure, the pressure
+// and velocity halos and viscosity brought up to date on the device, then clover.visit
+// (the block count once, a file name each call) and one VTK rectilinear grid per call
+// in the working directory, with the reference's file names and number formats.
+struct Visit {
+  bool first = true;
+  void operator()(clv_chunk *c, const clv_config &cfg, const clv_hydro_state &st) {
+    if (first) {
+      std::ofstream u("clover.visit");
+      u << "!NBLOCKS " << 1 << std::endl;  // chunks x tiles (one of each)
+      first = false;
+    }
+    CHECK(clv_ideal_gas(c, 0), "visit ideal_gas");
+    int fields[CLV_NUM_HALO_FIELDS] = {0};
+    fields[CLV_PRESSURE] = fields[CLV_XVEL0] = fields[CLV_YVEL0] = 1;
+    CHECK(clv_update_halo(c, fields, 1), "visit update_halo");
+    CHECK(clv_viscosity(c), "visit viscosity");
+    std::ostringstream name;
+    name << "clover." << std::setfill('0') << std::setw(5) << 0 << "." << std::setw(5) << 1 << "." << std::setw(5) << st.step << ".vtk";
+    {
+      std::ofstream u("clover.visit", std::ios::app);
+      u << name.str() << std::endl;
+    }
+    const int nx = cfg.x_cells, ny = cfg.y_cells, nxv = nx + 1, nyv = ny + 1;
+    const int64_t P = nx + 5;
+    std::vector<double> vx(clv_mesh_length(nx, ny, CLV_VERTEXX)), vy(clv_mesh_length(nx, ny, CLV_VERTEXY));
+    CHECK(clv_mesh_download(c, CLV_VERTEXX, vx.data()), "visit download");
+    CHECK(clv_mesh_download(c, CLV_VERTEXY, vy.data()), "visit download");
+    std::ofstream u(name.str());
+    u << "# vtk DataFile Version 3.0" << std::endl << "vtk output" << std::endl << "ASCII" << std::endl
+      << "DATASET RECTILINEAR_GRID" << std::endl << "DIMENSIONS " << nxv << " " << nyv << " 1" << std::endl
+      << "X_COORDINATES " << nxv << " double" << std::endl;
+    for (int i = 2; i <= nx + 2; ++i) u << vx[i] << std::endl;
+    u << "Y_COORDINATES " << nyv << " double" << std::endl;
+    for (int j = 2; j <= ny + 2; ++j) u << vy[j] << std::endl;
+    u << "Z_COORDINATES 1 double" << std::endl << "0" << std::endl;
+    std::vector<double> a((size_t)P * (ny + 5));
+    // cells [2, nx + 1] x [2, ny + 1] or nodes [2, nx + 2] x [2, ny + 2], x fastest;
+    // small: values below 1e-8 in magnitude written as 0 (viscosity, velocities)
+    auto block = [&](const char *what, int field, int n, int ext, bool small) {
+      CHECK(clv_field_download(c, field, a.data(), P), "visit download");
+      u << what << " 1 " << n << " double" << std::endl;
+      for (int j = 2; j <= ny + 1 + ext; ++j)
+        for (int i = 2; i <= nx + 1 + ext; ++i) {
+          const double v = a[(size_t)j * P + i];
+          u << std::scientific << std::setprecision(3) << (small && !(std::fabs(v) > 0.00000001) ? 0.0 : v) << std::endl;
+        }
+    };
+    u << "CELL_DATA " << nx * ny << std::endl << "FIELD FieldData 4" << std::endl;
+    block("density", CLV_DENSITY0, nx * ny, 0, false);
+    block("energy", CLV_ENERGY0, nx * ny, 0, false);
+    block("pressure", CLV_PRESSURE, nx * ny, 0, false);
+    block("viscosity", CLV_VISCOSITY, nx * ny, 0, true);
+    u << "POINT_DATA " << nxv * nyv << std::endl << "FIELD FieldData 2" << std::endl;
+    block("x_vel", CLV_XVEL0, nxv * nyv, 1, true);
+    block("y_vel", CLV_YVEL0, nxv * nyv, 1, true);
+  }
+};
+
 // ---- --profile (driver/hydro.cpp:125-200) ---------------------------------------------------
 // Kernel times from HIP events, grouped under the reference's profiler rows.  A fused
 // kernel counts under the first reference phase it contains: f_timestep under Timestep
@@ -496,6 +556,8 @@ int main(int argc, char **argv) {
   CHECK(summary(c, cfg.profiler_on, s), "field_summary");
   report_step(g_out, hs, s);
   dump(c, cfg, hs, "04_field_summary");
+  Visit visit;
+  if (cfg.visit_frequency != 0) visit(c, cfg, hs);  // driver/start.cpp:131
   g_out << "Starting the calculation" << std::endl;
   std::cout << " Launching hydro" << std::endl;
   if (cfg.profiler_on) CHECK(clv_profile_reset(c), "clv_profile_reset");
@@ -537,6 +599,7 @@ int main(int argc, char **argv) {
       CHECK(summary(c, cfg.profiler_on, s), "field_summary");
       report_step(g_out, hs, s);
     }
+    if (cfg.visit_frequency != 0 && hs.step % cfg.visit_frequency == 0) visit(c, cfg, hs);  // driver/hydro.cpp:95-97
     // the reference resets these every step, so its completion line prints 0 unless
     // the run ends at step 1 or 2 (driver/hydro.cpp:103-106)
     double first_step = 0, second_step = 0;
@@ -545,6 +608,7 @@ int main(int argc, char **argv) {
     if (hs.complete) {
       CHECK(summary(c, cfg.profiler_on, s), "field_summary");
       report_step(g_out, hs, s);
+      if (cfg.visit_frequency != 0) visit(c, cfg, hs);  // driver/hydro.cpp:112
       if (cfg.test_problem >= 1) {  // driver/report.cpp:73-111
         double qa_diff = 0.0, ke_exp = 0.0;
         if (expected_ke(cfg.test_problem, ke_exp)) {

@@ -120,3 +120,17 @@ def test_driver_small_deck_defaults_to_a_plan_that_runs(gpu, tmp_path):
     assert p.returncode == 0, p.stderr[-2000:]
     assert "This test is considered PASSED" in p.stdout
     assert " - Outcome: PASSED" in p.stdout
+
+
+@pytest.mark.parametrize("plan", ["fused", "reference"])
+def test_driver_visit_files_equal_reference(gpu, tmp_path, plan):
+    """visit_frequency (driver/visit.cpp:38-196): clover.visit and every VTK grid, written
+    after start-up, every second step and at the end, byte-identical to the
+    reference's (case visit: 12 x 10 cells, 5 steps)."""
+    g, _ = _run(tmp_path, "visit", ("--plan", plan))
+    files = sorted(f.name for f in tmp_path.iterdir() if f.name == "clover.visit" or f.suffix == ".vtk")
+    assert files == sorted(g["visit"])
+    for name in files:
+        got, want = (tmp_path / name).read_text().splitlines(), g["visit"][name].splitlines()
+        bad = [(k, a, b) for k, (a, b) in enumerate(zip(got, want)) if a != b]
+        assert len(got) == len(want) and not bad, (name, bad[:3])

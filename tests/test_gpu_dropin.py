@@ -96,3 +96,22 @@ def test_dropin_dumps_equal_reference(gpu, tmp_path):
             bad = next((k for k in range(min(len(a), len(b))) if a[k] != b[k]), min(len(a), len(b)))
             raise AssertionError(f"{name}: line {bad}: {a[bad] if bad < len(a) else None!r} != {b[bad] if bad < len(b) else None!r}")
 
+
+
+@pytest.mark.gpu
+def test_dropin_visit_files_equal_reference(gpu, tmp_path):
+    """visit_frequency: the reference driver's visit output (driver/visit.cpp:38-196)
+    with this model -- clover.visit and every VTK grid -- byte-identical to the
+    reference serial model's (tests/golden/driver_output.json.gz, case visit)."""
+    import gzip
+    import json
+    g = json.load(gzip.open(T.GOLDEN / "driver_output.json.gz", "rt"))["visit"]
+    (tmp_path / "deck.in").write_text((T.DECKS / g["deck"]).read_text())
+    p = subprocess.run([str(DROPIN), *g["args"]], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    files = sorted(f.name for f in tmp_path.iterdir() if f.name == "clover.visit" or f.suffix == ".vtk")
+    assert files == sorted(g["visit"])
+    for name in files:
+        got, want = (tmp_path / name).read_text().splitlines(), g["visit"][name].splitlines()
+        bad = [(k, a, b) for k, (a, b) in enumerate(zip(got, want)) if a != b]
+        assert len(got) == len(want) and not bad, (name, bad[:3])
