@@ -276,20 +276,24 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
     fcfg.dtinit = fcfg.dtmax = dt
     fcfg.dtc_safe = fcfg.dtu_safe = fcfg.dtv_safe = fcfg.dtdiv_safe = 1e30
     fcfg.end_step = steps
-    with clv.Chunk(nx, ny) as fc:
-        st, _ = fc.start(fcfg)
-        for name, a in (("density0", rho), ("energy0", e)):
-            full = fc.download(name)
-            full[2:ny + 2, 2:nx + 2] = a
-            fc.upload(name, full)
-        fc.update_halo(["density0", "energy0"], 2)
-        fc.synchronize()
-        t0 = time.perf_counter()
-        while not st.complete:
-            fc.step(fcfg, st, clv.PLAN_FUSED)
-        fc.synchronize()
-        fused_s = time.perf_counter() - t0
-        ke_fused = fc.summary()[3]
+    def plain_forward(plan):
+        with clv.Chunk(nx, ny) as fc:
+            st, _ = fc.start(fcfg)
+            for name, a in (("density0", rho), ("energy0", e)):
+                full = fc.download(name)
+                full[2:ny + 2, 2:nx + 2] = a
+                fc.upload(name, full)
+            fc.update_halo(["density0", "energy0"], 2)
+            fc.synchronize()
+            t0 = time.perf_counter()
+            while not st.complete:
+                fc.step(fcfg, st, plan)
+            fc.synchronize()
+            return time.perf_counter() - t0, fc.summary()[3]
+    fused_s, ke_fused = plain_forward(clv.PLAN_FUSED)
+    # and on the reference plan (one launch per reference loop), as the adjoint's forward ran before its
+    # Lagrangian phase and advection sweeps became single passes
+    ref_s, ke_ref = plain_forward(clv.PLAN_REFERENCE)
     # central differences along seeded smooth directions.  eps = 1e-8: over 200 steps
     # on 1920^2 a step of 1e-7 or more moves some van Leer limiter / upwind switch
     # across its kink, and the difference quotient then carries an O(1e-5) error
@@ -336,7 +340,10 @@ def adjoint_leg(clv, directions=32, eps=1e-8):
         "forward_s": fwd_s,
         "backward_s": bwd_s,
         "fused_forward_s": fused_s,
+        "reference_plan_forward_s": ref_s,
+        "ke_reference_plan_forward": ke_ref,
         "adjoint_cost_vs_forward": bwd_s / fwd_s,
+        "adjoint_cost_vs_reference_plan_forward": bwd_s / ref_s,
         "adjoint_cost_vs_fused_forward": bwd_s / fused_s,
         "fd_directions": directions,
         "fd_eps": eps,
