@@ -115,3 +115,51 @@ def test_dropin_visit_files_equal_reference(gpu, tmp_path):
         got, want = (tmp_path / name).read_text().splitlines(), g["visit"][name].splitlines()
         bad = [(k, a, b) for k, (a, b) in enumerate(zip(got, want)) if a != b]
         assert len(got) == len(want) and not bad, (name, bad[:3])
+
+
+def _printed_value_ok(printed: float, exact: float) -> bool:
+    """printed is exact rounded to the 7 significant digits of the reference's format
+    (std::scientific, driver/report.cpp), a last-digit flip allowed only within 1e-11
+    of a rounding boundary."""
+    if exact == 0.0:
+        return printed == 0.0
+    import math
+    half = 0.5 * 10.0 ** (math.floor(math.log10(abs(exact))) - 6)
+    return abs(printed - exact) <= half * (1 + 1e-9) + 1e-11 * abs(exact)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("deck", ["clover_bm16_300", "clover_bm256_short"])
+def test_dropin_large_decks_vs_reference_trace(gpu, tmp_path, deck):
+    """The drop-in on InputDecks/clover_bm16_300.in (3840^2, 300 steps) and the headline
+    deck clover_bm256_short.in (15360^2, 87 steps) against the reference omp model's
+    full-precision trace of the same deck (tests/golden/<deck>.json): every printed dt
+    and time equal to the reference's value in the driver's own format
+    (driver/timestep.cpp:98-103), and every printed field summary
+    (driver/report.cpp:48-71) equal to the exact sums of the reference's state rounded
+    to the printed digits."""
+    gold = T.golden(deck)["summaries"]
+    by_step = {g["step"]: g for g in gold}
+    (tmp_path / "deck.in").write_text((T.DECKS / f"{deck}.in").read_text())
+    print(f"# drop-in {deck}", flush=True)
+    p = subprocess.run([str(DROPIN), "--file", "deck.in", "--out", "clover.out"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=600)
+    assert p.returncode == 0, p.stderr[-2000:]
+    steps = [l for l in p.stdout.splitlines() if re.match(r"^ Step \d+ time", l)]
+    assert len(steps) == len(gold) - 1
+    for line in steps:
+        m = re.match(r"^ Step (\d+) time (\S+) control \S+ timestep  (\S+) ", line)
+        assert m, line
+        n = int(m.group(1))
+        assert m.group(3) == f"{by_step[n]['dt']:g}", (line, by_step[n]["dt"])
+        assert m.group(2) == f"{by_step[n - 1]['time']:g}", (line, by_step[n - 1]["time"])
+    rows = [l.split() for l in (tmp_path / "clover.out").read_text().splitlines() if l.startswith(" step:")]
+    assert rows and rows[-1][1] == str(len(gold) - 1)
+    bad = []
+    for r in rows:
+        g = by_step[int(r[1])]
+        vol, mass, ie, ke, press = g["x_vol"], g["x_mass"], g["x_ie"], g["x_ke"], g["x_press"]
+        want = [vol, mass, mass / vol, press / vol, ie, ke, ie + ke]
+        got = [float(x) for x in r[2:9]]
+        bad += [(r[1], k, a, b) for k, (a, b) in enumerate(zip(got, want)) if not _printed_value_ok(a, b)]
+    assert not bad, bad[:5]
