@@ -8,7 +8,8 @@
  * floating-point expression order, so that results are bit-identical.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
- * this library, and only as the checker.  The product library never calls it.
+ * this library, and only to check the GPU library's results against or to time a
+ * CPU run beside it.  The product library never calls it.
  *
  * Pinning (how each function is checked; the tests live in tests/):
  *   - every kernel: bitwise against the reference serial kernel compiled from

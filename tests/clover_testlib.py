@@ -2,7 +2,7 @@
 reference serial kernels (oracle/_ref/librefkern.so) behind ctypes.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-module, and only as the checker.
+module, and only to check the GPU library's results against or to time a CPU run beside it.
 
 The deck reader restates driver/read_input.cpp:37-277: defaults (:39-69),
 ``keyword=value`` / ``keyword value`` tokens split on spaces and '=' (:135),

@@ -6,7 +6,7 @@ reference's CMake then configures with -DMODEL=mi355x (cmake/register_models.cma
 115-146, the model's model.cmake supplying setup()/setup_target()), compiles the
 reference's unchanged driver with the model and links libclover_mi355x.so into
 mi355x-cloverleaf.  The binary's own --help runs without a GPU.  Skipped where the
-reference tree or cmake is absent (the GPU box has neither).
+reference tree or cmake is absent (as on machines that only run the GPU tests).
 """
 import os
 import shutil
