@@ -136,7 +136,7 @@ def alg_bytes(name, nx, ny, geom_mesh=False):
 
 
 def cpu_baseline(deck_text, steps):
-    """The OpenMP CPU oracle (oracle/liboracle.so, this build's restatement of the
+    """The OpenMP CPU oracle (oracle/liboracle.so, this repository's restatement of the
     reference) on a bounded sample: same problem (deck geometry and states) on a
     3840^2 grid, `steps` steps."""
     sys.path.insert(0, str(REPO / "tests"))
@@ -597,7 +597,7 @@ def main():
     ap.add_argument("--profile-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (one chunk per GPU); --gpus must be 1", file=sys.stderr)
+        print("bench.py: runs on one GPU only (one chunk per GPU); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     import cloverleaf_amd as clv
