@@ -57,6 +57,11 @@ namespace {
 #define MU(mid, i) ((const __attribute__((address_space(4))) double *)(d.m[mid]))[i]
 
 constexpr int WAVES = 4;       // waves (strips) per 256-thread block
+// waves (strips) per block of the plain y advection pass on a wide chunk (at least
+// kWideCmY strips; narrower ones keep WAVES): a block's walks keep in step, and 8 in
+// step read 8 adjacent strips of each row together (15360^2: 2-5 % faster than 4 a
+// block, 2 % faster than 16; 7680^2: 3-7 %; 3840^2: even; 1920^2: 5-10 % slower)
+constexpr int kWavesCmY = 8, kWideCmY = 64;
 
 __device__ __forceinline__ double wave_min(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -799,11 +804,11 @@ struct CmRing {
   __device__ double post(int row) const { return get(kRingMf + kRingRho + row % kRingPost); }
 };
 // RESET as for cm_x_tile.
-template <int SWEEP, bool MESH, bool LOCKSTEP = true, bool RESET = false, class V, class O>
+template <int SWEEP, bool MESH, bool LOCKSTEP = true, bool RESET = false, int NW = WAVES, class V, class O>
 __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 2 : 4;
   const int lane = threadIdx.x & 63;
-  const int strip = bx * WAVES + (threadIdx.x >> 6);
+  const int strip = bx * NW + (threadIdx.x >> 6);
   const bool live = strip < nstrips;  // past the chunk: walk on clamped loads, own nothing (or leave)
   if (!LOCKSTEP && !live) return;
   const int nx = d.nx, ny = d.ny;
@@ -815,7 +820,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   const int lane_f = lane + (i == 1 ? 2 : (i == nx + 2 ? -2 : 0));
   const int Fs = j0 - 2 > 2 ? j0 - 2 : 2, Fe = min(j1 + 1, ny + 2);  // faces the cell part walks
   const double cdx_ic = MESH ? M(CLV_CELLDX, ic) : 0.0;                 // ic <= nx + 3
-  __shared__ double ring_lds[WAVES * kRing * 64];
+  __shared__ double ring_lds[NW * kRing * 64];
   const CmRing ring{ring_lds + (threadIdx.x >> 6) * (kRing * 64) + lane};
   for (int q = 0; q < kRing; ++q) ring.put(q, 0.0);
   // pre_vol of cell row `row` (src/serial/advec_cell.cpp:139 / :149) from its bottom and
@@ -985,7 +990,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       v_m1 = v_0;
       v_0 = v_p1;
     }
-    // LOCKSTEP: the block's four walks stay on the same rows (3-6 % faster at 15360^2
+    // LOCKSTEP: the block's walks stay on the same rows (3-6 % faster at 15360^2
     // than free-running waves; F, j1 are uniform over the block).  The batched form
     // runs them free, and a walk past its problem's last strip exits at once: over
     // many small problems that is 7 % faster for the whole batch.
@@ -996,12 +1001,12 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   }
 }
 
-template <int SWEEP, bool MESH, bool RESET = false>
-__global__ __launch_bounds__(256, 4) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
+template <int SWEEP, bool MESH, bool RESET = false, int NW = WAVES>
+__global__ __launch_bounds__(64 * NW, 16 / NW) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  cm_y_tile<SWEEP, MESH, true, RESET>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+  cm_y_tile<SWEEP, MESH, true, RESET, NW>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
 // ---- batched form: persistent launches over the tiles of many chunks ----------------------
@@ -1180,13 +1185,14 @@ hipError_t batch_launch_dt(hipStream_t s, int nprob, const BatchConst *pc, Batch
 }
 
 // ---- host side ----------------------------------------------------------------------------
-#define LAUNCH_F(name, kern, grid, ...)                                          \
-  do {                                                                           \
-    before_launch(c, name);                                                      \
-    hipLaunchKernelGGL(kern, grid, dim3(256, 1, 1), 0, c->stream, __VA_ARGS__);  \
-    clv_status st_ = after_launch(c, name);                                      \
-    if (st_ != CLV_OK) return st_;                                               \
+#define LAUNCH_FT(name, kern, grid, threads, ...)                                    \
+  do {                                                                               \
+    before_launch(c, name);                                                          \
+    hipLaunchKernelGGL(kern, grid, dim3(threads, 1, 1), 0, c->stream, __VA_ARGS__);  \
+    clv_status st_ = after_launch(c, name);                                          \
+    if (st_ != CLV_OK) return st_;                                                   \
   } while (0)
+#define LAUNCH_F(name, kern, grid, ...) LAUNCH_FT(name, kern, grid, 256, __VA_ARGS__)
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
@@ -1196,7 +1202,7 @@ static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // smaller tail, a tighter working set per XCD L2).  At 15360^2 (ms per launch over
 // segments of 8 ... 256 rows) the row marches are fastest at 8-16 rows and the
 // per-lane y walk at 128 (64 and 256 within 1 %).  On a smaller grid a family's
-// segment is halved until it has 4 workgroups per CU (or 8 rows).  The timestep and
+// segment is halved until it has 4 groups of 4 waves per CU (or 8 rows).  The timestep and
 // lagrange marches halve until 16 per CU (3840^2: 8 rows, not 16, a 1.5 % faster
 // step; 15360^2 keeps 16, 2-3 % faster there than 8); the y walk, which spends 6
 // extra steps at each segment's start, stops at 3.5 (1920^2: 16 rows, not 8, 10 %
@@ -1348,22 +1354,27 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
 // reset: the pass also does reset_field's stores (the kernels' RESET form).
 using CmKernel = void (*)(Dev, int, int, double *, double *, double *, double *);
 template <bool MESH, bool RESET>
-static CmKernel cm_kernel(int sweep, int dir) {
+static CmKernel cm_kernel(int sweep, int dir, bool wide) {
   if (dir == CLV_XDIR) return sweep == 1 ? cm_x_k<1, MESH, RESET> : cm_x_k<2, MESH, RESET>;
+  if (wide) return sweep == 1 ? cm_y_k<1, MESH, RESET, kWavesCmY> : cm_y_k<2, MESH, RESET, kWavesCmY>;
   return sweep == 1 ? cm_y_k<1, MESH, RESET> : cm_y_k<2, MESH, RESET>;
 }
 
 static clv_status cm_sweep(clv_chunk *c, int sweep, int dir, bool reset = false) {
   const Dev &d = c->d;
   double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2], *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
-  const CmKernel kern = d.geom_mesh ? (reset ? cm_kernel<true, true>(sweep, dir) : cm_kernel<true, false>(sweep, dir))
-                                    : (reset ? cm_kernel<false, true>(sweep, dir) : cm_kernel<false, false>(sweep, dir));
+  const int own = dir == CLV_XDIR ? kOwnCmX : kOwnCmY, seg = c->seg[dir == CLV_XDIR ? SF_CM_X : SF_CM_Y];
+  const int ns = ceil_div(d.nx + 1, own);
+  const bool wide = dir == CLV_YDIR && ns >= kWideCmY;
+  const int waves = wide ? kWavesCmY : WAVES;
+  const CmKernel kern = d.geom_mesh
+                            ? (reset ? cm_kernel<true, true>(sweep, dir, wide) : cm_kernel<true, false>(sweep, dir, wide))
+                            : (reset ? cm_kernel<false, true>(sweep, dir, wide) : cm_kernel<false, false>(sweep, dir, wide));
   static const char *const names[2][2][2] = {{{"f_cm_x1", "f_cm_x1_reset"}, {"f_cm_x2", "f_cm_x2_reset"}},
                                              {{"f_cm_y1", "f_cm_y1_reset"}, {"f_cm_y2", "f_cm_y2_reset"}}};
   const char *name = names[dir == CLV_XDIR ? 0 : 1][sweep == 1 ? 0 : 1][reset ? 1 : 0];
-  const int own = dir == CLV_XDIR ? kOwnCmX : kOwnCmY, seg = c->seg[dir == CLV_XDIR ? SF_CM_X : SF_CM_Y];
-  const int ns = ceil_div(d.nx + 1, own);
-  LAUNCH_F(name, kern, dim3(ceil_div(ns, WAVES), ceil_div(d.ny + 1, seg), 1), d, ns, seg, rho_out, e_out, u_out, v_out);
+  LAUNCH_FT(name, kern, dim3(ceil_div(ns, waves), ceil_div(d.ny + 1, seg), 1), 64 * waves, d, ns, seg, rho_out, e_out,
+            u_out, v_out);
   swap_fields(c, CLV_DENSITY1, CLV_WORK1);
   swap_fields(c, CLV_ENERGY1, CLV_WORK2);
   swap_fields(c, CLV_XVEL1, CLV_WORK3);

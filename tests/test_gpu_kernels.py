@@ -205,8 +205,13 @@ def test_lagrange_kernels_equal_the_entry_point_sequence(gpu, nx, ny):
             assert np.array_equal(a.download(n), b.download(n), equal_nan=True), n
 
 
+# 4100 columns: 67 strips of the y pass, past the width where its blocks hold 8
+# walks, so the last block has 5 walks past the chunk
+ADVECTION_SIZES = [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130), (4100, 24)]
+
+
 @pytest.mark.parametrize("sweep,direction", [(1, 1), (1, 2), (2, 1), (2, 2)])
-@pytest.mark.parametrize("nx,ny", [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130)])
+@pytest.mark.parametrize("nx,ny", ADVECTION_SIZES)
 def test_advection_pass_equals_the_entry_point_sequence(gpu, nx, ny, sweep, direction):
     """clv_advection_pass against advec_cell, update_halo(density1, energy1, xvel1,
     yvel1, mass fluxes, 2), advec_mom(xvel), advec_mom(yvel) on the same arrays after
@@ -237,7 +242,7 @@ def test_advection_pass_equals_the_entry_point_sequence(gpu, nx, ny, sweep, dire
 
 
 @pytest.mark.parametrize("sweep,direction", [(1, 1), (1, 2), (2, 1), (2, 2)])
-@pytest.mark.parametrize("nx,ny", [(4, 4), (8, 8), (13, 7), (70, 33), (257, 130)])
+@pytest.mark.parametrize("nx,ny", ADVECTION_SIZES)
 def test_advection_pass_reset_equals_pass_then_reset_field(gpu, nx, ny, sweep, direction):
     """clv_advection_pass_reset against clv_advection_pass followed by clv_reset_field
     (src/serial/reset_field.cpp:29-42): every array but the work arrays bit for bit
