@@ -4,15 +4,14 @@
 //
 //   batch_timestep_k  (all members' tiles)   -> per-member timestep minima
 //   batch_dt_k        (one block per member) -> dt, time, step, complete on the device
-//   batch_halo_k      viscosity, depth 1
-//   batch_lagrange_k
+//   batch_lagrange_k  (q recomputed, as the timestep pass does not store it)
 //   batch_halo_k      energy1, density1, vol_flux_x/y, xvel1, yvel1, depth 2
 //   batch_advec_k     advection pass 1 (advec_cell + advec_mom), dir1
 //   batch_halo_k      density1, energy1, xvel1, yvel1, mass_flux_x/y, depth 2
 //   batch_advec_k     advection pass 2, dir2
 //   batch_halo_k      density1, energy1, mass_flux_x/y, depth 2
 //
-// with no host synchronisation: dt never leaves the device, so the nine
+// with no host synchronisation: dt never leaves the device, so the eight
 // launches are captured once as a hipGraph and replayed per step.
 //
 // Buffer renaming.  The fused step double-buffers the advected fields and ends
@@ -245,7 +244,6 @@ clv_status emit_step(clv_batch *b, int q) {
   const int fam1 = x_first ? CLV_BF_CM_X : CLV_BF_CM_Y, fam2 = x_first ? CLV_BF_CM_Y : CLV_BF_CM_X;
   TRY_B(launch_family(b, BK_TIMESTEP, CLV_BF_TIMESTEP, s0));
   TRY_B(hip_check(batch_launch_dt(b->stream, b->n, b->pc, b->ps), "batch dt launch"));
-  TRY_B(launch_halo(b, s0, {CLV_VISCOSITY}, 1));
   TRY_B(launch_family(b, BK_LAGRANGE, CLV_BF_LAGRANGE, s0));
   TRY_B(launch_halo(b, s0, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2));
   TRY_B(launch_family(b, x_first ? BK_CM_X1 : BK_CM_Y1, fam1, s0));

@@ -170,10 +170,52 @@ __device__ __forceinline__ double geo_yarea(const V &d, double cdx_i, int i, int
 // block of a plain launch (<name>_k) or in the loop of a persistent batched launch
 // over many chunks (batch_<name>_k).  Both run the same instructions per tile.
 
+// q of one cell, src/serial/viscosity.cpp:39-65: u/v 00, 10 (x + 1), 01 (y + 1), 11 the
+// cell's nodes; p_w, p_e, p_m, p_p the pressure of its x and y neighbours; cdx, cdx1
+// celldx of its column and the next; cdy, cdy2 celldy of its row and of row + 2.  The
+// reference sets q = 0 whenever div >= 0, whatever the limiter (:50), so the limiter and
+// its seven divisions are formed only where !(div >= 0): in the at-rest part of the box
+// (div exactly 0) whole waves skip them.  The same doubles come out: where the branch
+// runs, every expression is the reference's.
+__device__ __forceinline__ double viscosity_cell(double u00, double u10, double u01, double u11, double v00, double v10, double v01,
+                                                 double v11, double p_w, double p_e, double p_m, double p_p, double cdx, double cdx1,
+                                                 double cdy, double cdy2, double rho) {
+  double ugrad = (u10 + u11) - (u00 + u01);
+  double vgrad = (v01 + v11) - (v00 + v10);
+  double div = (cdx * (ugrad) + cdy * (vgrad));
+  double q = 0.0;
+  if (!(div >= 0.0)) {
+    double strain2 = 0.5 * (u01 + u11 - u00 - u10) / cdy + 0.5 * (v10 + v11 - v00 - v01) / cdx;
+    double pgradx = (p_e - p_w) / (cdx + cdx1);
+    double pgrady = (p_p - p_m) / (cdy + cdy2);
+    double pgradx2 = pgradx * pgradx;
+    double pgrady2 = pgrady * pgrady;
+    double limiter = ((0.5 * (ugrad) / cdx) * pgradx2 + (0.5 * (vgrad) / cdy) * pgrady2 + strain2 * pgradx * pgrady) /
+                     fmax(pgradx2 + pgrady2, g_small);
+    if (!(limiter > 0.0)) {
+      double dirx = 1.0;
+      if (pgradx < 0.0) dirx = -1.0;
+      pgradx = dirx * fmax(g_small, fabs(pgradx));
+      double diry = 1.0;
+      if (pgradx < 0.0) diry = -1.0;
+      pgrady = diry * fmax(g_small, fabs(pgrady));
+      double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
+      double xgrad = fabs(cdx * pgrad / pgradx);
+      double ygrad = fabs(cdy * pgrad / pgrady);
+      double grad = fmin(xgrad, ygrad);
+      double grad2 = grad * grad;
+      q = 2.0 * rho * grad2 * limiter * limiter;
+    }
+  }
+  return q;
+}
+
 // The wave's minimum goes to part[strip].  WRITE_PC: also store each owned cell's
 // pressure and sound speed, as ideal_gas writes them (the per-entry-point path, whose
-// later kernels read the arrays; the fused step recomputes them instead).
-template <bool MESH, bool WRITE_PC = false, class V>
+// later kernels read the arrays; the fused step recomputes them instead).  STORE_Q:
+// store each owned cell's q (the fused step leaves it out: lagrange_tile<..., OWN_Q>
+// recomputes q from the same operands).
+template <bool MESH, bool WRITE_PC = false, bool STORE_Q = true, class V>
 __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1, double dtc_safe, double dtu_safe, double dtv_safe,
                                               double dtdiv_safe, double *part, int nstrips) {
   const int lane = threadIdx.x & 63;
@@ -210,38 +252,7 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
     const double u10 = dn1(u_r), u11 = dn1(u_r1), v10 = dn1(v_r), v11 = dn1(v_r1), xa1 = dn1(xa);
     const double u00 = u_r, u01 = u_r1, v00 = v_r, v01 = v_r1;
     const double cdy = MU(CLV_CELLDY, r), cdy2 = MU(CLV_CELLDY, r + 2);
-    // viscosity, src/serial/viscosity.cpp:39-65.  The reference sets q = 0 whenever
-    // div >= 0, whatever the limiter (:50), so the limiter and its seven divisions are
-    // formed only where !(div >= 0): in the at-rest part of the box (div exactly 0)
-    // whole waves skip them.  The same doubles come out: where the branch runs, every
-    // expression is the reference's.
-    double ugrad = (u10 + u11) - (u00 + u01);
-    double vgrad = (v01 + v11) - (v00 + v10);
-    double div = (cdx * (ugrad) + cdy * (vgrad));
-    double q = 0.0;
-    if (!(div >= 0.0)) {
-      double strain2 = 0.5 * (u01 + u11 - u00 - u10) / cdy + 0.5 * (v10 + v11 - v00 - v01) / cdx;
-      double pgradx = (p_e - p_w) / (cdx + cdx1);
-      double pgrady = (p_p - p_m) / (cdy + cdy2);
-      double pgradx2 = pgradx * pgradx;
-      double pgrady2 = pgrady * pgrady;
-      double limiter = ((0.5 * (ugrad) / cdx) * pgradx2 + (0.5 * (vgrad) / cdy) * pgrady2 + strain2 * pgradx * pgrady) /
-                       fmax(pgradx2 + pgrady2, g_small);
-      if (!(limiter > 0.0)) {
-        double dirx = 1.0;
-        if (pgradx < 0.0) dirx = -1.0;
-        pgradx = dirx * fmax(g_small, fabs(pgradx));
-        double diry = 1.0;
-        if (pgradx < 0.0) diry = -1.0;
-        pgrady = diry * fmax(g_small, fabs(pgrady));
-        double pgrad = sqrt(pgradx * pgradx + pgrady * pgrady);
-        double xgrad = fabs(cdx * pgrad / pgradx);
-        double ygrad = fabs(cdy * pgrad / pgrady);
-        double grad = fmin(xgrad, ygrad);
-        double grad2 = grad * grad;
-        q = 2.0 * rho_c * grad2 * limiter * limiter;
-      }
-    }
+    const double q = viscosity_cell(u00, u10, u01, u11, v00, v10, v01, v11, p_w, p_e, p_m, p_p, cdx, cdx1, cdy, cdy2, rho_c);
     // calc_dt, src/serial/calc_dt.cpp:52-75
     {
       double dsx = cdx;
@@ -272,7 +283,7 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
       if (own) dtmin = fmin(mins, dtmin);
     }
     // p is not stored in the fused step: lagrange_tile recomputes it from the same rho0, e0
-    if (own) AT(CLV_VISCOSITY, i, r) = q;
+    if (STORE_Q && own) AT(CLV_VISCOSITY, i, r) = q;
     if (WRITE_PC && own) {
       AT(CLV_PRESSURE, i, r) = p_c;
       AT(CLV_SOUNDSPEED, i, r) = c_c;
@@ -290,13 +301,13 @@ __device__ __forceinline__ void timestep_tile(const V &d, int bx, int j0, int j1
 }
 
 // 5 waves per SIMD fit without spilling (96 VGPRs)
-template <bool MESH, bool WRITE_PC = false>
+template <bool MESH, bool WRITE_PC = false, bool STORE_Q = true>
 __global__ __launch_bounds__(256, 5) void timestep_k(Dev d, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe,
                                                   double *partials, int nstrips, int seg) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  timestep_tile<MESH, WRITE_PC>(d, bx, j0, min(j0 + seg, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe,
+  timestep_tile<MESH, WRITE_PC, STORE_Q>(d, bx, j0, min(j0 + seg, d.ny + 2), dtc_safe, dtu_safe, dtv_safe, dtdiv_safe,
                                 partials + (size_t)by * nstrips, nstrips);
 }
 
@@ -361,8 +372,14 @@ __device__ __forceinline__ double predicted_pressure(const PredIn &a, double dt,
 // path, whose caller may read them -- the fused step never does).  PRED (with
 // WRITE_PC): also the predictor's density1 and energy1 of the owned cells into
 // rho_pred / e_pred (what PdV(predict) leaves in density1 / energy1 before revert;
-// the adjoint's undo log).
-template <bool MESH, bool WRITE_PC = false, bool PRED = false, class V>
+// the adjoint's undo log).  OWN_Q: q of the cells the march needs (rows j0 - 1 .. jc1,
+// columns of lanes 1..62) computed here, as timestep_tile computes it, instead of read
+// (the fused step, whose timestep pass does not store q); its reflective ghosts (the
+// halo of driver/timestep.cpp:66-69) by lane shuffle and row carry.
+// Lanes: column x0 + L, x0 = kOwnLagrange * strip; lanes 2..1 + kOwnLagrange own
+// cells and nodes (lane 2 + kOwnLagrange the boundary node nx + 2); lanes 0, 1 and
+// 62, 63 hold the neighbours the stencils reach (q of lane 1 needs p of lane 0).
+template <bool MESH, bool WRITE_PC = false, bool PRED = false, bool OWN_Q = false, class V>
 __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc1, double dt, int nstrips, double *rho_pred = nullptr,
                                               double *e_pred = nullptr) {
   static_assert(!PRED || WRITE_PC, "the predictor's density and energy are stored with its pressure");
@@ -372,11 +389,12 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   // with clamped loads and owns nothing
   const bool live = strip < nstrips;
   const int nx = d.nx, ny = d.ny;
-  const int x0 = 1 + kOwnLagrange * strip;
+  const int x0 = kOwnLagrange * strip;
   const int i = x0 + lane;
   const int ic = i < nx + 4 ? i : nx + 4;
-  const bool own_cell = live && lane >= 1 && lane <= kOwnLagrange && i <= nx + 1;
-  const bool own_node = live && ((lane >= 1 && lane <= kOwnLagrange && i <= nx + 2) || (lane == kOwnLagrange + 1 && i == nx + 2));
+  const bool own_cell = live && lane >= 2 && lane <= kOwnLagrange + 1 && i <= nx + 1;
+  const bool own_node =
+      live && ((lane >= 2 && lane <= kOwnLagrange + 1 && i <= nx + 2) || (lane == kOwnLagrange + 2 && i == nx + 2));
   __shared__ double stage[2][PRED ? 10 : (WRITE_PC ? 8 : 6)][BlockRows<kOwnLagrange, true>::kSlots];
   const BlockRows<kOwnLagrange, true> out(bx, nx);
   const int pos = out.pos(i);
@@ -399,15 +417,17 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
     a.v10 = dn1(v_r);
     a.v11 = dn1(v_r1);
     a.vol = geo_volume_x<MESH>(d, cdx_i, ic, r);
-    // ghost rho0 through the mirror (the halo of driver/timestep.cpp:53-60)
-    a.rho0 = AT(CLV_DENSITY0, mirror_cell(ic, nx), mirror_cell(r, ny));
-    a.e0 = AT(CLV_ENERGY0, ic, r);
-    // ideal_gas of the timestep (driver/timestep.cpp:48), recomputed from the
-    // same operands with the same expression instead of re-read: bit-identical
-    // on the interior cells, the only cells whose predictor result is used (the
-    // ghost rows and columns of p' are mirrored from the interior below)
-    a.p = pressure_of(a.rho0, a.e0);
-    a.q = AT(CLV_VISCOSITY, ic, r);
+    if constexpr (!OWN_Q) {  // else carried by the march below
+      // ghost rho0 through the mirror (the halo of driver/timestep.cpp:53-60)
+      a.rho0 = AT(CLV_DENSITY0, mirror_cell(ic, nx), mirror_cell(r, ny));
+      a.e0 = AT(CLV_ENERGY0, ic, r);
+      // ideal_gas of the timestep (driver/timestep.cpp:48), recomputed from the
+      // same operands with the same expression instead of re-read: bit-identical
+      // on the interior cells, the only cells whose predictor result is used (the
+      // ghost rows and columns of p' are mirrored from the interior below)
+      a.p = pressure_of(a.rho0, a.e0);
+      a.q = AT(CLV_VISCOSITY, ic, r);
+    }
   };
   // predictor pressure of column i, with the reflective ghost columns taking the
   // interior value (the pressure halo of src/serial/PdV.cpp:121-125)
@@ -424,12 +444,45 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   // ghost row 1 for the first segment, whose p' is the mirror of row 2)
   int r = j0 - 1;
   double u_r = AT(CLV_XVEL0, ic, r), v_r = AT(CLV_YVEL0, ic, r);
+  // OWN_Q: rho0, e0 and p of cell row r and p of row r - 1, rho0 and e0 read through
+  // their mirrors (the halo of driver/timestep.cpp:53-60), p recomputed as the
+  // timestep's ideal_gas computes it; celldx of the column and the next (clamped
+  // past the chunk, where q is not used)
+  const int im = mirror_cell(ic, nx);
+  const double cdx_q = OWN_Q ? M(CLV_CELLDX, ic < nx + 3 ? ic : nx + 3) : 0.0;
+  const double cdx1_q = OWN_Q ? M(CLV_CELLDX, ic + 1 < nx + 3 ? ic + 1 : nx + 3) : 0.0;
+  double rho_c = 0.0, e_c = 0.0, p_c = 0.0, p_m = 0.0;
+  if constexpr (OWN_Q) {
+    const int rm = mirror_cell(r - 1, ny), rc = mirror_cell(r, ny);
+    p_m = pressure_of(AT(CLV_DENSITY0, im, rm), AT(CLV_ENERGY0, im, rm));
+    rho_c = AT(CLV_DENSITY0, im, rc);
+    e_c = AT(CLV_ENERGY0, im, rc);
+    p_c = pressure_of(rho_c, e_c);
+  }
   for (; r <= jc1; ++r) {
     const double u_r1 = AT(CLV_XVEL0, ic, r + 1);
     const double v_r1 = AT(CLV_YVEL0, ic, r + 1);
     // cell row r inputs
     PredIn a;
     load_pred(r, u_r, v_r, u_r1, v_r1, a);
+    if constexpr (OWN_Q) {
+      const int rn = mirror_cell(r + 1, ny);
+      const double rho_n = AT(CLV_DENSITY0, im, rn), e_n = AT(CLV_ENERGY0, im, rn);
+      const double p_n = pressure_of(rho_n, e_n);
+      const double cdy = MU(CLV_CELLDY, r), cdy2 = MU(CLV_CELLDY, r + 2 < ny + 3 ? r + 2 : ny + 3);
+      double q = viscosity_cell(u_r, dn1(u_r), u_r1, dn1(u_r1), v_r, dn1(v_r), v_r1, dn1(v_r1), up1(p_c), dn1(p_c), p_m, p_n, cdx_q,
+                                cdx1_q, cdy, cdy2, rho_c);
+      q = mirror_cols(q);           // ghost columns 1, nx + 2 from 2, nx + 1
+      if (r == ny + 2) q = q_prev;  // ghost row ny + 2 from ny + 1 (row 1 from row 2 below)
+      a.rho0 = rho_c;
+      a.e0 = e_c;
+      a.p = p_c;
+      a.q = q;
+      p_m = p_c;
+      p_c = p_n;
+      rho_c = rho_n;
+      e_c = e_n;
+    }
     double pp, cp = 0.0, rp = 0.0, ep = 0.0;
     if (r == 1) {
       pp = 0.0;  // set from row 2 below, before any use
@@ -439,6 +492,7 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
       pp = mirror_cols(predicted_pressure<WRITE_PC>(a, dt, cp, rp, ep));
     }
     if (r == 2) pp_prev = pp;  // ghost row 1 mirrors row 2
+    if (OWN_Q && r == 2) q_prev = a.q;
     const double m = a.rho0 * a.vol;
     if (r >= j0) {
       // accelerate at node (i, r), src/serial/accelerate.cpp:43-53
@@ -533,12 +587,14 @@ __device__ __forceinline__ void lagrange_tile(const V &d, int bx, int j0, int jc
   __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
-template <bool MESH, bool WRITE_PC = false, bool PRED = false>
-__global__ __launch_bounds__(256) void lagrange_k(Dev d, double dt, int nstrips, int seg, double *rho_pred, double *e_pred) {
+// 4 waves per SIMD fit without spilling (120-126 VGPRs; unbounded, the OWN_Q form with
+// the geometry from the mesh took 156 and 3 waves)
+template <bool MESH, bool WRITE_PC = false, bool PRED = false, bool OWN_Q = false>
+__global__ __launch_bounds__(256, 4) void lagrange_k(Dev d, double dt, int nstrips, int seg, double *rho_pred, double *e_pred) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  lagrange_tile<MESH, WRITE_PC, PRED>(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips, rho_pred, e_pred);
+  lagrange_tile<MESH, WRITE_PC, PRED, OWN_Q>(d, bx, j0, min(j0 + seg, d.ny + 2), dt, nstrips, rho_pred, e_pred);
 }
 
 // ---- advec_cell -------------------------------------------------------------------------
@@ -1082,13 +1138,13 @@ __global__ __launch_bounds__(256, 5) void batch_timestep_k(const Dev *__restrict
     const BatchConst c = pc[it.p];
     const int ns = strips_of(BK_TIMESTEP, d.nx);
     const int j0 = 2 + (int)it.k * it.seg;
-    timestep_tile<MESH>(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), c.dtc_safe, c.dtu_safe, c.dtv_safe, c.dtdiv_safe,
+    timestep_tile<MESH, false, false>(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), c.dtc_safe, c.dtu_safe, c.dtv_safe, c.dtdiv_safe,
                   c.partials + (size_t)it.k * ns, ns);
   }
 }
 
 template <bool MESH>
-__global__ __launch_bounds__(256) void batch_lagrange_k(const Dev *__restrict__ tab, const BatchState *__restrict__ ps,
+__global__ __launch_bounds__(256, 4) void batch_lagrange_k(const Dev *__restrict__ tab, const BatchState *__restrict__ ps,
                                                       const BatchItem *__restrict__ items, int nitems) {
   const ItemRange rg = batch_items(nitems);
   for (int w = rg.first; w < rg.end; w += rg.stride) {
@@ -1097,7 +1153,7 @@ __global__ __launch_bounds__(256) void batch_lagrange_k(const Dev *__restrict__ 
     const GDev d = global_view(tab[it.p]);
     const double dt = ps[it.p].dt;
     const int j0 = 2 + (int)it.k * it.seg;
-    lagrange_tile<MESH>(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), dt, strips_of(BK_LAGRANGE, d.nx));
+    lagrange_tile<MESH, false, false, true>(d, it.bx, j0, min(j0 + (int)it.seg, d.ny + 2), dt, strips_of(BK_LAGRANGE, d.nx));
   }
 }
 
@@ -1257,8 +1313,10 @@ static clv_status require_owning(clv_chunk *c) {
 }
 
 // timestep_k over the chunk and the two-level minimum of its partials into
-// c->result_dev, then copied to c->result_host (asynchronously)
-static clv_status timestep_launch(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, bool write_pc) {
+// c->result_dev, then copied to c->result_host (asynchronously).  write_pc: store p and
+// c too; store_q: store q (the fused step does not: its lagrange pass recomputes q).
+static clv_status timestep_launch(clv_chunk *c, double dtc_safe, double dtu_safe, double dtv_safe, double dtdiv_safe, bool write_pc,
+                                  bool store_q) {
   const Dev &d = c->d;
   const int seg = c->seg[SF_TIMESTEP];
   const int ns = ceil_div(d.nx, kOwnTimestep), nseg = ceil_div(d.ny, seg);
@@ -1268,8 +1326,11 @@ static clv_status timestep_launch(clv_chunk *c, double dtc_safe, double dtu_safe
   if (write_pc) {
     const auto kern = d.geom_mesh ? timestep_k<true, true> : timestep_k<false, true>;
     LAUNCH_F("f_timestep_pc", kern, g, d, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, c->partials, ns, seg);
-  } else {
+  } else if (store_q) {
     const auto kern = d.geom_mesh ? timestep_k<true> : timestep_k<false>;
+    LAUNCH_F("f_timestep_q", kern, g, d, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, c->partials, ns, seg);
+  } else {
+    const auto kern = d.geom_mesh ? timestep_k<true, false, false> : timestep_k<false, false, false>;
     LAUNCH_F("f_timestep", kern, g, d, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, c->partials, ns, seg);
   }
   before_launch(c, "calc_dt_final");
@@ -1288,9 +1349,8 @@ static clv_status timestep_launch(clv_chunk *c, double dtc_safe, double dtu_safe
 clv_status fused_timestep_kernels(clv_chunk *c, const clv_config *cfg, double *dt_min_val) {
   clv_status st = require_owning(c);
   if (st != CLV_OK) return st;
-  if ((st = timestep_launch(c, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe, false)) != CLV_OK) return st;
-  // q's reflective ghosts for the accelerate stencil (driver/timestep.cpp:66-69)
-  if ((st = halo(c, {CLV_VISCOSITY}, 1)) != CLV_OK) return st;
+  // q is not stored: the lagrange pass recomputes it with its ghosts (driver/timestep.cpp:62-69)
+  if ((st = timestep_launch(c, cfg->dtc_safe, cfg->dtu_safe, cfg->dtv_safe, cfg->dtdiv_safe, false, false)) != CLV_OK) return st;
   if ((st = hip_check(hipStreamSynchronize(c->stream), "fused dt sync")) != CLV_OK) return st;
   *dt_min_val = c->result_host[0];
   return CLV_OK;
@@ -1306,7 +1366,7 @@ clv_status fused_timestep_entry(clv_chunk *c, double dtc_safe, double dtu_safe, 
                                 double *dt_min_val) {
   if (c->d.nx < 4 || c->d.ny < 4) return set_error(CLV_ERR_ARG, "clv_timestep_kernels needs x_cells, y_cells >= 4");
   choose_segments(c);
-  clv_status st = timestep_launch(c, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, p_depth > 0);
+  clv_status st = timestep_launch(c, dtc_safe, dtu_safe, dtv_safe, dtdiv_safe, p_depth > 0, true);
   if (st != CLV_OK) return st;
   if (p_depth > 0 && (st = halo(c, {CLV_PRESSURE}, p_depth)) != CLV_OK) return st;
   if ((st = halo(c, {CLV_VISCOSITY}, 1)) != CLV_OK) return st;
@@ -1346,7 +1406,7 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
   const Dev &d = c->d;
   const int seg = c->seg[SF_LAGRANGE];
   const int ns = ceil_div(d.nx, kOwnLagrange), nseg = ceil_div(d.ny, seg);
-  const auto kern = d.geom_mesh ? lagrange_k<true> : lagrange_k<false>;
+  const auto kern = d.geom_mesh ? lagrange_k<true, false, false, true> : lagrange_k<false, false, false, true>;
   LAUNCH_F("f_lagrange", kern, dim3(ceil_div(ns, WAVES), nseg, 1), d, dt, ns, seg, nullptr, nullptr);
   return CLV_OK;
 }

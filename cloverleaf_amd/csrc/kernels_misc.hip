@@ -522,6 +522,7 @@ __global__ __launch_bounds__(256) void march_probe_k(MarchProbe p) {
   const int i = 2 + (bx * 4 + (int)(threadIdx.x >> 6)) * 64 + (int)(threadIdx.x & 63);
   if (i > p.nx + 1) return;
   const int j0 = 2 + by * p.seg, j1 = min(j0 + p.seg, p.ny + 2);
+  double acc = 0.0;
   for (int j = j0; j < j1; ++j) {
     const int64_t o = (int64_t)j * p.pitch + i;
     double s = 0.0;
@@ -529,7 +530,11 @@ __global__ __launch_bounds__(256) void march_probe_k(MarchProbe p) {
     for (int a = 0; a < R; ++a) s += p.in[a][o];
 #pragma unroll
     for (int w = 0; w < W; ++w) __builtin_nontemporal_store(s + (double)w, &p.out[w][o]);  // as lagrange_k / cm_x_k store
+    if (W == 0) acc += s;
   }
+  // a shape that writes nothing keeps its loads through a store that the zero-filled
+  // inputs never reach
+  if (W == 0 && acc == 1.0) const_cast<double *>(p.in[0])[(int64_t)j0 * p.pitch + i] = 0.0;
 }
 
 // ---- ghost frame copy ---------------------------------------------------------------------
@@ -864,8 +869,14 @@ extern "C" clv_status clv_march_probe(int x_cells, int y_cells, int reads, int w
   if (reads == 4 && writes == 1) kern = march_probe_k<4, 1>;
   if (reads == 5 && writes == 6) kern = march_probe_k<5, 6>;
   if (reads == 5 && writes == 5) kern = march_probe_k<5, 5>;
+  if (reads == 7 && writes == 0) kern = march_probe_k<7, 0>;
+  if (reads == 7 && writes == 6) kern = march_probe_k<7, 6>;
+  if (reads == 4 && writes == 0) kern = march_probe_k<4, 0>;
+  if (reads == 4 && writes == 6) kern = march_probe_k<4, 6>;
   if (!kern)
-    return set_error(CLV_ERR_ARG, "clv_march_probe: (reads, writes) must be (7, 1), (8, 6), (7, 5), (6, 5), (4, 1), (5, 6) or (5, 5)");
+    return set_error(CLV_ERR_ARG,
+                     "clv_march_probe: (reads, writes) must be (7, 1), (8, 6), (7, 5), (6, 5), (4, 1), (5, 6), (5, 5), (7, 0), (7, 6), "
+                     "(4, 0) or (4, 6)");
   MarchProbe p{};
   p.pitch = pitch_for(x_cells);
   p.nx = x_cells;

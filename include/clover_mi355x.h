@@ -357,9 +357,9 @@ clv_status clv_deck_parse(const char *text, clv_config *out);
  * Resets *st; summary0 (may be NULL) receives the step-0 field summary. */
 clv_status clv_hydro_start(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, double summary0[5]);
 /* driver/timestep.cpp:37-108: ideal_gas, halos, viscosity, calc_dt, dt selection.
- * With CLV_PLAN_FUSED the pressure and soundspeed arrays are not written (the
- * fused step recomputes them where it needs them); follow it with the fused
- * step, not with the per-kernel entry points. */
+ * With CLV_PLAN_FUSED the pressure, soundspeed and viscosity arrays are not
+ * written (the fused step recomputes them where it needs them); follow it with
+ * the fused step, not with the per-kernel entry points. */
 clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan);
 /* One step of driver/hydro.cpp:57-89 (timestep, PdV predict, accelerate, PdV
  * correct, flux_calc, advection, reset_field, advect_x flip, time += dt); sets
@@ -520,7 +520,8 @@ clv_status clv_bandwidth_probe(size_t bytes, int iters, double *copy_gbps);
  * x-fastest on one XCD band each) over the interior of an x_cells * y_cells
  * grid, reading `reads` and writing `writes` FP64 arrays per cell with no
  * arithmetic (streaming stores); the arrays are padded like a chunk's, element
- * (2, j) on a 128-byte line.  (reads, writes) is (7, 1), (8, 6), (7, 5), (6, 5), (4, 1), (5, 6) or (5, 5).
+ * (2, j) on a 128-byte line.  (reads, writes) is (7, 1), (8, 6), (7, 5), (6, 5), (4, 1),
+ * (5, 6), (5, 5), (7, 0), (7, 6), (4, 0) or (4, 6).
  * Allocates and frees the arrays; *ms_per_launch = the mean of `iters` timed
  * launches. */
 clv_status clv_march_probe(int x_cells, int y_cells, int reads, int writes, int seg, int iters, double *ms_per_launch);
