@@ -102,27 +102,32 @@ ALG_TERMS = {
     "f_timestep_pc": [(10, "C")],              # the same -> q, p, c (the per-entry-point path)
     "f_lagrange_pc": [(16, "C")],              # rho0, e0, q, u0, v0, vol, areas -> f_lagrange's + p', c' (the per-entry-point path)
     "f_lagrange": [(13, "C")],                 # rho0, e0, u0, v0, vol, xarea, yarea -> u1, v1, rho1, e1, vfx, vfy (q recomputed)
-    "f_cm_x1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
-    "f_cm_y1": [(12, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
-    "f_cm_x2": [(11, "C")],                    # vol, vfx, rho1, e1, u1, v1 -> rho1, e1, mfx, u1, v1
-    "f_cm_y2": [(11, "C")],                    # vol, vfy, rho1, e1, u1, v1 -> rho1, e1, mfy, u1, v1
+    "f_cm_x1": [(11, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, u1, v1 (mass flux not stored)
+    "f_cm_y1": [(11, "C")],                    # vol, vfx, vfy, rho1, e1, u1, v1 -> rho1, e1, u1, v1
+    "f_cm_x2": [(10, "C")],                    # vol, vfx, rho1, e1, u1, v1 -> rho1, e1, u1, v1
+    "f_cm_y2": [(10, "C")],                    # vol, vfy, rho1, e1, u1, v1 -> rho1, e1, u1, v1
+    "f_cm_x1_mf": [(12, "C")],                 # the same -> + mfx (the per-entry-point path)
+    "f_cm_y1_mf": [(12, "C")],
+    "f_cm_x2_mf": [(11, "C")],
+    "f_cm_y2_mf": [(11, "C")],
 }
 # of those, the volume / face-area arrays a kernel does not stream when the chunk's
 # geometry comes from the mesh (CLV_GEOMETRY_MESH), on the first term's extent
 GEOM_ARRAYS = {"calc_dt": 3, "PdV_predict": 3, "PdV_correct": 3, "accelerate": 3, "flux_calc": 2, "advec_cell_vol": 1,
                "advec_mom_vol": 1, "field_summary": 1, "field_summary_gas": 1, "f_timestep": 3, "f_timestep_q": 3, "f_timestep_pc": 3,
-               "f_lagrange": 3, "f_lagrange_pc": 3, "f_cm_x1": 1, "f_cm_y1": 1, "f_cm_x2": 1, "f_cm_y2": 1}
+               "f_lagrange": 3, "f_lagrange_pc": 3, "f_cm_x1": 1, "f_cm_y1": 1, "f_cm_x2": 1, "f_cm_y2": 1, "f_cm_x1_mf": 1,
+               "f_cm_y1_mf": 1, "f_cm_x2_mf": 1, "f_cm_y2_mf": 1}
 # compulsory traffic of one fused step, array sweeps: timestep 7 + lagrange 13 + the
-# first advection pass 12 + the second 11 (geometry read); 4 + 10 + 11 + 10 from the mesh
-B_FUSED_PER_CELL_STEP = {False: 43 * 8.0, True: 35 * 8.0}
+# first advection pass 11 + the second 10 (geometry read); 4 + 10 + 10 + 9 from the mesh
+B_FUSED_PER_CELL_STEP = {False: 41 * 8.0, True: 33 * 8.0}
 B_ALG_PER_CELL_STEP = 1440.0  # 180 array sweeps of 8 B per interior cell-step at reference kernel granularity
 
 
 # arrays read and written per interior cell by each fused kernel (the march probe's shape)
-PATTERN = {False: {"f_timestep": (7, 0), "f_lagrange": (7, 6), "f_cm_x1": (7, 5), "f_cm_y1": (7, 5), "f_cm_x2": (6, 5),
-                   "f_cm_y2": (6, 5)},
-           True: {"f_timestep": (4, 0), "f_lagrange": (4, 6), "f_cm_x1": (6, 5), "f_cm_y1": (6, 5), "f_cm_x2": (5, 5),
-                  "f_cm_y2": (5, 5)}}
+PATTERN = {False: {"f_timestep": (7, 0), "f_lagrange": (7, 6), "f_cm_x1": (7, 4), "f_cm_y1": (7, 4), "f_cm_x2": (6, 4),
+                   "f_cm_y2": (6, 4)},
+           True: {"f_timestep": (4, 0), "f_lagrange": (4, 6), "f_cm_x1": (6, 4), "f_cm_y1": (6, 4), "f_cm_x2": (5, 4),
+                  "f_cm_y2": (5, 4)}}
 
 
 def alg_bytes(name, nx, ny, geom_mesh=False):

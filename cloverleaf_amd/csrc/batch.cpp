@@ -7,9 +7,11 @@
 //   batch_lagrange_k  (q recomputed, as the timestep pass does not store it)
 //   batch_halo_k      energy1, density1, vol_flux_x/y, xvel1, yvel1, depth 2
 //   batch_advec_k     advection pass 1 (advec_cell + advec_mom), dir1
-//   batch_halo_k      density1, energy1, xvel1, yvel1, mass_flux_x/y, depth 2
+//   batch_halo_k      density1, energy1, xvel1, yvel1, depth 2
 //   batch_advec_k     advection pass 2, dir2
-//   batch_halo_k      density1, energy1, mass_flux_x/y, depth 2
+//   batch_halo_k      density1, energy1, depth 2
+//
+// (the advection passes leave the mass fluxes unstored, as the plain fused step does)
 //
 // with no host synchronisation: dt never leaves the device, so the eight
 // launches are captured once as a hipGraph and replayed per step.
@@ -247,9 +249,9 @@ clv_status emit_step(clv_batch *b, int q) {
   TRY_B(launch_family(b, BK_LAGRANGE, CLV_BF_LAGRANGE, s0));
   TRY_B(launch_halo(b, s0, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2));
   TRY_B(launch_family(b, x_first ? BK_CM_X1 : BK_CM_Y1, fam1, s0));
-  TRY_B(launch_halo(b, s1, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2));
+  TRY_B(launch_halo(b, s1, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1}, 2));
   TRY_B(launch_family(b, x_first ? BK_CM_Y2 : BK_CM_X2, fam2, s1));
-  TRY_B(launch_halo(b, s2, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2));
+  TRY_B(launch_halo(b, s2, {CLV_DENSITY1, CLV_ENERGY1}, 2));
 #undef TRY_B
   return CLV_OK;
 }

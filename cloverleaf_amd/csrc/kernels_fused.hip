@@ -677,8 +677,10 @@ __device__ __forceinline__ double mom_flux_of(double flux, double mass_pre_donor
 // columns.  RESET: the pass also stores its new density, energy and velocities into
 // density0, energy0, xvel0 and yvel0 -- reset_field (src/serial/reset_field.cpp:29-42:
 // cells [2, nx + 1] x [2, ny + 1], nodes [2, nx + 2] x [2, ny + 2]) -- over the
-// same cells and nodes as its own stores.
-template <int SWEEP, bool MESH, bool RESET = false, class V, class O>
+// same cells and nodes as its own stores.  STORE_MF: store the mass fluxes (the
+// per-entry-point path, whose caller may read them; in the fused step nothing reads
+// them after the pass, and the next step's pass rewrites them).
+template <int SWEEP, bool MESH, bool RESET = false, bool STORE_MF = true, class V, class O>
 __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 1 : 3;
   const int lane = threadIdx.x & 63;
@@ -746,7 +748,7 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
         stage[r & 1][0][pos] = cr.rho;
         stage[r & 1][1][pos] = cr.e;
       }
-      if (own_face) stage[r & 1][2][pos] = cr.mf;
+      if (STORE_MF && own_face) stage[r & 1][2][pos] = cr.mf;
     }
     // advec_mom's view of the cell row: ghost columns mirrored
     const double rho_h = __shfl(cr.rho, lane_c), mfx = sgn_f * __shfl(mf, lane_f);
@@ -793,7 +795,7 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
             ST_STREAM(AT(CLV_ENERGY0, xs, r), stage[r & 1][1][t]);
           }
         }
-        if (out.node(t)) ST_STREAM(AT(CLV_MASS_FLUX_X, xs, r), stage[r & 1][2][t]);
+        if (STORE_MF && out.node(t)) ST_STREAM(AT(CLV_MASS_FLUX_X, xs, r), stage[r & 1][2][t]);
       }
       if (out.node(t)) {
         ST_STREAM(u_out[o], stage[r & 1][3][t]);
@@ -810,12 +812,12 @@ __device__ __forceinline__ void cm_x_tile(const V &d, int bx, int j0, int j1, in
   __syncthreads();  // the stage is reused by the block's next tile (batched form)
 }
 
-template <int SWEEP, bool MESH, bool RESET = false>
+template <int SWEEP, bool MESH, bool RESET = false, bool STORE_MF = true>
 __global__ __launch_bounds__(256) void cm_x_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  cm_x_tile<SWEEP, MESH, RESET>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+  cm_x_tile<SWEEP, MESH, RESET, STORE_MF>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
 // y direction, advec_cell then advec_mom of the same sweep in one pass (cell sweep 1
@@ -859,8 +861,8 @@ struct CmRing {
   __device__ void put_post(int row, double v) const { put(kRingMf + kRingRho + row % kRingPost, v); }
   __device__ double post(int row) const { return get(kRingMf + kRingRho + row % kRingPost); }
 };
-// RESET as for cm_x_tile.
-template <int SWEEP, bool MESH, bool LOCKSTEP = true, bool RESET = false, int NW = WAVES, class V, class O>
+// RESET and STORE_MF as for cm_x_tile.
+template <int SWEEP, bool MESH, bool LOCKSTEP = true, bool RESET = false, int NW = WAVES, bool STORE_MF = true, class V, class O>
 __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, int nstrips, O rho_out, O e_out, O u_out, O v_out) {
   constexpr int MOM_SWEEP = SWEEP == 1 ? 2 : 4;
   const int lane = threadIdx.x & 63;
@@ -960,7 +962,7 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
       const double vd_face = cur.vd, rho_p1 = cur.rho_p1, e_p1 = cur.e_p1, vd_p1 = cur.vd_p1;
       double mf, ef;
       cell_face_flux(vf, pre_m1, pre_0, rho_m2, rho_m1, rho_0, rho_p1, e_m2, e_m1, e_0, e_p1, vd_face, vd_m1, vd_p1, mf, ef);
-      if (own_cell && F >= j0 && F < j1) AT(CLV_MASS_FLUX_Y, i, F) = mf;
+      if (STORE_MF && own_cell && F >= j0 && F < j1) AT(CLV_MASS_FLUX_Y, i, F) = mf;
       mf_h = mf;
       if (F > Fs) {
         // cell F - 1 from faces F - 1 and F, src/serial/advec_cell.cpp:211-216
@@ -1061,12 +1063,12 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   }
 }
 
-template <int SWEEP, bool MESH, bool RESET = false, int NW = WAVES>
+template <int SWEEP, bool MESH, bool RESET = false, int NW = WAVES, bool STORE_MF = true>
 __global__ __launch_bounds__(64 * NW, 16 / NW) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
-  cm_y_tile<SWEEP, MESH, true, RESET, NW>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
+  cm_y_tile<SWEEP, MESH, true, RESET, NW, STORE_MF>(d, bx, j0, min(j0 + seg, d.ny + 3), nstrips, rho_out, e_out, u_out, v_out);
 }
 
 // ---- batched form: persistent launches over the tiles of many chunks ----------------------
@@ -1169,10 +1171,14 @@ __global__ __launch_bounds__(256) void batch_advec_k(const Dev *__restrict__ tab
     const GDev d = global_view(tab[it.p]);
     const int ns = strips_of(KIND, d.nx);
     const int j0 = 2 + (int)it.k * it.seg, jn1 = min(j0 + (int)it.seg, d.ny + 3);
-    if constexpr (KIND == BK_CM_X1) cm_x_tile<1, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_X2) cm_x_tile<2, MESH>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_Y1) cm_y_tile<1, MESH, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
-    if constexpr (KIND == BK_CM_Y2) cm_y_tile<2, MESH, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_X1)
+      cm_x_tile<1, MESH, false, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_X2)
+      cm_x_tile<2, MESH, false, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y1)
+      cm_y_tile<1, MESH, false, false, WAVES, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
+    if constexpr (KIND == BK_CM_Y2)
+      cm_y_tile<2, MESH, false, false, WAVES, false>(d, it.bx, j0, jn1, ns, d.f[CLV_WORK1], d.f[CLV_WORK2], d.f[CLV_WORK3], d.f[CLV_WORK4]);
   }
 }
 
@@ -1417,26 +1423,30 @@ clv_status fused_lagrange(clv_chunk *c, double dt) {
 // sweep does not change those), and the kernels read the ghosts they need by mirror.
 // reset: the pass also does reset_field's stores (the kernels' RESET form).
 using CmKernel = void (*)(Dev, int, int, double *, double *, double *, double *);
-template <bool MESH, bool RESET>
+template <bool MESH, bool RESET, bool STORE_MF>
 static CmKernel cm_kernel(int sweep, int dir, bool wide) {
-  if (dir == CLV_XDIR) return sweep == 1 ? cm_x_k<1, MESH, RESET> : cm_x_k<2, MESH, RESET>;
-  if (wide) return sweep == 1 ? cm_y_k<1, MESH, RESET, kWavesCmY> : cm_y_k<2, MESH, RESET, kWavesCmY>;
-  return sweep == 1 ? cm_y_k<1, MESH, RESET> : cm_y_k<2, MESH, RESET>;
+  if (dir == CLV_XDIR) return sweep == 1 ? cm_x_k<1, MESH, RESET, STORE_MF> : cm_x_k<2, MESH, RESET, STORE_MF>;
+  if (wide) return sweep == 1 ? cm_y_k<1, MESH, RESET, kWavesCmY, STORE_MF> : cm_y_k<2, MESH, RESET, kWavesCmY, STORE_MF>;
+  return sweep == 1 ? cm_y_k<1, MESH, RESET, WAVES, STORE_MF> : cm_y_k<2, MESH, RESET, WAVES, STORE_MF>;
+}
+template <bool MESH>
+static CmKernel cm_kernel(int sweep, int dir, bool wide, bool reset, bool store_mf) {
+  if (reset) return cm_kernel<MESH, true, true>(sweep, dir, wide);
+  return store_mf ? cm_kernel<MESH, false, true>(sweep, dir, wide) : cm_kernel<MESH, false, false>(sweep, dir, wide);
 }
 
-static clv_status cm_sweep(clv_chunk *c, int sweep, int dir, bool reset = false) {
+// store_mf false: the fused step's form, the mass fluxes not stored (reset implies them stored)
+static clv_status cm_sweep(clv_chunk *c, int sweep, int dir, bool reset, bool store_mf) {
   const Dev &d = c->d;
   double *rho_out = d.f[CLV_WORK1], *e_out = d.f[CLV_WORK2], *u_out = d.f[CLV_WORK3], *v_out = d.f[CLV_WORK4];
   const int own = dir == CLV_XDIR ? kOwnCmX : kOwnCmY, seg = c->seg[dir == CLV_XDIR ? SF_CM_X : SF_CM_Y];
   const int ns = ceil_div(d.nx + 1, own);
   const bool wide = dir == CLV_YDIR && ns >= kWideCmY;
   const int waves = wide ? kWavesCmY : WAVES;
-  const CmKernel kern = d.geom_mesh
-                            ? (reset ? cm_kernel<true, true>(sweep, dir, wide) : cm_kernel<true, false>(sweep, dir, wide))
-                            : (reset ? cm_kernel<false, true>(sweep, dir, wide) : cm_kernel<false, false>(sweep, dir, wide));
-  static const char *const names[2][2][2] = {{{"f_cm_x1", "f_cm_x1_reset"}, {"f_cm_x2", "f_cm_x2_reset"}},
-                                             {{"f_cm_y1", "f_cm_y1_reset"}, {"f_cm_y2", "f_cm_y2_reset"}}};
-  const char *name = names[dir == CLV_XDIR ? 0 : 1][sweep == 1 ? 0 : 1][reset ? 1 : 0];
+  const CmKernel kern = d.geom_mesh ? cm_kernel<true>(sweep, dir, wide, reset, store_mf) : cm_kernel<false>(sweep, dir, wide, reset, store_mf);
+  static const char *const names[2][2][3] = {{{"f_cm_x1", "f_cm_x1_mf", "f_cm_x1_reset"}, {"f_cm_x2", "f_cm_x2_mf", "f_cm_x2_reset"}},
+                                             {{"f_cm_y1", "f_cm_y1_mf", "f_cm_y1_reset"}, {"f_cm_y2", "f_cm_y2_mf", "f_cm_y2_reset"}}};
+  const char *name = names[dir == CLV_XDIR ? 0 : 1][sweep == 1 ? 0 : 1][reset ? 2 : (store_mf ? 1 : 0)];
   LAUNCH_FT(name, kern, dim3(ceil_div(ns, waves), ceil_div(d.ny + 1, seg), 1), 64 * waves, d, ns, seg, rho_out, e_out,
             u_out, v_out);
   swap_fields(c, CLV_DENSITY1, CLV_WORK1);
@@ -1458,7 +1468,7 @@ clv_status fused_advection_entry(clv_chunk *c, int sweep, int dir, bool reset) {
   choose_segments(c);
   clv_status st = k_copy_frame(c, CLV_XVEL1, CLV_WORK3);
   if (st == CLV_OK) st = k_copy_frame(c, CLV_YVEL1, CLV_WORK4);
-  if (st == CLV_OK) st = cm_sweep(c, sweep, dir, reset);
+  if (st == CLV_OK) st = cm_sweep(c, sweep, dir, reset, true);
   if (st != CLV_OK) return st;
   return halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2);
 }
@@ -1469,11 +1479,13 @@ clv_status fused_advection(clv_chunk *c, int advect_x) {
   if (st != CLV_OK) return st;
   const int dir1 = advect_x ? CLV_XDIR : CLV_YDIR, dir2 = advect_x ? CLV_YDIR : CLV_XDIR;
   if ((st = halo(c, {CLV_ENERGY1, CLV_DENSITY1, CLV_VOL_FLUX_X, CLV_VOL_FLUX_Y, CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
-  if ((st = cm_sweep(c, 1, dir1)) != CLV_OK) return st;
+  // the passes leave the mass fluxes unstored: nothing reads them before the next
+  // step's passes rewrite them (driver/advection.cpp:47-103)
+  if ((st = cm_sweep(c, 1, dir1, false, false)) != CLV_OK) return st;
   // the first pass's halo after it and the second pass's velocity halo before it
-  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
-  if ((st = cm_sweep(c, 2, dir2)) != CLV_OK) return st;
-  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_MASS_FLUX_X, CLV_MASS_FLUX_Y}, 2)) != CLV_OK) return st;
+  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1, CLV_XVEL1, CLV_YVEL1}, 2)) != CLV_OK) return st;
+  if ((st = cm_sweep(c, 2, dir2, false, false)) != CLV_OK) return st;
+  if ((st = halo(c, {CLV_DENSITY1, CLV_ENERGY1}, 2)) != CLV_OK) return st;
   // reset_field: the end-of-step fields become the start-of-step fields
   swap_fields(c, CLV_DENSITY0, CLV_DENSITY1);
   swap_fields(c, CLV_ENERGY0, CLV_ENERGY1);

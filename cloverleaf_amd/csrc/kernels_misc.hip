@@ -873,10 +873,13 @@ extern "C" clv_status clv_march_probe(int x_cells, int y_cells, int reads, int w
   if (reads == 7 && writes == 6) kern = march_probe_k<7, 6>;
   if (reads == 4 && writes == 0) kern = march_probe_k<4, 0>;
   if (reads == 4 && writes == 6) kern = march_probe_k<4, 6>;
+  if (reads == 7 && writes == 4) kern = march_probe_k<7, 4>;
+  if (reads == 6 && writes == 4) kern = march_probe_k<6, 4>;
+  if (reads == 5 && writes == 4) kern = march_probe_k<5, 4>;
   if (!kern)
     return set_error(CLV_ERR_ARG,
                      "clv_march_probe: (reads, writes) must be (7, 1), (8, 6), (7, 5), (6, 5), (4, 1), (5, 6), (5, 5), (7, 0), (7, 6), "
-                     "(4, 0) or (4, 6)");
+                     "(4, 0), (4, 6), (7, 4), (6, 4) or (5, 4)");
   MarchProbe p{};
   p.pitch = pitch_for(x_cells);
   p.nx = x_cells;

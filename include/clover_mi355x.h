@@ -521,7 +521,7 @@ clv_status clv_bandwidth_probe(size_t bytes, int iters, double *copy_gbps);
  * grid, reading `reads` and writing `writes` FP64 arrays per cell with no
  * arithmetic (streaming stores); the arrays are padded like a chunk's, element
  * (2, j) on a 128-byte line.  (reads, writes) is (7, 1), (8, 6), (7, 5), (6, 5), (4, 1),
- * (5, 6), (5, 5), (7, 0), (7, 6), (4, 0) or (4, 6).
+ * (5, 6), (5, 5), (7, 0), (7, 6), (4, 0), (4, 6), (7, 4), (6, 4) or (5, 4).
  * Allocates and frees the arrays; *ms_per_launch = the mean of `iters` timed
  * launches. */
 clv_status clv_march_probe(int x_cells, int y_cells, int reads, int writes, int seg, int iters, double *ms_per_launch);

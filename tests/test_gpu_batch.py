@@ -41,12 +41,11 @@ SMALL = [(4, 5, 15), (37, 61, 15), (64, 48, 9), (130, 97, 15), (249, 33, 12), (6
 
 def _state_regions(nx, ny):
     """Arrays that carry the fused step's state (interior cells, interior and
-    boundary nodes, and the fluxes whole; q is recomputed where it is used)."""
+    boundary nodes, and the volume fluxes whole; q and the mass fluxes are
+    recomputed where they are used)."""
     return {"density0": (slice(2, ny + 2), slice(2, nx + 2)), "energy0": (slice(2, ny + 2), slice(2, nx + 2)),
             "xvel0": (slice(2, ny + 3), slice(2, nx + 3)), "yvel0": (slice(2, ny + 3), slice(2, nx + 3)),
-            "vol_flux_x": (slice(None), slice(None)),
-            "vol_flux_y": (slice(None), slice(None)), "mass_flux_x": (slice(None), slice(None)),
-            "mass_flux_y": (slice(None), slice(None))}
+            "vol_flux_x": (slice(None), slice(None)), "vol_flux_y": (slice(None), slice(None))}
 
 
 def _start(decks, geometry=clv.GEOMETRY_ARRAYS):

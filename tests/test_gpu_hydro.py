@@ -42,16 +42,14 @@ def _fields_equal(ch, o, names):
 
 def _fused_state_equal(ch, o, nx, ny):
     """Fields that carry state between steps under the fused plan: rho0/e0 on the
-    interior cells, u0/v0 on the interior and boundary nodes, and the volume and
-    mass fluxes over their whole arrays (ghosts included).  (The fused plan keeps
-    the pressure, soundspeed, viscosity and the work arrays in registers, and
-    reset_field is a buffer swap, so those arrays hold other, dead, values.)"""
+    interior cells, u0/v0 on the interior and boundary nodes, and the volume
+    fluxes over their whole arrays (ghosts included).  (The fused plan keeps the
+    pressure, soundspeed, viscosity, mass fluxes and the work arrays in registers,
+    and reset_field is a buffer swap, so those arrays hold other, dead, values.)"""
     bad = []
     regions = {"density0": (slice(2, ny + 2), slice(2, nx + 2)), "energy0": (slice(2, ny + 2), slice(2, nx + 2)),
                "xvel0": (slice(2, ny + 3), slice(2, nx + 3)), "yvel0": (slice(2, ny + 3), slice(2, nx + 3)),
-               "vol_flux_x": (slice(None), slice(None)),
-               "vol_flux_y": (slice(None), slice(None)), "mass_flux_x": (slice(None), slice(None)),
-               "mass_flux_y": (slice(None), slice(None))}
+               "vol_flux_x": (slice(None), slice(None)), "vol_flux_y": (slice(None), slice(None))}
     for n, reg in regions.items():
         a, b = ch.download(n)[reg], o.f(n)[reg]
         if not np.array_equal(a, b):

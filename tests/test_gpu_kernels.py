@@ -316,7 +316,7 @@ def test_measurement_probes(gpu):
     """The copy-bandwidth and access-pattern probes run and time something; the
     pattern probe accepts only the fused kernels' shapes."""
     assert clv.bandwidth_probe(64 << 20, 2) > 0.0
-    for r, w in [(4, 1), (5, 6), (6, 5), (5, 5), (7, 1), (8, 6), (7, 5), (7, 0), (7, 6), (4, 0), (4, 6)]:
+    for r, w in [(4, 1), (5, 6), (6, 5), (5, 5), (7, 1), (8, 6), (7, 5), (7, 0), (7, 6), (4, 0), (4, 6), (7, 4), (6, 4), (5, 4)]:
         assert clv.march_probe(300, 200, r, w, 16, 2) > 0.0
     with pytest.raises(clv.CloverError, match="reads, writes"):
         clv.march_probe(300, 200, 3, 3, 16, 1)
