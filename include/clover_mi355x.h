@@ -364,7 +364,11 @@ clv_status clv_timestep(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st
 /* One step of driver/hydro.cpp:57-89 (timestep, PdV predict, accelerate, PdV
  * correct, flux_calc, advection, reset_field, advect_x flip, time += dt); sets
  * st->complete when driver/hydro.cpp:108 says so.  Returns CLV_ERR_SMALL_DT
- * when dt < dtmin. */
+ * when dt < dtmin.  With CLV_PLAN_FUSED the state the next step reads equals the
+ * reference plan's (density0 and energy0 on the interior cells, xvel0 and yvel0 on
+ * the interior and boundary nodes, the volume fluxes); pressure, soundspeed,
+ * viscosity, the mass fluxes, the 1-fields and the work arrays hold other values
+ * (clv_hydro_summary recomputes what the summary reads). */
 clv_status clv_hydro_step(clv_chunk *c, const clv_config *cfg, clv_hydro_state *st, int plan);
 /* field_summary with its ideal_gas (src/serial/field_summary.cpp:51-99).  With the
  * tree summation order the two run as one pass over the cells: pressure and
