@@ -1063,8 +1063,11 @@ __device__ __forceinline__ void cm_y_tile(const V &d, int bx, int j0, int j1, in
   }
 }
 
+// 4 waves per SIMD (the bound's second argument is the minimum waves per SIMD) with
+// either block size: sweep 1 takes 128 VGPRs and spills 12-36 bytes a lane; bounded at
+// 2, its 8-walk form took 130-136 VGPRs, 3 waves and one block a CU, 2 % slower
 template <int SWEEP, bool MESH, bool RESET = false, int NW = WAVES, bool STORE_MF = true>
-__global__ __launch_bounds__(64 * NW, 16 / NW) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
+__global__ __launch_bounds__(64 * NW, 4) void cm_y_k(Dev d, int nstrips, int seg, double *rho_out, double *e_out, double *u_out, double *v_out) {
   int bx, by;
   xcd_tile(bx, by);
   const int j0 = 2 + seg * by;
